@@ -1,0 +1,233 @@
+/*
+ * pmg.h -- C ABI of the MI355X-native matrix-free multigrid library (libpmg.so).
+ *
+ * Drop-in boundary for the hot path of gshubhamk/portable-multigrid: the
+ * BK3-style continuous Q_p Laplace operator, the SIP-DG operator, h-/p- and
+ * DG->CG transfers, the Chebyshev/Jacobi smoother, the V-cycle and the
+ * (mixed-precision) preconditioned CG solve.  Every entry point names the
+ * reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *  - Return value: PMG_OK (0) or a PMG_ERR_* status; pmg_last_error() gives a
+ *    message (thread-local).  No C++ exception crosses this boundary.
+ *  - Vectors are caller-owned DEVICE pointers (hipMalloc / torch.cuda) of the
+ *    handle's precision (double for PMG_FP64, float for PMG_FP32), length m().
+ *  - Setup arrays (DoF maps, vertices, weights, G tensors) are HOST pointers;
+ *    the library copies them to device memory it owns.  Apply calls allocate
+ *    nothing.
+ *  - stream: a hipStream_t (NULL = the default stream).  All work is
+ *    stream-ordered and asynchronous unless stated otherwise.  A handle must
+ *    not be used from two host threads at once.
+ *  - DoF maps are lexicographic per cell (x fastest), [n_cells][(p+1)^3],
+ *    with 0xFFFFFFFF marking a constrained (Dirichlet) DoF, exactly as the
+ *    reference's dof_indices views (portable_laplace_operator_bk3.h:360-437).
+ */
+#ifndef PMG_H
+#define PMG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMG_INVALID_DOF 0xFFFFFFFFu
+
+enum pmg_status {
+  PMG_OK = 0,
+  PMG_ERR_INVALID_ARG = 1,
+  PMG_ERR_SIZE = 2,
+  PMG_ERR_HIP = 3,
+  PMG_ERR_NOT_CONVERGED = 4,
+  PMG_ERR_UNSUPPORTED = 5,
+  PMG_ERR_ALLOC = 6
+};
+
+enum pmg_precision { PMG_FP64 = 0, PMG_FP32 = 1 };
+
+enum pmg_transfer_kind { PMG_TRANSFER_H = 0, PMG_TRANSFER_P = 1, PMG_TRANSFER_DG_CG = 2 };
+
+enum pmg_precond_kind { PMG_PRECOND_NONE = 0, PMG_PRECOND_JACOBI = 1, PMG_PRECOND_MG = 2 };
+
+enum pmg_coarse_kind { PMG_COARSE_CHEBYSHEV = 0, PMG_COARSE_PCG = 1 };
+
+typedef struct pmg_operator_s *pmg_operator;
+typedef struct pmg_transfer_s *pmg_transfer;
+typedef struct pmg_smoother_s *pmg_smoother;
+typedef struct pmg_multigrid_s *pmg_multigrid;
+typedef void *pmg_stream;
+
+/* ------------------------------------------------------------ library */
+/* Message of the last failing call on this thread ("" if none). */
+const char *pmg_last_error(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int pmg_version(void);
+/* hipDeviceSynchronize(); reports asynchronous kernel errors. */
+int pmg_device_synchronize(void);
+
+/* ------------------------------------------------------------ structured meshes
+ * Host-side helpers that build the inputs deal.II's DoFHandler/MatrixFree
+ * produce in the reference (tests/poisson_cube_bk3/program.cc:281-566): a
+ * box [lo,hi] of nx*ny*nz hexahedra, optionally displaced by
+ * x += a sin(2 pi y) sin(2 pi z) and cyclic permutations (deform = 1), with a
+ * lexicographic global numbering on the (p nx+1)(p ny+1)(p nz+1) grid.
+ * All outputs are caller-allocated host arrays. */
+typedef struct {
+  int nx, ny, nz;
+  double lo[3], hi[3];
+  int deform;
+  double amplitude;
+} pmg_box;
+
+int64_t pmg_box_n_cells(const pmg_box *box);
+int64_t pmg_box_n_dofs(const pmg_box *box, int degree);
+/* [C][8][3], vertices of each cell in lexicographic order */
+int pmg_box_cell_vertices(const pmg_box *box, double *cell_vertices);
+/* [C][(p+1)^3]; dirichlet != 0 marks boundary DoFs 0xFFFFFFFF */
+int pmg_box_dofmap(const pmg_box *box, int degree, int dirichlet, uint32_t *dofmap);
+/* boundary DoF list; pass NULL to query the count in *n */
+int pmg_box_boundary_dofs(const pmg_box *box, int degree, uint32_t *dofs, int64_t *n);
+/* h-transfer patch maps of GeometricTransfer<dim,p>::setup_dof_indices /
+ * setup_weights (include/multigrid/portable_geometric_transfer.h:1344-1486):
+ * coarse map [C_c][(p+1)^3] (constrained -> invalid), fine patch map
+ * [C_c][(2p+1)^3] (plain), weights [C_c][(2p+1)^3] (1/multiplicity, 0 on
+ * constrained fine DoFs).  The fine box is the coarse box refined once. */
+int pmg_box_h_transfer_maps(const pmg_box *coarse, int degree, uint32_t *map_coarse,
+                            uint32_t *map_fine, double *weights);
+/* p-transfer maps of PolynomialTransfer<dim,pc,pf>::setup_weights_and_dof_indices
+ * (include/multigrid/portable_polynomial_transfer.h:1155-1360) */
+int pmg_box_p_transfer_maps(const pmg_box *box, int degree_coarse, int degree_fine,
+                            uint32_t *map_coarse, uint32_t *map_fine, double *weights);
+
+/* ------------------------------------------------------------ operator
+ * Replaces Portable::LaplaceOperatorBK3<dim,p,number>
+ * (include/operators/portable_laplace_operator_bk3.h:19-96, ctor :104-126) and
+ * the LaplaceOperatorBase<dim,Number> interface
+ * (include/base/portable_laplace_operator_base.h:16-61).  dim = 3,
+ * 1 <= degree <= 7, degree+1 <= n_q_points_1d <= 8 (the reference uses
+ * n_q_points_1d = degree+1, portable_laplace_operator_bk3.h:120).
+ * Geometry: the trilinear map of each cell's 8 vertices (what MappingQ(p)
+ * reproduces on straight-sided hexahedra), G computed on the device
+ * (compute_G_tensors, portable_laplace_operator_bk3.h:456-515). */
+int pmg_operator_create(int dim, int degree, int n_q_points_1d, int precision,
+                        int64_t n_cells, const uint32_t *dofmap, int64_t n_dofs,
+                        const uint32_t *constrained_dofs, int64_t n_constrained,
+                        const double *cell_vertices, pmg_operator *op);
+/* Same, with G[C][6][nq^3] (xx,xy,xz,yy,yz,zz, already times JxW) given. */
+int pmg_operator_create_from_G(int dim, int degree, int n_q_points_1d, int precision,
+                               int64_t n_cells, const uint32_t *dofmap, int64_t n_dofs,
+                               const uint32_t *constrained_dofs, int64_t n_constrained,
+                               const double *G, pmg_operator *op);
+int pmg_operator_destroy(pmg_operator op);
+/* m() / n() (portable_laplace_operator_bk3.h:565-577) */
+int pmg_operator_m(pmg_operator op, int64_t *m);
+int pmg_operator_info(pmg_operator op, int *degree, int *n_q_points_1d, int *precision,
+                      int64_t *n_cells);
+/* vmult: dst = A src with dst_c = src_c on constrained rows
+ * (LaplaceOperatorBK3::vmult, portable_laplace_operator_bk3.h:128-231).
+ * src and dst must not alias. */
+int pmg_operator_vmult(pmg_operator op, void *dst, const void *src, pmg_stream stream);
+/* Tvmult == vmult (portable_laplace_operator_bk3.h:440-453) */
+int pmg_operator_Tvmult(pmg_operator op, void *dst, const void *src, pmg_stream stream);
+/* dst += sum_cells A_e src, no framing (the raw cell loop a subdomain
+ * operator such as SubdomainLaplaceOperator::vmult_bk3 calls) */
+int pmg_operator_apply_add(pmg_operator op, void *dst, const void *src, pmg_stream stream);
+/* compute_diagonal(): diag(A), 1 on constrained rows, then inverted
+ * (portable_laplace_operator_bk3.h:533-561). Synchronous. */
+int pmg_operator_compute_diagonal(pmg_operator op);
+/* get_matrix_diagonal_inverse(): copies the inverse diagonal into out (device, m()) */
+int pmg_operator_diagonal_inverse(pmg_operator op, void *out, pmg_stream stream);
+/* Variable coefficient a(u,v) = sum_q kappa_q grad u . grad v JxW: kappa is a
+ * DEVICE array [C][nq^3] of doubles; G <- kappa * G_geometry.  Requires an
+ * operator created from cell vertices.  kappa = NULL restores kappa = 1. */
+int pmg_operator_set_coefficient(pmg_operator op, const double *kappa, pmg_stream stream);
+/* Adjoint gradient: out[c][q] = -(grad_xi u)^T G_q (grad_xi lambda) with the
+ * operator's current G (= kappa G_geometry), i.e. dJ/dg_q for kappa = exp(g)
+ * when lambda solves A lambda = dJ/du.  u, lambda: device vectors of the
+ * operator precision; out: device [C][nq^3] of the operator precision.
+ * No reference counterpart (new in this library). */
+int pmg_operator_gradient_coefficient(pmg_operator op, const void *u, const void *lambda,
+                                      void *out, pmg_stream stream);
+/* Bytes the vmult streams under the algorithmic model s(2N + 6 C nq^3). */
+int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops);
+
+/* ------------------------------------------------------------ transfers
+ * Replaces Portable::MGTransferBase<dim,number> (include/base/portable_mg_transfer_base.h:15-36)
+ * as implemented by GeometricTransfer<dim,p,number> (h), PolynomialTransfer<dim,pc,pf,number> (p)
+ * and ContinuousTransfer<dim,p,number> (DG->CG).
+ * kind H: degree_fine must equal degree_coarse; per coarse cell a (2p+1)^3 patch.
+ * kind P: same mesh, degree_coarse < degree_fine.
+ * kind DG_CG: same mesh and degree; map_coarse is the CG map (invalid on
+ *   constrained), map_fine the DG map (cell-major), weights may be NULL.
+ * prolongation_scale multiplies the weights of the prolongation only: 1.0 is
+ * the mathematical P; 2.0 reproduces the reference HEAD, whose h-prolongation
+ * launches two kernels into the same vector (portable_geometric_transfer.h:805-855). */
+int pmg_transfer_create(int kind, int precision, int degree_coarse, int degree_fine,
+                        int64_t n_cells, const uint32_t *map_coarse, const uint32_t *map_fine,
+                        const double *weights, int64_t n_dofs_coarse, int64_t n_dofs_fine,
+                        double prolongation_scale, pmg_transfer *tr);
+int pmg_transfer_destroy(pmg_transfer tr);
+/* dst_fine += P src_coarse (prolongate_and_add) */
+int pmg_transfer_prolongate_and_add(pmg_transfer tr, void *dst_fine, const void *src_coarse,
+                                    pmg_stream stream);
+/* dst_coarse += P^T src_fine (restrict_and_add) */
+int pmg_transfer_restrict_and_add(pmg_transfer tr, void *dst_coarse, const void *src_fine,
+                                  pmg_stream stream);
+
+/* ------------------------------------------------------------ smoother
+ * Replaces deal.II PreconditionChebyshev<LaplaceOperatorBase, Vector> with the
+ * operator's inverse diagonal as preconditioner, configured as at
+ * tests/poisson_cube_bk3/program.cc:474-515.  degree <= 0 selects the automatic
+ * degree for the relative accuracy smoothing_range < 1 (coarse level).
+ * Eigenvalues are estimated at creation (Lanczos through eig_cg_n_iterations
+ * CG steps on P^{-1}A); compute_diagonal() must have been called. Synchronous. */
+int pmg_chebyshev_create(pmg_operator op, int degree, double smoothing_range,
+                         int eig_cg_n_iterations, pmg_smoother *sm);
+int pmg_chebyshev_destroy(pmg_smoother sm);
+/* vmult: zero initial guess (dst overwritten) */
+int pmg_chebyshev_vmult(pmg_smoother sm, void *dst, const void *src, pmg_stream stream);
+/* step: dst is the initial guess */
+int pmg_chebyshev_step(pmg_smoother sm, void *dst, const void *src, pmg_stream stream);
+int pmg_chebyshev_info(pmg_smoother sm, double *max_eigenvalue, double *min_eigenvalue,
+                       int *degree);
+
+/* ------------------------------------------------------------ multigrid
+ * Replaces VCycleMultigrid<dim,number,Transfer>::vmult
+ * (include/multigrid/portable_v_cycle_multigrid.h:163-206) and
+ * MultigridSolver::vmult (tests/common/portable_multigrid_solver.h:171-271).
+ * Level l = 0 is the coarsest.  transfers[l] maps level l-1 -> l
+ * (transfers[0] ignored), smoothers[l] for l >= 1.  All levels share one
+ * precision (the V-cycle precision); outer_precision is the precision of the
+ * vectors passed to pmg_multigrid_vmult (FP64 outer with FP32 levels is the
+ * mixed-precision preconditioner; conversion happens at the V-cycle boundary).
+ * coarse: PMG_COARSE_PCG (Jacobi-PCG to coarse_tol relative) or
+ * PMG_COARSE_CHEBYSHEV (Chebyshev auto-degree to 1e-3, the reference). */
+int pmg_multigrid_create(int n_levels, const pmg_operator *level_operators,
+                         const pmg_transfer *transfers, const pmg_smoother *smoothers,
+                         int coarse_kind, double coarse_tol, int outer_precision,
+                         pmg_multigrid *mg);
+int pmg_multigrid_destroy(pmg_multigrid mg);
+int pmg_multigrid_vmult(pmg_multigrid mg, void *dst, const void *src, pmg_stream stream);
+
+/* ------------------------------------------------------------ CG
+ * Preconditioned CG of deal.II SolverCG as driven by MultigridSolver::solve_cg
+ * (tests/common/portable_multigrid_solver.h:147-169): x is the initial guess
+ * (device, op precision), stop when ||r|| <= rel_tol ||r_0|| or max_it.
+ * precond: NONE, JACOBI (op's inverse diagonal) or MG (mg, outer precision
+ * = op precision).  Returns PMG_ERR_NOT_CONVERGED if max_it was reached. */
+int pmg_cg_solve(pmg_operator op, int precond, pmg_multigrid mg, const void *b, void *x,
+                 double rel_tol, int max_it, int *iterations, double *res0, double *res,
+                 pmg_stream stream);
+
+/* ------------------------------------------------------------ vectors (device)
+ * Small helpers for callers without a device-vector library. */
+int pmg_vector_dot(int precision, int64_t n, const void *a, const void *b, double *result,
+                   pmg_stream stream);
+int pmg_vector_convert(int64_t n, int dst_precision, void *dst, int src_precision,
+                       const void *src, pmg_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMG_H */
