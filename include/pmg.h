@@ -149,6 +149,18 @@ int pmg_operator_set_coefficient(pmg_operator op, const double *kappa, pmg_strea
  * No reference counterpart (new in this library). */
 int pmg_operator_gradient_coefficient(pmg_operator op, const void *u, const void *lambda,
                                       void *out, pmg_stream stream);
+/* Right-hand side b_i = int phi_i f dx with Gauss(n_q_points_1d) (<= 0: degree+1)
+ * on the operator's cells, constrained entries 0 (LaplaceProblem::compute_rhs,
+ * tests/poisson_cube_bk3/program.cc:517-566).  f_kind 0: f = 1 (the reference);
+ * 1: f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z).  b: device, op precision.
+ * Needs an operator created from cell vertices.  Synchronous. */
+int pmg_operator_assemble_rhs(pmg_operator op, int f_kind, int n_q_points_1d, void *b,
+                              pmg_stream stream);
+/* ||u_h - u||_L2 with Gauss(n_q_points_1d) (<= 0: degree+2), u_kind 0: u = 0,
+ * 1: u = sin(pi x) sin(pi y) sin(pi z) (VectorTools::integrate_difference,
+ * source/geometric_multigrid_bk3/program.cc:391-400).  Synchronous. */
+int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_points_1d,
+                          double *error, pmg_stream stream);
 /* Bytes the vmult streams under the algorithmic model s(2N + 6 C nq^3). */
 int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops);
 

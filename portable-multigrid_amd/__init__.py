@@ -1,0 +1,492 @@
+"""MI355X-native matrix-free multigrid hot path (Python host mirror).
+
+Thin ctypes layer over libpmg.so (gfx950 HIP kernels + C++ host classes,
+C ABI in include/pmg.h).  The classes mirror the reference's interfaces
+(gshubhamk/portable-multigrid):
+
+  LaplaceOperatorBK3   include/operators/portable_laplace_operator_bk3.h
+  GeometricTransfer    include/multigrid/portable_geometric_transfer.h
+  PolynomialTransfer   include/multigrid/portable_polynomial_transfer.h
+  PreconditionChebyshev  deal.II PreconditionChebyshev as configured in
+                       tests/poisson_cube_bk3/program.cc:474-515
+  VCycleMultigrid      include/multigrid/portable_v_cycle_multigrid.h
+  solve_cg             tests/common/portable_multigrid_solver.h:147-169
+
+Vectors are torch tensors on the GPU (float64 for FP64 handles, float32 for
+FP32 handles); torch is only the device-memory and stream provider.  There is
+no CPU fallback: if libpmg.so is missing every call raises.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmg.so")
+
+FP64, FP32 = 0, 1
+TRANSFER_H, TRANSFER_P, TRANSFER_DG_CG = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
+COARSE_CHEBYSHEV, COARSE_PCG = 0, 1
+INVALID_DOF = 0xFFFFFFFF
+
+OK, ERR_INVALID_ARG, ERR_SIZE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED, ERR_ALLOC = range(7)
+
+_lib = None
+
+
+class PMGError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"libpmg status {status}: {msg}")
+        self.status = status
+
+
+def build(force=False, jobs=8):
+    """Compile libpmg.so for gfx950 with the package Makefile (hipcc)."""
+    cmd = ["make", "-s", "-C", _HERE, f"-j{jobs}"]
+    if force:
+        cmd.append("-B")
+    subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+class Box(ctypes.Structure):
+    """Structured box mesh description (pmg_box)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+                ("deform", ctypes.c_int), ("amplitude", ctypes.c_double)]
+
+
+def box(nx, ny=None, nz=None, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform=0, amplitude=0.0):
+    b = Box()
+    b.nx, b.ny, b.nz = nx, nx if ny is None else ny, nx if nz is None else nz
+    b.lo[:] = lo
+    b.hi[:] = hi
+    b.deform, b.amplitude = deform, amplitude
+    return b
+
+
+def refine(b):
+    return box(2 * b.nx, 2 * b.ny, 2 * b.nz, tuple(b.lo), tuple(b.hi), b.deform, b.amplitude)
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_BP = ctypes.POINTER(Box)
+
+
+def lib():
+    """Load libpmg.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run portable-multigrid_amd build() "
+                           "(hipcc --offload-arch=gfx950) first; there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    sig = {
+        "pmg_last_error": (ctypes.c_char_p, []),
+        "pmg_version": (_i, []),
+        "pmg_device_synchronize": (_i, []),
+        "pmg_box_n_cells": (_i64, [_BP]),
+        "pmg_box_n_dofs": (_i64, [_BP, _i]),
+        "pmg_box_cell_vertices": (_i, [_BP, _vp]),
+        "pmg_box_dofmap": (_i, [_BP, _i, _i, _vp]),
+        "pmg_box_boundary_dofs": (_i, [_BP, _i, _vp, P(_i64)]),
+        "pmg_box_h_transfer_maps": (_i, [_BP, _i, _vp, _vp, _vp]),
+        "pmg_box_p_transfer_maps": (_i, [_BP, _i, _i, _vp, _vp, _vp]),
+        "pmg_operator_create": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp, P(_vp)]),
+        "pmg_operator_create_from_G": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp,
+                                            P(_vp)]),
+        "pmg_operator_destroy": (_i, [_vp]),
+        "pmg_operator_m": (_i, [_vp, P(_i64)]),
+        "pmg_operator_info": (_i, [_vp, P(_i), P(_i), P(_i), P(_i64)]),
+        "pmg_operator_vmult": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_operator_Tvmult": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_operator_apply_add": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_operator_compute_diagonal": (_i, [_vp]),
+        "pmg_operator_diagonal_inverse": (_i, [_vp, _vp, _vp]),
+        "pmg_operator_set_coefficient": (_i, [_vp, _vp, _vp]),
+        "pmg_operator_gradient_coefficient": (_i, [_vp, _vp, _vp, _vp, _vp]),
+        "pmg_operator_algorithmic_bytes": (_i, [_vp, P(_d), P(_d)]),
+        "pmg_operator_assemble_rhs": (_i, [_vp, _i, _i, _vp, _vp]),
+        "pmg_operator_l2_error": (_i, [_vp, _vp, _i, _i, P(_d), _vp]),
+        "pmg_transfer_create": (_i, [_i, _i, _i, _i, _i64, _vp, _vp, _vp, _i64, _i64, _d,
+                                     P(_vp)]),
+        "pmg_transfer_destroy": (_i, [_vp]),
+        "pmg_transfer_prolongate_and_add": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_transfer_restrict_and_add": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_chebyshev_create": (_i, [_vp, _i, _d, _i, P(_vp)]),
+        "pmg_chebyshev_destroy": (_i, [_vp]),
+        "pmg_chebyshev_vmult": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_chebyshev_step": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_chebyshev_info": (_i, [_vp, P(_d), P(_d), P(_i)]),
+        "pmg_multigrid_create": (_i, [_i, _vp, _vp, _vp, _i, _d, _i, P(_vp)]),
+        "pmg_multigrid_destroy": (_i, [_vp]),
+        "pmg_multigrid_vmult": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_cg_solve": (_i, [_vp, _i, _vp, _vp, _vp, _d, _i, P(_i), P(_d), P(_d), _vp]),
+        "pmg_vector_dot": (_i, [_i, _i64, _vp, _vp, P(_d), _vp]),
+        "pmg_vector_convert": (_i, [_i64, _i, _vp, _i, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != OK:
+        raise PMGError(status, lib().pmg_last_error().decode())
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev_ptr(t, precision, n=None):
+    import torch
+    want = torch.float64 if precision == FP64 else torch.float32
+    if not (t.is_cuda and t.dtype == want and t.is_contiguous()):
+        raise PMGError(ERR_INVALID_ARG, f"expected a contiguous CUDA {want} tensor, got "
+                       f"{t.dtype} on {t.device}")
+    if n is not None and t.numel() != n:
+        raise PMGError(ERR_SIZE, f"vector has {t.numel()} entries, expected {n}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------------ meshes
+def box_n_cells(b):
+    return lib().pmg_box_n_cells(ctypes.byref(b))
+
+
+def box_n_dofs(b, p):
+    return lib().pmg_box_n_dofs(ctypes.byref(b), p)
+
+
+def box_cell_vertices(b):
+    out = np.zeros(box_n_cells(b) * 24)
+    _check(lib().pmg_box_cell_vertices(ctypes.byref(b), _np_ptr(out)))
+    return out
+
+
+def box_dofmap(b, p, dirichlet=True):
+    out = np.zeros(box_n_cells(b) * (p + 1) ** 3, dtype=np.uint32)
+    _check(lib().pmg_box_dofmap(ctypes.byref(b), p, int(dirichlet), _np_ptr(out)))
+    return out
+
+
+def box_boundary_dofs(b, p):
+    n = ctypes.c_int64()
+    _check(lib().pmg_box_boundary_dofs(ctypes.byref(b), p, None, ctypes.byref(n)))
+    out = np.zeros(n.value, dtype=np.uint32)
+    _check(lib().pmg_box_boundary_dofs(ctypes.byref(b), p, _np_ptr(out), ctypes.byref(n)))
+    return out
+
+
+def box_h_transfer_maps(coarse, p):
+    C = box_n_cells(coarse)
+    mc = np.zeros(C * (p + 1) ** 3, dtype=np.uint32)
+    mf = np.zeros(C * (2 * p + 1) ** 3, dtype=np.uint32)
+    w = np.zeros(C * (2 * p + 1) ** 3)
+    _check(lib().pmg_box_h_transfer_maps(ctypes.byref(coarse), p, _np_ptr(mc), _np_ptr(mf),
+                                         _np_ptr(w)))
+    return mc, mf, w
+
+
+def box_p_transfer_maps(b, pc, pf):
+    C = box_n_cells(b)
+    mc = np.zeros(C * (pc + 1) ** 3, dtype=np.uint32)
+    mf = np.zeros(C * (pf + 1) ** 3, dtype=np.uint32)
+    w = np.zeros(C * (pf + 1) ** 3)
+    _check(lib().pmg_box_p_transfer_maps(ctypes.byref(b), pc, pf, _np_ptr(mc), _np_ptr(mf),
+                                         _np_ptr(w)))
+    return mc, mf, w
+
+
+# ------------------------------------------------------------------ operator
+class LaplaceOperatorBK3:
+    """Continuous Q_p Laplace operator on the GPU (LaplaceOperatorBK3<3,p,number>).
+
+    Build from a structured box (``LaplaceOperatorBK3(box, p)``) or from raw
+    arrays (``LaplaceOperatorBK3.from_arrays``)."""
+
+    def __init__(self, b=None, degree=1, n_q_points_1d=None, precision=FP64, _raw=None):
+        self.degree = degree
+        self.nq = degree + 1 if n_q_points_1d is None else n_q_points_1d
+        self.precision = precision
+        self.h = ctypes.c_void_p()
+        if _raw is None:
+            self.box = b
+            dofmap = box_dofmap(b, degree, True)
+            cons = box_boundary_dofs(b, degree)
+            cv = box_cell_vertices(b)
+            n_cells, n_dofs = box_n_cells(b), box_n_dofs(b, degree)
+            _check(lib().pmg_operator_create(3, degree, self.nq, precision, n_cells,
+                                             _np_ptr(dofmap), n_dofs, _np_ptr(cons), len(cons),
+                                             _np_ptr(cv), ctypes.byref(self.h)))
+        else:
+            dofmap, n_dofs, cons, cv, G = _raw
+            dofmap = np.ascontiguousarray(dofmap, dtype=np.uint32)
+            cons = np.ascontiguousarray(cons, dtype=np.uint32)
+            n_cells = len(dofmap) // (degree + 1) ** 3
+            if G is not None:
+                G = np.ascontiguousarray(G, dtype=np.float64)
+                _check(lib().pmg_operator_create_from_G(3, degree, self.nq, precision, n_cells,
+                                                        _np_ptr(dofmap), n_dofs, _np_ptr(cons),
+                                                        len(cons), _np_ptr(G),
+                                                        ctypes.byref(self.h)))
+            else:
+                cv = np.ascontiguousarray(cv, dtype=np.float64)
+                _check(lib().pmg_operator_create(3, degree, self.nq, precision, n_cells,
+                                                 _np_ptr(dofmap), n_dofs, _np_ptr(cons),
+                                                 len(cons), _np_ptr(cv), ctypes.byref(self.h)))
+        self.n_cells = n_cells
+        self._m = n_dofs
+
+    @classmethod
+    def from_arrays(cls, degree, dofmap, n_dofs, constrained, cell_vertices=None, G=None,
+                    n_q_points_1d=None, precision=FP64):
+        return cls(None, degree, n_q_points_1d, precision,
+                   _raw=(dofmap, n_dofs, constrained, cell_vertices, G))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.pmg_operator_destroy(self.h)
+            self.h = None
+
+    def m(self):
+        return self._m
+
+    def n(self):
+        return self._m
+
+    @property
+    def dtype(self):
+        import torch
+        return torch.float64 if self.precision == FP64 else torch.float32
+
+    def initialize_dof_vector(self):
+        import torch
+        return torch.zeros(self._m, dtype=self.dtype, device="cuda")
+
+    def vmult(self, dst, src):
+        _check(lib().pmg_operator_vmult(self.h, _dev_ptr(dst, self.precision, self._m),
+                                        _dev_ptr(src, self.precision, self._m), _stream()))
+
+    def Tvmult(self, dst, src):
+        _check(lib().pmg_operator_Tvmult(self.h, _dev_ptr(dst, self.precision, self._m),
+                                         _dev_ptr(src, self.precision, self._m), _stream()))
+
+    def apply_add(self, dst, src):
+        _check(lib().pmg_operator_apply_add(self.h, _dev_ptr(dst, self.precision, self._m),
+                                            _dev_ptr(src, self.precision, self._m), _stream()))
+
+    def compute_diagonal(self):
+        _check(lib().pmg_operator_compute_diagonal(self.h))
+
+    def get_matrix_diagonal_inverse(self):
+        out = self.initialize_dof_vector()
+        _check(lib().pmg_operator_diagonal_inverse(self.h, _dev_ptr(out, self.precision),
+                                                   _stream()))
+        return out
+
+    def set_coefficient(self, kappa):
+        """kappa: CUDA float64 tensor [n_cells * nq^3] or None (kappa = 1)."""
+        if kappa is None:
+            _check(lib().pmg_operator_set_coefficient(self.h, None, _stream()))
+        else:
+            _check(lib().pmg_operator_set_coefficient(
+                self.h, _dev_ptr(kappa, FP64, self.n_cells * self.nq ** 3), _stream()))
+
+    def gradient_coefficient(self, u, lam, out):
+        _check(lib().pmg_operator_gradient_coefficient(
+            self.h, _dev_ptr(u, self.precision, self._m), _dev_ptr(lam, self.precision, self._m),
+            _dev_ptr(out, self.precision, self.n_cells * self.nq ** 3), _stream()))
+
+    def assemble_rhs(self, f_kind=1, n_q_points_1d=0):
+        """b = int phi_i f (f_kind 0: f = 1, 1: 3 pi^2 sin sin sin), constrained rows 0."""
+        b = self.initialize_dof_vector()
+        _check(lib().pmg_operator_assemble_rhs(self.h, f_kind, n_q_points_1d,
+                                               _dev_ptr(b, self.precision), _stream()))
+        return b
+
+    def l2_error(self, u, u_kind=1, n_q_points_1d=0):
+        e = ctypes.c_double()
+        _check(lib().pmg_operator_l2_error(self.h, _dev_ptr(u, self.precision, self._m), u_kind,
+                                           n_q_points_1d, ctypes.byref(e), _stream()))
+        return e.value
+
+    def algorithmic_bytes(self):
+        b, f = ctypes.c_double(), ctypes.c_double()
+        _check(lib().pmg_operator_algorithmic_bytes(self.h, ctypes.byref(b), ctypes.byref(f)))
+        return b.value, f.value
+
+
+# ------------------------------------------------------------------ transfers
+class _Transfer:
+    def __init__(self, kind, precision, pc, pf, mc, mf, w, n_c, n_f, scale):
+        self.precision = precision
+        self.n_c, self.n_f = n_c, n_f
+        self.h = ctypes.c_void_p()
+        mc = np.ascontiguousarray(mc, dtype=np.uint32)
+        mf = np.ascontiguousarray(mf, dtype=np.uint32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        n_cells = len(mc) // (pc + 1) ** 3
+        _check(lib().pmg_transfer_create(kind, precision, pc, pf, n_cells, _np_ptr(mc),
+                                         _np_ptr(mf), _np_ptr(w), n_c, n_f, scale,
+                                         ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.pmg_transfer_destroy(self.h)
+            self.h = None
+
+    def prolongate_and_add(self, dst, src):
+        _check(lib().pmg_transfer_prolongate_and_add(
+            self.h, _dev_ptr(dst, self.precision, self.n_f), _dev_ptr(src, self.precision, self.n_c),
+            _stream()))
+
+    def restrict_and_add(self, dst, src):
+        _check(lib().pmg_transfer_restrict_and_add(
+            self.h, _dev_ptr(dst, self.precision, self.n_c), _dev_ptr(src, self.precision, self.n_f),
+            _stream()))
+
+
+class GeometricTransfer(_Transfer):
+    """h-transfer coarse box -> once-refined box at degree p (GeometricTransfer<3,p>).
+
+    prolongation_scale = 2.0 reproduces the reference HEAD's doubled
+    h-prolongation (portable_geometric_transfer.h:805-855)."""
+
+    def __init__(self, coarse_box, degree, precision=FP64, prolongation_scale=1.0):
+        mc, mf, w = box_h_transfer_maps(coarse_box, degree)
+        super().__init__(TRANSFER_H, precision, degree, degree, mc, mf, w,
+                         box_n_dofs(coarse_box, degree), box_n_dofs(refine(coarse_box), degree),
+                         prolongation_scale)
+
+
+class PolynomialTransfer(_Transfer):
+    """p-transfer on one box, degree pc -> pf (PolynomialTransfer<3,pc,pf>)."""
+
+    def __init__(self, b, pc, pf, precision=FP64):
+        mc, mf, w = box_p_transfer_maps(b, pc, pf)
+        super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, w, box_n_dofs(b, pc),
+                         box_n_dofs(b, pf), 1.0)
+
+
+# ------------------------------------------------------------------ smoother
+class PreconditionChebyshev:
+    """Chebyshev smoother over the operator's inverse diagonal (deal.II semantics)."""
+
+    def __init__(self, op, degree=3, smoothing_range=15.0, eig_cg_n_iterations=10):
+        self.op = op
+        self.h = ctypes.c_void_p()
+        _check(lib().pmg_chebyshev_create(op.h, degree, smoothing_range, eig_cg_n_iterations,
+                                          ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.pmg_chebyshev_destroy(self.h)
+            self.h = None
+
+    def vmult(self, dst, src):
+        p, m = self.op.precision, self.op.m()
+        _check(lib().pmg_chebyshev_vmult(self.h, _dev_ptr(dst, p, m), _dev_ptr(src, p, m),
+                                         _stream()))
+
+    def step(self, dst, src):
+        p, m = self.op.precision, self.op.m()
+        _check(lib().pmg_chebyshev_step(self.h, _dev_ptr(dst, p, m), _dev_ptr(src, p, m),
+                                        _stream()))
+
+    def info(self):
+        mx, mn, deg = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(lib().pmg_chebyshev_info(self.h, ctypes.byref(mx), ctypes.byref(mn),
+                                        ctypes.byref(deg)))
+        return mx.value, mn.value, deg.value
+
+
+# ------------------------------------------------------------------ multigrid
+class VCycleMultigrid:
+    """V-cycle over levels 0 (coarsest) .. L-1 (VCycleMultigrid / MultigridSolver::vmult)."""
+
+    def __init__(self, operators, transfers, smoothers, coarse=COARSE_PCG, coarse_tol=1e-12,
+                 outer_precision=FP64):
+        L = len(operators)
+        self._keep = (operators, transfers, smoothers)
+        self.outer_precision = outer_precision
+        self.n_fine = operators[-1].m()
+        ops = (ctypes.c_void_p * L)(*[o.h.value for o in operators])
+        trs = (ctypes.c_void_p * L)(*[(t.h.value if t is not None else None) for t in transfers])
+        sms = (ctypes.c_void_p * L)(*[(s.h.value if s is not None else None) for s in smoothers])
+        self.h = ctypes.c_void_p()
+        _check(lib().pmg_multigrid_create(L, ops, trs, sms, coarse, coarse_tol, outer_precision,
+                                          ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.pmg_multigrid_destroy(self.h)
+            self.h = None
+
+    def vmult(self, dst, src):
+        p = self.outer_precision
+        _check(lib().pmg_multigrid_vmult(self.h, _dev_ptr(dst, p, self.n_fine),
+                                         _dev_ptr(src, p, self.n_fine), _stream()))
+
+
+def solve_cg(op, b, x, precond=PRECOND_JACOBI, mg=None, rel_tol=1e-10, max_it=1000,
+             raise_on_failure=True):
+    """Preconditioned CG (deal.II SolverCG semantics); returns (iterations, res0, res)."""
+    its, r0, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    st = lib().pmg_cg_solve(op.h, precond, mg.h if mg is not None else None,
+                            _dev_ptr(b, op.precision, op.m()), _dev_ptr(x, op.precision, op.m()),
+                            rel_tol, max_it, ctypes.byref(its), ctypes.byref(r0), ctypes.byref(r),
+                            _stream())
+    if st != OK and (raise_on_failure or st != ERR_NOT_CONVERGED):
+        _check(st)
+    return its.value, r0.value, r.value
+
+
+def vector_dot(a, b, precision=FP64):
+    out = ctypes.c_double()
+    _check(lib().pmg_vector_dot(precision, a.numel(), _dev_ptr(a, precision),
+                                _dev_ptr(b, precision), ctypes.byref(out), _stream()))
+    return out.value
+
+
+# ------------------------------------------------------------------ hierarchies
+def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precision=FP64,
+                        smoother_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
+                        coarse=COARSE_PCG, coarse_tol=1e-12, prolongation_scale=1.0,
+                        outer_precision=None):
+    """h-only hierarchy at fixed degree (source/geometric_multigrid_bk3/program.cc):
+    level 0 has fine_box coarsened n_levels-1 times."""
+    boxes = [fine_box]
+    for _ in range(n_levels - 1):
+        b = boxes[0]
+        assert b.nx % 2 == 0 and b.ny % 2 == 0 and b.nz % 2 == 0, "box not coarsenable"
+        boxes.insert(0, box(b.nx // 2, b.ny // 2, b.nz // 2, tuple(b.lo), tuple(b.hi), b.deform,
+                            b.amplitude))
+    ops, trs, sms = [], [None], [None]
+    for l, b in enumerate(boxes):
+        op = LaplaceOperatorBK3(b, degree, n_q_points_1d, precision)
+        op.compute_diagonal()
+        ops.append(op)
+        if l > 0:
+            trs.append(GeometricTransfer(boxes[l - 1], degree, precision, prolongation_scale))
+            sms.append(PreconditionChebyshev(op, smoother_degree, smoothing_range,
+                                             eig_cg_n_iterations))
+    mg = VCycleMultigrid(ops, trs, sms, coarse, coarse_tol,
+                         precision if outer_precision is None else outer_precision)
+    return mg, ops, trs, sms, boxes

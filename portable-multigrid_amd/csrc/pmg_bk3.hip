@@ -1,0 +1,503 @@
+// pmg_bk3.hip -- continuous Q_p Laplace operator kernels for gfx950.
+//
+// Operator (reference: BK3::Parallel::KokkosKernel, bk3_kokkos_kernel.h:25-746):
+//   y += sum_cells P_e^T S^T D^T G_e D S P_e x
+// Design (MI355X): a workgroup of <= 256 lanes holds E = 256/nq^2 cells; each
+// cell owns an nq x nq tile of lanes.  A lane keeps one 1D line of the current
+// tensor in registers and contracts along it, so every 1D sweep costs nm (or
+// nq) LDS reads + nq (or nm) LDS writes per line instead of one read per FMA.
+// The first z-interpolation and the last z-back-interpolation run on the
+// register column loaded from / scattered to HBM, the z-gradient and its
+// transpose run on a register column of the q-point tensor.  Three nq^3 LDS
+// buffers per cell.  G (85 % of the algorithmic bytes) is fetched as 6*nq
+// coalesced loads per lane issued at kernel entry so the HBM latency overlaps
+// the interpolation sweeps; it is read once, so the loads are non-temporal
+// and leave L2/MALL to x and y.  The DoF map column is loaded once and kept
+// in registers for the scatter.  The 1D matrices are kernel arguments (scalar
+// registers).  Blocks are remapped so each XCD (own L2) walks a contiguous
+// range of cells, which keeps neighbouring cells' shared DoFs in one L2.
+#include "pmg_common.hpp"
+#include "pmg_kernels.hpp"
+
+namespace pmg {
+
+template <int NQ>
+struct Bk3Tile {
+  static constexpr int NQ2 = NQ * NQ;
+  static constexpr int cells = (256 / NQ2) > 0 ? 256 / NQ2 : 1;
+  static constexpr int lanes = ((cells * NQ2 + 63) / 64) * 64;
+};
+
+template <typename T>
+__device__ inline T ldg_stream(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+template <int NM, int NQ, typename T>
+__global__ void __launch_bounds__(256)
+bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
+                 const uint32_t *__restrict__ map, const int64_t n_cells, const Shape1D<T> sh)
+{
+  constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
+  constexpr int E = Bk3Tile<NQ>::cells;
+  constexpr int BUF = NQ3;
+  __shared__ T lds[E * 3 * BUF];
+
+  const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int e = tid / NQ2, t = tid - e * NQ2, ta = t % NQ, tb = t / NQ;
+  const int64_t cell = static_cast<int64_t>(lb) * E + e;
+  const bool active = (e < E) && (cell < n_cells);
+  T *P0 = lds + (active ? e : 0) * 3 * BUF;
+  T *P1 = P0 + BUF;
+  T *P2 = P1 + BUF;
+
+  // ---- geometric factors for this lane's column (p,q) = (ta,tb), all r
+  T g[6][NQ];
+  if (active) {
+    const T *Gc = G + cell * (6 * NQ3) + tb * NQ + ta;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) g[c][r] = ldg_stream(Gc + c * NQ3 + r * NQ2);
+  }
+
+  // ---- phase 1: gather column (i,j)=(ta,tb) over k, interpolate in z
+  uint32_t idx[NM];
+  if (active && ta < NM && tb < NM) {
+    const uint32_t *mc = map + cell * NM3 + tb * NM + ta;
+    T u[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      idx[k] = mc[k * NM2];
+      u[k] = (idx[k] == kInvalid) ? T(0) : x[idx[k]];
+    }
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      T v = 0;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) v += sh.S[k * NQ + r] * u[k];
+      P0[r * NM2 + tb * NM + ta] = v;  // A[r][j][i]
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: y-interpolation, lane (i,r) = (ta,tb): B[r][q][i]
+  if (active && ta < NM) {
+    T a[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) a[j] = P0[tb * NM2 + j * NM + ta];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      T v = 0;
+#pragma unroll
+      for (int j = 0; j < NM; ++j) v += sh.S[j * NQ + q] * a[j];
+      P1[(tb * NQ + q) * NM + ta] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 3: x-interpolation + x-gradient, lane (q,r) = (ta,tb)
+  if (active) {
+    T b[NM], U[NQ];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) b[i] = P1[(tb * NQ + ta) * NM + i];
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) {
+      T v = 0;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) v += sh.S[i * NQ + p] * b[i];
+      U[p] = v;
+      P0[tb * NQ2 + ta * NQ + p] = v;  // U[r][q][p]
+    }
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) {
+      T v = 0;
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + p] * U[n];
+      P2[tb * NQ2 + ta * NQ + p] = v;  // du/dxi_0
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 4: y-gradient, lane (p,r) = (ta,tb)
+  if (active) {
+    T c[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) c[n] = P0[tb * NQ2 + n * NQ + ta];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      T v = 0;
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + q] * c[n];
+      P1[tb * NQ2 + q * NQ + ta] = v;  // du/dxi_1
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
+  if (active) {
+    T col[NQ], fz[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) col[n] = P0[n * NQ2 + tb * NQ + ta];
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      T uz = 0;
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) uz += sh.D[n * NQ + r] * col[n];
+      const int id = r * NQ2 + tb * NQ + ta;
+      const T ux = P2[id], uy = P1[id];
+      P2[id] = g[0][r] * ux + g[1][r] * uy + g[2][r] * uz;
+      P1[id] = g[1][r] * ux + g[3][r] * uy + g[4][r] * uz;
+      fz[r] = g[2][r] * ux + g[4][r] * uy + g[5][r] * uz;
+    }
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      T v = 0;
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) v += sh.D[r * NQ + n] * fz[n];
+      P0[r * NQ2 + tb * NQ + ta] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 6: x-integration accumulated onto the z part, lane (q,r)
+  if (active) {
+    T f[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) f[n] = P2[tb * NQ2 + ta * NQ + n];
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) {
+      T v = P0[tb * NQ2 + ta * NQ + p];
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) v += sh.D[p * NQ + n] * f[n];
+      P0[tb * NQ2 + ta * NQ + p] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 7: y-integration + y back-interpolation, lane (p,r)
+  if (active) {
+    T f[NQ], acc[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) f[n] = P1[tb * NQ2 + n * NQ + ta];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      T v = P0[tb * NQ2 + q * NQ + ta];
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) v += sh.D[q * NQ + n] * f[n];
+      acc[q] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      T v = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v += sh.S[j * NQ + q] * acc[q];
+      P1[tb * NQ2 + j * NQ + ta] = v;  // W[r][j][p]
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 8: x back-interpolation, lane (j,r) = (ta,tb)
+  if (active && ta < NM) {
+    T l[NQ];
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) l[p] = P1[tb * NQ2 + ta * NQ + p];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+      T v = 0;
+#pragma unroll
+      for (int p = 0; p < NQ; ++p) v += sh.S[i * NQ + p] * l[p];
+      P2[tb * NM2 + ta * NM + i] = v;  // Z[r][j][i]
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 9: z back-interpolation + scatter-add, lane (i,j) = (ta,tb)
+  if (active && ta < NM && tb < NM) {
+    T z[NQ];
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) z[r] = P2[r * NM2 + tb * NM + ta];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      T v = 0;
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) v += sh.S[k * NQ + r] * z[r];
+      if (idx[k] != kInvalid) atomic_add(y + idx[k], v);
+    }
+  }
+}
+
+template <int NM, int NQ, typename T>
+static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_t n_cells,
+                       const Shape1D<T> &sh, hipStream_t s) {
+  if (n_cells <= 0) return;
+  using Tile = Bk3Tile<NQ>;
+  const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
+  PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
+  const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
+  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T>), dim3(grid), dim3(Tile::lanes), 0, s, x, y, G,
+                     map, n_cells, sh);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
+                      int64_t n_cells, const Shape1D<T> &sh, hipStream_t s) {
+#define PMG_BK3_CASE(P, Q)                                       \
+  if (degree == P && nq == Q) {                                  \
+    bk3_launch<P + 1, Q, T>(x, y, G, map, n_cells, sh, s);       \
+    return;                                                      \
+  }
+  PMG_BK3_CASE(1, 2) PMG_BK3_CASE(1, 3)
+  PMG_BK3_CASE(2, 3) PMG_BK3_CASE(2, 4)
+  PMG_BK3_CASE(3, 4) PMG_BK3_CASE(3, 5)
+  PMG_BK3_CASE(4, 5) PMG_BK3_CASE(4, 6)
+  PMG_BK3_CASE(5, 6) PMG_BK3_CASE(5, 7)
+  PMG_BK3_CASE(6, 7) PMG_BK3_CASE(6, 8)
+#undef PMG_BK3_CASE
+  throw Error(PMG_ERR_UNSUPPORTED, "BK3 kernel: unsupported (degree, n_q_points_1d) = (" +
+                                       std::to_string(degree) + ", " + std::to_string(nq) + ")");
+}
+
+template void launch_bk3_cells<double>(int, int, const double *, double *, const double *,
+                                       const uint32_t *, int64_t, const Shape1D<double> &,
+                                       hipStream_t);
+template void launch_bk3_cells<float>(int, int, const float *, float *, const float *,
+                                      const uint32_t *, int64_t, const Shape1D<float> &,
+                                      hipStream_t);
+
+// ---------------------------------------------------------------- geometry
+// One lane per (cell, q): trilinear Jacobian J = dX/dxi at the Gauss point,
+// G = kappa * |det J| w * J^{-1} J^{-T}  (compute_G_tensors,
+// portable_laplace_operator_bk3.h:484-511), stored [c][comp][q].
+template <typename T>
+__global__ void __launch_bounds__(256)
+geometry_kernel(const int nq, const int64_t n_cells, const double *__restrict__ cv,
+                const double *__restrict__ qp, const double *__restrict__ qw,
+                const double *__restrict__ kappa, T *__restrict__ G)
+{
+  const int nq3 = nq * nq * nq;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n_cells * nq3) return;
+  const int64_t c = gid / nq3;
+  const int q = static_cast<int>(gid - c * nq3);
+  const int q0 = q % nq, q1 = (q / nq) % nq, q2 = q / (nq * nq);
+  const double xi[3] = {qp[q0], qp[q1], qp[q2]};
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double *v = cv + c * 24;
+  for (int vi = 0; vi < 8; ++vi) {
+    const int b0 = vi & 1, b1 = (vi >> 1) & 1, b2 = (vi >> 2) & 1;
+    const double f0 = b0 ? xi[0] : 1.0 - xi[0], f1 = b1 ? xi[1] : 1.0 - xi[1],
+                 f2 = b2 ? xi[2] : 1.0 - xi[2];
+    const double d0 = b0 ? 1.0 : -1.0, d1 = b1 ? 1.0 : -1.0, d2 = b2 ? 1.0 : -1.0;
+    const double dN[3] = {d0 * f1 * f2, f0 * d1 * f2, f0 * f1 * d2};
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 3; ++k) J[d][k] += dN[k] * v[3 * vi + d];
+  }
+  // cofactors: inv(J)[a][b] = cof(J)[b][a] / det
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / det;
+  // K[a][k] = d xi_a / d x_k = cof[k][a] / det
+  const double K[3][3] = {{c00 * id, c10 * id, c20 * id},
+                          {c01 * id, c11 * id, c21 * id},
+                          {c02 * id, c12 * id, c22 * id}};
+  const double s = fabs(det) * qw[q0] * qw[q1] * qw[q2] * (kappa ? kappa[gid] : 1.0);
+  T *out = G + c * 6 * nq3 + q;
+  int comp = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b, ++comp)
+      out[comp * nq3] =
+          static_cast<T>(s * (K[a][0] * K[b][0] + K[a][1] * K[b][1] + K[a][2] * K[b][2]));
+}
+
+template <typename T>
+void launch_geometry(int nq, int64_t n_cells, const double *cell_vertices, const double *qpts,
+                     const double *qwts, const double *kappa, T *G, hipStream_t s) {
+  const int64_t n = n_cells * nq * nq * nq;
+  if (n == 0) return;
+  const int64_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(geometry_kernel<T>, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, nq,
+                     n_cells, cell_vertices, qpts, qwts, kappa, G);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_geometry<double>(int, int64_t, const double *, const double *,
+                                      const double *, const double *, double *, hipStream_t);
+template void launch_geometry<float>(int, int64_t, const double *, const double *,
+                                     const double *, const double *, float *, hipStream_t);
+
+// ---------------------------------------------------------------- diagonal
+// One lane per (cell, local DoF i): (A_e)_ii = sum_q g_i(q)^T G_q g_i(q) with
+// g_i(q) = (phi'phi phi, phi phi' phi, phi phi phi') from the exact 1D
+// derivatives (sh.D holds Sd here).  Setup only (compute_diagonal,
+// portable_laplace_operator_bk3.h:533-561).
+template <typename T>
+__global__ void __launch_bounds__(256)
+bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const T *__restrict__ G,
+                    const uint32_t *__restrict__ map, const Shape1D<T> sh, T *__restrict__ diag)
+{
+  const int nm3 = nm * nm * nm, nq3 = nq * nq * nq;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n_cells * nm3) return;
+  const int64_t c = gid / nm3;
+  const int i = static_cast<int>(gid - c * nm3);
+  const uint32_t dof = map[gid];
+  if (dof == kInvalid) return;
+  const int i0 = i % nm, i1 = (i / nm) % nm, i2 = i / (nm * nm);
+  const T *Gc = G + c * 6 * nq3;
+  double acc = 0;
+  for (int q2 = 0; q2 < nq; ++q2)
+    for (int q1 = 0; q1 < nq; ++q1)
+      for (int q0 = 0; q0 < nq; ++q0) {
+        const int q = (q2 * nq + q1) * nq + q0;
+        const double s0 = sh.S[i0 * nq + q0], s1 = sh.S[i1 * nq + q1], s2 = sh.S[i2 * nq + q2];
+        const double d0 = sh.D[i0 * nq + q0], d1 = sh.D[i1 * nq + q1], d2 = sh.D[i2 * nq + q2];
+        const double g0 = d0 * s1 * s2, g1 = s0 * d1 * s2, g2 = s0 * s1 * d2;
+        acc += Gc[q] * g0 * g0 + Gc[3 * nq3 + q] * g1 * g1 + Gc[5 * nq3 + q] * g2 * g2 +
+               2.0 * (Gc[nq3 + q] * g0 * g1 + Gc[2 * nq3 + q] * g0 * g2 +
+                      Gc[4 * nq3 + q] * g1 * g2);
+      }
+  atomic_add(diag + dof, static_cast<T>(acc));
+}
+
+template <typename T>
+void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
+                         const Shape1D<T> &S_and_Sd, T *diag, hipStream_t s) {
+  const int nm = degree + 1;
+  const int64_t n = n_cells * nm * nm * nm;
+  if (n == 0) return;
+  hipLaunchKernelGGL(bk3_diagonal_kernel<T>, dim3(static_cast<unsigned>((n + 255) / 256)),
+                     dim3(256), 0, s, nm, nq, n_cells, G, map, S_and_Sd, diag);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_bk3_diagonal<double>(int, int, int64_t, const double *, const uint32_t *,
+                                          const Shape1D<double> &, double *, hipStream_t);
+template void launch_bk3_diagonal<float>(int, int, int64_t, const float *, const uint32_t *,
+                                         const Shape1D<float> &, float *, hipStream_t);
+
+// ---------------------------------------------------------------- adjoint gradient
+// One workgroup per cell, one lane per quadrature point column (p,q); reference
+// gradients of u and lambda by the forward half of the operator (phases 1-5),
+// then out = -(grad u)^T G (grad lambda) at every q-point.
+template <int NM, int NQ, typename T>
+__device__ void cell_gradient(const T *__restrict__ vec, const uint32_t *__restrict__ map,
+                              int64_t cell, const Shape1D<T> &sh, T *P0, T *P1, T *P2,
+                              int ta, int tb, T (&gz)[NQ])
+{
+  constexpr int NQ2 = NQ * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
+  if (ta < NM && tb < NM) {
+    T u[NM];
+    for (int k = 0; k < NM; ++k) {
+      const uint32_t d = map[cell * NM3 + k * NM2 + tb * NM + ta];
+      u[k] = d == kInvalid ? T(0) : vec[d];
+    }
+    for (int r = 0; r < NQ; ++r) {
+      T v = 0;
+      for (int k = 0; k < NM; ++k) v += sh.S[k * NQ + r] * u[k];
+      P0[r * NM2 + tb * NM + ta] = v;
+    }
+  }
+  __syncthreads();
+  if (ta < NM) {
+    for (int q = 0; q < NQ; ++q) {
+      T v = 0;
+      for (int j = 0; j < NM; ++j) v += sh.S[j * NQ + q] * P0[tb * NM2 + j * NM + ta];
+      P1[(tb * NQ + q) * NM + ta] = v;
+    }
+  }
+  __syncthreads();
+  {
+    T U[NQ];
+    for (int p = 0; p < NQ; ++p) {
+      T v = 0;
+      for (int i = 0; i < NM; ++i) v += sh.S[i * NQ + p] * P1[(tb * NQ + ta) * NM + i];
+      U[p] = v;
+    }
+    __syncthreads();
+    for (int p = 0; p < NQ; ++p) {
+      P0[tb * NQ2 + ta * NQ + p] = U[p];
+      T v = 0;
+      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + p] * U[n];
+      P2[tb * NQ2 + ta * NQ + p] = v;
+    }
+  }
+  __syncthreads();
+  for (int q = 0; q < NQ; ++q) {
+    T v = 0;
+    for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + q] * P0[tb * NQ2 + n * NQ + ta];
+    P1[tb * NQ2 + q * NQ + ta] = v;
+  }
+  __syncthreads();
+  for (int r = 0; r < NQ; ++r) {
+    T v = 0;
+    for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + r] * P0[n * NQ2 + tb * NQ + ta];
+    gz[r] = v;
+  }
+}
+
+template <int NM, int NQ, typename T>
+__global__ void __launch_bounds__(NQ * NQ)
+grad_coefficient_kernel(const int64_t n_cells, const T *__restrict__ G,
+                        const uint32_t *__restrict__ map, const Shape1D<T> sh,
+                        const T *__restrict__ u, const T *__restrict__ lam, T *__restrict__ out)
+{
+  constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
+  __shared__ T lds[6 * NQ3];
+  const int64_t cell = blockIdx.x;
+  const int ta = threadIdx.x % NQ, tb = threadIdx.x / NQ;
+  T uz[NQ], lz[NQ];
+  cell_gradient<NM, NQ, T>(u, map, cell, sh, lds, lds + NQ3, lds + 2 * NQ3, ta, tb, uz);
+  __syncthreads();
+  cell_gradient<NM, NQ, T>(lam, map, cell, sh, lds + 3 * NQ3, lds + 4 * NQ3, lds + 5 * NQ3, ta,
+                           tb, lz);
+  __syncthreads();
+  const T *Gc = G + cell * 6 * NQ3;
+  for (int r = 0; r < NQ; ++r) {
+    const int id = r * NQ2 + tb * NQ + ta;
+    const T ux = lds[2 * NQ3 + id], uy = lds[NQ3 + id], uzz = uz[r];
+    const T lx = lds[5 * NQ3 + id], ly = lds[4 * NQ3 + id], lzz = lz[r];
+    const T fx = Gc[id] * ux + Gc[NQ3 + id] * uy + Gc[2 * NQ3 + id] * uzz;
+    const T fy = Gc[NQ3 + id] * ux + Gc[3 * NQ3 + id] * uy + Gc[4 * NQ3 + id] * uzz;
+    const T fz = Gc[2 * NQ3 + id] * ux + Gc[4 * NQ3 + id] * uy + Gc[5 * NQ3 + id] * uzz;
+    out[cell * NQ3 + id] = -(fx * lx + fy * ly + fz * lzz);
+  }
+}
+
+template <typename T>
+void launch_bk3_grad_coefficient(int degree, int nq, int64_t n_cells, const T *G,
+                                 const uint32_t *map, const Shape1D<T> &sh, const T *u,
+                                 const T *lam, T *out, hipStream_t s) {
+  if (n_cells == 0) return;
+  PMG_REQUIRE(n_cells < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells");
+#define PMG_GRAD_CASE(P, Q)                                                                  \
+  if (degree == P && nq == Q) {                                                              \
+    hipLaunchKernelGGL((grad_coefficient_kernel<P + 1, Q, T>), dim3(unsigned(n_cells)),       \
+                       dim3(Q * Q), 0, s, n_cells, G, map, sh, u, lam, out);                  \
+    PMG_HIP_CHECK(hipGetLastError());                                                        \
+    return;                                                                                  \
+  }
+  PMG_GRAD_CASE(1, 2) PMG_GRAD_CASE(1, 3) PMG_GRAD_CASE(2, 3) PMG_GRAD_CASE(2, 4)
+  PMG_GRAD_CASE(3, 4) PMG_GRAD_CASE(3, 5) PMG_GRAD_CASE(4, 5) PMG_GRAD_CASE(4, 6)
+  PMG_GRAD_CASE(5, 6) PMG_GRAD_CASE(5, 7) PMG_GRAD_CASE(6, 7) PMG_GRAD_CASE(6, 8)
+#undef PMG_GRAD_CASE
+  throw Error(PMG_ERR_UNSUPPORTED, "gradient kernel: unsupported (degree, n_q_points_1d)");
+}
+template void launch_bk3_grad_coefficient<double>(int, int, int64_t, const double *,
+                                                  const uint32_t *, const Shape1D<double> &,
+                                                  const double *, const double *, double *,
+                                                  hipStream_t);
+template void launch_bk3_grad_coefficient<float>(int, int, int64_t, const float *,
+                                                 const uint32_t *, const Shape1D<float> &,
+                                                 const float *, const float *, float *,
+                                                 hipStream_t);
+
+}  // namespace pmg

@@ -1,0 +1,98 @@
+// pmg_kernels.hpp -- host-callable launchers of the gfx950 kernels.
+// All launchers are stream-ordered and asynchronous; they throw pmg::Error on
+// unsupported template parameters or launch failures.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pmg {
+
+// ---------------------------------------------------------------- BK3 operator
+// 1D shape data in the operator precision, passed to kernels by value.
+constexpr int kMaxNm = 8;
+constexpr int kMaxNq = 8;
+
+template <typename T>
+struct Shape1D {
+  T S[kMaxNm * kMaxNq];  // S[i*nq+q]
+  T D[kMaxNq * kMaxNq];  // D[n*nq+q]
+};
+
+// y += sum_cells A_e x  (atomics on shared DoFs)
+template <typename T>
+void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
+                      int64_t n_cells, const Shape1D<T> &sh, hipStream_t s);
+// G[c][6][nq^3] = kappa * JxW * J^{-1}J^{-T} of the trilinear map of the cell vertices
+template <typename T>
+void launch_geometry(int nq, int64_t n_cells, const double *cell_vertices, const double *qpts,
+                     const double *qwts, const double *kappa, T *G, hipStream_t s);
+// diag[map[c][i]] += (A_e)_ii
+template <typename T>
+void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
+                         const Shape1D<T> &S_and_Sd, T *diag, hipStream_t s);
+// out[c][q] = -(grad u)^T G_q (grad lambda)
+template <typename T>
+void launch_bk3_grad_coefficient(int degree, int nq, int64_t n_cells, const T *G,
+                                 const uint32_t *map, const Shape1D<T> &sh, const T *u,
+                                 const T *lam, T *out, hipStream_t s);
+
+// ---------------------------------------------------------------- transfers
+template <typename T>
+struct Transfer1D {
+  T M[kMaxNm * (2 * kMaxNm)];  // M[i*nmf + j]
+};
+
+template <typename T>
+void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
+                       const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xc,
+                       T *yf, hipStream_t s);
+template <typename T>
+void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
+                     const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
+                     T *yc, hipStream_t s);
+
+// ---------------------------------------------------------------- vectors
+template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s);
+template <typename T> void launch_copy(int64_t n, T *dst, const T *src, hipStream_t s);
+// dst[idx[i]] = src[idx[i]]
+template <typename T>
+void launch_copy_indexed(int64_t n, const uint32_t *idx, T *dst, const T *src, hipStream_t s);
+template <typename T>
+void launch_set_indexed(int64_t n, const uint32_t *idx, T *dst, T v, hipStream_t s);
+// x[i] = 1/x[i]
+template <typename T> void launch_invert(int64_t n, T *x, hipStream_t s);
+template <typename TD, typename TS>
+void launch_convert(int64_t n, TD *dst, const TS *src, hipStream_t s);
+// partial sums of a.b into `partials` (kDotBlocks entries), then final sum into out[0]
+constexpr int kDotBlocks = 1024;
+template <typename T>
+void launch_dot(int64_t n, const T *a, const T *b, double *partials, double *out, hipStream_t s);
+// t = b - t   (residual)
+template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStream_t s);
+
+// Chebyshev (first kind, Jacobi preconditioner); coefficient args are host scalars.
+// mode 0: x = f2 * P b                                     (vmult start)
+// mode 1: old = x; x = x + f2 * P (b - t)                   (step start)
+// mode 2: xn = (1+f1) x - f1*old + f2 P (b - t); old = x; x = xn   (old = 0 if first)
+template <typename T>
+void launch_cheby_update(int mode, int64_t n, T *x, T *old, const T *b, const T *t,
+                         const T *pinv, double f1, double f2, bool old_is_zero, hipStream_t s);
+
+// CG kernels with device-resident scalars (sc = array of doubles, see pmg_solver.cpp)
+template <typename T>
+void launch_cg_update_xr(int64_t n, T *x, T *r, const T *p, const T *q, const double *sc,
+                         double *partials, hipStream_t s);
+template <typename T>
+void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *partials,
+                           hipStream_t s);
+template <typename T>
+void launch_cg_update_p(int64_t n, T *p, const T *z, const double *sc, int first, hipStream_t s);
+// final reduction of kDotBlocks partial sums into sc[slot] (optionally sqrt)
+void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
+                            int n_sums, hipStream_t s);
+// scalar bookkeeping for CG on the device: see kernel for slot layout
+void launch_cg_scalars(double *sc, int op, hipStream_t s);
+
+}  // namespace pmg

@@ -1,0 +1,661 @@
+// pmg_library.cpp -- host-side C++ classes: operator, transfers, Chebyshev
+// smoother, V-cycle and preconditioned CG.  Every device operation is a
+// launch of a gfx950 kernel from pmg_*.hip on the caller's stream; host code
+// only sequences launches and reads back the scalars a solver must test.
+#include "pmg_library.hpp"
+
+#include "pmg_fields.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace pmg {
+
+namespace {
+template <typename T>
+Shape1D<T> to_shape(const std::vector<double> &S, const std::vector<double> &D) {
+  Shape1D<T> sh{};
+  PMG_REQUIRE(S.size() <= size_t(kMaxNm * kMaxNq) && D.size() <= size_t(kMaxNq * kMaxNq),
+              PMG_ERR_UNSUPPORTED, "1D shape data exceeds kernel limits");
+  for (size_t i = 0; i < S.size(); ++i) sh.S[i] = static_cast<T>(S[i]);
+  for (size_t i = 0; i < D.size(); ++i) sh.D[i] = static_cast<T>(D[i]);
+  return sh;
+}
+
+template <typename T>
+double dot_sync(int64_t n, const T *a, const T *b, DeviceArray<double> &partials,
+                DeviceArray<double> &sc, hipStream_t s) {
+  launch_dot<T>(n, a, b, partials.ptr, sc.ptr, s);
+  double v = 0;
+  PMG_HIP_CHECK(hipMemcpyAsync(&v, sc.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  return v;
+}
+
+// Symmetric tridiagonal eigenvalues by Sturm-sequence bisection.
+std::vector<double> tridiag_eigenvalues(const std::vector<double> &d, const std::vector<double> &o) {
+  const int n = static_cast<int>(d.size());
+  std::vector<double> ev(n);
+  if (n == 0) return ev;
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(o[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(o[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  auto count_below = [&](double x) {
+    int c = 0;
+    double q = d[0] - x;
+    if (q < 0) ++c;
+    for (int i = 1; i < n; ++i) {
+      if (q == 0) q = 1e-300;
+      q = d[i] - x - o[i - 1] * o[i - 1] / q;
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  for (int k = 0; k < n; ++k) {
+    double a = lo, b = hi;
+    for (int it = 0; it < 200 && b - a > 1e-15 * (std::fabs(a) + std::fabs(b)) + 1e-300; ++it) {
+      const double m = 0.5 * (a + b);
+      if (count_below(m) > k) b = m; else a = m;
+    }
+    ev[k] = 0.5 * (a + b);
+  }
+  return ev;
+}
+}  // namespace
+
+// ============================================================ operator
+LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_t n_cells,
+                                       const uint32_t *dofmap, int64_t n_dofs,
+                                       const uint32_t *constrained, int64_t n_constrained,
+                                       const double *cell_vertices, const double *G)
+    : degree_(degree), nq_(nq), precision_(precision), n_cells_(n_cells), n_dofs_(n_dofs),
+      n_constrained_(n_constrained) {
+  check_precision(precision);
+  PMG_REQUIRE(degree >= 1 && degree <= 6, PMG_ERR_UNSUPPORTED, "degree must be in [1,6]");
+  PMG_REQUIRE(nq == degree + 1 || nq == degree + 2, PMG_ERR_UNSUPPORTED,
+              "n_q_points_1d must be degree+1 or degree+2");
+  PMG_REQUIRE(n_cells >= 0 && n_dofs >= 0 && n_constrained >= 0, PMG_ERR_INVALID_ARG,
+              "negative size");
+  PMG_REQUIRE(n_dofs < int64_t(0xFFFFFFFFu), PMG_ERR_SIZE, "n_dofs exceeds 32-bit DoF maps");
+  PMG_REQUIRE(n_cells == 0 || dofmap, PMG_ERR_INVALID_ARG, "dofmap is NULL");
+  PMG_REQUIRE(n_constrained == 0 || constrained, PMG_ERR_INVALID_ARG, "constrained list is NULL");
+  PMG_REQUIRE((cell_vertices != nullptr) != (G != nullptr), PMG_ERR_INVALID_ARG,
+              "give exactly one of cell_vertices and G");
+  const int64_t nm3 = int64_t(degree + 1) * (degree + 1) * (degree + 1);
+  const int64_t nq3 = int64_t(nq) * nq * nq;
+  // host-side validation of the index data the kernels will dereference
+  for (int64_t e = 0; e < n_cells * nm3; ++e)
+    PMG_REQUIRE(dofmap[e] == kInvalid || int64_t(dofmap[e]) < n_dofs, PMG_ERR_INVALID_ARG,
+                "dofmap entry out of range at " + std::to_string(e));
+  for (int64_t e = 0; e < n_constrained; ++e)
+    PMG_REQUIRE(int64_t(constrained[e]) < n_dofs, PMG_ERR_INVALID_ARG,
+                "constrained DoF out of range");
+  shape_ = make_shape_data(degree, nq);
+  sh64_ = to_shape<double>(shape_.S, shape_.D);
+  sh32_ = to_shape<float>(shape_.S, shape_.D);
+  shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
+  shd32_ = to_shape<float>(shape_.S, shape_.Sd);
+  map_.upload(dofmap, static_cast<size_t>(n_cells * nm3));
+  constrained_.upload(constrained, static_cast<size_t>(n_constrained));
+  G_.reinit(precision, n_cells * 6 * nq3);
+  if (cell_vertices) {
+    vertices_.upload(cell_vertices, static_cast<size_t>(n_cells * 24));
+    qpts_.upload(shape_.quad.points);
+    qwts_.upload(shape_.quad.weights);
+    if (precision == PMG_FP64) fill_geometry_t<double>(nullptr, nullptr);
+    else fill_geometry_t<float>(nullptr, nullptr);
+    PMG_HIP_CHECK(hipDeviceSynchronize());
+  } else {
+    if (precision == PMG_FP64) {
+      PMG_HIP_CHECK(hipMemcpy(G_.data(), G, G_.raw.bytes(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> g(static_cast<size_t>(n_cells * 6 * nq3));
+      for (size_t i = 0; i < g.size(); ++i) g[i] = static_cast<float>(G[i]);
+      PMG_HIP_CHECK(hipMemcpy(G_.data(), g.data(), G_.raw.bytes(), hipMemcpyHostToDevice));
+    }
+  }
+}
+
+template <typename T>
+void LaplaceOperatorBK3::fill_geometry_t(const double *kappa, hipStream_t s) {
+  launch_geometry<T>(nq_, n_cells_, vertices_.ptr, qpts_.ptr, qwts_.ptr, kappa,
+                     static_cast<T *>(G_.data()), s);
+}
+
+template <typename T>
+void LaplaceOperatorBK3::vmult_t(T *dst, const T *src, hipStream_t s) const {
+  const Shape1D<T> &sh = *reinterpret_cast<const Shape1D<T> *>(
+      std::is_same<T, double>::value ? static_cast<const void *>(&sh64_) : static_cast<const void *>(&sh32_));
+  launch_fill<T>(n_dofs_, dst, T(0), s);
+  launch_bk3_cells<T>(degree_, nq_, src, dst, static_cast<const T *>(G_.data()), map_.ptr,
+                      n_cells_, sh, s);
+  launch_copy_indexed<T>(n_constrained_, constrained_.ptr, dst, src, s);
+}
+
+void LaplaceOperatorBK3::vmult(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  PMG_REQUIRE(dst != src, PMG_ERR_INVALID_ARG, "vmult: dst and src must not alias");
+  if (precision_ == PMG_FP64) vmult_t(static_cast<double *>(dst), static_cast<const double *>(src), s);
+  else vmult_t(static_cast<float *>(dst), static_cast<const float *>(src), s);
+}
+
+void LaplaceOperatorBK3::apply_add(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
+  if (precision_ == PMG_FP64)
+    launch_bk3_cells<double>(degree_, nq_, static_cast<const double *>(src), static_cast<double *>(dst),
+                             static_cast<const double *>(G_.data()), map_.ptr, n_cells_, sh64_, s);
+  else
+    launch_bk3_cells<float>(degree_, nq_, static_cast<const float *>(src), static_cast<float *>(dst),
+                            static_cast<const float *>(G_.data()), map_.ptr, n_cells_, sh32_, s);
+}
+
+template <typename T>
+void LaplaceOperatorBK3::compute_diagonal_t() {
+  const Shape1D<T> &shd = *reinterpret_cast<const Shape1D<T> *>(
+      std::is_same<T, double>::value ? static_cast<const void *>(&shd64_) : static_cast<const void *>(&shd32_));
+  T *d = static_cast<T *>(inv_diag_.data());
+  launch_fill<T>(n_dofs_, d, T(0), nullptr);
+  launch_bk3_diagonal<T>(degree_, nq_, n_cells_, static_cast<const T *>(G_.data()), map_.ptr, shd,
+                         d, nullptr);
+  launch_set_indexed<T>(n_constrained_, constrained_.ptr, d, T(1), nullptr);
+  // DoFs touched by no cell (possible only with exotic maps) would divide by 0
+  std::vector<T> h(static_cast<size_t>(n_dofs_));
+  PMG_HIP_CHECK(hipMemcpy(h.data(), d, sizeof(T) * n_dofs_, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n_dofs_; ++i)
+    PMG_REQUIRE(h[i] > T(0), PMG_ERR_INVALID_ARG,
+                "non-positive diagonal entry at DoF " + std::to_string(i));
+  launch_invert<T>(n_dofs_, d, nullptr);
+  PMG_HIP_CHECK(hipDeviceSynchronize());
+}
+
+void LaplaceOperatorBK3::compute_diagonal() {
+  inv_diag_.reinit(precision_, n_dofs_);
+  if (precision_ == PMG_FP64) compute_diagonal_t<double>();
+  else compute_diagonal_t<float>();
+  has_diag_ = true;
+}
+
+const void *LaplaceOperatorBK3::inverse_diagonal() const {
+  PMG_REQUIRE(has_diag_, PMG_ERR_INVALID_ARG, "compute_diagonal() has not been called");
+  return inv_diag_.data();
+}
+
+void LaplaceOperatorBK3::set_coefficient(const double *kappa, hipStream_t s) {
+  PMG_REQUIRE(vertices_.ptr != nullptr, PMG_ERR_INVALID_ARG,
+              "set_coefficient needs an operator created from cell vertices");
+  if (precision_ == PMG_FP64) fill_geometry_t<double>(kappa, s);
+  else fill_geometry_t<float>(kappa, s);
+  has_diag_ = false;
+}
+
+void LaplaceOperatorBK3::gradient_coefficient(const void *u, const void *lam, void *out,
+                                              hipStream_t s) const {
+  PMG_REQUIRE(u && lam && out, PMG_ERR_INVALID_ARG, "NULL vector");
+  if (precision_ == PMG_FP64)
+    launch_bk3_grad_coefficient<double>(degree_, nq_, n_cells_, static_cast<const double *>(G_.data()),
+                                        map_.ptr, sh64_, static_cast<const double *>(u),
+                                        static_cast<const double *>(lam), static_cast<double *>(out), s);
+  else
+    launch_bk3_grad_coefficient<float>(degree_, nq_, n_cells_, static_cast<const float *>(G_.data()),
+                                       map_.ptr, sh32_, static_cast<const float *>(u),
+                                       static_cast<const float *>(lam), static_cast<float *>(out), s);
+}
+
+void LaplaceOperatorBK3::assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const {
+  PMG_REQUIRE(b, PMG_ERR_INVALID_ARG, "NULL vector");
+  PMG_REQUIRE(f_kind == 0 || f_kind == 1, PMG_ERR_INVALID_ARG, "unknown right-hand side kind");
+  PMG_REQUIRE(vertices_.ptr != nullptr, PMG_ERR_INVALID_ARG,
+              "assemble_rhs needs an operator created from cell vertices");
+  if (nq <= 0) nq = degree_ + 1;
+  PMG_REQUIRE(nq >= 1 && nq <= 12, PMG_ERR_INVALID_ARG, "n_q_points_1d out of range");
+  const ShapeData sd = make_shape_data(degree_, std::max(nq, degree_ + 1));
+  PMG_REQUIRE(sd.n_q_points_1d == nq, PMG_ERR_INVALID_ARG, "need n_q_points_1d >= degree+1");
+  DeviceArray<double> S, qp, qw, F(static_cast<size_t>(n_cells_ * nq * nq * nq));
+  S.upload(sd.S);
+  qp.upload(sd.quad.points);
+  qw.upload(sd.quad.weights);
+  launch_f_at_q(f_kind, nq, n_cells_, vertices_.ptr, qp.ptr, qw.ptr, F.ptr, s);
+  if (precision_ == PMG_FP64) {
+    launch_fill<double>(n_dofs_, static_cast<double *>(b), 0.0, s);
+    launch_integrate<double>(degree_ + 1, nq, n_cells_, S.ptr, map_.ptr, F.ptr, static_cast<double *>(b), s);
+  } else {
+    launch_fill<float>(n_dofs_, static_cast<float *>(b), 0.0f, s);
+    launch_integrate<float>(degree_ + 1, nq, n_cells_, S.ptr, map_.ptr, F.ptr, static_cast<float *>(b), s);
+  }
+  PMG_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
+}
+
+double LaplaceOperatorBK3::l2_error(const void *u, int u_kind, int nq, hipStream_t s) const {
+  PMG_REQUIRE(u, PMG_ERR_INVALID_ARG, "NULL vector");
+  PMG_REQUIRE(u_kind == 0 || u_kind == 1, PMG_ERR_INVALID_ARG, "unknown solution kind");
+  PMG_REQUIRE(vertices_.ptr != nullptr, PMG_ERR_INVALID_ARG,
+              "l2_error needs an operator created from cell vertices");
+  if (nq <= 0) nq = degree_ + 2;
+  PMG_REQUIRE(nq >= degree_ + 1 && nq <= 12, PMG_ERR_INVALID_ARG, "n_q_points_1d out of range");
+  const ShapeData sd = make_shape_data(degree_, nq);
+  DeviceArray<double> S, qp, qw;
+  S.upload(sd.S);
+  qp.upload(sd.quad.points);
+  qw.upload(sd.quad.weights);
+  const int nb = 2048;
+  DeviceArray<double> partials(nb);
+  if (precision_ == PMG_FP64)
+    launch_l2_error<double>(u_kind, degree_ + 1, nq, n_cells_, vertices_.ptr, qp.ptr, qw.ptr, S.ptr,
+                            map_.ptr, static_cast<const double *>(u), partials.ptr, nb, s);
+  else
+    launch_l2_error<float>(u_kind, degree_ + 1, nq, n_cells_, vertices_.ptr, qp.ptr, qw.ptr, S.ptr,
+                           map_.ptr, static_cast<const float *>(u), partials.ptr, nb, s);
+  std::vector<double> h(nb);
+  PMG_HIP_CHECK(hipMemcpyAsync(h.data(), partials.ptr, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  double e2 = 0;
+  for (double v : h) e2 += v;
+  return std::sqrt(e2);
+}
+
+double LaplaceOperatorBK3::algorithmic_bytes() const {
+  const double s = element_size(precision_);
+  const double nq3 = double(nq_) * nq_ * nq_;
+  return s * (2.0 * n_dofs_ + 6.0 * n_cells_ * nq3);
+}
+
+double LaplaceOperatorBK3::algorithmic_flops() const {
+  const double nm = degree_ + 1, nq = nq_;
+  return n_cells_ * 2.0 *
+         (2.0 * (nm * nm * nm * nq + nm * nm * nq * nq + nm * nq * nq * nq) + 6.0 * nq * nq * nq * nq +
+          9.0 * nq * nq * nq);
+}
+
+// ============================================================ transfers
+MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
+                       const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
+                       int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale)
+    : precision_(precision), n_cells_(n_cells), n_dofs_c_(n_dofs_coarse),
+      n_dofs_f_(n_dofs_fine) {
+  check_precision(precision);
+  PMG_REQUIRE(kind == PMG_TRANSFER_H || kind == PMG_TRANSFER_P, PMG_ERR_UNSUPPORTED,
+              "transfer kind not supported");
+  PMG_REQUIRE(map_coarse && map_fine && weights, PMG_ERR_INVALID_ARG, "NULL setup array");
+  std::vector<double> M;
+  if (kind == PMG_TRANSFER_H) {
+    PMG_REQUIRE(pc == pf && pc >= 1 && pc <= 6, PMG_ERR_UNSUPPORTED,
+                "h-transfer needs degree_coarse == degree_fine in [1,6]");
+    nmc_ = pc + 1;
+    nmf_ = 2 * pc + 1;
+    M = h_transfer_matrix(pc);
+  } else {
+    PMG_REQUIRE(pc >= 1 && pf > pc && pf <= 6, PMG_ERR_UNSUPPORTED,
+                "p-transfer needs 1 <= degree_coarse < degree_fine <= 6");
+    nmc_ = pc + 1;
+    nmf_ = pf + 1;
+    M = p_transfer_matrix(pc, pf);
+  }
+  for (size_t i = 0; i < M.size(); ++i) {
+    M64_.M[i] = M[i];
+    M32_.M[i] = static_cast<float>(M[i]);
+  }
+  const int64_t nc3 = int64_t(nmc_) * nmc_ * nmc_, nf3 = int64_t(nmf_) * nmf_ * nmf_;
+  for (int64_t e = 0; e < n_cells * nc3; ++e)
+    PMG_REQUIRE(map_coarse[e] == kInvalid || int64_t(map_coarse[e]) < n_dofs_coarse,
+                PMG_ERR_INVALID_ARG, "coarse map entry out of range");
+  for (int64_t e = 0; e < n_cells * nf3; ++e)
+    PMG_REQUIRE(int64_t(map_fine[e]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
+                "fine map entry out of range (fine maps carry no invalid entries)");
+  map_c_.upload(map_coarse, static_cast<size_t>(n_cells * nc3));
+  map_f_.upload(map_fine, static_cast<size_t>(n_cells * nf3));
+  const size_t nw = static_cast<size_t>(n_cells * nf3);
+  w_prolong_.reinit(precision, n_cells * nf3);
+  w_restrict_.reinit(precision, n_cells * nf3);
+  if (precision == PMG_FP64) {
+    std::vector<double> wp(weights, weights + nw);
+    for (auto &v : wp) v *= scale;
+    PMG_HIP_CHECK(hipMemcpy(w_prolong_.data(), wp.data(), nw * 8, hipMemcpyHostToDevice));
+    PMG_HIP_CHECK(hipMemcpy(w_restrict_.data(), weights, nw * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> wp(nw), wr(nw);
+    for (size_t i = 0; i < nw; ++i) {
+      wp[i] = static_cast<float>(weights[i] * scale);
+      wr[i] = static_cast<float>(weights[i]);
+    }
+    PMG_HIP_CHECK(hipMemcpy(w_prolong_.data(), wp.data(), nw * 4, hipMemcpyHostToDevice));
+    PMG_HIP_CHECK(hipMemcpy(w_restrict_.data(), wr.data(), nw * 4, hipMemcpyHostToDevice));
+  }
+}
+
+void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  if (precision_ == PMG_FP64)
+    launch_prolongate<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
+                              static_cast<const double *>(w_prolong_.data()),
+                              static_cast<const double *>(src), static_cast<double *>(dst), s);
+  else
+    launch_prolongate<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
+                             static_cast<const float *>(w_prolong_.data()),
+                             static_cast<const float *>(src), static_cast<float *>(dst), s);
+}
+
+void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  if (precision_ == PMG_FP64)
+    launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
+                            static_cast<const double *>(w_restrict_.data()),
+                            static_cast<const double *>(src), static_cast<double *>(dst), s);
+  else
+    launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
+                           static_cast<const float *>(w_restrict_.data()),
+                           static_cast<const float *>(src), static_cast<float *>(dst), s);
+}
+
+// ============================================================ CG
+void CGWork::reinit(int precision, int64_t n) {
+  if (sc.n == 0) {
+    sc.resize(8);
+    partials.resize(kDotBlocks);
+  }
+  if (r.n != n || r.precision != precision) {
+    r.reinit(precision, n);
+    z.reinit(precision, n);
+    p.reinit(precision, n);
+    q.reinit(precision, n);
+  }
+}
+
+namespace {
+template <typename T>
+CGResult solve_cg_t(const LaplaceOperatorBK3 &op, int precond, const VCycleMultigrid *mg,
+                    const T *b, T *x, double rel_tol, int max_it, CGWork &w, hipStream_t s,
+                    int check_every) {
+  const int64_t n = op.m();
+  w.reinit(op.precision(), n);
+  T *r = static_cast<T *>(w.r.data()), *z = static_cast<T *>(w.z.data());
+  T *p = static_cast<T *>(w.p.data()), *q = static_cast<T *>(w.q.data());
+  const T *pinv = precond == PMG_PRECOND_JACOBI ? static_cast<const T *>(op.inverse_diagonal()) : nullptr;
+  double *sc = w.sc.ptr;
+  // r = b - A x
+  op.vmult(q, x, s);
+  launch_residual<T>(n, q, b, s);  // q = b - A x
+  launch_copy<T>(n, r, q, s);
+  launch_dot<T>(n, r, r, w.partials.ptr, sc + 2, s);
+  double rr = 0;
+  PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  CGResult res;
+  res.res0 = res.res = std::sqrt(rr);
+  const double target = rel_tol * res.res0;
+  if (res.res <= target || res.res0 == 0.0) {
+    res.converged = true;
+    return res;
+  }
+  int it = 0;
+  while (it < max_it) {
+    if (precond == PMG_PRECOND_MG) {
+      mg->vmult(z, r, s);
+      launch_dot<T>(n, r, z, w.partials.ptr, sc + 0, s);
+    } else {
+      launch_cg_precond_dot<T>(n, r, pinv, z, w.partials.ptr, s);
+      launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 0, 1, s);
+    }
+    launch_cg_update_p<T>(n, p, z, sc, it == 0 ? 1 : 0, s);
+    op.vmult(q, p, s);
+    launch_dot<T>(n, p, q, w.partials.ptr, sc + 1, s);
+    launch_cg_update_xr<T>(n, x, r, p, q, sc, w.partials.ptr, s);
+    launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 2, 1, s);
+    launch_cg_scalars(sc, 0, s);
+    ++it;
+    if (it % check_every == 0 || it == max_it) {
+      PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+      PMG_HIP_CHECK(hipStreamSynchronize(s));
+      res.res = std::sqrt(rr);
+      if (!(res.res > target)) {  // also stops on NaN
+        res.converged = res.res <= target;
+        break;
+      }
+    }
+  }
+  res.iterations = it;
+  return res;
+}
+}  // namespace
+
+CGResult solve_cg(const LaplaceOperatorBK3 &op, int precond, const VCycleMultigrid *mg,
+                  const void *b, void *x, double rel_tol, int max_it, CGWork &work,
+                  hipStream_t s, int check_every) {
+  PMG_REQUIRE(b && x && b != x, PMG_ERR_INVALID_ARG, "bad vectors");
+  PMG_REQUIRE(precond >= 0 && precond <= 2, PMG_ERR_INVALID_ARG, "unknown preconditioner");
+  PMG_REQUIRE(precond != PMG_PRECOND_MG || (mg && mg->outer_precision() == op.precision()),
+              PMG_ERR_INVALID_ARG, "MG preconditioner missing or of the wrong outer precision");
+  PMG_REQUIRE(rel_tol > 0 && max_it >= 0, PMG_ERR_INVALID_ARG, "bad tolerance or max_it");
+  if (op.precision() == PMG_FP64)
+    return solve_cg_t<double>(op, precond, mg, static_cast<const double *>(b),
+                              static_cast<double *>(x), rel_tol, max_it, work, s, check_every);
+  return solve_cg_t<float>(op, precond, mg, static_cast<const float *>(b), static_cast<float *>(x),
+                           rel_tol, max_it, work, s, check_every);
+}
+
+// ============================================================ Chebyshev
+namespace {
+// Lanczos estimate of the extreme eigenvalues of P^{-1}A through CG with
+// IterationNumberControl(n_it, 1e-10) semantics, right-hand side (i mod 11)
+// minus its mean, zero start (deal.II internal::estimate_eigenvalues).
+template <typename T>
+void estimate_eigenvalues(const LaplaceOperatorBK3 &op, int n_it, double &mn, double &mx) {
+  const int64_t n = op.m();
+  DeviceVector xv, rv, zv, pv, qv;
+  xv.reinit(op.precision(), n); rv.reinit(op.precision(), n); zv.reinit(op.precision(), n);
+  pv.reinit(op.precision(), n); qv.reinit(op.precision(), n);
+  DeviceArray<double> partials(kDotBlocks), sc(4);
+  T *x = static_cast<T *>(xv.data()), *r = static_cast<T *>(rv.data()), *z = static_cast<T *>(zv.data());
+  T *p = static_cast<T *>(pv.data()), *q = static_cast<T *>(qv.data());
+  const T *pinv = static_cast<const T *>(op.inverse_diagonal());
+  std::vector<T> h(static_cast<size_t>(n));
+  double mean = 0;
+  for (int64_t i = 0; i < n; ++i) mean += double(i % 11);
+  mean /= std::max<int64_t>(n, 1);
+  for (int64_t i = 0; i < n; ++i) h[i] = static_cast<T>(double(i % 11) - mean);
+  PMG_HIP_CHECK(hipMemcpy(r, h.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+  launch_fill<T>(n, x, T(0), nullptr);
+  double res = std::sqrt(dot_sync<T>(n, r, r, partials, sc, nullptr));
+  std::vector<double> al, be;
+  double rho_old = 0;
+  int it = 0;
+  while (it < n_it && res > 1e-10) {
+    launch_cg_precond_dot<T>(n, r, pinv, z, partials.ptr, nullptr);
+    launch_reduce_partials(partials.ptr, kDotBlocks, sc.ptr, 0, 1, nullptr);
+    double rho = 0;
+    PMG_HIP_CHECK(hipMemcpy(&rho, sc.ptr, sizeof(double), hipMemcpyDeviceToHost));
+    const double beta = it == 0 ? 0.0 : rho / rho_old;
+    if (it > 0) be.push_back(beta);
+    // p = z + beta p
+    PMG_HIP_CHECK(hipMemcpy(sc.ptr + 3, &rho_old, sizeof(double), hipMemcpyHostToDevice));
+    launch_cg_update_p<T>(n, p, z, sc.ptr, it == 0 ? 1 : 0, nullptr);
+    op.vmult(q, p, nullptr);
+    const double pq = dot_sync<T>(n, p, q, partials, sc, nullptr);
+    if (!(pq > 0)) break;
+    const double alpha = rho / pq;
+    al.push_back(alpha);
+    double s2[4] = {rho, pq, 0, 0};
+    PMG_HIP_CHECK(hipMemcpy(sc.ptr, s2, 2 * sizeof(double), hipMemcpyHostToDevice));
+    launch_cg_update_xr<T>(n, x, r, p, q, sc.ptr, partials.ptr, nullptr);
+    launch_reduce_partials(partials.ptr, kDotBlocks, sc.ptr, 2, 1, nullptr);
+    double rr = 0;
+    PMG_HIP_CHECK(hipMemcpy(&rr, sc.ptr + 2, sizeof(double), hipMemcpyDeviceToHost));
+    res = std::sqrt(rr);
+    rho_old = rho;
+    ++it;
+  }
+  const int k = static_cast<int>(al.size());
+  if (k == 0) {
+    mn = mx = 1.0;
+    return;
+  }
+  std::vector<double> d(k), o(std::max(k - 1, 0));
+  for (int i = 0; i < k; ++i) d[i] = 1.0 / al[i] + (i > 0 ? be[i - 1] / al[i - 1] : 0.0);
+  for (int i = 0; i + 1 < k; ++i) o[i] = std::sqrt(be[i]) / al[i];
+  const auto ev = tridiag_eigenvalues(d, o);
+  mn = ev.front();
+  mx = ev.back();
+}
+}  // namespace
+
+PreconditionChebyshev::PreconditionChebyshev(const LaplaceOperatorBK3 &op, int degree,
+                                             double smoothing_range, int eig_cg_n_iterations)
+    : op_(op), degree_(degree), range_(smoothing_range) {
+  PMG_REQUIRE(op.has_diagonal(), PMG_ERR_INVALID_ARG,
+              "Chebyshev needs compute_diagonal() on its operator first");
+  PMG_REQUIRE(smoothing_range > 0 && eig_cg_n_iterations > 2, PMG_ERR_INVALID_ARG,
+              "need smoothing_range > 0 and eig_cg_n_iterations > 2");
+  PMG_REQUIRE(degree > 0 || smoothing_range < 1, PMG_ERR_INVALID_ARG,
+              "automatic degree needs smoothing_range < 1");
+  double mn, mx;
+  if (op.precision() == PMG_FP64) estimate_eigenvalues<double>(op, eig_cg_n_iterations, mn, mx);
+  else estimate_eigenvalues<float>(op, eig_cg_n_iterations, mn, mx);
+  min_ev_ = mn;
+  max_ev_ = 1.2 * mx;  // safety factor: CG is not converged
+  const double alpha = range_ > 1.0 ? max_ev_ / range_ : std::min(0.9 * max_ev_, min_ev_);
+  if (degree_ <= 0) {
+    const double actual_range = max_ev_ / alpha;
+    const double sigma = (1.0 - std::sqrt(1.0 / actual_range)) / (1.0 + std::sqrt(1.0 / actual_range));
+    const double eps = range_;
+    degree_ = 1 + static_cast<int>(std::log(1.0 / eps + std::sqrt(1.0 / eps / eps - 1.0)) /
+                                   std::log(1.0 / sigma));
+  }
+  delta_ = 0.5 * (max_ev_ - alpha);
+  theta_ = 0.5 * (max_ev_ + alpha);
+  tmp_.reinit(op.precision(), op.m());
+  old_.reinit(op.precision(), op.m());
+}
+
+void PreconditionChebyshev::run(void *dst, const void *src, bool step, hipStream_t s) const {
+  PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
+  const int64_t n = op_.m();
+  const bool f64 = op_.precision() == PMG_FP64;
+  auto upd = [&](int mode, double f1, double f2, bool old_zero) {
+    if (f64)
+      launch_cheby_update<double>(mode, n, static_cast<double *>(dst), static_cast<double *>(old_.data()),
+                                  static_cast<const double *>(src), static_cast<const double *>(tmp_.data()),
+                                  static_cast<const double *>(op_.inverse_diagonal()), f1, f2, old_zero, s);
+    else
+      launch_cheby_update<float>(mode, n, static_cast<float *>(dst), static_cast<float *>(old_.data()),
+                                 static_cast<const float *>(src), static_cast<const float *>(tmp_.data()),
+                                 static_cast<const float *>(op_.inverse_diagonal()), f1, f2, old_zero, s);
+  };
+  if (!step) {
+    upd(0, 0.0, 1.0 / theta_, false);
+  } else {
+    op_.vmult(tmp_.data(), dst, s);
+    upd(1, 0.0, 1.0 / theta_, false);
+  }
+  if (degree_ < 2 || std::fabs(delta_) < 1e-40) return;
+  double rhok = delta_ / theta_;
+  const double sigma = theta_ / delta_;
+  for (int k = 0; k < degree_ - 1; ++k) {
+    op_.vmult(tmp_.data(), dst, s);
+    const double rhokp = 1.0 / (2.0 * sigma - rhok);
+    const double f1 = rhokp * rhok, f2 = 2.0 * rhokp / delta_;
+    rhok = rhokp;
+    upd(2, f1, f2, !step && k == 0);
+  }
+}
+
+void PreconditionChebyshev::vmult(void *dst, const void *src, hipStream_t s) const {
+  run(dst, src, false, s);
+}
+void PreconditionChebyshev::step(void *dst, const void *src, hipStream_t s) const {
+  run(dst, src, true, s);
+}
+
+// ============================================================ V-cycle
+VCycleMultigrid::VCycleMultigrid(std::vector<const LaplaceOperatorBK3 *> ops,
+                                 std::vector<const MGTransfer *> transfers,
+                                 std::vector<const PreconditionChebyshev *> smoothers,
+                                 int coarse_kind, double coarse_tol, int outer_precision)
+    : ops_(std::move(ops)), xfer_(std::move(transfers)), sm_(std::move(smoothers)),
+      coarse_kind_(coarse_kind), outer_precision_(outer_precision), coarse_tol_(coarse_tol) {
+  const int L = static_cast<int>(ops_.size());
+  PMG_REQUIRE(L >= 1, PMG_ERR_INVALID_ARG, "need at least one level");
+  PMG_REQUIRE(int(xfer_.size()) == L && int(sm_.size()) == L, PMG_ERR_INVALID_ARG,
+              "transfers/smoothers arrays must have n_levels entries");
+  check_precision(outer_precision);
+  PMG_REQUIRE(coarse_kind == PMG_COARSE_PCG || coarse_kind == PMG_COARSE_CHEBYSHEV,
+              PMG_ERR_INVALID_ARG, "unknown coarse solver");
+  precision_ = ops_[0]->precision();
+  for (int l = 0; l < L; ++l) {
+    PMG_REQUIRE(ops_[l] && ops_[l]->precision() == precision_, PMG_ERR_INVALID_ARG,
+                "all level operators must exist and share one precision");
+    PMG_REQUIRE(ops_[l]->has_diagonal(), PMG_ERR_INVALID_ARG,
+                "level operators need compute_diagonal()");
+    if (l > 0) {
+      PMG_REQUIRE(xfer_[l] && sm_[l], PMG_ERR_INVALID_ARG, "missing transfer or smoother");
+      PMG_REQUIRE(xfer_[l]->precision() == precision_ && &sm_[l]->op() == ops_[l],
+                  PMG_ERR_INVALID_ARG, "transfer/smoother precision or operator mismatch");
+      PMG_REQUIRE(xfer_[l]->n_dofs_coarse() == ops_[l - 1]->m() &&
+                      xfer_[l]->n_dofs_fine() == ops_[l]->m(),
+                  PMG_ERR_SIZE, "transfer sizes do not match the level operators");
+    }
+  }
+  if (coarse_kind == PMG_COARSE_CHEBYSHEV) {
+    const int n_it = static_cast<int>(std::max<int64_t>(ops_[0]->m(), 3));
+    coarse_cheby_ = std::make_unique<PreconditionChebyshev>(*ops_[0], 0, 1e-3, n_it);
+  }
+  sol_.resize(L);
+  def_.resize(L);
+  t_.resize(L);
+  for (int l = 0; l < L; ++l) {
+    sol_[l].reinit(precision_, ops_[l]->m());
+    def_[l].reinit(precision_, ops_[l]->m());
+    t_[l].reinit(precision_, ops_[l]->m());
+  }
+  coarse_work_.reinit(precision_, ops_[0]->m());
+}
+
+void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
+  const bool f64 = precision_ == PMG_FP64;
+  const int64_t n = ops_[l]->m();
+  if (l == 0) {
+    if (coarse_kind_ == PMG_COARSE_PCG) {
+      if (f64) launch_fill<double>(n, static_cast<double *>(sol_[0].data()), 0.0, s);
+      else launch_fill<float>(n, static_cast<float *>(sol_[0].data()), 0.0f, s);
+      solve_cg(*ops_[0], PMG_PRECOND_JACOBI, nullptr, def_[0].data(), sol_[0].data(), coarse_tol_,
+               100000, coarse_work_, s, 4);
+    } else {
+      coarse_cheby_->vmult(sol_[0].data(), def_[0].data(), s);
+    }
+    return;
+  }
+  sm_[l]->vmult(sol_[l].data(), def_[l].data(), s);
+  ops_[l]->vmult(t_[l].data(), sol_[l].data(), s);
+  const int64_t nc = ops_[l - 1]->m();
+  if (f64) {
+    launch_residual<double>(n, static_cast<double *>(t_[l].data()), static_cast<const double *>(def_[l].data()), s);
+    launch_fill<double>(nc, static_cast<double *>(def_[l - 1].data()), 0.0, s);
+  } else {
+    launch_residual<float>(n, static_cast<float *>(t_[l].data()), static_cast<const float *>(def_[l].data()), s);
+    launch_fill<float>(nc, static_cast<float *>(def_[l - 1].data()), 0.0f, s);
+  }
+  xfer_[l]->restrict_and_add(def_[l - 1].data(), t_[l].data(), s);
+  v_cycle(l - 1, s);
+  xfer_[l]->prolongate_and_add(sol_[l].data(), sol_[l - 1].data(), s);
+  sm_[l]->step(sol_[l].data(), def_[l].data(), s);
+}
+
+void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  const int top = levels() - 1;
+  const int64_t n = ops_[top]->m();
+  const bool in64 = outer_precision_ == PMG_FP64, lv64 = precision_ == PMG_FP64;
+  if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(def_[top].data()), static_cast<const double *>(src), s);
+  else if (in64) launch_convert<float, double>(n, static_cast<float *>(def_[top].data()), static_cast<const double *>(src), s);
+  else if (lv64) launch_convert<double, float>(n, static_cast<double *>(def_[top].data()), static_cast<const float *>(src), s);
+  else launch_copy<float>(n, static_cast<float *>(def_[top].data()), static_cast<const float *>(src), s);
+  v_cycle(top, s);
+  if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(dst), static_cast<const double *>(sol_[top].data()), s);
+  else if (in64) launch_convert<double, float>(n, static_cast<double *>(dst), static_cast<const float *>(sol_[top].data()), s);
+  else if (lv64) launch_convert<float, double>(n, static_cast<float *>(dst), static_cast<const double *>(sol_[top].data()), s);
+  else launch_copy<float>(n, static_cast<float *>(dst), static_cast<const float *>(sol_[top].data()), s);
+}
+
+}  // namespace pmg

@@ -1,0 +1,262 @@
+// pmg_vector.hip -- streaming vector kernels of the solver stack (gfx950).
+// These replace the deal.II device-vector operations that SolverCG and
+// PreconditionChebyshev perform between operator applications
+// (tests/common/portable_multigrid_solver.h:171-271); each is fused so that a
+// CG or Chebyshev iteration touches every vector entry the minimum number of
+// times.  Reductions are deterministic: a fixed grid of kDotBlocks blocks
+// writes partial sums that one block adds in a fixed order.
+#include "pmg_common.hpp"
+#include "pmg_kernels.hpp"
+
+namespace pmg {
+
+namespace {
+constexpr int kThreads = 256;
+inline unsigned grid_for(int64_t n) {
+  int64_t b = (n + kThreads - 1) / kThreads;
+  if (b > 65536) b = 65536;
+  return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+template <typename T> __global__ void fill_k(int64_t n, T *x, T v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+template <typename T> __global__ void copy_k(int64_t n, T *d, const T *s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    d[i] = s[i];
+}
+template <typename T>
+__global__ void copy_idx_k(int64_t n, const uint32_t *idx, T *d, const T *s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t j = idx[i];
+    d[j] = s[j];
+  }
+}
+template <typename T>
+__global__ void set_idx_k(int64_t n, const uint32_t *idx, T *d, T v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    d[idx[i]] = v;
+}
+template <typename T> __global__ void invert_k(int64_t n, T *x) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = T(1) / x[i];
+}
+template <typename TD, typename TS> __global__ void convert_k(int64_t n, TD *d, const TS *s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    d[i] = static_cast<TD>(s[i]);
+}
+template <typename T> __global__ void residual_k(int64_t n, T *t, const T *b) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    t[i] = b[i] - t[i];
+}
+
+// block-level sum of a double (wave64 shuffles, then LDS across waves)
+__device__ inline double block_sum(double v) {
+  __shared__ double part[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) part[wave] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; ++w) s += part[w];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) dot_k(int64_t n, const T *a, const T *b, double *partials) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// sc[slot + k] = sum_b partials[k * n_partials + b] (optionally also its sqrt)
+__global__ void __launch_bounds__(kThreads)
+reduce_k(const double *partials, int n_partials, double *sc, int slot, int n_sums) {
+  for (int k = 0; k < n_sums; ++k) {
+    double s = 0;
+    for (int i = threadIdx.x; i < n_partials; i += kThreads) s += partials[k * n_partials + i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) sc[slot + k] = s;
+  }
+}
+
+template <typename T>
+__global__ void cheby_k(int mode, int64_t n, T *x, T *old, const T *b, const T *t, const T *pinv,
+                        T f1, T f2, bool old_zero) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    if (mode == 0) {
+      x[i] = f2 * pinv[i] * b[i];
+    } else if (mode == 1) {
+      const T xi = x[i];
+      old[i] = xi;
+      x[i] = xi + f2 * pinv[i] * (b[i] - t[i]);
+    } else {
+      const T xi = x[i];
+      const T xo = old_zero ? T(0) : old[i];
+      old[i] = xi;
+      x[i] = (T(1) + f1) * xi - f1 * xo + f2 * pinv[i] * (b[i] - t[i]);
+    }
+  }
+}
+
+// CG scalar slots (device doubles):
+//  0 rho = r.z      1 p.q       2 r.r       3 rho_old   4 alpha   5 beta
+enum { kRho = 0, kPq = 1, kRr = 2, kRhoOld = 3, kAlpha = 4, kBeta = 5 };
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+cg_xr_k(int64_t n, T *x, T *r, const T *p, const T *q, const double *sc, double *partials) {
+  const double pq = sc[kPq];
+  const double alpha = pq != 0.0 ? sc[kRho] / pq : 0.0;
+  const T a = static_cast<T>(alpha);
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    x[i] += a * p[i];
+    const T ri = r[i] - a * q[i];
+    r[i] = ri;
+    s += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+cg_pz_k(int64_t n, const T *r, const T *pinv, T *z, double *partials) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const T zi = pinv ? pinv[i] * r[i] : r[i];
+    z[i] = zi;
+    s += static_cast<double>(zi) * static_cast<double>(r[i]);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void cg_p_k(int64_t n, T *p, const T *z, const double *sc, int first) {
+  const double ro = sc[kRhoOld];
+  const T beta = (first || ro == 0.0) ? T(0) : static_cast<T>(sc[kRho] / ro);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+__global__ void cg_scalars_k(double *sc, int op) {
+  if (threadIdx.x == 0) {
+    if (op == 0) sc[kRhoOld] = sc[kRho];  // end of iteration bookkeeping
+  }
+}
+}  // namespace
+
+template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, v);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T> void launch_copy(int64_t n, T *d, const T *src, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(copy_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, d, src);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_copy_indexed(int64_t n, const uint32_t *idx, T *d, const T *src, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(copy_idx_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, idx, d, src);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_set_indexed(int64_t n, const uint32_t *idx, T *d, T v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(set_idx_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, idx, d, v);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T> void launch_invert(int64_t n, T *x, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(invert_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename TD, typename TS>
+void launch_convert(int64_t n, TD *d, const TS *src, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((convert_k<TD, TS>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, d, src);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(residual_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, t, b);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_dot(int64_t n, const T *a, const T *b, double *partials, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(dot_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, a, b, partials);
+  PMG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, kDotBlocks, out, 0, 1);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
+                            int n_sums, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, n_partials, sc, slot,
+                     n_sums);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cheby_update(int mode, int64_t n, T *x, T *old, const T *b, const T *t,
+                         const T *pinv, double f1, double f2, bool old_is_zero, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cheby_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, mode, n, x, old, b, t,
+                     pinv, static_cast<T>(f1), static_cast<T>(f2), old_is_zero);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cg_update_xr(int64_t n, T *x, T *r, const T *p, const T *q, const double *sc,
+                         double *partials, hipStream_t s) {
+  hipLaunchKernelGGL(cg_xr_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, x, r, p, q, sc,
+                     partials);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *partials,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(cg_pz_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, pinv, z, partials);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cg_update_p(int64_t n, T *p, const T *z, const double *sc, int first, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cg_p_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, p, z, sc, first);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_cg_scalars(double *sc, int op, hipStream_t s) {
+  hipLaunchKernelGGL(cg_scalars_k, dim3(1), dim3(64), 0, s, sc, op);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+#define PMG_INST(T)                                                                           \
+  template void launch_fill<T>(int64_t, T *, T, hipStream_t);                                 \
+  template void launch_copy<T>(int64_t, T *, const T *, hipStream_t);                         \
+  template void launch_copy_indexed<T>(int64_t, const uint32_t *, T *, const T *, hipStream_t); \
+  template void launch_set_indexed<T>(int64_t, const uint32_t *, T *, T, hipStream_t);        \
+  template void launch_invert<T>(int64_t, T *, hipStream_t);                                  \
+  template void launch_residual<T>(int64_t, T *, const T *, hipStream_t);                     \
+  template void launch_dot<T>(int64_t, const T *, const T *, double *, double *, hipStream_t); \
+  template void launch_cheby_update<T>(int, int64_t, T *, T *, const T *, const T *,          \
+                                       const T *, double, double, bool, hipStream_t);         \
+  template void launch_cg_update_xr<T>(int64_t, T *, T *, const T *, const T *,               \
+                                       const double *, double *, hipStream_t);                \
+  template void launch_cg_precond_dot<T>(int64_t, const T *, const T *, T *, double *,        \
+                                         hipStream_t);                                        \
+  template void launch_cg_update_p<T>(int64_t, T *, const T *, const double *, int, hipStream_t);
+PMG_INST(double)
+PMG_INST(float)
+#undef PMG_INST
+template void launch_convert<double, float>(int64_t, double *, const float *, hipStream_t);
+template void launch_convert<float, double>(int64_t, float *, const double *, hipStream_t);
+template void launch_convert<double, double>(int64_t, double *, const double *, hipStream_t);
+template void launch_convert<float, float>(int64_t, float *, const float *, hipStream_t);
+
+}  // namespace pmg

@@ -1,0 +1,143 @@
+"""GPU parity of the BK3 operator (libpmg.so) against the CPU oracle.
+
+Bar: FP64 within 1e-12 relative (max-norm) of the oracle's restatement of
+BK3::Parallel::KokkosKernel + LaplaceOperatorBK3::vmult; FP32 within 2e-6.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err, seeded_uniform
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (cells per dir, degree, nq, deform)
+    (3, 1, 2, 0), (2, 1, 3, 0), (3, 2, 3, 0), (2, 2, 4, 1), (3, 3, 4, 1), (2, 3, 5, 0),
+    (4, 4, 5, 0), (3, 4, 6, 1), (2, 5, 6, 0), (2, 5, 7, 1), (2, 6, 7, 1), (2, 6, 8, 0),
+]
+
+
+def _pair(pmg, orc, n, p, nq, deform, precision):
+    b = pmg.box(n, deform=deform, amplitude=0.05)
+    m = orc.mesh(n, deform=deform, amplitude=0.05)
+    op = pmg.LaplaceOperatorBK3(b, p, nq, precision)
+    ref = orc.LaplaceBK3(m, p, nq)
+    return op, ref
+
+
+@pytest.mark.parametrize("n,p,nq,deform", CASES)
+def test_vmult_fp64_matches_oracle(pmg, orc, n, p, nq, deform):
+    import torch
+    print(f"[bk3 fp64] n={n} p={p} nq={nq} deform={deform}", flush=True)
+    op, ref = _pair(pmg, orc, n, p, nq, deform, pmg.FP64)
+    x = seeded_uniform(op.m(), 1000 + 10 * p + nq)
+    src = torch.tensor(x, device="cuda")
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, src)
+    y = ref.vmult(x)
+    assert rel_err(dst.cpu().numpy(), y) < 1e-12
+    # Tvmult == vmult, and repeated application is reproducible to round-off
+    dst2 = op.initialize_dof_vector()
+    op.Tvmult(dst2, src)
+    assert rel_err(dst2.cpu().numpy(), y) < 1e-12
+
+
+@pytest.mark.parametrize("n,p,nq,deform", [(3, 1, 2, 0), (3, 2, 3, 1), (3, 4, 5, 0), (2, 6, 7, 1)])
+def test_vmult_fp32_matches_oracle(pmg, orc, n, p, nq, deform):
+    import torch
+    print(f"[bk3 fp32] n={n} p={p} nq={nq}", flush=True)
+    op, ref = _pair(pmg, orc, n, p, nq, deform, pmg.FP32)
+    x = seeded_uniform(op.m(), 2000 + p)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, torch.tensor(x, dtype=torch.float32, device="cuda"))
+    y = ref.vmult(x.astype(np.float32).astype(np.float64))
+    assert rel_err(dst.cpu().numpy(), y) < 2e-6
+
+
+def test_apply_add_and_constrained_rows(pmg, orc):
+    import torch
+    op, ref = _pair(pmg, orc, 3, 3, 4, 1, pmg.FP64)
+    x = seeded_uniform(op.m(), 7)
+    src = torch.tensor(x, device="cuda")
+    base = seeded_uniform(op.m(), 8)
+    dst = torch.tensor(base, device="cuda")
+    op.apply_add(dst, src)
+    y = ref.cells_add(x, base.copy())
+    assert rel_err(dst.cpu().numpy(), y) < 1e-12
+    # constrained rows of vmult are the identity
+    out = op.initialize_dof_vector()
+    op.vmult(out, src)
+    cons = ref.cons
+    assert np.array_equal(out.cpu().numpy()[cons], x[cons])
+
+
+@pytest.mark.parametrize("n,p,nq,deform", [(3, 1, 2, 0), (3, 2, 3, 1), (2, 4, 5, 0), (2, 4, 6, 1),
+                                           (2, 6, 7, 0)])
+def test_diagonal_matches_dense(pmg, orc, n, p, nq, deform):
+    op, ref = _pair(pmg, orc, n, p, nq, deform, pmg.FP64)
+    op.compute_diagonal()
+    inv = op.get_matrix_diagonal_inverse().cpu().numpy()
+    d = ref.diagonal()
+    assert rel_err(1.0 / inv, d) < 1e-12
+
+
+def test_large_vmult_properties(pmg):
+    """Size-independent checks at the 64^3 Q4 workload: symmetry <Ax,y> = <x,Ay>
+    and A 1 = 0 on rows whose stencil touches no constrained DoF."""
+    import torch
+    b = pmg.box(64)
+    op = pmg.LaplaceOperatorBK3(b, 4, 5, pmg.FP64)
+    n = op.m()
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    y = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    cons = torch.tensor(pmg.box_boundary_dofs(b, 4).astype(np.int64), device="cuda")
+    x[cons] = 0
+    y[cons] = 0
+    ax, ay = op.initialize_dof_vector(), op.initialize_dof_vector()
+    op.vmult(ax, x)
+    op.vmult(ay, y)
+    lhs, rhs = torch.dot(ax, y).item(), torch.dot(x, ay).item()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    ones = torch.ones(n, dtype=torch.float64, device="cuda")
+    a1 = op.initialize_dof_vector()
+    op.vmult(a1, ones)
+    N = 4 * 64 + 1
+    a1 = a1.reshape(N, N, N)
+    inner = a1[5:-5, 5:-5, 5:-5]
+    assert inner.abs().max().item() < 1e-11 * a1.abs().max().item()
+
+
+def test_coefficient_and_gradient_match_oracle(pmg, orc):
+    import torch
+    n, p, nq = 3, 3, 4
+    op, ref = _pair(pmg, orc, n, p, nq, 1, pmg.FP64)
+    kappa = np.exp(seeded_uniform(op.n_cells * nq ** 3, 21, -0.5, 0.5))
+    op.set_coefficient(torch.tensor(kappa, device="cuda"))
+    ref.G = orc.scale_G(ref.G, kappa, nq)
+    x = seeded_uniform(op.m(), 22)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, torch.tensor(x, device="cuda"))
+    assert rel_err(dst.cpu().numpy(), ref.vmult(x)) < 1e-12
+    u, lam = seeded_uniform(op.m(), 23), seeded_uniform(op.m(), 24)
+    u[ref.cons] = 0
+    lam[ref.cons] = 0
+    out = torch.zeros(op.n_cells * nq ** 3, dtype=torch.float64, device="cuda")
+    op.gradient_coefficient(torch.tensor(u, device="cuda"), torch.tensor(lam, device="cuda"), out)
+    G0 = orc.geometry(ref.m, nq)[0]
+    ref.G = G0
+    g_ref = ref.grad_kappa(kappa, u, lam)
+    assert rel_err(out.cpu().numpy(), g_ref) < 1e-12
+
+
+def test_bad_arguments_raise(pmg):
+    import torch
+    b = pmg.box(2)
+    with pytest.raises(pmg.PMGError):
+        pmg.LaplaceOperatorBK3(b, 4, 9)  # unsupported nq
+    op = pmg.LaplaceOperatorBK3(b, 2)
+    x = torch.zeros(op.m() + 1, dtype=torch.float64, device="cuda")
+    with pytest.raises(pmg.PMGError):
+        op.vmult(op.initialize_dof_vector(), x)
+    with pytest.raises(pmg.PMGError):
+        op.get_matrix_diagonal_inverse()  # compute_diagonal() not called
