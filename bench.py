@@ -140,6 +140,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--schedule", choices=["atomic", "colored"], default=None,
+                    help="BK3 cell-loop schedule (default: the library default)")
+    ap.add_argument("--gload", type=int, default=None, help="BK3 G-load policy 0/1/2")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -153,6 +158,8 @@ def main():
     fine_box = pmg.box(n)
     log(f"[bench] building operator {n}^3 Q{degree} nq={nq}")
     op = pmg.LaplaceOperatorBK3(fine_box, degree, nq, pmg.FP64)
+    if args.schedule is not None or args.gload is not None:
+        op.set_schedule(args.schedule == "colored", args.gload or 0)
     N = op.m()
     g = torch.Generator(device="cuda").manual_seed(1234)
     x = torch.rand(N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
@@ -180,7 +187,21 @@ def main():
                      "fp64_tflops": alg_flops / t_kernel / 1e12,
                      "fp64_frac": alg_flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS},
     }
+    if args.schedule is not None or args.gload is not None:
+        out["config"]["schedule"] = {"scatter": args.schedule or "atomic", "g_load": args.gload or 0}
     del y
+    if args.calibrate_copy:
+        # plain copy kernel of known size: 2^27 doubles read + written per launch
+        n = 1 << 27
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(3):
+            pmg._check(pmg.lib().pmg_vector_convert(n, pmg.FP64, pmg._dev_ptr(c, pmg.FP64),
+                                                    pmg.FP64, pmg._dev_ptr(a, pmg.FP64),
+                                                    pmg._stream()))
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 3 launches of 1 GiB read + 1 GiB write")
+        del a, c
 
     if not args.no_solve:
         b = op.assemble_rhs(1)

@@ -114,6 +114,7 @@ def lib():
         "pmg_operator_gradient_coefficient": (_i, [_vp, _vp, _vp, _vp, _vp]),
         "pmg_operator_algorithmic_bytes": (_i, [_vp, P(_d), P(_d)]),
         "pmg_operator_assemble_rhs": (_i, [_vp, _i, _i, _vp, _vp]),
+        "pmg_operator_set_schedule": (_i, [_vp, _i, _i]),
         "pmg_operator_l2_error": (_i, [_vp, _vp, _i, _i, P(_d), _vp]),
         "pmg_transfer_create": (_i, [_i, _i, _i, _i, _i64, _vp, _vp, _vp, _i64, _i64, _d,
                                      P(_vp)]),
@@ -314,6 +315,10 @@ class LaplaceOperatorBK3:
             self.h, _dev_ptr(u, self.precision, self._m), _dev_ptr(lam, self.precision, self._m),
             _dev_ptr(out, self.precision, self.n_cells * self.nq ** 3), _stream()))
 
+    def set_schedule(self, colored=False, g_load=0):
+        """Cell-loop schedule: atomic scatter (default) or one launch per colour."""
+        _check(lib().pmg_operator_set_schedule(self.h, int(colored), g_load))
+
     def assemble_rhs(self, f_kind=1, n_q_points_1d=0):
         """b = int phi_i f (f_kind 0: f = 1, 1: 3 pi^2 sin sin sin), constrained rows 0."""
         b = self.initialize_dof_vector()
@@ -466,27 +471,57 @@ def vector_dot(a, b, precision=FP64):
 
 
 # ------------------------------------------------------------------ hierarchies
-def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precision=FP64,
-                        smoother_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
-                        coarse=COARSE_PCG, coarse_tol=1e-12, prolongation_scale=1.0,
-                        outer_precision=None):
-    """h-only hierarchy at fixed degree (source/geometric_multigrid_bk3/program.cc):
-    level 0 has fine_box coarsened n_levels-1 times."""
+def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=3,
+                    smoothing_range=15.0, eig_cg_n_iterations=10, coarse=COARSE_PCG,
+                    coarse_tol=1e-12, prolongation_scale=1.0, outer_precision=None):
+    """V-cycle over `levels` = [(box, degree), ...] from coarsest to finest.
+
+    Consecutive levels on the same box are joined by a PolynomialTransfer,
+    levels whose box is refined once by a GeometricTransfer (same degree),
+    as the reference's ph hierarchy (tests/poisson_cube_bk3/program.cc:281-351,
+    421-470).  n_q_points_1d: None -> degree+1 per level, an int offset as
+    "degree + k" is given by passing a callable degree -> nq."""
+    ops, trs, sms = [], [None], [None]
+    for l, (b, p) in enumerate(levels):
+        nq = None if n_q_points_1d is None else n_q_points_1d(p)
+        op = LaplaceOperatorBK3(b, p, nq, precision)
+        op.compute_diagonal()
+        ops.append(op)
+        if l == 0:
+            continue
+        bc, pc = levels[l - 1]
+        if (bc.nx, bc.ny, bc.nz) == (b.nx, b.ny, b.nz):
+            trs.append(PolynomialTransfer(b, pc, p, precision))
+        else:
+            assert (2 * bc.nx, 2 * bc.ny, 2 * bc.nz) == (b.nx, b.ny, b.nz) and pc == p, \
+                "consecutive levels must differ by one refinement or by the degree"
+            trs.append(GeometricTransfer(bc, p, precision, prolongation_scale))
+        sms.append(PreconditionChebyshev(op, smoother_degree, smoothing_range,
+                                         eig_cg_n_iterations))
+    mg = VCycleMultigrid(ops, trs, sms, coarse, coarse_tol,
+                         precision if outer_precision is None else outer_precision)
+    return mg, ops, trs, sms
+
+
+def coarsen_boxes(fine_box, n_levels):
     boxes = [fine_box]
     for _ in range(n_levels - 1):
         b = boxes[0]
         assert b.nx % 2 == 0 and b.ny % 2 == 0 and b.nz % 2 == 0, "box not coarsenable"
         boxes.insert(0, box(b.nx // 2, b.ny // 2, b.nz // 2, tuple(b.lo), tuple(b.hi), b.deform,
                             b.amplitude))
-    ops, trs, sms = [], [None], [None]
-    for l, b in enumerate(boxes):
-        op = LaplaceOperatorBK3(b, degree, n_q_points_1d, precision)
-        op.compute_diagonal()
-        ops.append(op)
-        if l > 0:
-            trs.append(GeometricTransfer(boxes[l - 1], degree, precision, prolongation_scale))
-            sms.append(PreconditionChebyshev(op, smoother_degree, smoothing_range,
-                                             eig_cg_n_iterations))
-    mg = VCycleMultigrid(ops, trs, sms, coarse, coarse_tol,
-                         precision if outer_precision is None else outer_precision)
+    return boxes
+
+
+def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precision=FP64,
+                        smoother_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
+                        coarse=COARSE_PCG, coarse_tol=1e-12, prolongation_scale=1.0,
+                        outer_precision=None):
+    """h-only hierarchy at fixed degree (source/geometric_multigrid_bk3/program.cc):
+    level 0 has fine_box coarsened n_levels-1 times."""
+    boxes = coarsen_boxes(fine_box, n_levels)
+    nqf = None if n_q_points_1d is None else (lambda p: n_q_points_1d)
+    mg, ops, trs, sms = build_hierarchy([(b, degree) for b in boxes], nqf, precision,
+                                        smoother_degree, smoothing_range, eig_cg_n_iterations,
+                                        coarse, coarse_tol, prolongation_scale, outer_precision)
     return mg, ops, trs, sms, boxes

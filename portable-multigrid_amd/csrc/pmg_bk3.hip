@@ -33,33 +33,46 @@ __device__ inline T ldg_stream(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
-template <int NM, int NQ, typename T>
-__global__ void __launch_bounds__(256)
-bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
-                 const uint32_t *__restrict__ map, const int64_t n_cells, const Shape1D<T> sh)
+// G load policy of the BK3 kernel
+enum : int {
+  kGEarlyNT = 0,  // non-temporal loads issued at kernel entry
+  kGEarly = 1,    // ordinary loads issued at kernel entry
+  kGLateNT = 2    // non-temporal loads issued where G is used
+};
+
+// One batch of E = Bk3Tile<NQ>::cells cells at positions cell_base .. cell_base+E-1
+// of a range of n_range cells (cell index = cell_ids[pos] if cell_ids, else pos),
+// processed by the first E*NQ^2 lanes of the workgroup (all lanes must call it:
+// it holds workgroup barriers).  lds: E * 3 * NQ^3 elements.  ATOMIC = false is
+// only valid when no two cells of the range share a DoF (one colour).
+template <int NM, int NQ, typename T, bool ATOMIC = true, int GMODE = kGEarlyNT>
+__device__ __forceinline__ void
+bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
+          const uint32_t *__restrict__ map, const uint32_t *__restrict__ cell_ids,
+          const int64_t n_range, const Shape1D<T> &sh, const int64_t cell_base, T *lds)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells;
   constexpr int BUF = NQ3;
-  __shared__ T lds[E * 3 * BUF];
 
-  const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int e = tid / NQ2, t = tid - e * NQ2, ta = t % NQ, tb = t / NQ;
-  const int64_t cell = static_cast<int64_t>(lb) * E + e;
-  const bool active = (e < E) && (cell < n_cells);
+  const int64_t pos = cell_base + e;
+  const bool active = (e < E) && (pos < n_range);
+  const int64_t cell = active ? (cell_ids ? static_cast<int64_t>(cell_ids[pos]) : pos) : 0;
   T *P0 = lds + (active ? e : 0) * 3 * BUF;
   T *P1 = P0 + BUF;
   T *P2 = P1 + BUF;
+  const T *Gc = G + cell * (6 * NQ3) + tb * NQ + ta;
 
   // ---- geometric factors for this lane's column (p,q) = (ta,tb), all r
   T g[6][NQ];
-  if (active) {
-    const T *Gc = G + cell * (6 * NQ3) + tb * NQ + ta;
+  if (GMODE != kGLateNT && active) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-      for (int r = 0; r < NQ; ++r) g[c][r] = ldg_stream(Gc + c * NQ3 + r * NQ2);
+      for (int r = 0; r < NQ; ++r)
+        g[c][r] = GMODE == kGEarlyNT ? ldg_stream(Gc + c * NQ3 + r * NQ2) : Gc[c * NQ3 + r * NQ2];
   }
 
   // ---- phase 1: gather column (i,j)=(ta,tb) over k, interpolate in z
@@ -137,6 +150,12 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 
   // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
   if (active) {
+    if (GMODE == kGLateNT) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) g[c][r] = ldg_stream(Gc + c * NQ3 + r * NQ2);
+    }
     T col[NQ], fz[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; ++n) col[n] = P0[n * NQ2 + tb * NQ + ta];
@@ -223,38 +242,200 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
       T v = 0;
 #pragma unroll
       for (int r = 0; r < NQ; ++r) v += sh.S[k * NQ + r] * z[r];
-      if (idx[k] != kInvalid) atomic_add(y + idx[k], v);
+      if (idx[k] != kInvalid) {
+        if (ATOMIC) atomic_add(y + idx[k], v);
+        else y[idx[k]] += v;  // one colour: this cell is the only writer
+      }
     }
   }
 }
 
+template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
+__global__ void __launch_bounds__(256)
+bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
+                 const uint32_t *__restrict__ map, const uint32_t *__restrict__ cell_ids,
+                 const int64_t n_range, const Shape1D<T> sh)
+{
+  constexpr int E = Bk3Tile<NQ>::cells;
+  __shared__ T lds[E * 3 * NQ * NQ * NQ];
+  const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  bk3_batch<NM, NQ, T, ATOMIC, GMODE>(x, y, G, map, cell_ids, n_range, sh,
+                                      static_cast<int64_t>(lb) * E, lds);
+}
+
+// ---------------------------------------------------------------- coarse-grid PCG
+// Jacobi-preconditioned CG on a small level run entirely inside ONE workgroup:
+// operator applications loop over cell batches with bk3_batch, dot products
+// are workgroup reductions, and the convergence test ||r|| <= tol ||r0|| is
+// evaluated on the device, so a coarse solve costs one launch instead of
+// ~10 launches and a host round trip per iteration.  Every lane derives the
+// loop decisions from the same LDS-broadcast scalars, so barriers stay uniform.
+__device__ inline double wg_sum_bcast(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s += red[w];
+    red[8] = s;
+  }
+  __syncthreads();
+  const double s = red[8];
+  __syncthreads();
+  return s;
+}
+
 template <int NM, int NQ, typename T>
-static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_t n_cells,
-                       const Shape1D<T> &sh, hipStream_t s) {
-  if (n_cells <= 0) return;
+__global__ void __launch_bounds__(256)
+coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, const int64_t n_cells,
+                  const Shape1D<T> sh, const uint32_t *__restrict__ cons, const int64_t n_cons,
+                  const T *__restrict__ pinv, const int64_t n, const T *__restrict__ b, T *x, T *r,
+                  T *z, T *p, T *q, const double rel_tol, const int max_it, double *info)
+{
+  constexpr int E = Bk3Tile<NQ>::cells;
+  __shared__ T lds[E * 3 * NQ * NQ * NQ];
+  __shared__ double red[9];
+  const int tid = threadIdx.x, nt = blockDim.x;
+
+  auto apply = [&](const T *in, T *out) {
+    for (int64_t i = tid; i < n; i += nt) out[i] = T(0);
+    __syncthreads();
+    for (int64_t base = 0; base < n_cells; base += E) {
+      bk3_batch<NM, NQ, T>(in, out, G, map, nullptr, n_cells, sh, base, lds);
+      __syncthreads();
+    }
+    for (int64_t i = tid; i < n_cons; i += nt) out[cons[i]] = in[cons[i]];
+    __syncthreads();
+  };
+
+  apply(x, q);
+  double s = 0;
+  for (int64_t i = tid; i < n; i += nt) {
+    const T ri = b[i] - q[i];
+    r[i] = ri;
+    s += double(ri) * double(ri);
+  }
+  const double r0 = sqrt(wg_sum_bcast(s, red));
+  double res = r0, rho_old = 1.0;
+  int it = 0;
+  while (it < max_it && res > rel_tol * r0) {
+    s = 0;
+    for (int64_t i = tid; i < n; i += nt) {
+      const T zi = pinv[i] * r[i];
+      z[i] = zi;
+      s += double(zi) * double(r[i]);
+    }
+    const double rho = wg_sum_bcast(s, red);
+    // first iteration: p = z exactly (p holds stale memory, 0 * NaN would leak)
+    const T beta = it == 0 ? T(0) : static_cast<T>(rho / rho_old);
+    for (int64_t i = tid; i < n; i += nt) p[i] = it == 0 ? z[i] : z[i] + beta * p[i];
+    __syncthreads();
+    apply(p, q);
+    s = 0;
+    for (int64_t i = tid; i < n; i += nt) s += double(p[i]) * double(q[i]);
+    const double pq = wg_sum_bcast(s, red);
+    const T alpha = pq != 0.0 ? static_cast<T>(rho / pq) : T(0);
+    s = 0;
+    for (int64_t i = tid; i < n; i += nt) {
+      x[i] += alpha * p[i];
+      const T ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      s += double(ri) * double(ri);
+    }
+    res = sqrt(wg_sum_bcast(s, red));
+    rho_old = rho;
+    ++it;
+    if (pq == 0.0) break;
+  }
+  if (tid == 0) {
+    info[0] = it;
+    info[1] = r0;
+    info[2] = res;
+  }
+}
+
+template <typename T>
+void launch_bk3_coarse_pcg(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
+                           const Shape1D<T> &sh, const uint32_t *cons, int64_t n_cons,
+                           const T *pinv, int64_t n, const T *b, T *x, T *r, T *z, T *p, T *q,
+                           double rel_tol, int max_it, double *info, hipStream_t s) {
+#define PMG_CPCG_CASE(P, Q)                                                                     \
+  if (degree == P && nq == Q) {                                                                 \
+    hipLaunchKernelGGL((coarse_pcg_kernel<P + 1, Q, T>), dim3(1), dim3(Bk3Tile<Q>::lanes), 0, s, \
+                       G, map, n_cells, sh, cons, n_cons, pinv, n, b, x, r, z, p, q, rel_tol,    \
+                       max_it, info);                                                           \
+    PMG_HIP_CHECK(hipGetLastError());                                                           \
+    return;                                                                                     \
+  }
+  PMG_CPCG_CASE(1, 2) PMG_CPCG_CASE(1, 3) PMG_CPCG_CASE(2, 3) PMG_CPCG_CASE(2, 4)
+  PMG_CPCG_CASE(3, 4) PMG_CPCG_CASE(3, 5) PMG_CPCG_CASE(4, 5) PMG_CPCG_CASE(4, 6)
+  PMG_CPCG_CASE(5, 6) PMG_CPCG_CASE(5, 7) PMG_CPCG_CASE(6, 7) PMG_CPCG_CASE(6, 8)
+#undef PMG_CPCG_CASE
+  throw Error(PMG_ERR_UNSUPPORTED, "coarse PCG kernel: unsupported (degree, n_q_points_1d)");
+}
+template void launch_bk3_coarse_pcg<double>(int, int, int64_t, const double *, const uint32_t *,
+                                            const Shape1D<double> &, const uint32_t *, int64_t,
+                                            const double *, int64_t, const double *, double *,
+                                            double *, double *, double *, double *, double,
+                                            int, double *, hipStream_t);
+template void launch_bk3_coarse_pcg<float>(int, int, int64_t, const float *, const uint32_t *,
+                                           const Shape1D<float> &, const uint32_t *, int64_t,
+                                           const float *, int64_t, const float *, float *, float *,
+                                           float *, float *, float *, double, int, double *,
+                                           hipStream_t);
+
+template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
+static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
+                             const uint32_t *cell_ids, int64_t n_range, const Shape1D<T> &sh,
+                             hipStream_t s) {
+  if (n_range <= 0) return;
   using Tile = Bk3Tile<NQ>;
-  const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
+  const int64_t nb = (n_range + Tile::cells - 1) / Tile::cells;
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
-  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T>), dim3(grid), dim3(Tile::lanes), 0, s, x, y, G,
-                     map, n_cells, sh);
+  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, ATOMIC, GMODE>), dim3(grid), dim3(Tile::lanes), 0,
+                     s, x, y, G, map, cell_ids, n_range, sh);
   PMG_HIP_CHECK(hipGetLastError());
+}
+
+// FULL instantiates every G-load policy (used for the production degrees).
+template <int NM, int NQ, typename T, bool FULL>
+static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_t n_cells,
+                       const Shape1D<T> &sh, hipStream_t s, const Bk3Schedule *sc) {
+  if (sc && sc->colored) {
+    // one launch per colour, no atomics; launches are stream-ordered, so a
+    // colour sees the updates of the previous ones
+    for (size_t c = 0; c + 1 < sc->color_offsets.size(); ++c) {
+      const int64_t b = sc->color_offsets[c], n = sc->color_offsets[c + 1] - b;
+      bk3_launch_range<NM, NQ, T, false, kGEarlyNT>(x, y, G, map, sc->cell_ids + b, n, sh, s);
+    }
+    return;
+  }
+  const int gmode = sc ? sc->gmode : kGEarlyNT;
+  if (FULL && gmode == kGEarly)
+    bk3_launch_range<NM, NQ, T, true, (FULL ? kGEarly : kGEarlyNT)>(x, y, G, map, nullptr, n_cells, sh, s);
+  else if (FULL && gmode == kGLateNT)
+    bk3_launch_range<NM, NQ, T, true, (FULL ? kGLateNT : kGEarlyNT)>(x, y, G, map, nullptr, n_cells, sh, s);
+  else
+    bk3_launch_range<NM, NQ, T, true, kGEarlyNT>(x, y, G, map, nullptr, n_cells, sh, s);
 }
 
 template <typename T>
 void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
-                      int64_t n_cells, const Shape1D<T> &sh, hipStream_t s) {
-#define PMG_BK3_CASE(P, Q)                                       \
-  if (degree == P && nq == Q) {                                  \
-    bk3_launch<P + 1, Q, T>(x, y, G, map, n_cells, sh, s);       \
-    return;                                                      \
+                      int64_t n_cells, const Shape1D<T> &sh, hipStream_t s,
+                      const Bk3Schedule *sched) {
+#define PMG_BK3_CASE(P, Q, FULL)                                          \
+  if (degree == P && nq == Q) {                                           \
+    bk3_launch<P + 1, Q, T, FULL>(x, y, G, map, n_cells, sh, s, sched);   \
+    return;                                                               \
   }
-  PMG_BK3_CASE(1, 2) PMG_BK3_CASE(1, 3)
-  PMG_BK3_CASE(2, 3) PMG_BK3_CASE(2, 4)
-  PMG_BK3_CASE(3, 4) PMG_BK3_CASE(3, 5)
-  PMG_BK3_CASE(4, 5) PMG_BK3_CASE(4, 6)
-  PMG_BK3_CASE(5, 6) PMG_BK3_CASE(5, 7)
-  PMG_BK3_CASE(6, 7) PMG_BK3_CASE(6, 8)
+  PMG_BK3_CASE(1, 2, false) PMG_BK3_CASE(1, 3, false)
+  PMG_BK3_CASE(2, 3, false) PMG_BK3_CASE(2, 4, false)
+  PMG_BK3_CASE(3, 4, false) PMG_BK3_CASE(3, 5, false)
+  PMG_BK3_CASE(4, 5, true)  PMG_BK3_CASE(4, 6, true)
+  PMG_BK3_CASE(5, 6, false) PMG_BK3_CASE(5, 7, false)
+  PMG_BK3_CASE(6, 7, false) PMG_BK3_CASE(6, 8, false)
 #undef PMG_BK3_CASE
   throw Error(PMG_ERR_UNSUPPORTED, "BK3 kernel: unsupported (degree, n_q_points_1d) = (" +
                                        std::to_string(degree) + ", " + std::to_string(nq) + ")");
@@ -262,10 +443,10 @@ void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const ui
 
 template void launch_bk3_cells<double>(int, int, const double *, double *, const double *,
                                        const uint32_t *, int64_t, const Shape1D<double> &,
-                                       hipStream_t);
+                                       hipStream_t, const Bk3Schedule *);
 template void launch_bk3_cells<float>(int, int, const float *, float *, const float *,
                                       const uint32_t *, int64_t, const Shape1D<float> &,
-                                      hipStream_t);
+                                      hipStream_t, const Bk3Schedule *);
 
 // ---------------------------------------------------------------- geometry
 // One lane per (cell, q): trilinear Jacobian J = dX/dxi at the Gauss point,
@@ -275,7 +456,8 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 geometry_kernel(const int nq, const int64_t n_cells, const double *__restrict__ cv,
                 const double *__restrict__ qp, const double *__restrict__ qw,
-                const double *__restrict__ kappa, T *__restrict__ G)
+                const double *__restrict__ kappa, const uint32_t *__restrict__ perm,
+                T *__restrict__ G)
 {
   const int nq3 = nq * nq * nq;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -311,7 +493,8 @@ geometry_kernel(const int nq, const int64_t n_cells, const double *__restrict__ 
   const double K[3][3] = {{c00 * id, c10 * id, c20 * id},
                           {c01 * id, c11 * id, c21 * id},
                           {c02 * id, c12 * id, c22 * id}};
-  const double s = fabs(det) * qw[q0] * qw[q1] * qw[q2] * (kappa ? kappa[gid] : 1.0);
+  const double kap = kappa ? kappa[(perm ? static_cast<int64_t>(perm[c]) : c) * nq3 + q] : 1.0;
+  const double s = fabs(det) * qw[q0] * qw[q1] * qw[q2] * kap;
   T *out = G + c * 6 * nq3 + q;
   int comp = 0;
   for (int a = 0; a < 3; ++a)
@@ -322,18 +505,21 @@ geometry_kernel(const int nq, const int64_t n_cells, const double *__restrict__ 
 
 template <typename T>
 void launch_geometry(int nq, int64_t n_cells, const double *cell_vertices, const double *qpts,
-                     const double *qwts, const double *kappa, T *G, hipStream_t s) {
+                     const double *qwts, const double *kappa, const uint32_t *perm, T *G,
+                     hipStream_t s) {
   const int64_t n = n_cells * nq * nq * nq;
   if (n == 0) return;
   const int64_t nb = (n + 255) / 256;
   hipLaunchKernelGGL(geometry_kernel<T>, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, nq,
-                     n_cells, cell_vertices, qpts, qwts, kappa, G);
+                     n_cells, cell_vertices, qpts, qwts, kappa, perm, G);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_geometry<double>(int, int64_t, const double *, const double *,
-                                      const double *, const double *, double *, hipStream_t);
+                                      const double *, const double *, const uint32_t *, double *,
+                                      hipStream_t);
 template void launch_geometry<float>(int, int64_t, const double *, const double *,
-                                     const double *, const double *, float *, hipStream_t);
+                                     const double *, const double *, const uint32_t *, float *,
+                                     hipStream_t);
 
 // ---------------------------------------------------------------- diagonal
 // One lane per (cell, local DoF i): (A_e)_ii = sum_q g_i(q)^T G_q g_i(q) with
@@ -447,7 +633,8 @@ __device__ void cell_gradient(const T *__restrict__ vec, const uint32_t *__restr
 template <int NM, int NQ, typename T>
 __global__ void __launch_bounds__(NQ * NQ)
 grad_coefficient_kernel(const int64_t n_cells, const T *__restrict__ G,
-                        const uint32_t *__restrict__ map, const Shape1D<T> sh,
+                        const uint32_t *__restrict__ map, const uint32_t *__restrict__ perm,
+                        const Shape1D<T> sh,
                         const T *__restrict__ u, const T *__restrict__ lam, T *__restrict__ out)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
@@ -468,20 +655,20 @@ grad_coefficient_kernel(const int64_t n_cells, const T *__restrict__ G,
     const T fx = Gc[id] * ux + Gc[NQ3 + id] * uy + Gc[2 * NQ3 + id] * uzz;
     const T fy = Gc[NQ3 + id] * ux + Gc[3 * NQ3 + id] * uy + Gc[4 * NQ3 + id] * uzz;
     const T fz = Gc[2 * NQ3 + id] * ux + Gc[4 * NQ3 + id] * uy + Gc[5 * NQ3 + id] * uzz;
-    out[cell * NQ3 + id] = -(fx * lx + fy * ly + fz * lzz);
+    out[(perm ? static_cast<int64_t>(perm[cell]) : cell) * NQ3 + id] = -(fx * lx + fy * ly + fz * lzz);
   }
 }
 
 template <typename T>
 void launch_bk3_grad_coefficient(int degree, int nq, int64_t n_cells, const T *G,
-                                 const uint32_t *map, const Shape1D<T> &sh, const T *u,
-                                 const T *lam, T *out, hipStream_t s) {
+                                 const uint32_t *map, const uint32_t *perm, const Shape1D<T> &sh,
+                                 const T *u, const T *lam, T *out, hipStream_t s) {
   if (n_cells == 0) return;
   PMG_REQUIRE(n_cells < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells");
 #define PMG_GRAD_CASE(P, Q)                                                                  \
   if (degree == P && nq == Q) {                                                              \
     hipLaunchKernelGGL((grad_coefficient_kernel<P + 1, Q, T>), dim3(unsigned(n_cells)),       \
-                       dim3(Q * Q), 0, s, n_cells, G, map, sh, u, lam, out);                  \
+                       dim3(Q * Q), 0, s, n_cells, G, map, perm, sh, u, lam, out);                  \
     PMG_HIP_CHECK(hipGetLastError());                                                        \
     return;                                                                                  \
   }
@@ -492,11 +679,13 @@ void launch_bk3_grad_coefficient(int degree, int nq, int64_t n_cells, const T *G
   throw Error(PMG_ERR_UNSUPPORTED, "gradient kernel: unsupported (degree, n_q_points_1d)");
 }
 template void launch_bk3_grad_coefficient<double>(int, int, int64_t, const double *,
-                                                  const uint32_t *, const Shape1D<double> &,
+                                                  const uint32_t *, const uint32_t *,
+                                                  const Shape1D<double> &,
                                                   const double *, const double *, double *,
                                                   hipStream_t);
 template void launch_bk3_grad_coefficient<float>(int, int, int64_t, const float *,
-                                                 const uint32_t *, const Shape1D<float> &,
+                                                 const uint32_t *, const uint32_t *,
+                                                 const Shape1D<float> &,
                                                  const float *, const float *, float *,
                                                  hipStream_t);
 

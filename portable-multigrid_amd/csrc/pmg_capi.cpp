@@ -181,6 +181,14 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int nq, do
   });
 }
 
+int pmg_operator_set_schedule(pmg_operator op, int colored, int g_load) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    PMG_REQUIRE(g_load >= 0 && g_load <= 2, PMG_ERR_INVALID_ARG, "g_load must be 0, 1 or 2");
+    op->impl->set_schedule(colored != 0, g_load);
+  });
+}
+
 int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops) {
   return guard([&] {
     PMG_NOT_NULL(op);

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace pmg {
 
@@ -20,23 +21,44 @@ struct Shape1D {
   T D[kMaxNq * kMaxNq];  // D[n*nq+q]
 };
 
-// y += sum_cells A_e x  (atomics on shared DoFs)
+// How the cell loop is scheduled: all cells in one launch with atomic
+// scatter-add (default), or one launch per colour of a cell colouring with a
+// plain read-modify-write scatter (no two cells of a colour share a DoF).
+struct Bk3Schedule {
+  bool colored = false;
+  int gmode = 0;                        // G loads: 0 early non-temporal, 1 early, 2 late non-temporal
+  const uint32_t *cell_ids = nullptr;   // device: cells sorted by colour
+  std::vector<int64_t> color_offsets;   // host: n_colors + 1 offsets into cell_ids
+};
+
+// y += sum_cells A_e x
 template <typename T>
 void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
-                      int64_t n_cells, const Shape1D<T> &sh, hipStream_t s);
-// G[c][6][nq^3] = kappa * JxW * J^{-1}J^{-T} of the trilinear map of the cell vertices
+                      int64_t n_cells, const Shape1D<T> &sh, hipStream_t s,
+                      const Bk3Schedule *sched = nullptr);
+// G[c][6][nq^3] = kappa * JxW * J^{-1}J^{-T} of the trilinear map of the cell
+// vertices; kappa (optional) is indexed by the caller's cell perm[c]
 template <typename T>
 void launch_geometry(int nq, int64_t n_cells, const double *cell_vertices, const double *qpts,
-                     const double *qwts, const double *kappa, T *G, hipStream_t s);
+                     const double *qwts, const double *kappa, const uint32_t *perm, T *G,
+                     hipStream_t s);
 // diag[map[c][i]] += (A_e)_ii
 template <typename T>
 void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
                          const Shape1D<T> &S_and_Sd, T *diag, hipStream_t s);
-// out[c][q] = -(grad u)^T G_q (grad lambda)
+// out[perm[c]][q] = -(grad u)^T G_q (grad lambda)
 template <typename T>
 void launch_bk3_grad_coefficient(int degree, int nq, int64_t n_cells, const T *G,
-                                 const uint32_t *map, const Shape1D<T> &sh, const T *u,
-                                 const T *lam, T *out, hipStream_t s);
+                                 const uint32_t *map, const uint32_t *perm, const Shape1D<T> &sh,
+                                 const T *u, const T *lam, T *out, hipStream_t s);
+
+// Jacobi-PCG with the BK3 operator inside one workgroup (small coarse levels);
+// info[0..2] = iterations, ||r0||, ||r|| (device doubles).  x holds the initial guess.
+template <typename T>
+void launch_bk3_coarse_pcg(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
+                           const Shape1D<T> &sh, const uint32_t *cons, int64_t n_cons,
+                           const T *pinv, int64_t n, const T *b, T *x, T *r, T *z, T *p, T *q,
+                           double rel_tol, int max_it, double *info, hipStream_t s);
 
 // ---------------------------------------------------------------- transfers
 template <typename T>

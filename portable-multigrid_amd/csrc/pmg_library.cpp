@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace pmg {
@@ -99,11 +101,96 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   sh32_ = to_shape<float>(shape_.S, shape_.D);
   shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
   shd32_ = to_shape<float>(shape_.S, shape_.Sd);
-  map_.upload(dofmap, static_cast<size_t>(n_cells * nm3));
+  // Internal cell order: Morton (Z-order) of the cell centroids when the
+  // geometry is given by vertices, so that the cells one XCD processes at a
+  // time form a compact 3D region whose shared DoFs stay in its L2.  perm_
+  // maps internal -> caller cell index; cell-indexed user data (kappa, the
+  // coefficient gradient) is translated through it.
+  std::vector<uint32_t> perm(static_cast<size_t>(n_cells));
+  for (int64_t c = 0; c < n_cells; ++c) perm[c] = static_cast<uint32_t>(c);
+  const char *order_env = std::getenv("PMG_CELL_ORDER");  // "morton" reorders, default keeps
+  if (cell_vertices && n_cells > 1 && order_env && std::string(order_env) == "morton") {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    std::vector<double> cen(static_cast<size_t>(3 * n_cells));
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int d = 0; d < 3; ++d) {
+        double m = 0;
+        for (int v = 0; v < 8; ++v) m += cell_vertices[c * 24 + 3 * v + d];
+        cen[3 * c + d] = m / 8;
+        lo[d] = std::min(lo[d], m / 8);
+        hi[d] = std::max(hi[d], m / 8);
+      }
+    auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+      v &= 0x1fffff;
+      v = (v | v << 32) & 0x1f00000000ffffULL;
+      v = (v | v << 16) & 0x1f0000ff0000ffULL;
+      v = (v | v << 8) & 0x100f00f00f00f00fULL;
+      v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+      v = (v | v << 2) & 0x1249249249249249ULL;
+      return v;
+    };
+    std::vector<uint64_t> key(static_cast<size_t>(n_cells));
+    for (int64_t c = 0; c < n_cells; ++c) {
+      uint64_t q[3];
+      for (int d = 0; d < 3; ++d) {
+        const double ext = hi[d] - lo[d];
+        const double f = ext > 0 ? (cen[3 * c + d] - lo[d]) / ext : 0.0;
+        q[d] = static_cast<uint64_t>(std::min(std::max(f, 0.0), 1.0) * ((1u << 20) - 1));
+      }
+      key[c] = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
+    }
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  }
+  std::vector<uint32_t> imap(static_cast<size_t>(n_cells * nm3));
+  for (int64_t c = 0; c < n_cells; ++c)
+    std::copy(dofmap + perm[c] * nm3, dofmap + (perm[c] + 1) * nm3, imap.begin() + c * nm3);
+  map_.upload(imap);
+  perm_.upload(perm);
   constrained_.upload(constrained, static_cast<size_t>(n_constrained));
+  // Greedy cell colouring (cells sharing a DoF get different colours), cells
+  // sorted by colour: the atomics-free schedule launches one colour at a time.
+  {
+    std::vector<uint64_t> used(static_cast<size_t>(n_dofs), 0);
+    std::vector<int> color(static_cast<size_t>(n_cells), -1);
+    bool ok = true;
+    int n_col = 0;
+    for (int64_t c = 0; c < n_cells && ok; ++c) {
+      uint64_t forbidden = 0;
+      for (int64_t i = 0; i < nm3; ++i) {
+        const uint32_t d = imap[c * nm3 + i];
+        if (d != kInvalid) forbidden |= used[d];
+      }
+      int col = 0;
+      while (col < 64 && ((forbidden >> col) & 1u)) ++col;
+      if (col == 64) { ok = false; break; }
+      color[c] = col;
+      n_col = std::max(n_col, col + 1);
+      for (int64_t i = 0; i < nm3; ++i) {
+        const uint32_t d = imap[c * nm3 + i];
+        if (d != kInvalid) used[d] |= (uint64_t(1) << col);
+      }
+    }
+    if (ok && n_cells > 0) {
+      std::vector<int64_t> off(n_col + 1, 0);
+      for (int64_t c = 0; c < n_cells; ++c) ++off[color[c] + 1];
+      for (int k = 0; k < n_col; ++k) off[k + 1] += off[k];
+      std::vector<uint32_t> ids(static_cast<size_t>(n_cells));
+      std::vector<int64_t> fill(off.begin(), off.end() - 1);
+      for (int64_t c = 0; c < n_cells; ++c) ids[fill[color[c]]++] = static_cast<uint32_t>(c);
+      cell_ids_.upload(ids);
+      sched_.cell_ids = cell_ids_.ptr;
+      sched_.color_offsets = off;
+    }
+    const char *env = std::getenv("PMG_BK3_SCHEDULE");
+    const char *genv = std::getenv("PMG_BK3_GLOAD");
+    set_schedule(env && std::string(env) == "colored", genv ? std::atoi(genv) : 0);
+  }
   G_.reinit(precision, n_cells * 6 * nq3);
   if (cell_vertices) {
-    vertices_.upload(cell_vertices, static_cast<size_t>(n_cells * 24));
+    std::vector<double> iv(static_cast<size_t>(n_cells * 24));
+    for (int64_t c = 0; c < n_cells; ++c)
+      std::copy(cell_vertices + perm[c] * 24, cell_vertices + (perm[c] + 1) * 24, iv.begin() + c * 24);
+    vertices_.upload(iv);
     qpts_.upload(shape_.quad.points);
     qwts_.upload(shape_.quad.weights);
     if (precision == PMG_FP64) fill_geometry_t<double>(nullptr, nullptr);
@@ -122,7 +209,7 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
 
 template <typename T>
 void LaplaceOperatorBK3::fill_geometry_t(const double *kappa, hipStream_t s) {
-  launch_geometry<T>(nq_, n_cells_, vertices_.ptr, qpts_.ptr, qwts_.ptr, kappa,
+  launch_geometry<T>(nq_, n_cells_, vertices_.ptr, qpts_.ptr, qwts_.ptr, kappa, perm_.ptr,
                      static_cast<T *>(G_.data()), s);
 }
 
@@ -132,7 +219,7 @@ void LaplaceOperatorBK3::vmult_t(T *dst, const T *src, hipStream_t s) const {
       std::is_same<T, double>::value ? static_cast<const void *>(&sh64_) : static_cast<const void *>(&sh32_));
   launch_fill<T>(n_dofs_, dst, T(0), s);
   launch_bk3_cells<T>(degree_, nq_, src, dst, static_cast<const T *>(G_.data()), map_.ptr,
-                      n_cells_, sh, s);
+                      n_cells_, sh, s, &sched_);
   launch_copy_indexed<T>(n_constrained_, constrained_.ptr, dst, src, s);
 }
 
@@ -147,10 +234,10 @@ void LaplaceOperatorBK3::apply_add(void *dst, const void *src, hipStream_t s) co
   PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
   if (precision_ == PMG_FP64)
     launch_bk3_cells<double>(degree_, nq_, static_cast<const double *>(src), static_cast<double *>(dst),
-                             static_cast<const double *>(G_.data()), map_.ptr, n_cells_, sh64_, s);
+                             static_cast<const double *>(G_.data()), map_.ptr, n_cells_, sh64_, s, &sched_);
   else
     launch_bk3_cells<float>(degree_, nq_, static_cast<const float *>(src), static_cast<float *>(dst),
-                            static_cast<const float *>(G_.data()), map_.ptr, n_cells_, sh32_, s);
+                            static_cast<const float *>(G_.data()), map_.ptr, n_cells_, sh32_, s, &sched_);
 }
 
 template <typename T>
@@ -197,11 +284,11 @@ void LaplaceOperatorBK3::gradient_coefficient(const void *u, const void *lam, vo
   PMG_REQUIRE(u && lam && out, PMG_ERR_INVALID_ARG, "NULL vector");
   if (precision_ == PMG_FP64)
     launch_bk3_grad_coefficient<double>(degree_, nq_, n_cells_, static_cast<const double *>(G_.data()),
-                                        map_.ptr, sh64_, static_cast<const double *>(u),
+                                        map_.ptr, perm_.ptr, sh64_, static_cast<const double *>(u),
                                         static_cast<const double *>(lam), static_cast<double *>(out), s);
   else
     launch_bk3_grad_coefficient<float>(degree_, nq_, n_cells_, static_cast<const float *>(G_.data()),
-                                       map_.ptr, sh32_, static_cast<const float *>(u),
+                                       map_.ptr, perm_.ptr, sh32_, static_cast<const float *>(u),
                                        static_cast<const float *>(lam), static_cast<float *>(out), s);
 }
 
@@ -255,6 +342,28 @@ double LaplaceOperatorBK3::l2_error(const void *u, int u_kind, int nq, hipStream
   double e2 = 0;
   for (double v : h) e2 += v;
   return std::sqrt(e2);
+}
+
+void LaplaceOperatorBK3::coarse_pcg(const void *b, void *x, void *r, void *z, void *p, void *q,
+                                    double rel_tol, int max_it, double *info,
+                                    hipStream_t s) const {
+  PMG_REQUIRE(has_diag_, PMG_ERR_INVALID_ARG, "coarse PCG needs compute_diagonal()");
+  if (precision_ == PMG_FP64)
+    launch_bk3_coarse_pcg<double>(degree_, nq_, n_cells_, static_cast<const double *>(G_.data()),
+                                  map_.ptr, sh64_, constrained_.ptr, n_constrained_,
+                                  static_cast<const double *>(inv_diag_.data()), n_dofs_,
+                                  static_cast<const double *>(b), static_cast<double *>(x),
+                                  static_cast<double *>(r), static_cast<double *>(z),
+                                  static_cast<double *>(p), static_cast<double *>(q), rel_tol,
+                                  max_it, info, s);
+  else
+    launch_bk3_coarse_pcg<float>(degree_, nq_, n_cells_, static_cast<const float *>(G_.data()),
+                                 map_.ptr, sh32_, constrained_.ptr, n_constrained_,
+                                 static_cast<const float *>(inv_diag_.data()), n_dofs_,
+                                 static_cast<const float *>(b), static_cast<float *>(x),
+                                 static_cast<float *>(r), static_cast<float *>(z),
+                                 static_cast<float *>(p), static_cast<float *>(q), rel_tol, max_it,
+                                 info, s);
 }
 
 double LaplaceOperatorBK3::algorithmic_bytes() const {
@@ -391,6 +500,7 @@ CGResult solve_cg_t(const LaplaceOperatorBK3 &op, int precond, const VCycleMulti
     return res;
   }
   int it = 0;
+  static const bool verbose = std::getenv("PMG_CG_VERBOSE") != nullptr;
   while (it < max_it) {
     if (precond == PMG_PRECOND_MG) {
       mg->vmult(z, r, s);
@@ -410,6 +520,7 @@ CGResult solve_cg_t(const LaplaceOperatorBK3 &op, int precond, const VCycleMulti
       PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
       PMG_HIP_CHECK(hipStreamSynchronize(s));
       res.res = std::sqrt(rr);
+      if (verbose) std::fprintf(stderr, "[pmg cg] it %d  |r|/|r0| = %.3e\n", it, res.res / res.res0);
       if (!(res.res > target)) {  // also stops on NaN
         res.converged = res.res <= target;
         break;
@@ -619,8 +730,17 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
     if (coarse_kind_ == PMG_COARSE_PCG) {
       if (f64) launch_fill<double>(n, static_cast<double *>(sol_[0].data()), 0.0, s);
       else launch_fill<float>(n, static_cast<float *>(sol_[0].data()), 0.0f, s);
-      solve_cg(*ops_[0], PMG_PRECOND_JACOBI, nullptr, def_[0].data(), sol_[0].data(), coarse_tol_,
-               100000, coarse_work_, s, 4);
+      // FP32 arithmetic cannot reduce the residual below ~1e-7 relative
+      const double tol = f64 ? coarse_tol_ : std::max(coarse_tol_, 1e-6);
+      const int max_it = static_cast<int>(std::min<int64_t>(100000, 4 * n + 100));
+      if (ops_[0]->n_cells() <= kCoarseSingleWorkgroupCells) {
+        CGWork &w = coarse_work_;
+        ops_[0]->coarse_pcg(def_[0].data(), sol_[0].data(), w.r.data(), w.z.data(), w.p.data(),
+                            w.q.data(), tol, max_it, w.sc.ptr + 4, s);
+      } else {
+        solve_cg(*ops_[0], PMG_PRECOND_JACOBI, nullptr, def_[0].data(), sol_[0].data(), tol,
+                 max_it, coarse_work_, s, 4);
+      }
     } else {
       coarse_cheby_->vmult(sol_[0].data(), def_[0].data(), s);
     }

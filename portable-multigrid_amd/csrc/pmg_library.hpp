@@ -52,6 +52,10 @@ class LaplaceOperatorBK3 {
   double algorithmic_bytes() const;
   double algorithmic_flops() const;
   bool has_diagonal() const { return has_diag_; }
+  // Jacobi-PCG solved inside one workgroup (for small coarse levels); x is the
+  // initial guess; r,z,p,q work vectors of m() entries; info = 3 device doubles
+  void coarse_pcg(const void *b, void *x, void *r, void *z, void *p, void *q, double rel_tol,
+                  int max_it, double *info, hipStream_t s) const;
 
  private:
   template <typename T> void vmult_t(T *dst, const T *src, hipStream_t s) const;
@@ -63,10 +67,23 @@ class LaplaceOperatorBK3 {
   ShapeData shape_;
   Shape1D<double> sh64_{}, shd64_{};
   Shape1D<float> sh32_{}, shd32_{};
-  DeviceArray<uint32_t> map_, constrained_;
+  DeviceArray<uint32_t> map_, constrained_, perm_;
   DeviceArray<double> vertices_, qpts_, qwts_;
   DeviceVector G_, inv_diag_;
   bool has_diag_ = false;
+  // cell colouring for the atomics-free schedule
+  DeviceArray<uint32_t> cell_ids_;
+  Bk3Schedule sched_;
+
+ public:
+  // "atomic" (one launch, hardware FP atomics) or "colored" (one launch per
+  // colour, plain scatter); gmode: G load policy (see pmg_bk3.hip)
+  void set_schedule(bool colored, int gmode) {
+    sched_.colored = colored && !sched_.color_offsets.empty();
+    sched_.gmode = gmode;
+  }
+  int n_colors() const { return static_cast<int>(sched_.color_offsets.size()) - 1; }
+  const Bk3Schedule &schedule() const { return sched_; }
 };
 
 class MGTransfer {
@@ -127,6 +144,9 @@ class VCycleMultigrid;
 CGResult solve_cg(const LaplaceOperatorBK3 &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, CGWork &work,
                   hipStream_t s, int check_every = 1);
+
+// Coarse levels with at most this many cells are solved by the one-workgroup PCG kernel.
+constexpr int64_t kCoarseSingleWorkgroupCells = 4096;
 
 class VCycleMultigrid {
  public:

@@ -6,9 +6,12 @@
 // fine weights and scatter-add into the fine vector.  Restriction
 // (KokkosRestrictionBatchedKernel, :761-1051) is the exact transpose with the
 // weight applied on the gather.
-// Layout: one nmf x nmf lane tile per cell (256/nmf^2 cells per workgroup);
-// the z sweep runs on a register column, x and y sweeps on register lines
-// read from LDS; M is a kernel argument held in scalar registers.
+// Layout: one nmf x nmf lane tile per cell (256/nmf^2 cells per workgroup),
+// LDS for the x and y sweeps, a register column for the z sweep.  The
+// prolongation does z LAST and the restriction z FIRST, so the fine-side
+// scatter-add / gather (nmf^3 per cell, the dominant traffic) is issued by
+// lanes (a,b) over consecutive fine DoFs along x; M is a kernel argument held
+// in scalar registers.
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
@@ -36,50 +39,57 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
   const bool active = e < E && cell < n_cells;
   T *L0 = lds + (active ? e : 0) * 2 * NF3, *L1 = L0 + NF3;
 
-  // z: column (i,j) of coarse values -> L0[c][j][i]
+  // gather the coarse cell values, lane (i,j): L0[k][j][i]
   if (active && ta < NC && tb < NC) {
-    T u[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       const uint32_t d = map_c[cell * NC3 + k * NC2 + tb * NC + ta];
-      u[k] = d == kInvalid ? T(0) : xc[d];
-    }
-#pragma unroll
-    for (int c = 0; c < NF; ++c) {
-      T v = 0;
-#pragma unroll
-      for (int k = 0; k < NC; ++k) v += M.M[k * NF + c] * u[k];
-      L0[c * NC2 + tb * NC + ta] = v;
+      L0[k * NC2 + tb * NC + ta] = d == kInvalid ? T(0) : xc[d];
     }
   }
   __syncthreads();
-  // y: lane (i,c) -> L1[c][b][i]
-  if (active && ta < NC) {
-    T a[NC];
-#pragma unroll
-    for (int j = 0; j < NC; ++j) a[j] = L0[tb * NC2 + j * NC + ta];
-#pragma unroll
-    for (int b = 0; b < NF; ++b) {
-      T v = 0;
-#pragma unroll
-      for (int j = 0; j < NC; ++j) v += M.M[j * NF + b] * a[j];
-      L1[(tb * NF + b) * NC + ta] = v;
-    }
-  }
-  __syncthreads();
-  // x: lane (b,c) -> fine values along a, weighted scatter-add
-  if (active) {
+  // x sweep, lane (j,k): L1[k][j][a]
+  if (active && ta < NC && tb < NC) {
     T l[NC];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) l[i] = L1[(tb * NF + ta) * NC + i];
-    const int64_t base = cell * NF3 + tb * NF2 + ta * NF;
+    for (int i = 0; i < NC; ++i) l[i] = L0[tb * NC2 + ta * NC + i];
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
       T v = 0;
 #pragma unroll
       for (int i = 0; i < NC; ++i) v += M.M[i * NF + a] * l[i];
-      const T wt = w[base + a];
-      if (wt != T(0)) atomic_add(yf + map_f[base + a], wt * v);
+      L1[(tb * NC + ta) * NF + a] = v;
+    }
+  }
+  __syncthreads();
+  // y sweep, lane (a,k): L0[k][b][a]
+  if (active && tb < NC) {
+    T l[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) l[j] = L1[(tb * NC + j) * NF + ta];
+#pragma unroll
+    for (int b = 0; b < NF; ++b) {
+      T v = 0;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) v += M.M[j * NF + b] * l[j];
+      L0[(tb * NF + b) * NF + ta] = v;
+    }
+  }
+  __syncthreads();
+  // z sweep on a register column, lane (a,b); the weighted scatter-add runs
+  // along a across lanes, i.e. over consecutive fine DoFs
+  if (active) {
+    T l[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) l[k] = L0[(k * NF + tb) * NF + ta];
+    const int64_t base = cell * NF3 + tb * NF + ta;
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+      T v = 0;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) v += M.M[k * NF + c] * l[k];
+      const T wt = w[base + c * NF2];
+      if (wt != T(0)) atomic_add(yf + map_f[base + c * NF2], wt * v);
     }
   }
 }
@@ -99,50 +109,59 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
   const bool active = e < E && cell < n_cells;
   T *L0 = lds + (active ? e : 0) * 2 * NF3, *L1 = L0 + NF3;
 
-  // x: lane (b,c) gathers w*x along a, contracts to i -> L1[c][b][i]
+  // weighted gather of the fine column over c by lane (a,b) (consecutive fine
+  // DoFs across lanes), z sweep in registers: L0[k][b][a]
   if (active) {
     T f[NF];
-    const int64_t base = cell * NF3 + tb * NF2 + ta * NF;
+    const int64_t base = cell * NF3 + tb * NF + ta;
 #pragma unroll
-    for (int a = 0; a < NF; ++a) {
-      const T wt = w[base + a];
-      f[a] = wt == T(0) ? T(0) : wt * xf[map_f[base + a]];
+    for (int c = 0; c < NF; ++c) {
+      const T wt = w[base + c * NF2];
+      f[c] = wt == T(0) ? T(0) : wt * xf[map_f[base + c * NF2]];
     }
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-      T v = 0;
-#pragma unroll
-      for (int a = 0; a < NF; ++a) v += M.M[i * NF + a] * f[a];
-      L1[(tb * NF + ta) * NC + i] = v;
-    }
-  }
-  __syncthreads();
-  // y: lane (i,c) -> L0[c][j][i]
-  if (active && ta < NC) {
-    T a[NF];
-#pragma unroll
-    for (int b = 0; b < NF; ++b) a[b] = L1[(tb * NF + b) * NC + ta];
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      T v = 0;
-#pragma unroll
-      for (int b = 0; b < NF; ++b) v += M.M[j * NF + b] * a[b];
-      L0[tb * NC2 + j * NC + ta] = v;
-    }
-  }
-  __syncthreads();
-  // z: lane (i,j) -> coarse column, scatter-add (skip constrained)
-  if (active && ta < NC && tb < NC) {
-    T col[NF];
-#pragma unroll
-    for (int c = 0; c < NF; ++c) col[c] = L0[c * NC2 + tb * NC + ta];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       T v = 0;
 #pragma unroll
-      for (int c = 0; c < NF; ++c) v += M.M[k * NF + c] * col[c];
+      for (int c = 0; c < NF; ++c) v += M.M[k * NF + c] * f[c];
+      L0[(k * NF + tb) * NF + ta] = v;
+    }
+  }
+  __syncthreads();
+  // y sweep, lane (a,k): L1[k][j][a]
+  if (active && tb < NC) {
+    T l[NF];
+#pragma unroll
+    for (int b = 0; b < NF; ++b) l[b] = L0[(tb * NF + b) * NF + ta];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      T v = 0;
+#pragma unroll
+      for (int b = 0; b < NF; ++b) v += M.M[j * NF + b] * l[b];
+      L1[(tb * NC + j) * NF + ta] = v;
+    }
+  }
+  __syncthreads();
+  // x sweep, lane (j,k): L0[k][j][i]
+  if (active && ta < NC && tb < NC) {
+    T l[NF];
+#pragma unroll
+    for (int a = 0; a < NF; ++a) l[a] = L1[(tb * NC + ta) * NF + a];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      T v = 0;
+#pragma unroll
+      for (int a = 0; a < NF; ++a) v += M.M[i * NF + a] * l[a];
+      L0[tb * NC2 + ta * NC + i] = v;
+    }
+  }
+  __syncthreads();
+  // scatter-add to the coarse DoFs by lane (i,j) over k (skip constrained)
+  if (active && ta < NC && tb < NC) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
       const uint32_t d = map_c[cell * NC3 + k * NC2 + tb * NC + ta];
-      if (d != kInvalid) atomic_add(yc + d, v);
+      if (d != kInvalid) atomic_add(yc + d, L0[k * NC2 + tb * NC + ta]);
     }
   }
 }

@@ -141,3 +141,18 @@ def test_bad_arguments_raise(pmg):
         op.vmult(op.initialize_dof_vector(), x)
     with pytest.raises(pmg.PMGError):
         op.get_matrix_diagonal_inverse()  # compute_diagonal() not called
+
+
+@pytest.mark.parametrize("p,nq,colored,gload", [(4, 5, True, 0), (4, 5, False, 1), (4, 5, False, 2),
+                                                (4, 6, True, 0), (4, 6, False, 2), (2, 3, True, 0),
+                                                (6, 7, True, 0)])
+def test_schedules_agree(pmg, orc, p, nq, colored, gload):
+    """Atomics-free colour-by-colour scatter and the G-load variants give the
+    oracle's result as well."""
+    import torch
+    op, ref = _pair(pmg, orc, 3, p, nq, 1, pmg.FP64)
+    op.set_schedule(colored, gload)
+    x = seeded_uniform(op.m(), 300 + p)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, torch.tensor(x, device="cuda"))
+    assert rel_err(dst.cpu().numpy(), ref.vmult(x)) < 1e-12

@@ -1,0 +1,204 @@
+"""GPU parity of transfers, Chebyshev smoother, V-cycle and (mixed-precision)
+MG-CG against the CPU oracle.
+
+Bars: transfers and smoother applications within 1e-12 relative (FP64) and
+2e-6 (FP32); V-cycle within 1e-9 relative (its coarse solve is an iterative
+1e-12 PCG on both sides); CG iteration counts within +-1 of the oracle and
+solutions within 1e-9 relative; the FP32 V-cycle inside FP64 CG reaches the
+same 1e-10 residual and a solution within 1e-8 of the all-FP64 one.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err, seeded_uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype=None):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), dtype=dtype or torch.float64, device="cuda")
+
+
+@pytest.mark.parametrize("n,p,deform", [(2, 1, 0), (2, 2, 1), (2, 4, 0), (1, 6, 0), (3, 3, 1)])
+def test_h_transfer_matches_oracle(pmg, orc, n, p, deform):
+    import torch
+    print(f"[h-transfer] n={n} p={p}", flush=True)
+    bc = pmg.box(n, deform=deform, amplitude=0.05)
+    tr = pmg.GeometricTransfer(bc, p)
+    ref = orc.Transfer(orc.mesh(n, deform=deform, amplitude=0.05), p, kind="h")
+    nc, nf = pmg.box_n_dofs(bc, p), pmg.box_n_dofs(pmg.refine(bc), p)
+    xc, xf = seeded_uniform(nc, 31), seeded_uniform(nf, 32)
+    yf0, yc0 = seeded_uniform(nf, 33), seeded_uniform(nc, 34)
+    yf = _t(yf0)
+    tr.prolongate_and_add(yf, _t(xc))
+    assert rel_err(yf.cpu().numpy(), ref.prolongate_add(xc, yf0.copy())) < 1e-12
+    yc = _t(yc0)
+    tr.restrict_and_add(yc, _t(xf))
+    assert rel_err(yc.cpu().numpy(), ref.restrict_add(xf, yc0.copy())) < 1e-12
+
+
+@pytest.mark.parametrize("pc,pf", [(1, 2), (2, 4), (1, 3), (3, 6), (4, 6), (5, 6), (2, 3)])
+def test_p_transfer_matches_oracle(pmg, orc, pc, pf):
+    print(f"[p-transfer] {pc}->{pf}", flush=True)
+    b = pmg.box(3, deform=1, amplitude=0.05)
+    tr = pmg.PolynomialTransfer(b, pc, pf)
+    ref = orc.Transfer(orc.mesh(3, deform=1, amplitude=0.05), pc, pf, kind="p")
+    nc, nf = pmg.box_n_dofs(b, pc), pmg.box_n_dofs(b, pf)
+    xc, xf = seeded_uniform(nc, 41), seeded_uniform(nf, 42)
+    yf = _t(np.zeros(nf))
+    tr.prolongate_and_add(yf, _t(xc))
+    assert rel_err(yf.cpu().numpy(), ref.prolongate_add(xc, np.zeros(nf))) < 1e-12
+    yc = _t(np.zeros(nc))
+    tr.restrict_and_add(yc, _t(xf))
+    assert rel_err(yc.cpu().numpy(), ref.restrict_add(xf, np.zeros(nc))) < 1e-12
+
+
+def test_transfer_fp32_and_doubled_prolongation(pmg, orc):
+    import torch
+    bc = pmg.box(2)
+    ref = orc.Transfer(orc.mesh(2), 2, kind="h", h_prolong_scale=2.0)
+    nc, nf = pmg.box_n_dofs(bc, 2), pmg.box_n_dofs(pmg.refine(bc), 2)
+    xc = seeded_uniform(nc, 51)
+    tr2 = pmg.GeometricTransfer(bc, 2, prolongation_scale=2.0)
+    yf = _t(np.zeros(nf))
+    tr2.prolongate_and_add(yf, _t(xc))
+    assert rel_err(yf.cpu().numpy(), ref.prolongate_add(xc, np.zeros(nf))) < 1e-12
+    tr32 = pmg.GeometricTransfer(bc, 2, precision=pmg.FP32)
+    y32 = torch.zeros(nf, dtype=torch.float32, device="cuda")
+    tr32.prolongate_and_add(y32, _t(xc, torch.float32))
+    assert rel_err(2.0 * y32.double().cpu().numpy(), ref.prolongate_add(xc, np.zeros(nf))) < 2e-6
+
+
+def _oracle_levels(orc, boxes_pm):
+    return [orc.mesh(b.nx, b.ny, b.nz, deform=b.deform, amplitude=b.amplitude) for b in boxes_pm]
+
+
+@pytest.mark.parametrize("p,deform", [(1, 0), (2, 1), (4, 0)])
+def test_chebyshev_matches_oracle(pmg, orc, p, deform):
+    print(f"[chebyshev] p={p}", flush=True)
+    b = pmg.box(4, deform=deform, amplitude=0.05)
+    m = orc.mesh(4, deform=deform, amplitude=0.05)
+    op = pmg.LaplaceOperatorBK3(b, p)
+    op.compute_diagonal()
+    sm = pmg.PreconditionChebyshev(op, 3, 15.0, 10)
+    ref = orc.Multigrid([orc.mesh(2, deform=deform, amplitude=0.05), m], [p, p])
+    mx, mn, deg = sm.info()
+    rmx, rmn, rdeg = ref.smoother_info(1)
+    assert deg == rdeg == 3
+    assert abs(mx - rmx) <= 1e-10 * rmx and abs(mn - rmn) <= 1e-8 * abs(rmn)
+    rhs = seeded_uniform(op.m(), 61)
+    x = op.initialize_dof_vector()
+    sm.vmult(x, _t(rhs))
+    xr = ref.smooth(1, np.zeros(op.m()), rhs, 0)
+    assert rel_err(x.cpu().numpy(), xr) < 1e-11
+    sm.step(x, _t(rhs))
+    xr = ref.smooth(1, xr, rhs, 1)
+    assert rel_err(x.cpu().numpy(), xr) < 1e-11
+
+
+def test_coarse_chebyshev_auto_degree(pmg, orc):
+    """Coarse smoother with automatic degree (range 1e-3, eig its = m())."""
+    b = pmg.box(2)
+    op = pmg.LaplaceOperatorBK3(b, 2)
+    op.compute_diagonal()
+    sm = pmg.PreconditionChebyshev(op, 0, 1e-3, op.m())
+    ref = orc.Multigrid([orc.mesh(2)], [2], coarse_kind=0)
+    mx, mn, deg = sm.info()
+    rmx, rmn, rdeg = ref.smoother_info(0)
+    assert deg == rdeg and deg > 3
+    assert abs(mx - rmx) <= 1e-8 * rmx and abs(mn - rmn) <= 1e-6 * rmn
+
+
+@pytest.mark.parametrize("levels_spec", [
+    [(2, 2), (4, 2), (8, 2)],                 # h-only, Q2
+    [(2, 1), (4, 1), (4, 2), (4, 4)],         # ph: h at Q1 then p 1->2->4
+    [(2, 3), (4, 3)],                         # two-level Q3
+])
+def test_vcycle_and_mgcg_match_oracle(pmg, orc, levels_spec):
+    import torch
+    print(f"[mg] {levels_spec}", flush=True)
+    boxes = [pmg.box(n, deform=1, amplitude=0.05) for n, _ in levels_spec]
+    degs = [p for _, p in levels_spec]
+    mg, ops, trs, sms = pmg.build_hierarchy(list(zip(boxes, degs)))
+    ref = orc.Multigrid([orc.mesh(n, deform=1, amplitude=0.05) for n, _ in levels_spec], degs,
+                        coarse_kind=1, coarse_tol=1e-12)
+    fine = ops[-1]
+    r = seeded_uniform(fine.m(), 71)
+    r[orc.boundary_dofs(orc.mesh(levels_spec[-1][0]), degs[-1])] = 0
+    z = fine.initialize_dof_vector()
+    mg.vmult(z, _t(r))
+    zr = ref.vmult(r)
+    assert rel_err(z.cpu().numpy(), zr) < 1e-9
+    # MG-preconditioned CG
+    b = fine.assemble_rhs(1)
+    x = fine.initialize_dof_vector()
+    its, r0, res = pmg.solve_cg(fine, b, x, pmg.PRECOND_MG, mg, 1e-10, 200)
+    xr, rits, rr0, rres = ref.pcg(b.cpu().numpy(), precond=2, rel_tol=1e-10)
+    assert abs(its - rits) <= 1, (its, rits)
+    assert res <= 1e-10 * r0
+    assert rel_err(x.cpu().numpy(), xr) < 1e-9
+    # mixed precision: FP32 V-cycle inside FP64 CG
+    mg32, ops32, _, _ = pmg.build_hierarchy(list(zip(boxes, degs)), precision=pmg.FP32,
+                                            outer_precision=pmg.FP64)
+    x32 = fine.initialize_dof_vector()
+    its32, _, res32 = pmg.solve_cg(fine, b, x32, pmg.PRECOND_MG, mg32, 1e-10, 200)
+    assert res32 <= 1e-10 * r0 and its32 <= its + 2
+    assert rel_err(x32.cpu().numpy(), x.cpu().numpy()) < 1e-8
+
+
+def test_jacobi_pcg_matches_oracle(pmg, orc):
+    """Small version of the Jacobi-PCG configuration: seeded U[-1,1] RHS."""
+    b = pmg.box(4)
+    op = pmg.LaplaceOperatorBK3(b, 4, 6)
+    op.compute_diagonal()
+    ref = orc.Multigrid([orc.mesh(4)], [4], nqs=[6])
+    rhs = seeded_uniform(op.m(), 81)
+    rhs[orc.boundary_dofs(orc.mesh(4), 4)] = 0
+    x = op.initialize_dof_vector()
+    its, r0, res = pmg.solve_cg(op, _t(rhs), x, pmg.PRECOND_JACOBI, None, 1e-8, 2000)
+    xr, rits, _, _ = ref.pcg(rhs, level=0, precond=1, rel_tol=1e-8, max_it=2000)
+    assert abs(its - rits) <= 1, (its, rits)
+    # both iterates stop at 1e-8 relative residual; they agree far below that
+    assert res <= 1e-8 * r0
+    assert rel_err(x.cpu().numpy(), xr) < 1e-7
+
+
+def test_coarse_chebyshev_vcycle_runs(pmg):
+    """The reference's coarse solver (Chebyshev, automatic degree to 1e-3)."""
+    boxes = [pmg.box(2), pmg.box(4)]
+    mg, ops, _, _ = pmg.build_hierarchy([(boxes[0], 2), (boxes[1], 2)], coarse=pmg.COARSE_CHEBYSHEV)
+    b = ops[-1].assemble_rhs(0)
+    x = ops[-1].initialize_dof_vector()
+    its, r0, res = pmg.solve_cg(ops[-1], b, x, pmg.PRECOND_MG, mg, 1e-10, 100)
+    assert res <= 1e-10 * r0 and its < 30
+
+
+def test_cg_reports_non_convergence(pmg):
+    b = pmg.box(4)
+    op = pmg.LaplaceOperatorBK3(b, 2)
+    op.compute_diagonal()
+    rhs = op.assemble_rhs(1)
+    x = op.initialize_dof_vector()
+    with pytest.raises(pmg.PMGError) as e:
+        pmg.solve_cg(op, rhs, x, pmg.PRECOND_JACOBI, None, 1e-14, 3)
+    assert e.value.status == pmg.ERR_NOT_CONVERGED
+
+
+def test_vcycle_is_finite_over_poisoned_memory(pmg):
+    """Work vectors come from fresh device allocations whose contents are
+    arbitrary: the V-cycle (including the one-workgroup coarse PCG) must never
+    read them before writing.  Fill freed device memory with NaN first."""
+    import torch
+    junk = torch.full((1 << 27,), float("nan"), dtype=torch.float64, device="cuda")
+    del junk
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    for prec in (pmg.FP64, pmg.FP32):
+        mg, ops, _, _, _ = pmg.geometric_hierarchy(pmg.box(16), 4, 4, precision=prec,
+                                                   outer_precision=pmg.FP64)
+        r = torch.rand(ops[-1].m(), dtype=torch.float64, device="cuda")
+        z = torch.zeros_like(r)
+        for _ in range(3):
+            mg.vmult(z, r)
+            assert torch.isfinite(z).all().item()
