@@ -63,7 +63,11 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   T *P0 = lds + (active ? e : 0) * 3 * BUF;
   T *P1 = P0 + BUF;
   T *P2 = P1 + BUF;
-  const T *Gc = G + cell * (6 * NQ3) + tb * NQ + ta;
+  // batch-interleaved G: component c, plane r of this lane's column sits at
+  // Gc[(c * NQ + r) * LANES]; for a batch of consecutive cells a wave reads
+  // 64 consecutive values per load
+  constexpr int LANES = Bk3Tile<NQ>::lanes;
+  const T *Gc = G + g_index(NQ, cell, 0, t);
 
   // ---- geometric factors for this lane's column (p,q) = (ta,tb), all r
   T g[6][NQ];
@@ -72,7 +76,7 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
     for (int c = 0; c < 6; ++c)
 #pragma unroll
       for (int r = 0; r < NQ; ++r)
-        g[c][r] = GMODE == kGEarlyNT ? ldg_stream(Gc + c * NQ3 + r * NQ2) : Gc[c * NQ3 + r * NQ2];
+        g[c][r] = GMODE == kGEarlyNT ? ldg_stream(Gc + (c * NQ + r) * LANES) : Gc[(c * NQ + r) * LANES];
   }
 
   // ---- phase 1: gather column (i,j)=(ta,tb) over k, interpolate in z
@@ -154,7 +158,7 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
 #pragma unroll
       for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int r = 0; r < NQ; ++r) g[c][r] = ldg_stream(Gc + c * NQ3 + r * NQ2);
+        for (int r = 0; r < NQ; ++r) g[c][r] = ldg_stream(Gc + (c * NQ + r) * LANES);
     }
     T col[NQ], fz[NQ];
 #pragma unroll
@@ -412,7 +416,7 @@ static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_
     }
     return;
   }
-  const int gmode = sc ? sc->gmode : kGEarlyNT;
+  const int gmode = sc ? sc->gmode : kGLateNT;
   if (FULL && gmode == kGEarly)
     bk3_launch_range<NM, NQ, T, true, (FULL ? kGEarly : kGEarlyNT)>(x, y, G, map, nullptr, n_cells, sh, s);
   else if (FULL && gmode == kGLateNT)
@@ -495,11 +499,10 @@ geometry_kernel(const int nq, const int64_t n_cells, const double *__restrict__ 
                           {c02 * id, c12 * id, c22 * id}};
   const double kap = kappa ? kappa[(perm ? static_cast<int64_t>(perm[c]) : c) * nq3 + q] : 1.0;
   const double s = fabs(det) * qw[q0] * qw[q1] * qw[q2] * kap;
-  T *out = G + c * 6 * nq3 + q;
   int comp = 0;
   for (int a = 0; a < 3; ++a)
     for (int b = a; b < 3; ++b, ++comp)
-      out[comp * nq3] =
+      G[g_index(nq, c, comp, q)] =
           static_cast<T>(s * (K[a][0] * K[b][0] + K[a][1] * K[b][1] + K[a][2] * K[b][2]));
 }
 
@@ -539,7 +542,7 @@ bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const T *
   const uint32_t dof = map[gid];
   if (dof == kInvalid) return;
   const int i0 = i % nm, i1 = (i / nm) % nm, i2 = i / (nm * nm);
-  const T *Gc = G + c * 6 * nq3;
+  (void)nq3;
   double acc = 0;
   for (int q2 = 0; q2 < nq; ++q2)
     for (int q1 = 0; q1 < nq; ++q1)
@@ -548,9 +551,10 @@ bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const T *
         const double s0 = sh.S[i0 * nq + q0], s1 = sh.S[i1 * nq + q1], s2 = sh.S[i2 * nq + q2];
         const double d0 = sh.D[i0 * nq + q0], d1 = sh.D[i1 * nq + q1], d2 = sh.D[i2 * nq + q2];
         const double g0 = d0 * s1 * s2, g1 = s0 * d1 * s2, g2 = s0 * s1 * d2;
-        acc += Gc[q] * g0 * g0 + Gc[3 * nq3 + q] * g1 * g1 + Gc[5 * nq3 + q] * g2 * g2 +
-               2.0 * (Gc[nq3 + q] * g0 * g1 + Gc[2 * nq3 + q] * g0 * g2 +
-                      Gc[4 * nq3 + q] * g1 * g2);
+        const T *Gq = G + g_index(nq, c, 0, q);
+        const int64_t cs = static_cast<int64_t>(nq) * g_batch_lanes(nq);  // component stride
+        acc += Gq[0] * g0 * g0 + Gq[3 * cs] * g1 * g1 + Gq[5 * cs] * g2 * g2 +
+               2.0 * (Gq[cs] * g0 * g1 + Gq[2 * cs] * g0 * g2 + Gq[4 * cs] * g1 * g2);
       }
   atomic_add(diag + dof, static_cast<T>(acc));
 }
@@ -647,14 +651,15 @@ grad_coefficient_kernel(const int64_t n_cells, const T *__restrict__ G,
   cell_gradient<NM, NQ, T>(lam, map, cell, sh, lds + 3 * NQ3, lds + 4 * NQ3, lds + 5 * NQ3, ta,
                            tb, lz);
   __syncthreads();
-  const T *Gc = G + cell * 6 * NQ3;
+  constexpr int64_t CS = static_cast<int64_t>(NQ) * Bk3Tile<NQ>::lanes;  // component stride
   for (int r = 0; r < NQ; ++r) {
     const int id = r * NQ2 + tb * NQ + ta;
+    const T *Gq = G + g_index(NQ, cell, 0, id);
     const T ux = lds[2 * NQ3 + id], uy = lds[NQ3 + id], uzz = uz[r];
     const T lx = lds[5 * NQ3 + id], ly = lds[4 * NQ3 + id], lzz = lz[r];
-    const T fx = Gc[id] * ux + Gc[NQ3 + id] * uy + Gc[2 * NQ3 + id] * uzz;
-    const T fy = Gc[NQ3 + id] * ux + Gc[3 * NQ3 + id] * uy + Gc[4 * NQ3 + id] * uzz;
-    const T fz = Gc[2 * NQ3 + id] * ux + Gc[4 * NQ3 + id] * uy + Gc[5 * NQ3 + id] * uzz;
+    const T fx = Gq[0] * ux + Gq[CS] * uy + Gq[2 * CS] * uzz;
+    const T fy = Gq[CS] * ux + Gq[3 * CS] * uy + Gq[4 * CS] * uzz;
+    const T fz = Gq[2 * CS] * ux + Gq[4 * CS] * uy + Gq[5 * CS] * uzz;
     out[(perm ? static_cast<int64_t>(perm[cell]) : cell) * NQ3 + id] = -(fx * lx + fy * ly + fz * lzz);
   }
 }

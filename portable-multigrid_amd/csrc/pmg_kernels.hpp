@@ -21,12 +21,36 @@ struct Shape1D {
   T D[kMaxNq * kMaxNq];  // D[n*nq+q]
 };
 
+// Storage of the geometric factors G (6 symmetric components per q-point).
+// The cell kernel gives each cell an nq x nq lane tile and E = 256/nq^2 cells
+// one workgroup ("batch"); G is stored batch-interleaved,
+//   G[((batch * 6 + comp) * nq + r) * lanes + e * nq^2 + (q mod nq^2)],
+// r = q / nq^2, e = cell mod E, so that every G load instruction of a wave
+// reads 64 consecutive values.  lanes = E nq^2 rounded up to 64.
+__host__ __device__ inline int g_batch_cells(int nq) {
+  return (256 / (nq * nq)) > 0 ? 256 / (nq * nq) : 1;
+}
+__host__ __device__ inline int g_batch_lanes(int nq) {
+  return ((g_batch_cells(nq) * nq * nq + 63) / 64) * 64;
+}
+__host__ __device__ inline int64_t g_index(int nq, int64_t cell, int comp, int q) {
+  const int nq2 = nq * nq, E = g_batch_cells(nq);
+  const int r = q / nq2, t = q - r * nq2;
+  const int64_t b = cell / E;
+  const int e = static_cast<int>(cell - b * E);
+  return ((b * 6 + comp) * nq + r) * g_batch_lanes(nq) + e * nq2 + t;
+}
+__host__ __device__ inline int64_t g_size(int nq, int64_t n_cells) {
+  const int64_t nb = (n_cells + g_batch_cells(nq) - 1) / g_batch_cells(nq);
+  return nb * 6 * nq * g_batch_lanes(nq);
+}
+
 // How the cell loop is scheduled: all cells in one launch with atomic
 // scatter-add (default), or one launch per colour of a cell colouring with a
 // plain read-modify-write scatter (no two cells of a colour share a DoF).
 struct Bk3Schedule {
   bool colored = false;
-  int gmode = 0;                        // G loads: 0 early non-temporal, 1 early, 2 late non-temporal
+  int gmode = 2;                        // G loads: 0 early non-temporal, 1 early, 2 late non-temporal
   const uint32_t *cell_ids = nullptr;   // device: cells sorted by colour
   std::vector<int64_t> color_offsets;   // host: n_colors + 1 offsets into cell_ids
 };

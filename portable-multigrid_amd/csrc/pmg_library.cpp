@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace pmg {
 
@@ -183,9 +184,9 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     }
     const char *env = std::getenv("PMG_BK3_SCHEDULE");
     const char *genv = std::getenv("PMG_BK3_GLOAD");
-    set_schedule(env && std::string(env) == "colored", genv ? std::atoi(genv) : 0);
+    set_schedule(env && std::string(env) == "colored", genv ? std::atoi(genv) : 2);  // late non-temporal G loads measured fastest
   }
-  G_.reinit(precision, n_cells * 6 * nq3);
+  G_.reinit(precision, g_size(nq, n_cells));
   if (cell_vertices) {
     std::vector<double> iv(static_cast<size_t>(n_cells * 24));
     for (int64_t c = 0; c < n_cells; ++c)
@@ -197,11 +198,21 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     else fill_geometry_t<float>(nullptr, nullptr);
     PMG_HIP_CHECK(hipDeviceSynchronize());
   } else {
+    // caller layout [cell][comp][q] -> batch-interleaved device layout (padding 0)
+    auto relayout = [&](auto *dst) {
+      for (int64_t c = 0; c < n_cells; ++c)
+        for (int comp = 0; comp < 6; ++comp)
+          for (int q = 0; q < nq3; ++q)
+            dst[g_index(nq, c, comp, q)] =
+                static_cast<std::remove_pointer_t<decltype(dst)>>(G[(c * 6 + comp) * nq3 + q]);
+    };
     if (precision == PMG_FP64) {
-      PMG_HIP_CHECK(hipMemcpy(G_.data(), G, G_.raw.bytes(), hipMemcpyHostToDevice));
+      std::vector<double> g(static_cast<size_t>(g_size(nq, n_cells)), 0.0);
+      relayout(g.data());
+      PMG_HIP_CHECK(hipMemcpy(G_.data(), g.data(), G_.raw.bytes(), hipMemcpyHostToDevice));
     } else {
-      std::vector<float> g(static_cast<size_t>(n_cells * 6 * nq3));
-      for (size_t i = 0; i < g.size(); ++i) g[i] = static_cast<float>(G[i]);
+      std::vector<float> g(static_cast<size_t>(g_size(nq, n_cells)), 0.0f);
+      relayout(g.data());
       PMG_HIP_CHECK(hipMemcpy(G_.data(), g.data(), G_.raw.bytes(), hipMemcpyHostToDevice));
     }
   }
