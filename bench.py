@@ -50,6 +50,22 @@ def load_pmg():
     return mod
 
 
+PROFILE = os.path.join(ROOT, "profiles", "bk3_cells_q4_64cube_fp64.md")
+DEFAULT_KERNEL = "bk3_cells_kernel<5, 5, double, true, 2>"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the default cell kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:") and DEFAULT_KERNEL in line:
+                return float(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def time_vmult(pmg, op, x, y, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -181,8 +197,12 @@ def main():
                                f"hex cells, {N} DoF, Gauss {nq}/dir, homogeneous Dirichlet",
                    "cells": n ** 3, "dofs": N, "degree": degree, "n_q_points_1d": nq},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "bk3_cells_kernel<5,5,double>",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic() if args.schedule is None and args.gload is None
+                                 else None),
+                     "traffic_source": "profiles/bk3_cells_q4_64cube_fp64.md (rocprofv3 PMC, "
+                                       "calibrated against a 1 GiB copy)",
+                     "kernel": DEFAULT_KERNEL,
                      "kernel_time_us": t_kernel * 1e6, "algorithmic_bytes": alg_bytes,
                      "fp64_tflops": alg_flops / t_kernel / 1e12,
                      "fp64_frac": alg_flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS},
