@@ -70,7 +70,8 @@ int pmg_device_synchronize(void);
  * Host-side helpers that build the inputs deal.II's DoFHandler/MatrixFree
  * produce in the reference (tests/poisson_cube_bk3/program.cc:281-566): a
  * box [lo,hi] of nx*ny*nz hexahedra, optionally displaced by
- * x += a sin(2 pi y) sin(2 pi z) and cyclic permutations (deform = 1), with a
+ * x += a sin(2 pi y) sin(2 pi z) and cyclic permutations (deform = 1) or
+ * sheared by x += a y, y += a z, z += a x (deform = 2), with a
  * lexicographic global numbering on the (p nx+1)(p ny+1)(p nz+1) grid.
  * All outputs are caller-allocated host arrays. */
 typedef struct {
@@ -99,14 +100,21 @@ int pmg_box_h_transfer_maps(const pmg_box *coarse, int degree, uint32_t *map_coa
  * (include/multigrid/portable_polynomial_transfer.h:1155-1360) */
 int pmg_box_p_transfer_maps(const pmg_box *box, int degree_coarse, int degree_fine,
                             uint32_t *map_coarse, uint32_t *map_fine, double *weights);
+/* DG face lists of the box (the face_info of LaplaceOperatorDG::compute_face_info,
+ * include/operators/portable_laplace_operator_dg.h:623-675): inner faces
+ * [F][4] = (cell-, cell+, face-, face+) with face- = 2d+1, face+ = 2d between
+ * neighbours along axis d; boundary faces [Fb][3] = (cell, face, boundary id 0).
+ * Pass NULL arrays to query the counts. */
+int pmg_box_dg_faces(const pmg_box *box, uint32_t *inner_faces, int64_t *n_inner,
+                     uint32_t *boundary_faces, int64_t *n_boundary);
 
 /* ------------------------------------------------------------ operator
  * Replaces Portable::LaplaceOperatorBK3<dim,p,number>
  * (include/operators/portable_laplace_operator_bk3.h:19-96, ctor :104-126) and
  * the LaplaceOperatorBase<dim,Number> interface
  * (include/base/portable_laplace_operator_base.h:16-61).  dim = 3,
- * 1 <= degree <= 7, degree+1 <= n_q_points_1d <= 8 (the reference uses
- * n_q_points_1d = degree+1, portable_laplace_operator_bk3.h:120).
+ * 1 <= degree <= 6 (OperatorDispatchFactory, max 6), n_q_points_1d = degree+1
+ * (the reference, portable_laplace_operator_bk3.h:120) or degree+2.
  * Geometry: the trilinear map of each cell's 8 vertices (what MappingQ(p)
  * reproduces on straight-sided hexahedra), G computed on the device
  * (compute_G_tensors, portable_laplace_operator_bk3.h:456-515). */
@@ -169,8 +177,36 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_po
  * 2 non-temporal loads at first use (variants 1, 2 only for degree 4).
  * Defaults come from PMG_BK3_SCHEDULE=colored|atomic and PMG_BK3_GLOAD. */
 int pmg_operator_set_schedule(pmg_operator op, int colored, int g_load);
-/* Bytes the vmult streams under the algorithmic model s(2N + 6 C nq^3). */
+/* Bytes one vmult must move (BK3: s(2N + 6 C nq^3); DG: adds the face data
+ * and connectivity) and its flop count. */
 int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops);
+
+/* ------------------------------------------------------------ SIP-DG operator
+ * Replaces Portable::LaplaceOperatorDG<dim,p,n_q_points_1d,number>
+ * (include/operators/portable_laplace_operator_dg.h:20-151, ctor :154-200,
+ * vmult :252-351, compute_diagonal :887-915) on trilinear cells.  The handle
+ * is a pmg_operator: vmult, apply_add (= vmult added to dst), m, info,
+ * compute_diagonal / diagonal_inverse (cell integrals only, as the reference),
+ * assemble_rhs, l2_error, Chebyshev, multigrid and CG accept it.
+ * DoFs are cell-major: cell c owns [c (p+1)^3, (c+1) (p+1)^3), lexicographic.
+ * inner_faces [F][4] = (cell-, cell+, face-, face+), face+ = face- ^ 1;
+ * boundary_faces [Fb][3] = (cell, face, boundary_id): id 0 weak homogeneous
+ * Dirichlet (u+ = -u-), other ids homogeneous Neumann
+ * (laplace_dg_kokkos_kernels.h:1563-1576).  Faces numbered 0: x=0, 1: x=1,
+ * 2: y=0, 3: y=1, 4: z=0, 5: z=1 in the reference cell.
+ * penalty: NULL, or [F + Fb] values (inner faces first); NULL computes
+ * penalty_kind 0: p(p+1)/2 (1/h- + 1/h+) (get_penalty_factor, :147-151) or
+ * penalty_kind 1: (p+1)^2 (1/h- + 1/h+), h = |K| / |F|, h+ = h- on the boundary.
+ * metric 0: d_n u = (J^{-1} n) . grad_xi u (exact normal derivative);
+ * metric 1: (J^{-T} n) . grad_xi u as the reference GPU face data stores it
+ * (portable_laplace_operator_dg.h:781-789); equal on axis-aligned cells. */
+int pmg_dg_operator_create(int dim, int degree, int n_q_points_1d, int precision,
+                           int64_t n_cells, const double *cell_vertices, int64_t n_inner_faces,
+                           const uint32_t *inner_faces, int64_t n_boundary_faces,
+                           const uint32_t *boundary_faces, const double *penalty,
+                           int penalty_kind, int metric, pmg_operator *op);
+/* penalty values in use, [F + Fb] (host array) */
+int pmg_dg_operator_penalty(pmg_operator op, double *penalty);
 
 /* ------------------------------------------------------------ transfers
  * Replaces Portable::MGTransferBase<dim,number> (include/base/portable_mg_transfer_base.h:15-36)
@@ -179,7 +215,9 @@ int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops
  * kind H: degree_fine must equal degree_coarse; per coarse cell a (2p+1)^3 patch.
  * kind P: same mesh, degree_coarse < degree_fine.
  * kind DG_CG: same mesh and degree; map_coarse is the CG map (invalid on
- *   constrained), map_fine the DG map (cell-major), weights may be NULL.
+ *   constrained), map_fine the DG DoF of each cell-local index, or NULL for the
+ *   cell-major numbering of pmg_dg_operator_create; weights are ignored
+ *   (ContinuousTransfer, include/multigrid/portable_continuous_transfer.h:112-266).
  * prolongation_scale multiplies the weights of the prolongation only: 1.0 is
  * the mathematical P; 2.0 reproduces the reference HEAD, whose h-prolongation
  * launches two kernels into the same vector (portable_geometric_transfer.h:805-855). */

@@ -47,7 +47,9 @@ class MGParams(ctypes.Structure):
     _fields_ = [("n_levels", ctypes.c_int), ("n_threads", ctypes.c_int),
                 ("cheby_degree", ctypes.c_int), ("smoothing_range", ctypes.c_double),
                 ("eig_cg_n_iterations", ctypes.c_int), ("coarse_kind", ctypes.c_int),
-                ("coarse_tol", ctypes.c_double), ("h_prolong_scale", ctypes.c_double)]
+                ("coarse_tol", ctypes.c_double), ("h_prolong_scale", ctypes.c_double),
+                ("dg_top", ctypes.c_int), ("dg_penalty_kind", ctypes.c_int),
+                ("dg_metric", ctypes.c_int)]
 
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -104,6 +106,11 @@ def lib():
             "pmo_pcg": (_i, [_vp, _i, _i, _D, _D, _d, _i, ctypes.POINTER(_d),
                              ctypes.POINTER(_d)]),
             "pmo_tridiag_eigs": (None, [_i, _D, _D, _D]),
+            "pmo_dg_n_dofs": (_L, [_MP, _i]),
+            "pmo_dg_penalty": (None, [_MP, _i, _i, _D]),
+            "pmo_dg_vmult": (None, [_MP, _i, _i, _D, _i, _i, _D, _D]),
+            "pmo_dg_cg_prolongate": (None, [_i, _L, _U, _D, _D]),
+            "pmo_dg_cg_restrict": (None, [_i, _L, _U, _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -254,6 +261,63 @@ class LaplaceBK3:
         return out
 
 
+class LaplaceDG:
+    """Oracle counterpart of LaplaceOperatorDG (SIP) on a structured mesh.
+
+    DoFs are cell-major; sigma is [C][6] (per cell face, the same value seen
+    from both sides of an inner face); metric 0 uses J^{-1}n, 1 J^{-T}n."""
+
+    def __init__(self, m, p, nq=None, penalty_kind=0, sigma=None, metric=0, neumann_mask=0):
+        self.m, self.p = m, p
+        self.nq = p + 1 if nq is None else nq
+        self.C = n_cells(m)
+        self.N = self.C * (p + 1) ** 3
+        self.metric, self.neumann_mask = metric, neumann_mask
+        if sigma is None:
+            sigma = np.zeros(6 * self.C)
+            lib().pmo_dg_penalty(ctypes.byref(m), p, penalty_kind, sigma)
+        self.sigma = np.ascontiguousarray(sigma, np.float64)
+        self.map = np.arange(self.N, dtype=np.uint32)
+
+    def vmult(self, src):
+        dst = np.zeros(self.N)
+        lib().pmo_dg_vmult(ctypes.byref(self.m), self.p, self.nq, self.sigma, self.metric,
+                           self.neumann_mask, np.ascontiguousarray(src, np.float64), dst)
+        return dst
+
+    def cell_diagonal(self):
+        """Diagonal of the cell integrals only (what LaplaceOperatorDG uses)."""
+        G = geometry(self.m, self.nq)[0]
+        d = np.zeros(self.N)
+        lib().pmo_bk3_diagonal(self.p, self.nq, self.C, G, self.map, self.N,
+                               np.zeros(1, np.uint32), 0, d)
+        return d
+
+    def rhs(self, f_kind, nq=None):
+        b = np.zeros(self.N)
+        lib().pmo_rhs(ctypes.byref(self.m), self.p, self.p + 1 if nq is None else nq, f_kind,
+                      self.map, self.N, b)
+        return b
+
+    def l2_error(self, u, u_kind=1, nq=None):
+        return lib().pmo_l2_error(ctypes.byref(self.m), self.p, self.p + 2 if nq is None else nq,
+                                  u_kind, self.map, np.ascontiguousarray(u, np.float64))
+
+
+def dg_cg_prolongate(p, cg_map, src_cg, dst_dg):
+    C = len(cg_map) // (p + 1) ** 3
+    lib().pmo_dg_cg_prolongate(p, C, np.ascontiguousarray(cg_map, np.uint32),
+                               np.ascontiguousarray(src_cg, np.float64), dst_dg)
+    return dst_dg
+
+
+def dg_cg_restrict(p, cg_map, src_dg, dst_cg):
+    C = len(cg_map) // (p + 1) ** 3
+    lib().pmo_dg_cg_restrict(p, C, np.ascontiguousarray(cg_map, np.uint32),
+                             np.ascontiguousarray(src_dg, np.float64), dst_cg)
+    return dst_cg
+
+
 def scale_G(G, kappa, nq):
     C = len(kappa) // nq ** 3
     out = np.zeros_like(G)
@@ -313,13 +377,15 @@ class Multigrid:
 
     def __init__(self, meshes, degrees, nqs=None, cheby_degree=3, smoothing_range=15.0,
                  eig_cg_n_iterations=10, coarse_kind=1, coarse_tol=1e-12, h_prolong_scale=1.0,
-                 n_threads=1):
+                 n_threads=1, dg_top=False, dg_penalty_kind=0, dg_metric=0):
+        """dg_top: the finest level is SIP-DG on the mesh and degree of the level below."""
         L = len(meshes)
         self._meshes = (Mesh * L)(*meshes)
         self._ps = np.array(degrees, dtype=np.int32)
         self._nqs = None if nqs is None else np.array(nqs, dtype=np.int32)
         prm = MGParams(L, n_threads, cheby_degree, smoothing_range, eig_cg_n_iterations,
-                       coarse_kind, coarse_tol, h_prolong_scale)
+                       coarse_kind, coarse_tol, h_prolong_scale, int(dg_top), dg_penalty_kind,
+                       dg_metric)
         self.h = lib().pmo_mg_create(L, self._meshes, self._ps, _ptr(self._nqs),
                                      ctypes.byref(prm))
         self.L = L

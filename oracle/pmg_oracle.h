@@ -54,8 +54,8 @@ void pmo_shape_gradients(int p, int nq, double *Sd);
 typedef struct {
   int nx, ny, nz;          /* cells per direction */
   double lo[3], hi[3];     /* box before deformation */
-  int deform;              /* 0: none, 1: sinusoidal cyclic displacement */
-  double amplitude;        /* displacement amplitude for deform == 1 */
+  int deform;              /* 0: none, 1: sinusoidal cyclic displacement, 2: affine shear */
+  double amplitude;        /* displacement amplitude / shear factor */
 } pmo_mesh;
 
 long pmo_n_cells(const pmo_mesh *m);
@@ -118,6 +118,22 @@ void pmo_prolongate(int nmc, int nmf, long n_cells, const double *M, const uint3
 void pmo_restrict(int nmc, int nmf, long n_cells, const double *M, const uint32_t *map_c,
                   const uint32_t *map_f, const double *w, const double *xf, double *yc);
 
+/* ---------------- SIP-DG (pmo_dg.c) ---------------- */
+/* cell-major DG(p) numbering: DoF c*(p+1)^3 + lexicographic local index */
+long pmo_dg_n_dofs(const pmo_mesh *m, int p);
+/* sigma[c][f] of the face f (0: x=0, 1: x=1, 2: y=0, ...) of cell c;
+ * kind 0: p(p+1)/2 (1/h- + 1/h+), kind 1: (p+1)^2 (1/h- + 1/h+), h = |K|/|F| */
+void pmo_dg_penalty(const pmo_mesh *m, int p, int kind, double *sigma /*[C][6]*/);
+/* dst = A_DG src; metric 0: d_n = (J^{-1} n).grad_xi, 1: (J^{-T} n).grad_xi;
+ * bit f of neumann_mask marks box side f as Neumann (else weak Dirichlet) */
+void pmo_dg_vmult(const pmo_mesh *m, int p, int nq, const double *sigma, int metric,
+                  int neumann_mask, const double *src, double *dst);
+/* DG(p) <-> CG(p) on one mesh; cg_map[C][(p+1)^3] with 0xFFFFFFFF = constrained */
+void pmo_dg_cg_prolongate(int p, long n_cells, const uint32_t *cg_map, const double *src_cg,
+                          double *dst_dg);
+void pmo_dg_cg_restrict(int p, long n_cells, const uint32_t *cg_map, const double *src_dg,
+                        double *dst_cg);
+
 /* ---------------- solvers (pmo_solver.c) ---------------- */
 typedef struct pmo_level pmo_level;
 /* A level of a geometric/polynomial hierarchy on the unit-cube family */
@@ -129,7 +145,10 @@ typedef struct {
   int eig_cg_n_iterations;   /* 10 */
   int coarse_kind;           /* 0: Chebyshev auto-degree to 1e-3, 1: PCG to coarse_tol */
   double coarse_tol;         /* 1e-12 */
-  double h_prolong_scale;    /* 1 (mathematical P) or 2 (reference HEAD, F1) */
+  double h_prolong_scale;    /* 1 (mathematical P) or 2 (reference HEAD's doubled h-prolongation) */
+  int dg_top;                /* 1: the finest level is SIP-DG on the mesh/degree of the level below */
+  int dg_penalty_kind;       /* see pmo_dg_penalty */
+  int dg_metric;             /* see pmo_dg_vmult */
 } pmo_mg_params;
 
 typedef struct pmo_mg pmo_mg;

@@ -38,6 +38,13 @@ static void vertex_coord(const pmo_mesh *m, int a, int b, int c, double *X)
     X[0] = x + A * sin(tp * y) * sin(tp * z);
     X[1] = y + A * sin(tp * z) * sin(tp * x);
     X[2] = z + A * sin(tp * x) * sin(tp * y);
+  } else if (m->deform == 2) {
+    /* affine shear x += a y, y += a z, z += a x (parallelepiped cells with a
+     * non-symmetric Jacobian) */
+    const double A = m->amplitude;
+    X[0] = x + A * y;
+    X[1] = y + A * z;
+    X[2] = z + A * x;
   }
 }
 

@@ -30,6 +30,8 @@
 struct pmo_level {
   pmo_mesh mesh;
   int p, nq;
+  int dg;          /* 1: SIP-DG level (finest only), DoFs cell-major */
+  double *sigma;   /* DG penalty [C][6] */
   long C, N;
   double *S, *D, *G, *inv_diag;
   uint32_t *map, *cons;
@@ -39,7 +41,7 @@ struct pmo_level {
   double theta, delta, max_ev, min_ev;
   int degree;
   /* transfer (level-1) -> level */
-  int xfer_h; /* 1: h, 0: p */
+  int xfer_h; /* 1: h, 0: p, 2: CG(p) -> DG(p) */
   int nmc, nmf;
   long xcells;
   double *M, *w;
@@ -64,6 +66,10 @@ static double dot(long n, const double *a, const double *b)
 void pmo_mg_level_vmult(pmo_mg *mg, int l, double *dst, const double *src)
 {
   struct pmo_level *v = &mg->lv[l];
+  if (v->dg) {
+    pmo_dg_vmult(&v->mesh, v->p, v->nq, v->sigma, mg->prm.dg_metric, 0, src, dst);
+    return;
+  }
   pmo_bk3_vmult(v->p, v->nq, v->C, v->S, v->D, v->G, v->map, v->N, v->cons, v->ncons, src, dst,
                 mg->prm.n_threads, v->colors);
 }
@@ -227,10 +233,13 @@ static void v_cycle(pmo_mg *mg, int l)
   pmo_mg_level_vmult(mg, l, v->t, v->sol);
   for (long i = 0; i < v->N; ++i) v->t[i] = v->def[i] - v->t[i];
   memset(c->def, 0, sizeof(double) * c->N);
-  pmo_restrict(v->nmc, v->nmf, v->xcells, v->M, v->map_c, v->map_f, v->w, v->t, c->def);
+  if (v->xfer_h == 2) pmo_dg_cg_restrict(v->p, v->C, v->map_c, v->t, c->def);
+  else pmo_restrict(v->nmc, v->nmf, v->xcells, v->M, v->map_c, v->map_f, v->w, v->t, c->def);
   v_cycle(mg, l - 1);
-  pmo_prolongate(v->nmc, v->nmf, v->xcells, v->M, v->map_c, v->map_f, v->w,
-                 v->xfer_h ? mg->prm.h_prolong_scale : 1.0, c->sol, v->sol);
+  if (v->xfer_h == 2) pmo_dg_cg_prolongate(v->p, v->C, v->map_c, c->sol, v->sol);
+  else
+    pmo_prolongate(v->nmc, v->nmf, v->xcells, v->M, v->map_c, v->map_f, v->w,
+                   v->xfer_h ? mg->prm.h_prolong_scale : 1.0, c->sol, v->sol);
   pmo_mg_smooth(mg, l, v->sol, v->def, 1);
 }
 
@@ -257,7 +266,8 @@ pmo_mg *pmo_mg_create(int n_levels, const pmo_mesh *ms, const int *ps, const int
     v->p = ps[l];
     v->nq = nqs ? nqs[l] : ps[l] + 1;
     v->C = pmo_n_cells(&ms[l]);
-    v->N = pmo_n_dofs_cg(&ms[l], v->p);
+    v->dg = prm->dg_top && l == n_levels - 1 && l > 0;
+    v->N = v->dg ? pmo_dg_n_dofs(&ms[l], v->p) : pmo_n_dofs_cg(&ms[l], v->p);
     const int nm = v->p + 1, nm3 = nm * nm * nm, nq3 = v->nq * v->nq * v->nq;
     v->S = malloc(sizeof(double) * nm * v->nq);
     v->D = malloc(sizeof(double) * v->nq * v->nq);
@@ -265,10 +275,19 @@ pmo_mg *pmo_mg_create(int n_levels, const pmo_mesh *ms, const int *ps, const int
     v->G = malloc(sizeof(double) * v->C * 6 * nq3);
     pmo_geometry(&ms[l], v->nq, v->G, NULL, NULL);
     v->map = malloc(sizeof(uint32_t) * v->C * nm3);
-    pmo_dofmap_cg(&ms[l], v->p, 1, v->map);
-    v->ncons = pmo_boundary_dofs_cg(&ms[l], v->p, NULL);
+    if (v->dg) {
+      /* cell-major DG numbering, no constraints; cell-only diagonal
+       * (LaplaceOperatorDG::compute_diagonal, portable_laplace_operator_dg.h:887-915) */
+      for (long i = 0; i < v->C * nm3; ++i) v->map[i] = (uint32_t)i;
+      v->ncons = 0;
+      v->sigma = malloc(sizeof(double) * 6 * v->C);
+      pmo_dg_penalty(&ms[l], v->p, prm->dg_penalty_kind, v->sigma);
+    } else {
+      pmo_dofmap_cg(&ms[l], v->p, 1, v->map);
+      v->ncons = pmo_boundary_dofs_cg(&ms[l], v->p, NULL);
+    }
     v->cons = malloc(sizeof(uint32_t) * (v->ncons + 1));
-    pmo_boundary_dofs_cg(&ms[l], v->p, v->cons);
+    if (!v->dg) pmo_boundary_dofs_cg(&ms[l], v->p, v->cons);
     v->colors = malloc(sizeof(int) * v->C);
     pmo_cell_colors(&ms[l], v->colors);
     v->inv_diag = malloc(sizeof(double) * v->N);
@@ -282,7 +301,14 @@ pmo_mg *pmo_mg_create(int n_levels, const pmo_mesh *ms, const int *ps, const int
     if (l > 0) {
       const pmo_mesh *mc = &ms[l - 1];
       const int pc = ps[l - 1];
-      if (mc->nx != ms[l].nx) {
+      if (v->dg) {
+        /* ContinuousTransfer: CG map of the level below (same mesh, degree) */
+        v->xfer_h = 2;
+        v->nmc = v->nmf = nm;
+        v->xcells = v->C;
+        v->map_c = malloc(sizeof(uint32_t) * v->C * nm3);
+        pmo_dofmap_cg(mc, pc, 1, v->map_c);
+      } else if (mc->nx != ms[l].nx) {
         v->xfer_h = 1;
         v->nmc = v->p + 1;
         v->nmf = 2 * v->p + 1;
@@ -323,7 +349,7 @@ void pmo_mg_destroy(pmo_mg *mg)
     struct pmo_level *v = &mg->lv[l];
     free(v->S); free(v->D); free(v->G); free(v->inv_diag); free(v->map); free(v->cons);
     free(v->colors); free(v->sol); free(v->def); free(v->t); free(v->old); free(v->tmp);
-    free(v->M); free(v->w); free(v->map_c); free(v->map_f);
+    free(v->M); free(v->w); free(v->map_c); free(v->map_f); free(v->sigma);
   }
   free(mg->lv);
   free(mg);

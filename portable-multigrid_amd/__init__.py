@@ -86,6 +86,11 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run portable-multigrid_amd build() "
                            "(hipcc --offload-arch=gfx950) first; there is no CPU fallback")
+    # torch brings its own libamdhip64.so.7; loading it first makes libpmg.so
+    # bind to that same HIP runtime, so the library and torch share devices,
+    # streams and allocations (a second runtime loaded ahead of torch's leaves
+    # torch without visible GPUs)
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.POINTER
     sig = {
@@ -99,6 +104,10 @@ def lib():
         "pmg_box_boundary_dofs": (_i, [_BP, _i, _vp, P(_i64)]),
         "pmg_box_h_transfer_maps": (_i, [_BP, _i, _vp, _vp, _vp]),
         "pmg_box_p_transfer_maps": (_i, [_BP, _i, _i, _vp, _vp, _vp]),
+        "pmg_box_dg_faces": (_i, [_BP, _vp, P(_i64), _vp, P(_i64)]),
+        "pmg_dg_operator_create": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp, _vp,
+                                        _i, _i, P(_vp)]),
+        "pmg_dg_operator_penalty": (_i, [_vp, _vp]),
         "pmg_operator_create": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp, P(_vp)]),
         "pmg_operator_create_from_G": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp,
                                             P(_vp)]),
@@ -213,6 +222,18 @@ def box_p_transfer_maps(b, pc, pf):
     _check(lib().pmg_box_p_transfer_maps(ctypes.byref(b), pc, pf, _np_ptr(mc), _np_ptr(mf),
                                          _np_ptr(w)))
     return mc, mf, w
+
+
+def box_dg_faces(b):
+    """DG face lists of a box: inner [F][4] = (cell-, cell+, face-, face+),
+    boundary [Fb][3] = (cell, face, boundary id 0)."""
+    ni, nb = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().pmg_box_dg_faces(ctypes.byref(b), None, ctypes.byref(ni), None, ctypes.byref(nb)))
+    inner = np.zeros(4 * ni.value, dtype=np.uint32)
+    bdry = np.zeros(3 * nb.value, dtype=np.uint32)
+    _check(lib().pmg_box_dg_faces(ctypes.byref(b), _np_ptr(inner), ctypes.byref(ni),
+                                  _np_ptr(bdry), ctypes.byref(nb)))
+    return inner, bdry
 
 
 # ------------------------------------------------------------------ operator
@@ -338,6 +359,56 @@ class LaplaceOperatorBK3:
         return b.value, f.value
 
 
+class LaplaceOperatorDG(LaplaceOperatorBK3):
+    """Symmetric interior penalty DG Laplacian on the GPU (LaplaceOperatorDG<3,p,nq,number>).
+
+    Cell-major DoFs.  penalty: None (computed from penalty_kind, 0 = the
+    reference's p(p+1)/2 (1/h- + 1/h+), 1 = (p+1)^2 (1/h- + 1/h+)) or an
+    array over the faces (inner faces first).  metric 0 uses J^{-1}n, 1 the
+    J^{-T}n of the reference's GPU face data.  Boundary faces of a box are
+    weak homogeneous Dirichlet unless neumann_sides (bit f = box side f) says
+    otherwise."""
+
+    def __init__(self, b=None, degree=1, n_q_points_1d=None, precision=FP64, penalty=None,
+                 penalty_kind=0, metric=0, neumann_sides=0, _raw=None):
+        self.degree = degree
+        self.nq = degree + 1 if n_q_points_1d is None else n_q_points_1d
+        self.precision = precision
+        self.h = ctypes.c_void_p()
+        if _raw is None:
+            self.box = b
+            cv = box_cell_vertices(b)
+            inner, bdry = box_dg_faces(b)
+            if neumann_sides:
+                f = bdry.reshape(-1, 3)
+                f[:, 2] = ((neumann_sides >> f[:, 1].astype(np.int64)) & 1).astype(np.uint32)
+        else:
+            cv, inner, bdry = _raw
+            cv = np.ascontiguousarray(cv, dtype=np.float64)
+            inner = np.ascontiguousarray(inner, dtype=np.uint32)
+            bdry = np.ascontiguousarray(bdry, dtype=np.uint32)
+        self.n_cells = len(cv) // 24
+        self.n_faces = len(inner) // 4 + len(bdry) // 3
+        pen = None if penalty is None else np.ascontiguousarray(penalty, dtype=np.float64)
+        _check(lib().pmg_dg_operator_create(3, degree, self.nq, precision, self.n_cells,
+                                            _np_ptr(cv), len(inner) // 4, _np_ptr(inner),
+                                            len(bdry) // 3, _np_ptr(bdry),
+                                            None if pen is None else _np_ptr(pen), penalty_kind,
+                                            metric, ctypes.byref(self.h)))
+        self._m = self.n_cells * (degree + 1) ** 3
+
+    @classmethod
+    def from_arrays(cls, degree, cell_vertices, inner_faces, boundary_faces, n_q_points_1d=None,
+                    precision=FP64, penalty=None, penalty_kind=0, metric=0):
+        return cls(None, degree, n_q_points_1d, precision, penalty, penalty_kind, metric,
+                   _raw=(cell_vertices, inner_faces, boundary_faces))
+
+    def penalty(self):
+        out = np.zeros(self.n_faces)
+        _check(lib().pmg_dg_operator_penalty(self.h, _np_ptr(out)))
+        return out
+
+
 # ------------------------------------------------------------------ transfers
 class _Transfer:
     def __init__(self, kind, precision, pc, pf, mc, mf, w, n_c, n_f, scale):
@@ -345,11 +416,12 @@ class _Transfer:
         self.n_c, self.n_f = n_c, n_f
         self.h = ctypes.c_void_p()
         mc = np.ascontiguousarray(mc, dtype=np.uint32)
-        mf = np.ascontiguousarray(mf, dtype=np.uint32)
-        w = np.ascontiguousarray(w, dtype=np.float64)
+        mf = None if mf is None else np.ascontiguousarray(mf, dtype=np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
         n_cells = len(mc) // (pc + 1) ** 3
         _check(lib().pmg_transfer_create(kind, precision, pc, pf, n_cells, _np_ptr(mc),
-                                         _np_ptr(mf), _np_ptr(w), n_c, n_f, scale,
+                                         None if mf is None else _np_ptr(mf),
+                                         None if w is None else _np_ptr(w), n_c, n_f, scale,
                                          ctypes.byref(self.h)))
 
     def __del__(self):
@@ -388,6 +460,17 @@ class PolynomialTransfer(_Transfer):
         mc, mf, w = box_p_transfer_maps(b, pc, pf)
         super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, w, box_n_dofs(b, pc),
                          box_n_dofs(b, pf), 1.0)
+
+
+class ContinuousTransfer(_Transfer):
+    """CG(p) -> DG(p) on one box (ContinuousTransfer<3,p>): the DG DoFs of a
+    cell take the values of the CG DoFs at the same support points; the
+    restriction sums them back (constrained CG DoFs skipped)."""
+
+    def __init__(self, b, p, precision=FP64):
+        mc = box_dofmap(b, p, True)
+        super().__init__(TRANSFER_DG_CG, precision, p, p, mc, None, None, box_n_dofs(b, p),
+                         box_n_cells(b) * (p + 1) ** 3, 1.0)
 
 
 # ------------------------------------------------------------------ smoother
@@ -473,24 +556,40 @@ def vector_dot(a, b, precision=FP64):
 # ------------------------------------------------------------------ hierarchies
 def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=3,
                     smoothing_range=15.0, eig_cg_n_iterations=10, coarse=COARSE_PCG,
-                    coarse_tol=1e-12, prolongation_scale=1.0, outer_precision=None):
-    """V-cycle over `levels` = [(box, degree), ...] from coarsest to finest.
+                    coarse_tol=1e-12, prolongation_scale=1.0, outer_precision=None,
+                    dg_penalty_kind=0, dg_metric=0):
+    """V-cycle over `levels` = [(box, degree) or (box, degree, "dg"), ...]
+    from coarsest to finest.
 
-    Consecutive levels on the same box are joined by a PolynomialTransfer,
-    levels whose box is refined once by a GeometricTransfer (same degree),
-    as the reference's ph hierarchy (tests/poisson_cube_bk3/program.cc:281-351,
-    421-470).  n_q_points_1d: None -> degree+1 per level, an int offset as
-    "degree + k" is given by passing a callable degree -> nq."""
+    Consecutive levels on the same box are joined by a PolynomialTransfer
+    (degree change) or a ContinuousTransfer (same degree, CG -> DG), levels
+    whose box is refined once by a GeometricTransfer (same degree), as the
+    reference's ph hierarchies (tests/poisson_cube_bk3/program.cc:281-351,
+    421-470; tests/poisson_cube_bk3_fp64_dg/program.cc:294-460).
+    n_q_points_1d: None -> degree+1 per level, or a callable degree -> nq."""
     ops, trs, sms = [], [None], [None]
-    for l, (b, p) in enumerate(levels):
+    for l, lev in enumerate(levels):
+        b, p = lev[0], lev[1]
+        dg = len(lev) > 2 and lev[2] == "dg"
         nq = None if n_q_points_1d is None else n_q_points_1d(p)
-        op = LaplaceOperatorBK3(b, p, nq, precision)
+        if dg:
+            op = LaplaceOperatorDG(b, p, nq, precision, penalty_kind=dg_penalty_kind,
+                                   metric=dg_metric)
+        else:
+            op = LaplaceOperatorBK3(b, p, nq, precision)
         op.compute_diagonal()
         ops.append(op)
         if l == 0:
+            assert not dg, "the coarsest level must be continuous"
             continue
-        bc, pc = levels[l - 1]
-        if (bc.nx, bc.ny, bc.nz) == (b.nx, b.ny, b.nz):
+        bc, pc = levels[l - 1][0], levels[l - 1][1]
+        assert not (len(levels[l - 1]) > 2 and levels[l - 1][2] == "dg"), \
+            "a DG level can only be the finest"
+        same = (bc.nx, bc.ny, bc.nz) == (b.nx, b.ny, b.nz)
+        if dg:
+            assert same and pc == p, "a DG level sits on the CG level of the same box and degree"
+            trs.append(ContinuousTransfer(b, p, precision))
+        elif same:
             trs.append(PolynomialTransfer(b, pc, p, precision))
         else:
             assert (2 * bc.nx, 2 * bc.ny, 2 * bc.nz) == (b.nx, b.ny, b.nz) and pc == p, \
@@ -501,6 +600,23 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
     mg = VCycleMultigrid(ops, trs, sms, coarse, coarse_tol,
                          precision if outer_precision is None else outer_precision)
     return mg, ops, trs, sms
+
+
+def ph_levels(fine_box, degree, n_h_levels, dg=False):
+    """Level list of the reference's ph/DG drivers: Q1 on fine_box coarsened
+    n_h_levels-1 times up to fine_box, then degrees 1 -> ... -> p on fine_box
+    with p_levels = p, max(p-2,1), ... (tests/poisson_cube_bk3_fp64_dg/program.cc:294-300),
+    then (dg) DG(p) on fine_box."""
+    boxes = coarsen_boxes(fine_box, n_h_levels)
+    levels = [(b, 1) for b in boxes]
+    ps = [degree]
+    while ps[-1] > 1:
+        ps.append(max(ps[-1] - 2, 1))
+    for p in reversed(ps[:-1]):
+        levels.append((fine_box, p))
+    if dg:
+        levels.append((fine_box, degree, "dg"))
+    return levels
 
 
 def coarsen_boxes(fine_box, n_levels):

@@ -16,29 +16,11 @@
 // in registers for the scatter.  The 1D matrices are kernel arguments (scalar
 // registers).  Blocks are remapped so each XCD (own L2) walks a contiguous
 // range of cells, which keeps neighbouring cells' shared DoFs in one L2.
+#include "pmg_bk3_cell.hpp"
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
 namespace pmg {
-
-template <int NQ>
-struct Bk3Tile {
-  static constexpr int NQ2 = NQ * NQ;
-  static constexpr int cells = (256 / NQ2) > 0 ? 256 / NQ2 : 1;
-  static constexpr int lanes = ((cells * NQ2 + 63) / 64) * 64;
-};
-
-template <typename T>
-__device__ inline T ldg_stream(const T *p) {
-  return __builtin_nontemporal_load(p);
-}
-
-// G load policy of the BK3 kernel
-enum : int {
-  kGEarlyNT = 0,  // non-temporal loads issued at kernel entry
-  kGEarly = 1,    // ordinary loads issued at kernel entry
-  kGLateNT = 2    // non-temporal loads issued where G is used
-};
 
 // One batch of E = Bk3Tile<NQ>::cells cells at positions cell_base .. cell_base+E-1
 // of a range of n_range cells (cell index = cell_ids[pos] if cell_ids, else pos),
@@ -53,204 +35,38 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells;
-  constexpr int BUF = NQ3;
 
   const int tid = threadIdx.x;
   const int e = tid / NQ2, t = tid - e * NQ2, ta = t % NQ, tb = t / NQ;
   const int64_t pos = cell_base + e;
   const bool active = (e < E) && (pos < n_range);
   const int64_t cell = active ? (cell_ids ? static_cast<int64_t>(cell_ids[pos]) : pos) : 0;
-  T *P0 = lds + (active ? e : 0) * 3 * BUF;
-  T *P1 = P0 + BUF;
-  T *P2 = P1 + BUF;
-  // batch-interleaved G: component c, plane r of this lane's column sits at
-  // Gc[(c * NQ + r) * LANES]; for a batch of consecutive cells a wave reads
-  // 64 consecutive values per load
-  constexpr int LANES = Bk3Tile<NQ>::lanes;
+  T *P0 = lds + (active ? e : 0) * 3 * NQ3;
   const T *Gc = G + g_index(NQ, cell, 0, t);
 
-  // ---- geometric factors for this lane's column (p,q) = (ta,tb), all r
   T g[6][NQ];
-  if (GMODE != kGLateNT && active) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-      for (int r = 0; r < NQ; ++r)
-        g[c][r] = GMODE == kGEarlyNT ? ldg_stream(Gc + (c * NQ + r) * LANES) : Gc[(c * NQ + r) * LANES];
-  }
+  if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
 
-  // ---- phase 1: gather column (i,j)=(ta,tb) over k, interpolate in z
+  // gather column (i,j) = (ta,tb) over k; invalid (constrained) entries read 0
   uint32_t idx[NM];
+  T u[NM], v[NM];
   if (active && ta < NM && tb < NM) {
     const uint32_t *mc = map + cell * NM3 + tb * NM + ta;
-    T u[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
       idx[k] = mc[k * NM2];
       u[k] = (idx[k] == kInvalid) ? T(0) : x[idx[k]];
     }
-#pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      T v = 0;
-#pragma unroll
-      for (int k = 0; k < NM; ++k) v += sh.S[k * NQ + r] * u[k];
-      P0[r * NM2 + tb * NM + ta] = v;  // A[r][j][i]
-    }
   }
-  __syncthreads();
-
-  // ---- phase 2: y-interpolation, lane (i,r) = (ta,tb): B[r][q][i]
-  if (active && ta < NM) {
-    T a[NM];
-#pragma unroll
-    for (int j = 0; j < NM; ++j) a[j] = P0[tb * NM2 + j * NM + ta];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      T v = 0;
-#pragma unroll
-      for (int j = 0; j < NM; ++j) v += sh.S[j * NQ + q] * a[j];
-      P1[(tb * NQ + q) * NM + ta] = v;
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 3: x-interpolation + x-gradient, lane (q,r) = (ta,tb)
-  if (active) {
-    T b[NM], U[NQ];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) b[i] = P1[(tb * NQ + ta) * NM + i];
-#pragma unroll
-    for (int p = 0; p < NQ; ++p) {
-      T v = 0;
-#pragma unroll
-      for (int i = 0; i < NM; ++i) v += sh.S[i * NQ + p] * b[i];
-      U[p] = v;
-      P0[tb * NQ2 + ta * NQ + p] = v;  // U[r][q][p]
-    }
-#pragma unroll
-    for (int p = 0; p < NQ; ++p) {
-      T v = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + p] * U[n];
-      P2[tb * NQ2 + ta * NQ + p] = v;  // du/dxi_0
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 4: y-gradient, lane (p,r) = (ta,tb)
-  if (active) {
-    T c[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; ++n) c[n] = P0[tb * NQ2 + n * NQ + ta];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      T v = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + q] * c[n];
-      P1[tb * NQ2 + q * NQ + ta] = v;  // du/dxi_1
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
-  if (active) {
-    if (GMODE == kGLateNT) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int r = 0; r < NQ; ++r) g[c][r] = ldg_stream(Gc + (c * NQ + r) * LANES);
-    }
-    T col[NQ], fz[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; ++n) col[n] = P0[n * NQ2 + tb * NQ + ta];
-#pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      T uz = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) uz += sh.D[n * NQ + r] * col[n];
-      const int id = r * NQ2 + tb * NQ + ta;
-      const T ux = P2[id], uy = P1[id];
-      P2[id] = g[0][r] * ux + g[1][r] * uy + g[2][r] * uz;
-      P1[id] = g[1][r] * ux + g[3][r] * uy + g[4][r] * uz;
-      fz[r] = g[2][r] * ux + g[4][r] * uy + g[5][r] * uz;
-    }
-#pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      T v = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[r * NQ + n] * fz[n];
-      P0[r * NQ2 + tb * NQ + ta] = v;
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 6: x-integration accumulated onto the z part, lane (q,r)
-  if (active) {
-    T f[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; ++n) f[n] = P2[tb * NQ2 + ta * NQ + n];
-#pragma unroll
-    for (int p = 0; p < NQ; ++p) {
-      T v = P0[tb * NQ2 + ta * NQ + p];
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[p * NQ + n] * f[n];
-      P0[tb * NQ2 + ta * NQ + p] = v;
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 7: y-integration + y back-interpolation, lane (p,r)
-  if (active) {
-    T f[NQ], acc[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; ++n) f[n] = P1[tb * NQ2 + n * NQ + ta];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      T v = P0[tb * NQ2 + q * NQ + ta];
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[q * NQ + n] * f[n];
-      acc[q] = v;
-    }
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-      T v = 0;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) v += sh.S[j * NQ + q] * acc[q];
-      P1[tb * NQ2 + j * NQ + ta] = v;  // W[r][j][p]
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 8: x back-interpolation, lane (j,r) = (ta,tb)
-  if (active && ta < NM) {
-    T l[NQ];
-#pragma unroll
-    for (int p = 0; p < NQ; ++p) l[p] = P1[tb * NQ2 + ta * NQ + p];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) {
-      T v = 0;
-#pragma unroll
-      for (int p = 0; p < NQ; ++p) v += sh.S[i * NQ + p] * l[p];
-      P2[tb * NM2 + ta * NM + i] = v;  // Z[r][j][i]
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 9: z back-interpolation + scatter-add, lane (i,j) = (ta,tb)
+  bk3_cell_apply<NM, NQ, T, GMODE>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3, active, ta, tb);
+  // scatter-add
   if (active && ta < NM && tb < NM) {
-    T z[NQ];
 #pragma unroll
-    for (int r = 0; r < NQ; ++r) z[r] = P2[r * NM2 + tb * NM + ta];
-#pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      T v = 0;
-#pragma unroll
-      for (int r = 0; r < NQ; ++r) v += sh.S[k * NQ + r] * z[r];
+    for (int k = 0; k < NM; ++k)
       if (idx[k] != kInvalid) {
-        if (ATOMIC) atomic_add(y + idx[k], v);
-        else y[idx[k]] += v;  // one colour: this cell is the only writer
+        if (ATOMIC) atomic_add(y + idx[k], v[k]);
+        else y[idx[k]] += v[k];  // one colour: this cell is the only writer
       }
-    }
   }
 }
 

@@ -1,13 +1,14 @@
 // pmg_capi.cpp -- extern "C" boundary over the host classes.  Each entry point
 // converts exceptions into a status code and records the message for
 // pmg_last_error(); handles own their device metadata.
+#include <algorithm>
 #include <memory>
 #include <string>
 
 #include "pmg_library.hpp"
 
 struct pmg_operator_s {
-  std::unique_ptr<pmg::LaplaceOperatorBK3> impl;
+  std::unique_ptr<pmg::Operator> impl;
 };
 struct pmg_transfer_s {
   std::unique_ptr<pmg::MGTransfer> impl;
@@ -48,6 +49,11 @@ int guard(F &&f) {
   }
 }
 hipStream_t as_stream(pmg_stream s) { return static_cast<hipStream_t>(s); }
+pmg::LaplaceOperatorBK3 &bk3_of(pmg_operator op) {
+  auto *p = dynamic_cast<pmg::LaplaceOperatorBK3 *>(op->impl.get());
+  PMG_REQUIRE(p != nullptr, PMG_ERR_UNSUPPORTED, "this entry point needs a continuous (BK3) operator");
+  return *p;
+}
 #define PMG_NOT_NULL(p) PMG_REQUIRE((p) != nullptr, PMG_ERR_INVALID_ARG, #p " is NULL")
 }  // namespace
 
@@ -90,6 +96,33 @@ int pmg_operator_create_from_G(int dim, int degree, int nq, int precision, int64
     h->impl = std::make_unique<pmg::LaplaceOperatorBK3>(degree, nq, precision, n_cells, dofmap, n_dofs,
                                                         constrained, n_constrained, nullptr, G);
     *op = h.release();
+  });
+}
+
+int pmg_dg_operator_create(int dim, int degree, int nq, int precision, int64_t n_cells,
+                           const double *cell_vertices, int64_t n_inner_faces,
+                           const uint32_t *inner_faces, int64_t n_boundary_faces,
+                           const uint32_t *boundary_faces, const double *penalty, int penalty_kind,
+                           int metric, pmg_operator *op) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    *op = nullptr;
+    PMG_REQUIRE(dim == 3, PMG_ERR_UNSUPPORTED, "only dim = 3 is supported");
+    auto h = std::make_unique<pmg_operator_s>();
+    h->impl = std::make_unique<pmg::LaplaceOperatorDG>(degree, nq, precision, n_cells, cell_vertices,
+                                                       n_inner_faces, inner_faces, n_boundary_faces,
+                                                       boundary_faces, penalty, penalty_kind, metric);
+    *op = h.release();
+  });
+}
+
+int pmg_dg_operator_penalty(pmg_operator op, double *penalty) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    PMG_NOT_NULL(penalty);
+    auto *dg = dynamic_cast<pmg::LaplaceOperatorDG *>(op->impl.get());
+    PMG_REQUIRE(dg != nullptr, PMG_ERR_UNSUPPORTED, "not a DG operator");
+    std::copy(dg->penalty().begin(), dg->penalty().end(), penalty);
   });
 }
 
@@ -153,7 +186,7 @@ int pmg_operator_diagonal_inverse(pmg_operator op, void *out, pmg_stream s) {
 int pmg_operator_set_coefficient(pmg_operator op, const double *kappa, pmg_stream s) {
   return guard([&] {
     PMG_NOT_NULL(op);
-    op->impl->set_coefficient(kappa, as_stream(s));
+    bk3_of(op).set_coefficient(kappa, as_stream(s));
   });
 }
 
@@ -161,7 +194,7 @@ int pmg_operator_gradient_coefficient(pmg_operator op, const void *u, const void
                                       pmg_stream s) {
   return guard([&] {
     PMG_NOT_NULL(op);
-    op->impl->gradient_coefficient(u, lam, out, as_stream(s));
+    bk3_of(op).gradient_coefficient(u, lam, out, as_stream(s));
   });
 }
 
@@ -185,7 +218,7 @@ int pmg_operator_set_schedule(pmg_operator op, int colored, int g_load) {
   return guard([&] {
     PMG_NOT_NULL(op);
     PMG_REQUIRE(g_load >= 0 && g_load <= 2, PMG_ERR_INVALID_ARG, "g_load must be 0, 1 or 2");
-    op->impl->set_schedule(colored != 0, g_load);
+    bk3_of(op).set_schedule(colored != 0, g_load);
   });
 }
 
@@ -276,7 +309,7 @@ int pmg_multigrid_create(int n_levels, const pmg_operator *ops, const pmg_transf
     PMG_NOT_NULL(mg);
     *mg = nullptr;
     PMG_REQUIRE(n_levels >= 1 && ops && trs && sms, PMG_ERR_INVALID_ARG, "bad level arrays");
-    std::vector<const pmg::LaplaceOperatorBK3 *> o(n_levels);
+    std::vector<const pmg::Operator *> o(n_levels);
     std::vector<const pmg::MGTransfer *> t(n_levels, nullptr);
     std::vector<const pmg::PreconditionChebyshev *> m(n_levels, nullptr);
     for (int l = 0; l < n_levels; ++l) {

@@ -84,6 +84,31 @@ void launch_bk3_coarse_pcg(int degree, int nq, int64_t n_cells, const T *G, cons
                            const T *pinv, int64_t n, const T *b, T *x, T *r, T *z, T *p, T *q,
                            double rel_tol, int max_it, double *info, hipStream_t s);
 
+// ---------------------------------------------------------------- SIP-DG operator
+// Face data of the DG kernel beyond Shape1D: exact derivatives of the nodal
+// (Gauss-Lobatto) basis at the Gauss points, and at the cell ends 0 and 1.
+template <typename T>
+struct DgShape {
+  T Sd[kMaxNm * kMaxNq];  // Sd[i*nq+q] = l_i'(xi_q)
+  T dl[2 * kMaxNm];       // dl[s*nm+i] = l_i'(s)
+};
+
+// y (=, or += if add) = A_DG x; see pmg_dg.hip for the face data layout
+template <typename T>
+void launch_dg_apply(int degree, int nq, bool add, const T *x, T *y, const T *G,
+                     const uint32_t *cface, const uint32_t *cnbr, const T *fdata, const T *fsig,
+                     int64_t n_cells, const Shape1D<T> &sh, const DgShape<T> &fs, hipStream_t s);
+// fdata[F][7][nq^2] from cell vertices; finfo[F] = (cell-, cell+ or invalid, face of cell-)
+template <typename T>
+void launch_dg_face_geometry(int nq, int64_t n_faces, const uint32_t *finfo, const double *cv,
+                             const double *qp, const double *qw, int metric, T *fdata,
+                             hipStream_t s);
+// DG(p) <-> CG(p): prolongate dst[dg(e)] += src[cg_map[e]], restrict
+// dst[cg_map[e]] += src[dg(e)]; dg(e) = dg_map ? dg_map[e] : e; invalid CG entries skipped
+template <typename T>
+void launch_dg_cg_transfer(bool prolongate, int64_t n_entries, const uint32_t *cg_map,
+                           const uint32_t *dg_map, const T *src, T *dst, hipStream_t s);
+
 // ---------------------------------------------------------------- transfers
 template <typename T>
 struct Transfer1D {

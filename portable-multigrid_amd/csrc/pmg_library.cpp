@@ -303,56 +303,68 @@ void LaplaceOperatorBK3::gradient_coefficient(const void *u, const void *lam, vo
                                        static_cast<const float *>(lam), static_cast<float *>(out), s);
 }
 
-void LaplaceOperatorBK3::assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const {
+void assemble_rhs_cells(int precision, int degree, int64_t n_cells, int64_t n_dofs,
+                        const double *vertices, const uint32_t *map, int f_kind, int nq, void *b,
+                        hipStream_t s) {
   PMG_REQUIRE(b, PMG_ERR_INVALID_ARG, "NULL vector");
   PMG_REQUIRE(f_kind == 0 || f_kind == 1, PMG_ERR_INVALID_ARG, "unknown right-hand side kind");
-  PMG_REQUIRE(vertices_.ptr != nullptr, PMG_ERR_INVALID_ARG,
+  PMG_REQUIRE(vertices != nullptr, PMG_ERR_INVALID_ARG,
               "assemble_rhs needs an operator created from cell vertices");
-  if (nq <= 0) nq = degree_ + 1;
+  if (nq <= 0) nq = degree + 1;
   PMG_REQUIRE(nq >= 1 && nq <= 12, PMG_ERR_INVALID_ARG, "n_q_points_1d out of range");
-  const ShapeData sd = make_shape_data(degree_, std::max(nq, degree_ + 1));
+  const ShapeData sd = make_shape_data(degree, std::max(nq, degree + 1));
   PMG_REQUIRE(sd.n_q_points_1d == nq, PMG_ERR_INVALID_ARG, "need n_q_points_1d >= degree+1");
-  DeviceArray<double> S, qp, qw, F(static_cast<size_t>(n_cells_ * nq * nq * nq));
+  DeviceArray<double> S, qp, qw, F(static_cast<size_t>(n_cells * nq * nq * nq));
   S.upload(sd.S);
   qp.upload(sd.quad.points);
   qw.upload(sd.quad.weights);
-  launch_f_at_q(f_kind, nq, n_cells_, vertices_.ptr, qp.ptr, qw.ptr, F.ptr, s);
-  if (precision_ == PMG_FP64) {
-    launch_fill<double>(n_dofs_, static_cast<double *>(b), 0.0, s);
-    launch_integrate<double>(degree_ + 1, nq, n_cells_, S.ptr, map_.ptr, F.ptr, static_cast<double *>(b), s);
+  launch_f_at_q(f_kind, nq, n_cells, vertices, qp.ptr, qw.ptr, F.ptr, s);
+  if (precision == PMG_FP64) {
+    launch_fill<double>(n_dofs, static_cast<double *>(b), 0.0, s);
+    launch_integrate<double>(degree + 1, nq, n_cells, S.ptr, map, F.ptr, static_cast<double *>(b), s);
   } else {
-    launch_fill<float>(n_dofs_, static_cast<float *>(b), 0.0f, s);
-    launch_integrate<float>(degree_ + 1, nq, n_cells_, S.ptr, map_.ptr, F.ptr, static_cast<float *>(b), s);
+    launch_fill<float>(n_dofs, static_cast<float *>(b), 0.0f, s);
+    launch_integrate<float>(degree + 1, nq, n_cells, S.ptr, map, F.ptr, static_cast<float *>(b), s);
   }
   PMG_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
 }
 
-double LaplaceOperatorBK3::l2_error(const void *u, int u_kind, int nq, hipStream_t s) const {
+double l2_error_cells(int precision, int degree, int64_t n_cells, const double *vertices,
+                      const uint32_t *map, const void *u, int u_kind, int nq, hipStream_t s) {
   PMG_REQUIRE(u, PMG_ERR_INVALID_ARG, "NULL vector");
   PMG_REQUIRE(u_kind == 0 || u_kind == 1, PMG_ERR_INVALID_ARG, "unknown solution kind");
-  PMG_REQUIRE(vertices_.ptr != nullptr, PMG_ERR_INVALID_ARG,
+  PMG_REQUIRE(vertices != nullptr, PMG_ERR_INVALID_ARG,
               "l2_error needs an operator created from cell vertices");
-  if (nq <= 0) nq = degree_ + 2;
-  PMG_REQUIRE(nq >= degree_ + 1 && nq <= 12, PMG_ERR_INVALID_ARG, "n_q_points_1d out of range");
-  const ShapeData sd = make_shape_data(degree_, nq);
+  if (nq <= 0) nq = degree + 2;
+  PMG_REQUIRE(nq >= degree + 1 && nq <= 12, PMG_ERR_INVALID_ARG, "n_q_points_1d out of range");
+  const ShapeData sd = make_shape_data(degree, nq);
   DeviceArray<double> S, qp, qw;
   S.upload(sd.S);
   qp.upload(sd.quad.points);
   qw.upload(sd.quad.weights);
   const int nb = 2048;
   DeviceArray<double> partials(nb);
-  if (precision_ == PMG_FP64)
-    launch_l2_error<double>(u_kind, degree_ + 1, nq, n_cells_, vertices_.ptr, qp.ptr, qw.ptr, S.ptr,
-                            map_.ptr, static_cast<const double *>(u), partials.ptr, nb, s);
+  if (precision == PMG_FP64)
+    launch_l2_error<double>(u_kind, degree + 1, nq, n_cells, vertices, qp.ptr, qw.ptr, S.ptr, map,
+                            static_cast<const double *>(u), partials.ptr, nb, s);
   else
-    launch_l2_error<float>(u_kind, degree_ + 1, nq, n_cells_, vertices_.ptr, qp.ptr, qw.ptr, S.ptr,
-                           map_.ptr, static_cast<const float *>(u), partials.ptr, nb, s);
+    launch_l2_error<float>(u_kind, degree + 1, nq, n_cells, vertices, qp.ptr, qw.ptr, S.ptr, map,
+                           static_cast<const float *>(u), partials.ptr, nb, s);
   std::vector<double> h(nb);
   PMG_HIP_CHECK(hipMemcpyAsync(h.data(), partials.ptr, nb * sizeof(double), hipMemcpyDeviceToHost, s));
   PMG_HIP_CHECK(hipStreamSynchronize(s));
   double e2 = 0;
   for (double v : h) e2 += v;
   return std::sqrt(e2);
+}
+
+void LaplaceOperatorBK3::assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const {
+  assemble_rhs_cells(precision_, degree_, n_cells_, n_dofs_, vertices_.ptr, map_.ptr, f_kind, nq,
+                     b, s);
+}
+
+double LaplaceOperatorBK3::l2_error(const void *u, int u_kind, int nq, hipStream_t s) const {
+  return l2_error_cells(precision_, degree_, n_cells_, vertices_.ptr, map_.ptr, u, u_kind, nq, s);
 }
 
 void LaplaceOperatorBK3::coarse_pcg(const void *b, void *x, void *r, void *z, void *p, void *q,
@@ -394,11 +406,33 @@ double LaplaceOperatorBK3::algorithmic_flops() const {
 MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
                        const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
                        int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale)
-    : precision_(precision), n_cells_(n_cells), n_dofs_c_(n_dofs_coarse),
+    : kind_(kind), precision_(precision), n_cells_(n_cells), n_dofs_c_(n_dofs_coarse),
       n_dofs_f_(n_dofs_fine) {
   check_precision(precision);
-  PMG_REQUIRE(kind == PMG_TRANSFER_H || kind == PMG_TRANSFER_P, PMG_ERR_UNSUPPORTED,
-              "transfer kind not supported");
+  PMG_REQUIRE(kind == PMG_TRANSFER_H || kind == PMG_TRANSFER_P || kind == PMG_TRANSFER_DG_CG,
+              PMG_ERR_UNSUPPORTED, "transfer kind not supported");
+  PMG_REQUIRE(n_cells >= 0 && n_dofs_coarse >= 0 && n_dofs_fine >= 0, PMG_ERR_INVALID_ARG,
+              "negative size");
+  if (kind == PMG_TRANSFER_DG_CG) {
+    // ContinuousTransfer (portable_continuous_transfer.h:18-485): CG(p) coarse,
+    // DG(p) fine on the same cells; fine map NULL = cell-major DG numbering
+    PMG_REQUIRE(pc == pf && pc >= 1 && pc <= 6, PMG_ERR_UNSUPPORTED,
+                "DG-CG transfer needs degree_coarse == degree_fine in [1,6]");
+    PMG_REQUIRE(map_coarse, PMG_ERR_INVALID_ARG, "coarse (CG) map is NULL");
+    nmc_ = nmf_ = pc + 1;
+    const int64_t n3 = int64_t(nmc_) * nmc_ * nmc_;
+    PMG_REQUIRE(map_fine || n_dofs_fine == n_cells * n3, PMG_ERR_SIZE,
+                "implicit DG numbering needs n_dofs_fine = n_cells (p+1)^3");
+    for (int64_t e = 0; e < n_cells * n3; ++e) {
+      PMG_REQUIRE(map_coarse[e] == kInvalid || int64_t(map_coarse[e]) < n_dofs_coarse,
+                  PMG_ERR_INVALID_ARG, "coarse map entry out of range");
+      PMG_REQUIRE(!map_fine || int64_t(map_fine[e]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
+                  "fine map entry out of range");
+    }
+    map_c_.upload(map_coarse, static_cast<size_t>(n_cells * n3));
+    if (map_fine) map_f_.upload(map_fine, static_cast<size_t>(n_cells * n3));
+    return;
+  }
   PMG_REQUIRE(map_coarse && map_fine && weights, PMG_ERR_INVALID_ARG, "NULL setup array");
   std::vector<double> M;
   if (kind == PMG_TRANSFER_H) {
@@ -448,6 +482,16 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
 
 void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) const {
   PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  if (kind_ == PMG_TRANSFER_DG_CG) {
+    const int64_t n = n_cells_ * nmc_ * nmc_ * nmc_;
+    if (precision_ == PMG_FP64)
+      launch_dg_cg_transfer<double>(true, n, map_c_.ptr, map_f_.ptr, static_cast<const double *>(src),
+                                    static_cast<double *>(dst), s);
+    else
+      launch_dg_cg_transfer<float>(true, n, map_c_.ptr, map_f_.ptr, static_cast<const float *>(src),
+                                   static_cast<float *>(dst), s);
+    return;
+  }
   if (precision_ == PMG_FP64)
     launch_prolongate<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
                               static_cast<const double *>(w_prolong_.data()),
@@ -460,6 +504,16 @@ void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) c
 
 void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) const {
   PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  if (kind_ == PMG_TRANSFER_DG_CG) {
+    const int64_t n = n_cells_ * nmc_ * nmc_ * nmc_;
+    if (precision_ == PMG_FP64)
+      launch_dg_cg_transfer<double>(false, n, map_c_.ptr, map_f_.ptr, static_cast<const double *>(src),
+                                    static_cast<double *>(dst), s);
+    else
+      launch_dg_cg_transfer<float>(false, n, map_c_.ptr, map_f_.ptr, static_cast<const float *>(src),
+                                   static_cast<float *>(dst), s);
+    return;
+  }
   if (precision_ == PMG_FP64)
     launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
                             static_cast<const double *>(w_restrict_.data()),
@@ -486,7 +540,7 @@ void CGWork::reinit(int precision, int64_t n) {
 
 namespace {
 template <typename T>
-CGResult solve_cg_t(const LaplaceOperatorBK3 &op, int precond, const VCycleMultigrid *mg,
+CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
                     const T *b, T *x, double rel_tol, int max_it, CGWork &w, hipStream_t s,
                     int check_every) {
   const int64_t n = op.m();
@@ -543,7 +597,7 @@ CGResult solve_cg_t(const LaplaceOperatorBK3 &op, int precond, const VCycleMulti
 }
 }  // namespace
 
-CGResult solve_cg(const LaplaceOperatorBK3 &op, int precond, const VCycleMultigrid *mg,
+CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, CGWork &work,
                   hipStream_t s, int check_every) {
   PMG_REQUIRE(b && x && b != x, PMG_ERR_INVALID_ARG, "bad vectors");
@@ -564,7 +618,7 @@ namespace {
 // IterationNumberControl(n_it, 1e-10) semantics, right-hand side (i mod 11)
 // minus its mean, zero start (deal.II internal::estimate_eigenvalues).
 template <typename T>
-void estimate_eigenvalues(const LaplaceOperatorBK3 &op, int n_it, double &mn, double &mx) {
+void estimate_eigenvalues(const Operator &op, int n_it, double &mn, double &mx) {
   const int64_t n = op.m();
   DeviceVector xv, rv, zv, pv, qv;
   xv.reinit(op.precision(), n); rv.reinit(op.precision(), n); zv.reinit(op.precision(), n);
@@ -623,7 +677,7 @@ void estimate_eigenvalues(const LaplaceOperatorBK3 &op, int n_it, double &mn, do
 }
 }  // namespace
 
-PreconditionChebyshev::PreconditionChebyshev(const LaplaceOperatorBK3 &op, int degree,
+PreconditionChebyshev::PreconditionChebyshev(const Operator &op, int degree,
                                              double smoothing_range, int eig_cg_n_iterations)
     : op_(op), degree_(degree), range_(smoothing_range) {
   PMG_REQUIRE(op.has_diagonal(), PMG_ERR_INVALID_ARG,
@@ -691,7 +745,7 @@ void PreconditionChebyshev::step(void *dst, const void *src, hipStream_t s) cons
 }
 
 // ============================================================ V-cycle
-VCycleMultigrid::VCycleMultigrid(std::vector<const LaplaceOperatorBK3 *> ops,
+VCycleMultigrid::VCycleMultigrid(std::vector<const Operator *> ops,
                                  std::vector<const MGTransfer *> transfers,
                                  std::vector<const PreconditionChebyshev *> smoothers,
                                  int coarse_kind, double coarse_tol, int outer_precision)
@@ -744,9 +798,10 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
       // FP32 arithmetic cannot reduce the residual below ~1e-7 relative
       const double tol = f64 ? coarse_tol_ : std::max(coarse_tol_, 1e-6);
       const int max_it = static_cast<int>(std::min<int64_t>(100000, 4 * n + 100));
-      if (ops_[0]->n_cells() <= kCoarseSingleWorkgroupCells) {
+      const auto *bk3 = dynamic_cast<const LaplaceOperatorBK3 *>(ops_[0]);
+      if (bk3 && bk3->n_cells() <= kCoarseSingleWorkgroupCells) {
         CGWork &w = coarse_work_;
-        ops_[0]->coarse_pcg(def_[0].data(), sol_[0].data(), w.r.data(), w.z.data(), w.p.data(),
+        bk3->coarse_pcg(def_[0].data(), sol_[0].data(), w.r.data(), w.z.data(), w.p.data(),
                             w.q.data(), tol, max_it, w.sc.ptr + 4, s);
       } else {
         solve_cg(*ops_[0], PMG_PRECOND_JACOBI, nullptr, def_[0].data(), sol_[0].data(), tol,

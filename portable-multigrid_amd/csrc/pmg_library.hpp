@@ -27,31 +27,63 @@ struct DeviceVector {
   const void *data() const { return raw.ptr; }
 };
 
-// Continuous Q_p Laplace operator (LaplaceOperatorBK3 / LaplaceOperatorBase).
-class LaplaceOperatorBK3 {
+// Right-hand side b = int phi_i f and L2 error over cells given by their
+// vertices (device [C][8][3]) and DoF map (device [C][(p+1)^3]).
+void assemble_rhs_cells(int precision, int degree, int64_t n_cells, int64_t n_dofs,
+                        const double *vertices, const uint32_t *map, int f_kind, int nq, void *b,
+                        hipStream_t s);
+double l2_error_cells(int precision, int degree, int64_t n_cells, const double *vertices,
+                      const uint32_t *map, const void *u, int u_kind, int nq, hipStream_t s);
+
+// Level operator interface (Portable::LaplaceOperatorBase,
+// include/base/portable_laplace_operator_base.h:16-61): what the Chebyshev
+// smoother, the V-cycle and CG need from an operator.
+class Operator {
+ public:
+  virtual ~Operator() = default;
+  // dst = A src (constrained rows: dst_c = src_c)
+  virtual void vmult(void *dst, const void *src, hipStream_t s) const = 0;
+  // dst += (cell/face part of) A src, no constrained-row handling
+  virtual void apply_add(void *dst, const void *src, hipStream_t s) const = 0;
+  virtual void compute_diagonal() = 0;
+  virtual const void *inverse_diagonal() const = 0;
+  virtual bool has_diagonal() const = 0;
+  // b = int phi_i f (f kind 0: 1, 1: 3 pi^2 sin sin sin), Gauss(nq), constrained rows 0
+  virtual void assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const = 0;
+  // ||u_h - u||_L2 with Gauss(nq) (u kind 0: 0, 1: sin sin sin); synchronous
+  virtual double l2_error(const void *u, int u_kind, int nq, hipStream_t s) const = 0;
+  virtual int64_t m() const = 0;
+  virtual int degree() const = 0;
+  virtual int n_q_points_1d() const = 0;
+  virtual int precision() const = 0;
+  virtual int64_t n_cells() const = 0;
+  virtual double algorithmic_bytes() const = 0;
+  virtual double algorithmic_flops() const = 0;
+};
+
+// Continuous Q_p Laplace operator (LaplaceOperatorBK3).
+class LaplaceOperatorBK3 : public Operator {
  public:
   LaplaceOperatorBK3(int degree, int nq, int precision, int64_t n_cells, const uint32_t *dofmap,
                      int64_t n_dofs, const uint32_t *constrained, int64_t n_constrained,
                      const double *cell_vertices, const double *G);
-  void vmult(void *dst, const void *src, hipStream_t s) const;
-  void apply_add(void *dst, const void *src, hipStream_t s) const;
-  void compute_diagonal();
-  const void *inverse_diagonal() const;
+  void vmult(void *dst, const void *src, hipStream_t s) const override;
+  void apply_add(void *dst, const void *src, hipStream_t s) const override;
+  void compute_diagonal() override;
+  const void *inverse_diagonal() const override;
   void set_coefficient(const double *kappa, hipStream_t s);
   void gradient_coefficient(const void *u, const void *lam, void *out, hipStream_t s) const;
-  // b = int phi_i f (f kind 0: 1, 1: 3 pi^2 sin sin sin), Gauss(nq), constrained rows 0
-  void assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const;
-  // ||u_h - u||_L2 with Gauss(nq) (u kind 0: 0, 1: sin sin sin); synchronous
-  double l2_error(const void *u, int u_kind, int nq, hipStream_t s) const;
+  void assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const override;
+  double l2_error(const void *u, int u_kind, int nq, hipStream_t s) const override;
 
-  int64_t m() const { return n_dofs_; }
-  int degree() const { return degree_; }
-  int n_q_points_1d() const { return nq_; }
-  int precision() const { return precision_; }
-  int64_t n_cells() const { return n_cells_; }
-  double algorithmic_bytes() const;
-  double algorithmic_flops() const;
-  bool has_diagonal() const { return has_diag_; }
+  int64_t m() const override { return n_dofs_; }
+  int degree() const override { return degree_; }
+  int n_q_points_1d() const override { return nq_; }
+  int precision() const override { return precision_; }
+  int64_t n_cells() const override { return n_cells_; }
+  double algorithmic_bytes() const override;
+  double algorithmic_flops() const override;
+  bool has_diagonal() const override { return has_diag_; }
   // Jacobi-PCG solved inside one workgroup (for small coarse levels); x is the
   // initial guess; r,z,p,q work vectors of m() entries; info = 3 device doubles
   void coarse_pcg(const void *b, void *x, void *r, void *z, void *p, void *q, double rel_tol,
@@ -86,6 +118,58 @@ class LaplaceOperatorBK3 {
   const Bk3Schedule &schedule() const { return sched_; }
 };
 
+// Symmetric interior penalty DG Laplacian (LaplaceOperatorDG,
+// include/operators/portable_laplace_operator_dg.h:20-151) on trilinear cells
+// with cell-major FE_DGQ numbering.  inner_faces[F] = (cell-, cell+, face-,
+// face+) with face+ = face- ^ 1 (standard orientation); boundary_faces[F] =
+// (cell, face, boundary_id), id 0 weak homogeneous Dirichlet, others
+// homogeneous Neumann.  penalty: NULL (computed from penalty_kind, 0:
+// p(p+1)/2 (1/h- + 1/h+), 1: (p+1)^2 (1/h- + 1/h+), h = |K|/|F|) or one value
+// per face, inner faces first.  metric 0: d_n = (J^{-1} n).grad_xi, 1: (J^{-T} n).
+class LaplaceOperatorDG : public Operator {
+ public:
+  LaplaceOperatorDG(int degree, int nq, int precision, int64_t n_cells,
+                    const double *cell_vertices, int64_t n_inner, const uint32_t *inner_faces,
+                    int64_t n_boundary, const uint32_t *boundary_faces, const double *penalty,
+                    int penalty_kind, int metric);
+  void vmult(void *dst, const void *src, hipStream_t s) const override;
+  void apply_add(void *dst, const void *src, hipStream_t s) const override;
+  // cell-integral diagonal only, as LaplaceOperatorDG::compute_diagonal
+  // (portable_laplace_operator_dg.h:887-915)
+  void compute_diagonal() override;
+  const void *inverse_diagonal() const override;
+  bool has_diagonal() const override { return has_diag_; }
+  void assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const override;
+  double l2_error(const void *u, int u_kind, int nq, hipStream_t s) const override;
+  int64_t m() const override { return n_cells_ * nm3_; }
+  int degree() const override { return degree_; }
+  int n_q_points_1d() const override { return nq_; }
+  int precision() const override { return precision_; }
+  int64_t n_cells() const override { return n_cells_; }
+  int64_t n_faces() const { return n_faces_; }
+  double algorithmic_bytes() const override;
+  double algorithmic_flops() const override;
+  // penalty per face (host copy, inner faces first)
+  const std::vector<double> &penalty() const { return sigma_; }
+
+ private:
+  template <typename T> void apply_t(bool add, T *dst, const T *src, hipStream_t s) const;
+  const uint32_t *identity_map() const;
+  int degree_, nq_, precision_;
+  int64_t n_cells_, nm3_, n_faces_ = 0;
+  ShapeData shape_;
+  Shape1D<double> sh64_{}, shd64_{};
+  Shape1D<float> sh32_{}, shd32_{};
+  DgShape<double> fs64_{};
+  DgShape<float> fs32_{};
+  std::vector<double> sigma_;
+  DeviceArray<uint32_t> cface_, cnbr_;
+  DeviceArray<double> vertices_;
+  DeviceVector G_, fdata_, fsig_, inv_diag_;
+  mutable DeviceArray<uint32_t> ident_;
+  bool has_diag_ = false;
+};
+
 class MGTransfer {
  public:
   MGTransfer(int kind, int precision, int degree_coarse, int degree_fine, int64_t n_cells,
@@ -98,7 +182,7 @@ class MGTransfer {
   int64_t n_dofs_fine() const { return n_dofs_f_; }
 
  private:
-  int precision_, nmc_, nmf_;
+  int kind_, precision_, nmc_, nmf_;
   int64_t n_cells_, n_dofs_c_, n_dofs_f_;
   Transfer1D<double> M64_{};
   Transfer1D<float> M32_{};
@@ -108,18 +192,18 @@ class MGTransfer {
 
 class PreconditionChebyshev {
  public:
-  PreconditionChebyshev(const LaplaceOperatorBK3 &op, int degree, double smoothing_range,
+  PreconditionChebyshev(const Operator &op, int degree, double smoothing_range,
                         int eig_cg_n_iterations);
   void vmult(void *dst, const void *src, hipStream_t s) const;
   void step(void *dst, const void *src, hipStream_t s) const;
   double max_eigenvalue() const { return max_ev_; }
   double min_eigenvalue() const { return min_ev_; }
   int degree() const { return degree_; }
-  const LaplaceOperatorBK3 &op() const { return op_; }
+  const Operator &op() const { return op_; }
 
  private:
   void run(void *dst, const void *src, bool step, hipStream_t s) const;
-  const LaplaceOperatorBK3 &op_;
+  const Operator &op_;
   int degree_;
   double range_, max_ev_ = 1, min_ev_ = 1, theta_ = 1, delta_ = 0;
   mutable DeviceVector tmp_, old_;
@@ -141,7 +225,7 @@ struct CGResult {
 
 class VCycleMultigrid;
 
-CGResult solve_cg(const LaplaceOperatorBK3 &op, int precond, const VCycleMultigrid *mg,
+CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, CGWork &work,
                   hipStream_t s, int check_every = 1);
 
@@ -150,7 +234,7 @@ constexpr int64_t kCoarseSingleWorkgroupCells = 4096;
 
 class VCycleMultigrid {
  public:
-  VCycleMultigrid(std::vector<const LaplaceOperatorBK3 *> ops,
+  VCycleMultigrid(std::vector<const Operator *> ops,
                   std::vector<const MGTransfer *> transfers,
                   std::vector<const PreconditionChebyshev *> smoothers, int coarse_kind,
                   double coarse_tol, int outer_precision);
@@ -160,7 +244,7 @@ class VCycleMultigrid {
 
  private:
   void v_cycle(int level, hipStream_t s) const;
-  std::vector<const LaplaceOperatorBK3 *> ops_;
+  std::vector<const Operator *> ops_;
   std::vector<const MGTransfer *> xfer_;
   std::vector<const PreconditionChebyshev *> sm_;
   int coarse_kind_, outer_precision_, precision_;

@@ -8,6 +8,8 @@
 // Deformation 1: x += a sin(2 pi y) sin(2 pi z), y += a sin(2 pi z) sin(2 pi x),
 // z += a sin(2 pi x) sin(2 pi y), evaluated at the undeformed vertex positions;
 // cells stay trilinear (straight edges between displaced vertices).
+// Deformation 2: the affine shear x += a y, y += a z, z += a x (parallelepiped
+// cells whose Jacobian is not symmetric).
 #include <cmath>
 #include <vector>
 
@@ -25,7 +27,7 @@ void check_box(const pmg_box *b) {
   PMG_REQUIRE(b != nullptr, PMG_ERR_INVALID_ARG, "box is NULL");
   PMG_REQUIRE(b->nx > 0 && b->ny > 0 && b->nz > 0, PMG_ERR_INVALID_ARG,
               "box needs at least one cell per direction");
-  PMG_REQUIRE(b->deform == 0 || b->deform == 1, PMG_ERR_INVALID_ARG, "unknown deformation");
+  PMG_REQUIRE(b->deform >= 0 && b->deform <= 2, PMG_ERR_INVALID_ARG, "unknown deformation");
 }
 
 void vertex(const pmg_box *b, int a, int c1, int c2, double *X) {
@@ -40,6 +42,11 @@ void vertex(const pmg_box *b, int a, int c1, int c2, double *X) {
     X[0] += A * std::sin(w * y) * std::sin(w * z);
     X[1] += A * std::sin(w * z) * std::sin(w * x);
     X[2] += A * std::sin(w * x) * std::sin(w * y);
+  } else if (b->deform == 2) {
+    const double A = b->amplitude;
+    X[0] += A * y;
+    X[1] += A * z;
+    X[2] += A * x;
   }
 }
 
@@ -184,6 +191,52 @@ int pmg_box_p_transfer_maps(const pmg_box *b, int pc, int pf, uint32_t *map_c, u
     PMG_REQUIRE(pmg_box_dofmap(b, pf, 0, map_f) == PMG_OK, PMG_ERR_INVALID_ARG, "fine map failed");
     const int64_t nf3 = int64_t(pf + 1) * (pf + 1) * (pf + 1);
     patch_weights(pmg_box_n_cells(b), nf3, map_f, Grid(b, pf), w);
+  });
+}
+
+int pmg_box_dg_faces(const pmg_box *b, uint32_t *inner, int64_t *n_inner, uint32_t *bdry,
+                     int64_t *n_bdry) {
+  return guard([&] {
+    check_box(b);
+    PMG_REQUIRE(n_inner && n_bdry, PMG_ERR_INVALID_ARG, "NULL count");
+    const int64_t n[3] = {b->nx, b->ny, b->nz};
+    PMG_REQUIRE(n[0] * n[1] * n[2] < int64_t(0xFFFFFFFFu), PMG_ERR_SIZE, "too many cells");
+    auto cell = [&](int64_t i, int64_t j, int64_t k) { return uint32_t(i + n[0] * (j + n[1] * k)); };
+    int64_t fi = 0, fb = 0;
+    for (int d = 0; d < 3; ++d)
+      for (int64_t k = 0; k < n[2]; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+          for (int64_t i = 0; i < n[0]; ++i) {
+            const int64_t c[3] = {i, j, k};
+            if (c[d] == 0) {  // lower boundary face of the first cell layer
+              if (bdry) {
+                bdry[3 * fb] = cell(i, j, k);
+                bdry[3 * fb + 1] = uint32_t(2 * d);
+                bdry[3 * fb + 2] = 0;
+              }
+              ++fb;
+            }
+            if (c[d] + 1 < n[d]) {
+              if (inner) {
+                int64_t e[3] = {i, j, k};
+                ++e[d];
+                inner[4 * fi] = cell(i, j, k);
+                inner[4 * fi + 1] = cell(e[0], e[1], e[2]);
+                inner[4 * fi + 2] = uint32_t(2 * d + 1);
+                inner[4 * fi + 3] = uint32_t(2 * d);
+              }
+              ++fi;
+            } else {
+              if (bdry) {
+                bdry[3 * fb] = cell(i, j, k);
+                bdry[3 * fb + 1] = uint32_t(2 * d + 1);
+                bdry[3 * fb + 2] = 0;
+              }
+              ++fb;
+            }
+          }
+    *n_inner = fi;
+    *n_bdry = fb;
   });
 }
 
