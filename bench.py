@@ -18,7 +18,14 @@ Also reported on the same problem:
   cpu_baseline  the CPU oracle's vmult (plain-C restatement of the reference
                 kernel, OpenMP over 8 cell colours) on the host cores.
 
+  configs       the other workloads on their own sizes: BK3 Q4 16^3 Gauss-6
+                vmult + Jacobi-PCG to 1e-8; SIP-DG Q3 on 32^3 deformed cells
+                (vmult roofline, DG MG-CG FP64 and mixed); ph-MG Q6 on 48^3
+                deformed cells (FP64 vs FP32 V-cycle); adjoint gradient 32^3 Q4
+                with a 20-direction finite-difference check.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+                       [--no-configs]
 """
 import argparse
 import importlib.util
@@ -147,6 +154,159 @@ def cpu_baseline(n_cells_dir, degree, nq, seconds):
             "single_thread_value": single, "single_thread_sample": f"{n1} vmults, {t1:.1f}s"}
 
 
+def solve_timed(pmg, outer, b, mg, precond, tol, reps=2, max_it=500):
+    """MG- or Jacobi-preconditioned CG from x = 0; the last of `reps` solves is
+    reported (the first warms code objects and caches)."""
+    import torch
+    for _ in range(reps):
+        x = outer.initialize_dof_vector()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its, r0, r = pmg.solve_cg(outer, b, x, precond, mg, tol, max_it)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+    return {"time_s": t, "iterations": its, "rel_residual": r / r0}, x
+
+
+def time_events(fn, n):
+    import torch
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(n):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / n
+
+
+def bench_config1(pmg):
+    """Plain BK3 Poisson: 16^3 affine cells, Q4, Gauss 6/dir; 100 operator
+    applications (best of 5), then Jacobi-PCG to 1e-8 with a seeded U[-1,1]
+    right-hand side (constrained entries zeroed)."""
+    import numpy as np
+    import torch
+    b = pmg.box(16)
+    op = pmg.LaplaceOperatorBK3(b, 4, 6, pmg.FP64)
+    op.compute_diagonal()
+    x = torch.rand(op.m(), dtype=torch.float64, device="cuda") * 2 - 1
+    y = op.initialize_dof_vector()
+    t = min(time_events(lambda: op.vmult(y, x), 100) for _ in range(5))
+    by, fl = op.algorithmic_bytes()
+    rhs = np.random.Generator(np.random.PCG64(1)).uniform(-1, 1, op.m())
+    rhs[pmg.box_boundary_dofs(b, 4)] = 0.0
+    r, _ = solve_timed(pmg, op, torch.tensor(rhs, device="cuda"), None, pmg.PRECOND_JACOBI, 1e-8,
+                       max_it=5000)
+    return {"workload": "BK3 Q4, unit cube 16^3 affine cells, Gauss 6/dir, FP64", "dofs": op.m(),
+            "vmult_gdofs": op.m() / t / 1e9, "vmult_us": t * 1e6,
+            "achieved_gbs": by / t / 1e9, "hbm_frac": by / t / 1e9 / HBM_PEAK_GBS,
+            "note": "47 MB working set: MALL-resident, not an HBM measurement",
+            "jacobi_pcg_1e-8": r}
+
+
+def bench_config3(pmg):
+    """SIP-DG: 32^3 cells with the sinusoidal deformation (amplitude 0.05),
+    DG Q3, Gauss 4/dir, penalty 2(p+1)^2/h (kind 1); 100 vmults; MG-CG to
+    1e-10 (DG(3) -> CG(3) -> CG(1) h-levels 32^3..2^3), FP64 and FP32 V-cycle."""
+    import torch
+    fine = pmg.box(32, deform=1, amplitude=0.05)
+    op = pmg.LaplaceOperatorDG(fine, 3, 4, pmg.FP64, penalty_kind=1)
+    x = torch.rand(op.m(), dtype=torch.float64, device="cuda") * 2 - 1
+    y = op.initialize_dof_vector()
+    t = min(time_events(lambda: op.vmult(y, x), 100) for _ in range(3))
+    by, fl = op.algorithmic_bytes()
+    out = {"workload": "SIP-DG Q3, 32^3 deformed cells, Gauss 4/dir, penalty (p+1)^2(1/h-+1/h+)",
+           "dofs": op.m(), "vmult_gdofs": op.m() / t / 1e9, "vmult_us": t * 1e6,
+           "roofline": {"bound": "hbm", "achieved": by / t / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / t / 1e9 / HBM_PEAK_GBS,
+                        "algorithmic_bytes": by, "kernel": "dg_apply_kernel<4, 4, double, false>",
+                        "fp64_tflops": fl / t / 1e12}}
+    del x, y
+    levels = pmg.ph_levels(fine, 3, 5, dg=True)
+    rhs = op.assemble_rhs(0)
+    mg, ops, _, _ = pmg.build_hierarchy(levels, dg_penalty_kind=1)
+    r64, x64 = solve_timed(pmg, ops[-1], rhs, mg, pmg.PRECOND_MG, 1e-10)
+    del mg, ops
+    mg32, ops32, _, _ = pmg.build_hierarchy(levels, precision=pmg.FP32,
+                                            outer_precision=pmg.FP64, dg_penalty_kind=1)
+    r32, x32 = solve_timed(pmg, op, rhs, mg32, pmg.PRECOND_MG, 1e-10)
+    r32["rel_diff_vs_fp64_solution"] = ((x32 - x64).abs().max() / x64.abs().max()).item()
+    out["mgcg"] = {"fp64": r64, "fp32_vcycle_fp64_cg": r32, "levels": len(levels),
+                   "speedup_mixed": r64["time_s"] / r32["time_s"], "rhs": "f = 1"}
+    return out
+
+
+def bench_config4(pmg):
+    """ph-multigrid: 48^3 cells with the sinusoidal deformation, continuous Q6,
+    p-levels 6 -> 3 -> 1, then h-levels 48^3 .. 3^3 at Q1; FP64 V-cycle vs FP32
+    V-cycle inside FP64 CG, to 1e-10, f = 3 pi^2 sin sin sin."""
+    fine = pmg.box(48, deform=1, amplitude=0.05)
+    levels = pmg.ph_levels(fine, 6, 5, p_sequence=[6, 3, 1])
+    t0 = time.perf_counter()
+    mg, ops, _, _ = pmg.build_hierarchy(levels)
+    setup = time.perf_counter() - t0
+    outer = ops[-1]
+    rhs = outer.assemble_rhs(1)
+    r64, x64 = solve_timed(pmg, outer, rhs, mg, pmg.PRECOND_MG, 1e-10)
+    r64["setup_s"] = setup
+    r64["l2_error_vs_sin"] = outer.l2_error(x64, 1)
+    del mg, ops[:-1]
+    t0 = time.perf_counter()
+    mg32, ops32, _, _ = pmg.build_hierarchy(levels, precision=pmg.FP32, outer_precision=pmg.FP64)
+    setup = time.perf_counter() - t0
+    r32, x32 = solve_timed(pmg, outer, rhs, mg32, pmg.PRECOND_MG, 1e-10)
+    r32["setup_s"] = setup
+    r32["l2_error_vs_sin"] = outer.l2_error(x32, 1)
+    r32["rel_diff_vs_fp64_solution"] = ((x32 - x64).abs().max() / x64.abs().max()).item()
+    return {"workload": "ph-MG Q6, 48^3 deformed cells, p 6->3->1, h 48->3 at Q1",
+            "dofs": outer.m(), "levels": len(levels), "fp64": r64, "fp32_vcycle_fp64_cg": r32,
+            "speedup_mixed": r64["time_s"] / r32["time_s"],
+            "note": "the deformation moves the boundary, so sin sin sin is not the exact "
+                    "solution there: l2_error_vs_sin is a fixed offset, the FP64/mixed "
+                    "difference is the precision figure"}
+
+
+def bench_config5(pmg, n_dir=20):
+    """Adjoint gradient: -div(exp(g) grad u) = 1, 32^3 cells, Q4, FP32 V-cycle in
+    FP64 CG (solves to 1e-12), g = 16 random Fourier modes (seed 5), u_d = solution
+    for g = 0, J = 1/2 |u - u_d|^2; dJ/dg at the 5^3 Gauss points per cell,
+    checked on n_dir random directions by central differences (step 1e-6)."""
+    import numpy as np
+    import torch
+    gf = pmg.random_fourier_field(5)
+    prob = pmg.CoefficientPoisson(pmg.box(32), 4, 5, gf, pmg.FP32, rel_tol=1e-12)
+    prob.set_g(torch.zeros(len(prob.X), dtype=torch.float64, device="cuda"))
+    u_d = prob.solve()
+    g0 = torch.tensor(gf(prob.X), device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prob.set_g(g0)
+    u = prob.solve()
+    lam = prob.solve((u - u_d).contiguous())
+    grad = prob.gradient(u, lam)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    t_kernel = time_events(lambda: prob.gradient(u, lam), 10)
+
+    def J(gv):
+        prob.set_g(gv)
+        return 0.5 * float(torch.sum((prob.solve() - u_d) ** 2))
+
+    h, errs = 1e-6, []
+    rng = np.random.Generator(np.random.PCG64(7))
+    for _ in range(n_dir):
+        d = torch.tensor(rng.standard_normal(g0.numel()), device="cuda")
+        fd = (J(g0 + h * d) - J(g0 - h * d)) / (2 * h)
+        errs.append(abs(fd - float(torch.dot(grad, d))) / abs(fd))
+    return {"workload": "adjoint dJ/dlog(kappa), 32^3 Q4, 5^3 Gauss points/cell, FP32-MG/FP64-CG",
+            "dofs": prob.op.m(), "gradient_points": g0.numel(),
+            "forward_adjoint_gradient_s": t_grad, "gradient_kernel_us": t_kernel * 1e6,
+            "solve_iterations": prob.iterations[1:4],
+            "fd_directions": n_dir, "fd_step": h, "fd_max_rel_err": max(errs),
+            "fd_median_rel_err": float(np.median(errs))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +321,11 @@ def main():
     ap.add_argument("--gload", type=int, default=None, help="BK3 G-load policy 0/1/2")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (Jacobi-PCG 16^3, SIP-DG, ph-MG Q6, adjoint)")
+    ap.add_argument("--config", action="append", default=None,
+                    help="run only these configuration legs (repeatable): bk3_q4_16cube_jacobi_pcg, "
+                         "sipdg_q3_32cube_deformed, ph_mg_q6_48cube_deformed, adjoint_q4_32cube")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -243,6 +408,24 @@ def main():
 
     del op, x
     torch.cuda.synchronize()
+    if not args.no_configs:
+        extra = {}
+        for name, fn in [("bk3_q4_16cube_jacobi_pcg", bench_config1),
+                         ("sipdg_q3_32cube_deformed", bench_config3),
+                         ("ph_mg_q6_48cube_deformed", bench_config4),
+                         ("adjoint_q4_32cube", bench_config5)]:
+            if args.config and name not in args.config:
+                continue
+            log(f"[bench] configuration {name}")
+            t0 = time.perf_counter()
+            try:
+                extra[name] = fn(pmg)
+            except (RuntimeError, ValueError) as e:  # a failed leg is reported, not hidden
+                extra[name] = {"error": str(e)}
+            extra[name]["wall_s"] = time.perf_counter() - t0
+            log(f"[bench] {name}: {json.dumps(extra[name])}")
+            torch.cuda.empty_cache()
+        out["configs"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, degree, nq, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -13,8 +13,10 @@
  *  - Vectors are caller-owned DEVICE pointers (hipMalloc / torch.cuda) of the
  *    handle's precision (double for PMG_FP64, float for PMG_FP32), length m().
  *  - Setup arrays (DoF maps, vertices, weights, G tensors) are HOST pointers;
- *    the library copies them to device memory it owns.  Apply calls allocate
- *    nothing.
+ *    the library copies them to device memory it owns.  Operator, transfer,
+ *    smoother and V-cycle applications allocate nothing; pmg_cg_solve,
+ *    pmg_vector_dot and the set-up/post-processing calls (create,
+ *    compute_diagonal, assemble_rhs, l2_error) allocate temporaries.
  *  - stream: a hipStream_t (NULL = the default stream).  All work is
  *    stream-ordered and asynchronous unless stated otherwise.  A handle must
  *    not be used from two host threads at once.

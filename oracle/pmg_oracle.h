@@ -12,7 +12,7 @@
  * vectors and cannot be built here: it needs deal.II 9.8, Kokkos, p4est):
  *   1D data           closed forms (quadrature exactness, partition of unity,
  *                     D*1 = 0, GLL endpoint/symmetry)          tests/test_oracle_basis.py
- *   G tensors         closed forms on affine boxes, volume = sum JxW       test_oracle_geometry.py
+ *   G tensors         closed forms on affine boxes, volume = sum JxW       test_oracle_bk3.py
  *   BK3 cell apply    dense element matrices with exact Lagrange
  *                     derivatives, Q1 27-point stencil, x^T A x = int |grad u|^2
  *                                                             test_oracle_bk3.py
@@ -23,7 +23,9 @@
  *                     CG on a closed-form SPD system           test_oracle_solver.py
  *   SIP-DG            reduces to the continuous operator for smooth u,
  *                     symmetry, J^{-1}n vs J^{-T}n sheared cell test_oracle_dg.py
- *   adjoint gradient  central finite differences               test_oracle_adjoint.py
+ *   adjoint gradient  difference of the bilinear form under a kappa change,
+ *                     central finite differences of J          test_oracle_bk3.py,
+ *                                                             test_oracle_adjoint.py
  * Parity unpinned (deal.II internals no reference test pins): the
  * eigenvalue-estimate details of deal.II PreconditionChebyshev (initial
  * vector, stopping rule), see pmo_cheby_setup.

@@ -5,8 +5,10 @@ C ABI in include/pmg.h).  The classes mirror the reference's interfaces
 (gshubhamk/portable-multigrid):
 
   LaplaceOperatorBK3   include/operators/portable_laplace_operator_bk3.h
+  LaplaceOperatorDG    include/operators/portable_laplace_operator_dg.h
   GeometricTransfer    include/multigrid/portable_geometric_transfer.h
   PolynomialTransfer   include/multigrid/portable_polynomial_transfer.h
+  ContinuousTransfer   include/multigrid/portable_continuous_transfer.h
   PreconditionChebyshev  deal.II PreconditionChebyshev as configured in
                        tests/poisson_cube_bk3/program.cc:474-515
   VCycleMultigrid      include/multigrid/portable_v_cycle_multigrid.h
@@ -557,7 +559,7 @@ def vector_dot(a, b, precision=FP64):
 def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=3,
                     smoothing_range=15.0, eig_cg_n_iterations=10, coarse=COARSE_PCG,
                     coarse_tol=1e-12, prolongation_scale=1.0, outer_precision=None,
-                    dg_penalty_kind=0, dg_metric=0):
+                    dg_penalty_kind=0, dg_metric=0, coefficient=None):
     """V-cycle over `levels` = [(box, degree) or (box, degree, "dg"), ...]
     from coarsest to finest.
 
@@ -566,7 +568,10 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
     whose box is refined once by a GeometricTransfer (same degree), as the
     reference's ph hierarchies (tests/poisson_cube_bk3/program.cc:281-351,
     421-470; tests/poisson_cube_bk3_fp64_dg/program.cc:294-460).
-    n_q_points_1d: None -> degree+1 per level, or a callable degree -> nq."""
+    n_q_points_1d: None -> degree+1 per level, or a callable degree -> nq.
+    coefficient: None, or a callable (box, n_q_points_1d) -> CUDA float64
+    tensor kappa [n_cells * nq^3] applied to each continuous level before its
+    diagonal and smoother are set up."""
     ops, trs, sms = [], [None], [None]
     for l, lev in enumerate(levels):
         b, p = lev[0], lev[1]
@@ -577,6 +582,8 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
                                    metric=dg_metric)
         else:
             op = LaplaceOperatorBK3(b, p, nq, precision)
+            if coefficient is not None:
+                op.set_coefficient(coefficient(b, op.nq))
         op.compute_diagonal()
         ops.append(op)
         if l == 0:
@@ -602,14 +609,18 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
     return mg, ops, trs, sms
 
 
-def ph_levels(fine_box, degree, n_h_levels, dg=False):
+def ph_levels(fine_box, degree, n_h_levels, dg=False, p_sequence=None):
     """Level list of the reference's ph/DG drivers: Q1 on fine_box coarsened
     n_h_levels-1 times up to fine_box, then degrees 1 -> ... -> p on fine_box
-    with p_levels = p, max(p-2,1), ... (tests/poisson_cube_bk3_fp64_dg/program.cc:294-300),
-    then (dg) DG(p) on fine_box."""
+    with p_levels = p, max(p-2,1), ... (tests/poisson_cube_bk3_fp64_dg/program.cc:294-300)
+    or the given descending p_sequence (e.g. [6, 3, 1]), then (dg) DG(p) on
+    fine_box."""
     boxes = coarsen_boxes(fine_box, n_h_levels)
     levels = [(b, 1) for b in boxes]
     ps = [degree]
+    if p_sequence is not None:
+        assert p_sequence[0] == degree and p_sequence[-1] == 1, "p_sequence runs from p down to 1"
+        ps = list(p_sequence)
     while ps[-1] > 1:
         ps.append(max(ps[-1] - 2, 1))
     for p in reversed(ps[:-1]):
@@ -617,6 +628,94 @@ def ph_levels(fine_box, degree, n_h_levels, dg=False):
     if dg:
         levels.append((fine_box, degree, "dg"))
     return levels
+
+
+def random_fourier_field(seed, n_modes=16, kmax=4):
+    """Synthetic smooth field g(x) = sum_m a_m cos(2 pi k_m . x + phi_m): integer
+    wave vectors k_m with max-norm 1..kmax, a_m ~ N(0, 0.5/|k_m|^2), phases
+    uniform (numpy PCG64 with the seed).  Returns a callable X[n,3] -> g[n]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    modes = []
+    for _ in range(n_modes):
+        while True:
+            k = rng.integers(-kmax, kmax + 1, 3)
+            if np.abs(k).max() >= 1:
+                break
+        modes.append((k, rng.normal(0.0, np.sqrt(0.5 / float(k @ k))), rng.uniform(0, 2 * np.pi)))
+
+    def g(X):
+        X = np.asarray(X, dtype=np.float64)
+        out = np.zeros(X.shape[0])
+        for k, a, phi in modes:
+            out += a * np.cos(2 * np.pi * (X @ k) + phi)
+        return out
+
+    return g
+
+
+def quadrature_points(b, n_q_points_1d):
+    """Physical Gauss points of the box's trilinear cells, [n_cells * nq^3, 3]
+    in caller cell order, q-points lexicographic (x fastest) -- the layout of
+    kappa for LaplaceOperatorBK3.set_coefficient (host-side set-up helper)."""
+    nq = n_q_points_1d
+    x, _ = np.polynomial.legendre.leggauss(nq)
+    x = 0.5 * (x + 1.0)
+    cv = box_cell_vertices(b).reshape(-1, 8, 3)
+    xi = np.stack(np.meshgrid(x, x, x, indexing="ij"), -1)[..., ::-1].reshape(-1, 3)  # [q][xyz]
+    N = np.empty((xi.shape[0], 8))
+    for v in range(8):
+        f = [xi[:, d] if (v >> d) & 1 else 1.0 - xi[:, d] for d in range(3)]
+        N[:, v] = f[0] * f[1] * f[2]
+    return np.einsum("qv,cvd->cqd", N, cv).reshape(-1, 3)
+
+
+class CoefficientPoisson:
+    """-div(kappa grad u) = f with kappa = exp(g) on a box: MG-CG forward and
+    adjoint solves and the gradient dJ/dg at the quadrature points.
+
+    The outer FP64 operator carries kappa = exp(g_q) at the fine Gauss points
+    (any g_q); the V-cycle preconditioner (h-levels 2^k..fine, Chebyshev,
+    coarse PCG, levels in `level_precision`) carries exp(g_smooth(x))
+    evaluated at every level's Gauss points and stays fixed, so perturbing
+    g_q needs no re-setup.  The gradient of J(u) for dJ/du = w is
+    -kappa_q (grad u)^T G_q (grad lambda) with A lambda = w (A symmetric)."""
+
+    def __init__(self, fine_box, degree, n_levels, g_smooth, level_precision=FP32,
+                 n_q_points_1d=None, f_kind=0, rel_tol=1e-10, max_it=500):
+        import torch
+        self.box, self.degree = fine_box, degree
+        self.nq = degree + 1 if n_q_points_1d is None else n_q_points_1d
+        self.rel_tol, self.max_it = rel_tol, max_it
+
+        def kappa_of(b, nq):
+            return torch.tensor(np.exp(g_smooth(quadrature_points(b, nq))), device="cuda")
+
+        self.mg, self.levels, _, _, _ = geometric_hierarchy(
+            fine_box, degree, n_levels, self.nq, level_precision, outer_precision=FP64,
+            coefficient=kappa_of)
+        self.op = LaplaceOperatorBK3(fine_box, degree, self.nq, FP64)
+        self.X = quadrature_points(fine_box, self.nq)
+        self.b = self.op.assemble_rhs(f_kind)
+        self.iterations = []
+
+    def set_g(self, g_q):
+        """g_q: CUDA float64 tensor [n_cells * nq^3]; kappa = exp(g_q)."""
+        import torch
+        self.op.set_coefficient(torch.exp(g_q).contiguous())
+
+    def solve(self, rhs=None):
+        x = self.op.initialize_dof_vector()
+        its, _, _ = solve_cg(self.op, self.b if rhs is None else rhs, x, PRECOND_MG, self.mg,
+                             self.rel_tol, self.max_it)
+        self.iterations.append(its)
+        return x
+
+    def gradient(self, u, lam):
+        """dJ/dg_q for the current kappa (forward u, adjoint lambda)."""
+        import torch
+        out = torch.zeros(self.op.n_cells * self.nq ** 3, dtype=torch.float64, device="cuda")
+        self.op.gradient_coefficient(u, lam, out)
+        return out
 
 
 def coarsen_boxes(fine_box, n_levels):
@@ -632,12 +731,13 @@ def coarsen_boxes(fine_box, n_levels):
 def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precision=FP64,
                         smoother_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
                         coarse=COARSE_PCG, coarse_tol=1e-12, prolongation_scale=1.0,
-                        outer_precision=None):
+                        outer_precision=None, coefficient=None):
     """h-only hierarchy at fixed degree (source/geometric_multigrid_bk3/program.cc):
     level 0 has fine_box coarsened n_levels-1 times."""
     boxes = coarsen_boxes(fine_box, n_levels)
     nqf = None if n_q_points_1d is None else (lambda p: n_q_points_1d)
     mg, ops, trs, sms = build_hierarchy([(b, degree) for b in boxes], nqf, precision,
                                         smoother_degree, smoothing_range, eig_cg_n_iterations,
-                                        coarse, coarse_tol, prolongation_scale, outer_precision)
+                                        coarse, coarse_tol, prolongation_scale, outer_precision,
+                                        coefficient=coefficient)
     return mg, ops, trs, sms, boxes

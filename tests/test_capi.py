@@ -61,3 +61,49 @@ def test_p_transfer_maps_match_oracle(pmg, orc, pc, pf):
     t = orc.Transfer(m, pc, pf, kind="p")
     assert np.array_equal(mc, t.map_c) and np.array_equal(mf, t.map_f)
     assert np.allclose(w, t.w, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("cells", [(1, 1, 1), (2, 3, 2), (4, 1, 3)])
+def test_box_dg_faces_cover_every_cell_face_once(pmg, cells):
+    """Every (cell, face) pair appears exactly once; inner faces join
+    neighbours along one axis with face+ = face- ^ 1 (standard orientation)."""
+    b = pmg.box(*cells)
+    inner, bdry = pmg.box_dg_faces(b)
+    inner, bdry = inner.reshape(-1, 4), bdry.reshape(-1, 3)
+    C = pmg.box_n_cells(b)
+    seen = np.zeros((C, 6), dtype=int)
+    np.add.at(seen, (inner[:, 0], inner[:, 2]), 1)
+    np.add.at(seen, (inner[:, 1], inner[:, 3]), 1)
+    np.add.at(seen, (bdry[:, 0], bdry[:, 1]), 1)
+    assert np.all(seen == 1)
+    assert np.all(inner[:, 3] == (inner[:, 2] ^ 1)) and np.all(inner[:, 2] % 2 == 1)
+    nx, ny, _ = cells
+    stride = np.array([1, nx, nx * ny])
+    assert np.all(inner[:, 1] - inner[:, 0] == stride[inner[:, 2] // 2])
+    assert np.all(bdry[:, 2] == 0)
+
+
+def test_quadrature_points_match_oracle(pmg, orc):
+    b = pmg.box(3, 2, 2, deform=1, amplitude=0.05)
+    X = pmg.quadrature_points(b, 4)
+    _, _, Xo = orc.geometry(orc.mesh(3, 2, 2, deform=1, amplitude=0.05), 4, want_x=True)
+    assert np.abs(X - Xo.reshape(-1, 3)).max() < 1e-15
+
+
+def test_cpp_header_compiles_and_links(pmg, tmp_path):
+    """include/pmg.hpp (the C++ mirror) compiles standalone against the C ABI."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "pmg.hpp"\n'
+                   'int main() { pmg_box b{}; b.nx = b.ny = b.nz = 2; b.hi[0] = b.hi[1] = b.hi[2] = 1;\n'
+                   '  return pmg_box_n_cells(&b) == 8 ? 0 : 1; }\n')
+    exe = tmp_path / "use"
+    pkg = os.path.dirname(pmg.LIB_PATH)
+    subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L", pkg, "-lpmg", f"-Wl,-rpath,{pkg}"],
+                   check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
