@@ -316,7 +316,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cells", type=int, default=64)
-    ap.add_argument("--schedule", choices=["atomic", "colored"], default=None,
+    ap.add_argument("--schedule", choices=["atomic", "colored", "patch"], default=None,
                     help="BK3 cell-loop schedule (default: the library default)")
     ap.add_argument("--gload", type=int, default=None, help="BK3 G-load policy 0/1/2")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -340,7 +340,8 @@ def main():
     log(f"[bench] building operator {n}^3 Q{degree} nq={nq}")
     op = pmg.LaplaceOperatorBK3(fine_box, degree, nq, pmg.FP64)
     if args.schedule is not None or args.gload is not None:
-        op.set_schedule(args.schedule == "colored", args.gload or 0)
+        kinds = {"atomic": 0, "colored": 1, "patch": 2}
+        op.set_schedule(kinds[args.schedule or "atomic"], 2 if args.gload is None else args.gload)
     N = op.m()
     g = torch.Generator(device="cuda").manual_seed(1234)
     x = torch.rand(N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
@@ -373,7 +374,8 @@ def main():
                      "fp64_frac": alg_flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS},
     }
     if args.schedule is not None or args.gload is not None:
-        out["config"]["schedule"] = {"scatter": args.schedule or "atomic", "g_load": args.gload or 0}
+        out["config"]["schedule"] = {"scatter": args.schedule or "atomic",
+                                     "g_load": 2 if args.gload is None else args.gload}
     del y
     if args.calibrate_copy:
         # plain copy kernel of known size: 2^27 doubles read + written per launch

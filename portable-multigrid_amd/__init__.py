@@ -31,6 +31,7 @@ FP64, FP32 = 0, 1
 TRANSFER_H, TRANSFER_P, TRANSFER_DG_CG = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
 COARSE_CHEBYSHEV, COARSE_PCG = 0, 1
+SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH = 0, 1, 2
 INVALID_DOF = 0xFFFFFFFF
 
 OK, ERR_INVALID_ARG, ERR_SIZE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED, ERR_ALLOC = range(7)
@@ -338,9 +339,11 @@ class LaplaceOperatorBK3:
             self.h, _dev_ptr(u, self.precision, self._m), _dev_ptr(lam, self.precision, self._m),
             _dev_ptr(out, self.precision, self.n_cells * self.nq ** 3), _stream()))
 
-    def set_schedule(self, colored=False, g_load=0):
-        """Cell-loop schedule: atomic scatter (default) or one launch per colour."""
-        _check(lib().pmg_operator_set_schedule(self.h, int(colored), g_load))
+    def set_schedule(self, schedule=SCHEDULE_ATOMIC, g_load=2):
+        """Cell-loop schedule: SCHEDULE_ATOMIC (default, per-cell device
+        atomics), SCHEDULE_COLORED (one launch per colour) or SCHEDULE_PATCH
+        (batch patches in LDS)."""
+        _check(lib().pmg_operator_set_schedule(self.h, int(schedule), g_load))
 
     def assemble_rhs(self, f_kind=1, n_q_points_1d=0):
         """b = int phi_i f (f_kind 0: f = 1, 1: 3 pi^2 sin sin sin), constrained rows 0."""

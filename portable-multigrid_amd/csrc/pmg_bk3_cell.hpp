@@ -11,8 +11,9 @@ namespace pmg {
 template <int NQ>
 struct Bk3Tile {
   static constexpr int NQ2 = NQ * NQ;
-  static constexpr int cells = (256 / NQ2) > 0 ? 256 / NQ2 : 1;
+  static constexpr int cells = g_batch_cells(NQ);
   static constexpr int lanes = ((cells * NQ2 + 63) / 64) * 64;
+  static_assert(cells * NQ2 <= 256, "a batch must fit one 256-lane workgroup");
 };
 
 template <typename T>
@@ -45,10 +46,13 @@ __device__ __forceinline__ void bk3_load_g(const T *Gc, T (&g)[6][NQ]) {
 // ta,tb < NM), out[k] receives (A_e u)_ijk on the same lanes.  P0..P2: three
 // NQ^3 LDS buffers of this cell.  g holds the column's G (loaded here when
 // GMODE is late).  All lanes of the workgroup must call it (barriers).
+// zbuf[0..zn) (LDS, optional) is cleared by the whole workgroup between the
+// first and the last barrier, i.e. after every lane has consumed u.
 template <int NM, int NQ, typename T, int GMODE>
 __device__ __forceinline__ void
 bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh,
-               T *P0, T *P1, T *P2, const bool active, const int ta, const int tb)
+               T *P0, T *P1, T *P2, const bool active, const int ta, const int tb,
+               T *zbuf = nullptr, const int zn = 0)
 {
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
   // ---- phase 1: interpolate the column in z
@@ -62,6 +66,9 @@ bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const
     }
   }
   __syncthreads();
+
+  if (zbuf)
+    for (int i = threadIdx.x; i < zn; i += blockDim.x) zbuf[i] = T(0);
 
   // ---- phase 2: y-interpolation, lane (i,r) = (ta,tb): B[r][q][i]
   if (active && ta < NM) {

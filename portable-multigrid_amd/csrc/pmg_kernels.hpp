@@ -21,15 +21,18 @@ struct Shape1D {
   T D[kMaxNq * kMaxNq];  // D[n*nq+q]
 };
 
-// Storage of the geometric factors G (6 symmetric components per q-point).
-// The cell kernel gives each cell an nq x nq lane tile and E = 256/nq^2 cells
-// one workgroup ("batch"); G is stored batch-interleaved,
+// Cells per workgroup ("batch") of the cell kernels: each cell owns an
+// nq x nq lane tile of a 256-lane workgroup, E = floor(256 / nq^2).  (Cubes of
+// 2^3 cells at nq = 5 were measured slower: fewer active lanes and shorter
+// contiguous DoF runs per batch.)
+__host__ __device__ constexpr int g_batch_cells(int nq) {
+  return (256 / (nq * nq)) > 0 ? 256 / (nq * nq) : 1;
+}
+// Storage of the geometric factors G (6 symmetric components per q-point),
+// batch-interleaved:
 //   G[((batch * 6 + comp) * nq + r) * lanes + e * nq^2 + (q mod nq^2)],
 // r = q / nq^2, e = cell mod E, so that every G load instruction of a wave
 // reads 64 consecutive values.  lanes = E nq^2 rounded up to 64.
-__host__ __device__ inline int g_batch_cells(int nq) {
-  return (256 / (nq * nq)) > 0 ? 256 / (nq * nq) : 1;
-}
 __host__ __device__ inline int g_batch_lanes(int nq) {
   return ((g_batch_cells(nq) * nq * nq + 63) / 64) * 64;
 }
@@ -45,21 +48,34 @@ __host__ __device__ inline int64_t g_size(int nq, int64_t n_cells) {
   return nb * 6 * nq * g_batch_lanes(nq);
 }
 
-// How the cell loop is scheduled: all cells in one launch with atomic
-// scatter-add (default), or one launch per colour of a cell colouring with a
-// plain read-modify-write scatter (no two cells of a colour share a DoF).
+// How the cell loop is scheduled:
+//  kAtomic  all cells in one launch, every cell scatter-adds its nm^3 values
+//           with device-scope FP atomics;
+//  kColored one launch per colour of a cell colouring, plain read-modify-write
+//           (no two cells of a colour share a DoF);
+//  kPatch   all cells in one launch; a batch gathers the union of its cells'
+//           DoFs ("patch") into LDS, its cells accumulate into the patch with
+//           LDS atomics, and the patch is written once: DoFs no other batch
+//           touches with plain stores, shared ones with device atomics.
+enum Bk3ScheduleKind : int { kAtomic = 0, kColored = 1, kPatch = 2 };
 struct Bk3Schedule {
-  bool colored = false;
+  int kind = kAtomic;
   int gmode = 2;                        // G loads: 0 early non-temporal, 1 early, 2 late non-temporal
   const uint32_t *cell_ids = nullptr;   // device: cells sorted by colour
   std::vector<int64_t> color_offsets;   // host: n_colors + 1 offsets into cell_ids
+  // patch data (kPatch): per batch b, pcount[b] patch DoFs at
+  // pdof[b * E nm^3 ...] (bit 31 set: shared with another batch), and per
+  // cell-local DoF its patch index pidx[cell][i] (0xFFFF: constrained)
+  const uint32_t *pdof = nullptr, *pcount = nullptr;
+  const uint16_t *pidx = nullptr;
 };
 
-// y += sum_cells A_e x
+// y += sum_cells A_e x.  assign (patch schedule only): DoFs owned by one
+// batch are overwritten instead of added to (y must be zero on the others)
 template <typename T>
 void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
                       int64_t n_cells, const Shape1D<T> &sh, hipStream_t s,
-                      const Bk3Schedule *sched = nullptr);
+                      const Bk3Schedule *sched = nullptr, bool assign = false);
 // G[c][6][nq^3] = kappa * JxW * J^{-1}J^{-T} of the trilinear map of the cell
 // vertices; kappa (optional) is indexed by the caller's cell perm[c]
 template <typename T>

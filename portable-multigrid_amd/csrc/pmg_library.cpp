@@ -4,6 +4,7 @@
 // only sequences launches and reads back the scalars a solver must test.
 #include "pmg_library.hpp"
 
+#include "pmg_cellgroups.hpp"
 #include "pmg_fields.hpp"
 
 #include <algorithm>
@@ -102,15 +103,20 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   sh32_ = to_shape<float>(shape_.S, shape_.D);
   shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
   shd32_ = to_shape<float>(shape_.S, shape_.Sd);
-  // Internal cell order: Morton (Z-order) of the cell centroids when the
-  // geometry is given by vertices, so that the cells one XCD processes at a
-  // time form a compact 3D region whose shared DoFs stay in its L2.  perm_
-  // maps internal -> caller cell index; cell-indexed user data (kappa, the
-  // coefficient gradient) is translated through it.
+  // Internal cell order.  Default: the caller's order (on lexicographic
+  // structured meshes a batch is a run of cells along x, whose DoF rows are
+  // long and contiguous -- measured fastest for the atomic scatter).
+  // PMG_CELL_ORDER=grouped forms compact groups of E cells that share many
+  // DoFs (group_cells), =morton sorts by the Morton key of the cell centroids.
+  // perm_ maps internal -> caller cell index; cell-indexed user data (G,
+  // kappa, the coefficient gradient) is translated through it.
   std::vector<uint32_t> perm(static_cast<size_t>(n_cells));
   for (int64_t c = 0; c < n_cells; ++c) perm[c] = static_cast<uint32_t>(c);
-  const char *order_env = std::getenv("PMG_CELL_ORDER");  // "morton" reorders, default keeps
-  if (cell_vertices && n_cells > 1 && order_env && std::string(order_env) == "morton") {
+  const char *order_env = std::getenv("PMG_CELL_ORDER");
+  const std::string order_kind = order_env ? order_env : "caller";
+  if (order_kind == "grouped" && n_cells > 1)
+    perm = group_cells(n_cells, nm3, dofmap, n_dofs, g_batch_cells(nq));
+  if (cell_vertices && n_cells > 1 && order_kind == "morton") {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     std::vector<double> cen(static_cast<size_t>(3 * n_cells));
     for (int64_t c = 0; c < n_cells; ++c)
@@ -148,43 +154,12 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   map_.upload(imap);
   perm_.upload(perm);
   constrained_.upload(constrained, static_cast<size_t>(n_constrained));
-  // Greedy cell colouring (cells sharing a DoF get different colours), cells
-  // sorted by colour: the atomics-free schedule launches one colour at a time.
   {
-    std::vector<uint64_t> used(static_cast<size_t>(n_dofs), 0);
-    std::vector<int> color(static_cast<size_t>(n_cells), -1);
-    bool ok = true;
-    int n_col = 0;
-    for (int64_t c = 0; c < n_cells && ok; ++c) {
-      uint64_t forbidden = 0;
-      for (int64_t i = 0; i < nm3; ++i) {
-        const uint32_t d = imap[c * nm3 + i];
-        if (d != kInvalid) forbidden |= used[d];
-      }
-      int col = 0;
-      while (col < 64 && ((forbidden >> col) & 1u)) ++col;
-      if (col == 64) { ok = false; break; }
-      color[c] = col;
-      n_col = std::max(n_col, col + 1);
-      for (int64_t i = 0; i < nm3; ++i) {
-        const uint32_t d = imap[c * nm3 + i];
-        if (d != kInvalid) used[d] |= (uint64_t(1) << col);
-      }
-    }
-    if (ok && n_cells > 0) {
-      std::vector<int64_t> off(n_col + 1, 0);
-      for (int64_t c = 0; c < n_cells; ++c) ++off[color[c] + 1];
-      for (int k = 0; k < n_col; ++k) off[k + 1] += off[k];
-      std::vector<uint32_t> ids(static_cast<size_t>(n_cells));
-      std::vector<int64_t> fill(off.begin(), off.end() - 1);
-      for (int64_t c = 0; c < n_cells; ++c) ids[fill[color[c]]++] = static_cast<uint32_t>(c);
-      cell_ids_.upload(ids);
-      sched_.cell_ids = cell_ids_.ptr;
-      sched_.color_offsets = off;
-    }
-    const char *env = std::getenv("PMG_BK3_SCHEDULE");
+    const char *env = std::getenv("PMG_BK3_SCHEDULE");  // atomic | colored | patch
     const char *genv = std::getenv("PMG_BK3_GLOAD");
-    set_schedule(env && std::string(env) == "colored", genv ? std::atoi(genv) : 2);  // late non-temporal G loads measured fastest
+    const std::string kind = env ? env : "atomic";
+    set_schedule(kind == "colored" ? kColored : kind == "atomic" ? kAtomic : kPatch,
+                 genv ? std::atoi(genv) : 2);  // late non-temporal G loads measured fastest
   }
   G_.reinit(precision, g_size(nq, n_cells));
   if (cell_vertices) {
@@ -203,8 +178,8 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
       for (int64_t c = 0; c < n_cells; ++c)
         for (int comp = 0; comp < 6; ++comp)
           for (int q = 0; q < nq3; ++q)
-            dst[g_index(nq, c, comp, q)] =
-                static_cast<std::remove_pointer_t<decltype(dst)>>(G[(c * 6 + comp) * nq3 + q]);
+            dst[g_index(nq, c, comp, q)] = static_cast<std::remove_pointer_t<decltype(dst)>>(
+                G[(int64_t(perm[c]) * 6 + comp) * nq3 + q]);
     };
     if (precision == PMG_FP64) {
       std::vector<double> g(static_cast<size_t>(g_size(nq, n_cells)), 0.0);
@@ -216,6 +191,76 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
       PMG_HIP_CHECK(hipMemcpy(G_.data(), g.data(), G_.raw.bytes(), hipMemcpyHostToDevice));
     }
   }
+}
+
+// Greedy cell colouring (cells sharing a DoF get different colours), cells
+// sorted by colour: the colored schedule launches one colour at a time.  Built
+// on first use from the device map.
+void LaplaceOperatorBK3::build_coloring() {
+  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
+  std::vector<uint32_t> imap(static_cast<size_t>(n_cells_ * nm3));
+  if (!imap.empty())
+    PMG_HIP_CHECK(hipMemcpy(imap.data(), map_.ptr, imap.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost));
+  const int64_t n_cells = n_cells_;
+  std::vector<uint64_t> used(static_cast<size_t>(n_dofs_), 0);
+  std::vector<int> color(static_cast<size_t>(n_cells), -1);
+  bool ok = true;
+  int n_col = 0;
+  for (int64_t c = 0; c < n_cells && ok; ++c) {
+    uint64_t forbidden = 0;
+    for (int64_t i = 0; i < nm3; ++i) {
+      const uint32_t d = imap[c * nm3 + i];
+      if (d != kInvalid) forbidden |= used[d];
+    }
+    int col = 0;
+    while (col < 64 && ((forbidden >> col) & 1u)) ++col;
+    if (col == 64) { ok = false; break; }
+    color[c] = col;
+    n_col = std::max(n_col, col + 1);
+    for (int64_t i = 0; i < nm3; ++i) {
+      const uint32_t d = imap[c * nm3 + i];
+      if (d != kInvalid) used[d] |= (uint64_t(1) << col);
+    }
+  }
+  if (ok && n_cells > 0) {
+    std::vector<int64_t> off(n_col + 1, 0);
+    for (int64_t c = 0; c < n_cells; ++c) ++off[color[c] + 1];
+    for (int k = 0; k < n_col; ++k) off[k + 1] += off[k];
+    std::vector<uint32_t> ids(static_cast<size_t>(n_cells));
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (int64_t c = 0; c < n_cells; ++c) ids[fill[color[c]]++] = static_cast<uint32_t>(c);
+    cell_ids_.upload(ids);
+    sched_.cell_ids = cell_ids_.ptr;
+    sched_.color_offsets = off;
+  }
+}
+
+// DoF patches of the batches (patch schedule), built on first use from the
+// device map.
+void LaplaceOperatorBK3::build_patches_once() {
+  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
+  const int E = g_batch_cells(nq_);
+  if (n_cells_ == 0 || n_dofs_ >= (int64_t(1) << 31) || E * nm3 >= 0xFFFF) return;
+  std::vector<uint32_t> imap(static_cast<size_t>(n_cells_ * nm3));
+  PMG_HIP_CHECK(hipMemcpy(imap.data(), map_.ptr, imap.size() * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost));
+  const CellPatches P = build_patches(n_cells_, nm3, imap.data(), n_dofs_, E);
+  pdof_.upload(P.pdof);
+  pcount_.upload(P.pcount);
+  pidx_.upload(P.pidx);
+  sched_.pdof = pdof_.ptr;
+  sched_.pcount = pcount_.ptr;
+  sched_.pidx = pidx_.ptr;
+}
+
+void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {
+  if (kind == kPatch && sched_.pdof == nullptr) build_patches_once();
+  if (kind == kColored && sched_.color_offsets.empty()) build_coloring();
+  if (kind == kColored && sched_.color_offsets.empty()) kind = kAtomic;
+  if (kind == kPatch && sched_.pdof == nullptr) kind = kAtomic;
+  sched_.kind = kind;
+  sched_.gmode = gmode;
 }
 
 template <typename T>
@@ -230,7 +275,7 @@ void LaplaceOperatorBK3::vmult_t(T *dst, const T *src, hipStream_t s) const {
       std::is_same<T, double>::value ? static_cast<const void *>(&sh64_) : static_cast<const void *>(&sh32_));
   launch_fill<T>(n_dofs_, dst, T(0), s);
   launch_bk3_cells<T>(degree_, nq_, src, dst, static_cast<const T *>(G_.data()), map_.ptr,
-                      n_cells_, sh, s, &sched_);
+                      n_cells_, sh, s, &sched_, /*assign=*/true);
   launch_copy_indexed<T>(n_constrained_, constrained_.ptr, dst, src, s);
 }
 

@@ -93,6 +93,8 @@ class LaplaceOperatorBK3 : public Operator {
   template <typename T> void vmult_t(T *dst, const T *src, hipStream_t s) const;
   template <typename T> void compute_diagonal_t();
   template <typename T> void fill_geometry_t(const double *kappa, hipStream_t s);
+  void build_coloring();
+  void build_patches_once();
 
   int degree_, nq_, precision_;
   int64_t n_cells_, n_dofs_, n_constrained_;
@@ -103,17 +105,17 @@ class LaplaceOperatorBK3 : public Operator {
   DeviceArray<double> vertices_, qpts_, qwts_;
   DeviceVector G_, inv_diag_;
   bool has_diag_ = false;
-  // cell colouring for the atomics-free schedule
-  DeviceArray<uint32_t> cell_ids_;
+  // cell colouring for the atomics-free schedule, DoF patches for the patch schedule
+  DeviceArray<uint32_t> cell_ids_, pdof_, pcount_;
+  DeviceArray<uint16_t> pidx_;
   Bk3Schedule sched_;
 
  public:
-  // "atomic" (one launch, hardware FP atomics) or "colored" (one launch per
-  // colour, plain scatter); gmode: G load policy (see pmg_bk3.hip)
-  void set_schedule(bool colored, int gmode) {
-    sched_.colored = colored && !sched_.color_offsets.empty();
-    sched_.gmode = gmode;
-  }
+  // kAtomic (one launch, device FP atomics per cell), kColored (one launch per
+  // colour, plain scatter) or kPatch (batch patches in LDS, see pmg_bk3.hip);
+  // a schedule whose set-up data is missing falls back to kAtomic.
+  // gmode: G load policy (see pmg_bk3.hip)
+  void set_schedule(int kind, int gmode);
   int n_colors() const { return static_cast<int>(sched_.color_offsets.size()) - 1; }
   const Bk3Schedule &schedule() const { return sched_; }
 };

@@ -143,16 +143,58 @@ def test_bad_arguments_raise(pmg):
         op.get_matrix_diagonal_inverse()  # compute_diagonal() not called
 
 
-@pytest.mark.parametrize("p,nq,colored,gload", [(4, 5, True, 0), (4, 5, False, 1), (4, 5, False, 2),
-                                                (4, 6, True, 0), (4, 6, False, 2), (2, 3, True, 0),
-                                                (6, 7, True, 0)])
-def test_schedules_agree(pmg, orc, p, nq, colored, gload):
-    """Atomics-free colour-by-colour scatter and the G-load variants give the
-    oracle's result as well."""
+@pytest.mark.parametrize("p,nq,sched,gload", [
+    (4, 5, 1, 0), (4, 5, 0, 1), (4, 5, 0, 0), (4, 5, 2, 2), (4, 6, 1, 0), (4, 6, 2, 2),
+    (2, 3, 1, 0), (6, 7, 1, 0), (1, 2, 2, 2), (3, 4, 2, 2), (5, 7, 2, 2), (6, 8, 2, 2)])
+def test_schedules_agree(pmg, orc, p, nq, sched, gload):
+    """Atomics-free colour-by-colour scatter, the batch-patch scatter and the
+    G-load variants give the oracle's result as the default atomic scatter."""
     import torch
     op, ref = _pair(pmg, orc, 3, p, nq, 1, pmg.FP64)
-    op.set_schedule(colored, gload)
+    op.set_schedule(sched, gload)
     x = seeded_uniform(op.m(), 300 + p)
     dst = op.initialize_dof_vector()
     op.vmult(dst, torch.tensor(x, device="cuda"))
     assert rel_err(dst.cpu().numpy(), ref.vmult(x)) < 1e-12
+
+
+@pytest.mark.parametrize("order", ["grouped", "morton"])
+def test_internal_cell_orders(pmg, orc, order, monkeypatch):
+    """Internal cell reorderings (and the patch schedule on compact groups)
+    leave vmult, the diagonal, the coefficient and its gradient unchanged,
+    also for an operator created from a caller-given G."""
+    import torch
+    monkeypatch.setenv("PMG_CELL_ORDER", order)
+    n, p, nq = 4, 3, 4
+    op, ref = _pair(pmg, orc, n, p, nq, 1, pmg.FP64)
+    x = seeded_uniform(op.m(), 61)
+    y = ref.vmult(x)
+    for sched in (0, 2):
+        op.set_schedule(sched, 2)
+        dst = op.initialize_dof_vector()
+        op.vmult(dst, torch.tensor(x, device="cuda"))
+        assert rel_err(dst.cpu().numpy(), y) < 1e-12
+    op.compute_diagonal()
+    assert rel_err(1.0 / op.get_matrix_diagonal_inverse().cpu().numpy(), ref.diagonal()) < 1e-12
+    kappa = np.exp(seeded_uniform(op.n_cells * nq ** 3, 62, -0.5, 0.5))
+    op.set_coefficient(torch.tensor(kappa, device="cuda"))
+    G0 = ref.G
+    ref.G = orc.scale_G(G0, kappa, nq)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, torch.tensor(x, device="cuda"))
+    assert rel_err(dst.cpu().numpy(), ref.vmult(x)) < 1e-12
+    u, lam = seeded_uniform(op.m(), 63), seeded_uniform(op.m(), 64)
+    u[ref.cons] = 0
+    lam[ref.cons] = 0
+    out = torch.zeros(op.n_cells * nq ** 3, dtype=torch.float64, device="cuda")
+    op.gradient_coefficient(torch.tensor(u, device="cuda"), torch.tensor(lam, device="cuda"), out)
+    ref.G = G0
+    assert rel_err(out.cpu().numpy(), ref.grad_kappa(kappa, u, lam)) < 1e-12
+    # caller-given G (reference layout [cell][comp][q]) is re-laid out through the order
+    b = op.box
+    opg = pmg.LaplaceOperatorBK3.from_arrays(p, pmg.box_dofmap(b, p), op.m(),
+                                             pmg.box_boundary_dofs(b, p), G=ref.G,
+                                             n_q_points_1d=nq)
+    dst = opg.initialize_dof_vector()
+    opg.vmult(dst, torch.tensor(x, device="cuda"))
+    assert rel_err(dst.cpu().numpy(), y) < 1e-12
