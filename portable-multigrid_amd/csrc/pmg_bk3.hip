@@ -41,7 +41,7 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   const int64_t pos = cell_base + e;
   const bool active = (e < E) && (pos < n_range);
   const int64_t cell = active ? (cell_ids ? static_cast<int64_t>(cell_ids[pos]) : pos) : 0;
-  T *P0 = lds + (active ? e : 0) * 3 * NQ3;
+  T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
   const T *Gc = G + g_index(NQ, cell, 0, t);
 
   T g[6][NQ];
@@ -77,7 +77,7 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
                  const int64_t n_range, const Shape1D<T> sh)
 {
   constexpr int E = Bk3Tile<NQ>::cells;
-  __shared__ T lds[E * 3 * NQ * NQ * NQ];
+  __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
   bk3_batch<NM, NQ, T, ATOMIC, GMODE>(x, y, G, map, cell_ids, n_range, sh,
                                       static_cast<int64_t>(lb) * E, lds);
@@ -101,7 +101,7 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells, PMAX = E * NM3;
-  __shared__ T lds[E * 3 * NQ3];
+  __shared__ T lds[E * lds_cell_stride<T>(3 * NQ3)];
   __shared__ T patch[PMAX];
   const int64_t batch = xcd_logical_block(blockIdx.x, gridDim.x);
   const int64_t cell_base = batch * E;
@@ -116,7 +116,7 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   const int64_t cell = cell_base + e;
   const bool active = (e < E) && (cell < n_cells);
   const bool col = active && ta < NM && tb < NM;
-  T *P0 = lds + (active ? e : 0) * 3 * NQ3;
+  T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
   const T *Gc = G + g_index(NQ, active ? cell : 0, 0, t);
   T g[6][NQ];
   if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
@@ -195,7 +195,7 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
                   T *z, T *p, T *q, const double rel_tol, const int max_it, double *info)
 {
   constexpr int E = Bk3Tile<NQ>::cells;
-  __shared__ T lds[E * 3 * NQ * NQ * NQ];
+  __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   __shared__ double red[9];
   const int tid = threadIdx.x, nt = blockDim.x;
 

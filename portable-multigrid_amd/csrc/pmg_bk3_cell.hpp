@@ -16,6 +16,15 @@ struct Bk3Tile {
   static_assert(cells * NQ2 <= 256, "a batch must fit one 256-lane workgroup");
 };
 
+// LDS elements per cell for a per-cell need of `base` elements of type T:
+// padded when base is a multiple of the 64 x 4-byte bank row, so that the
+// cells sharing a wave do not start on the same bank (they would otherwise
+// conflict on every access of the same local index).
+template <typename T>
+__host__ __device__ constexpr int lds_cell_stride(int base) {
+  return base % (256 / int(sizeof(T))) == 0 ? base + 64 / int(sizeof(T)) : base;
+}
+
 template <typename T>
 __device__ inline T ldg_stream(const T *p) {
   return __builtin_nontemporal_load(p);
@@ -42,17 +51,17 @@ __device__ __forceinline__ void bk3_load_g(const T *Gc, T (&g)[6][NQ]) {
 }
 
 // Cell stiffness applied to one cell held by an NQ x NQ lane tile (lane
-// (ta,tb)): u[k] is the source column (i,j) = (ta,tb) over k (read only where
-// ta,tb < NM), out[k] receives (A_e u)_ijk on the same lanes.  P0..P2: three
-// NQ^3 LDS buffers of this cell.  g holds the column's G (loaded here when
-// GMODE is late).  All lanes of the workgroup must call it (barriers).
-// zbuf[0..zn) (LDS, optional) is cleared by the whole workgroup between the
-// first and the last barrier, i.e. after every lane has consumed u.
-template <int NM, int NQ, typename T, int GMODE>
+// (ta,tb)), in two halves that all lanes of the workgroup must call (they
+// hold barriers).  P0..P2: three NQ^3 LDS buffers of this cell.
+//
+// bk3_forward: u[k] is the source column (i,j) = (ta,tb) over k (read only
+// where ta,tb < NM).  Leaves the values at the Gauss points in
+// P0[r][q][p] (p along x) and d/dxi_0 of them in P2.  zbuf[0..zn) (LDS,
+// optional) is cleared by the whole workgroup after every lane has consumed u.
+template <int NM, int NQ, typename T>
 __device__ __forceinline__ void
-bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh,
-               T *P0, T *P1, T *P2, const bool active, const int ta, const int tb,
-               T *zbuf = nullptr, const int zn = 0)
+bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const bool active,
+            const int ta, const int tb, T *zbuf = nullptr, const int zn = 0)
 {
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
   // ---- phase 1: interpolate the column in z
@@ -107,7 +116,19 @@ bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const
     }
   }
   __syncthreads();
+}
 
+// bk3_backward: from the state bk3_forward leaves, the remaining gradient,
+// the flux G grad u (g holds the column's G, loaded here when GMODE is late)
+// and the test with grad phi; with HASV also sum_q phi(q) V[q] for a value
+// coefficient V[r][q][p] at the Gauss points (LDS).  out[k] receives the
+// result for column (i,j) = (ta,tb) over k.
+template <int NM, int NQ, typename T, int GMODE, bool HASV>
+__device__ __forceinline__ void
+bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *P0, T *P1,
+             T *P2, const T *V, const bool active, const int ta, const int tb)
+{
+  constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
   // ---- phase 4: y-gradient, lane (p,r) = (ta,tb)
   if (active) {
     T c[NQ];
@@ -173,6 +194,7 @@ bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       T v = P0[tb * NQ2 + q * NQ + ta];
+      if (HASV) v += V[tb * NQ2 + q * NQ + ta];
 #pragma unroll
       for (int n = 0; n < NQ; ++n) v += sh.D[q * NQ + n] * f[n];
       acc[q] = v;
@@ -215,6 +237,17 @@ bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const
       out[k] = v;
     }
   }
+}
+
+// Both halves: out = A_e u (see bk3_forward / bk3_backward).
+template <int NM, int NQ, typename T, int GMODE>
+__device__ __forceinline__ void
+bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh,
+               T *P0, T *P1, T *P2, const bool active, const int ta, const int tb,
+               T *zbuf = nullptr, const int zn = 0)
+{
+  bk3_forward<NM, NQ, T>(u, sh, P0, P1, P2, active, ta, tb, zbuf, zn);
+  bk3_backward<NM, NQ, T, GMODE, false>(out, g, Gc, sh, P0, P1, P2, nullptr, active, ta, tb);
 }
 
 }  // namespace pmg

@@ -11,8 +11,19 @@
 // basis functions, reading the neighbour's nodal values (from L2/MALL) to get
 // u+ and d_n u+ on the face.  Every DG DoF belongs to one cell, so the result
 // is written with plain coalesced stores: no atomics, no face buffers, no
-// second pass.  The price is that each inner face's flux is evaluated twice
-// (once from each side), which costs FP64 ALU time, not HBM bytes.
+// second pass.  Each inner face's flux is evaluated from both sides, which
+// costs ALU time, not HBM bytes.  Per face, on the cell's Gauss-point grid:
+//  - own value and normal derivative at the face point: 1D contraction of the
+//    Gauss-point values along the normal with the Gauss-Lagrange basis L_r(s),
+//    L_r'(s) (exact: u is a polynomial of degree < nq along every line);
+//  - neighbour value and normal derivative: its nodal line along the normal,
+//    then the 2D interpolation to the face points;
+//  - tangential derivatives: collocation derivative on the face Gauss grid;
+//  - test: tangential-gradient terms become value terms through the
+//    transposed collocation derivative, and value and normal-derivative terms
+//    are spread back along the normal with L_r(s), L_r'(s) into a value
+//    coefficient V at the Gauss points, which the cell's final
+//    interpolation sweeps integrate together with the cell flux.
 //
 // Layout: cell-major DoFs (cell c owns x[c*NM^3 .. (c+1)*NM^3), lexicographic
 // inside); G batch-interleaved as for the continuous operator; per face F
@@ -46,7 +57,7 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = DgTile<NQ>::cells;
-  constexpr int CELL_LDS = 3 * NQ3 + NM3;
+  constexpr int CELL_LDS = lds_cell_stride<T>(4 * NQ3 + 4 * NQ2);
   __shared__ T lds[E * CELL_LDS];
 
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -56,115 +67,123 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
   const bool active = (e < E) && (cell < n_cells);
   const bool col = active && ta < NM && tb < NM;
   T *P0 = lds + (active ? e : 0) * CELL_LDS;
-  T *P1 = P0 + NQ3, *P2 = P1 + NQ3, *U = P2 + NQ3;
+  T *P1 = P0 + NQ3, *P2 = P1 + NQ3;
+  T *V = P2 + NQ3;      // value coefficient at the Gauss points [r][q][p]
+  T *A = V + NQ3;       // face scratch, 2 NQ^2
+  T *B = A + 2 * NQ2;   // face scratch, 2 NQ^2
   const T *Gc = G + g_index(NQ, active ? cell : 0, 0, t);
 
-  // ---- own column (i,j) = (ta,tb) over k: contiguous cell-major loads
+  // ---- own column (i,j) = (ta,tb) over k; V = 0
   T u[NM], out[NM];
   if (col) {
     const T *xc = x + cell * NM3 + tb * NM + ta;
 #pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      u[k] = xc[k * NM2];
-      U[k * NM2 + tb * NM + ta] = u[k];
-    }
+    for (int k = 0; k < NM; ++k) u[k] = xc[k * NM2];
   }
-  T g[6][NQ];
-  bk3_cell_apply<NM, NQ, T, kGLateNT>(u, out, g, Gc, sh, P0, P1, P2, active, ta, tb);
-  __syncthreads();
-
-  // ---- face terms; Y (= P0) accumulates them on the nodal grid
-  T *Y = P0;
   if (active)
-    for (int i = t; i < NM3; i += NQ2) Y[i] = T(0);
-  T *NV = P1;  // 4 nodal face arrays [NM][NM]: v-, d_n v-, v+, d_n v+
-  T *TA = P1;  // 4 face q-point arrays [NQ][NQ] (aliases NV after a barrier)
+    for (int i = t; i < NQ3; i += NQ2) V[i] = T(0);
+  bk3_forward<NM, NQ, T>(u, sh, P0, P1, P2, active, ta, tb);  // values at Gauss points in P0
 
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
     const int d = f >> 1, s = f & 1;
-    const int sn = d == 0 ? 1 : (d == 1 ? NM : NM2);
-    const int st1 = d == 0 ? NM : 1;
-    const int st2 = d == 2 ? NM : NM2;
+    // strides of the normal and the tangential axes (ascending) on the nodal
+    // grid [k][j][i] and on the Gauss-point grid [r][q][p]
+    const int snn = d == 0 ? 1 : (d == 1 ? NM : NM2);
+    const int t1n = d == 0 ? NM : 1, t2n = d == 2 ? NM : NM2;
+    const int snq = d == 0 ? 1 : (d == 1 ? NQ : NQ2);
+    const int t1q = d == 0 ? NQ : 1, t2q = d == 2 ? NQ : NQ2;
     const uint32_t cf = active ? cface[cell * 6 + f] : 0u;
     const int role = static_cast<int>(cf >> 30);
     const int64_t F = cf & 0x3FFFFFFFu;
 
-    // F1: traces and normal derivatives on the nodal face grid, lane (alpha,beta)
-    if (col) {
-      const int base = ta * st1 + tb * st2;
-      T vm = 0, dm = 0, vp = 0, dp = 0;
+    // face geometry of this face point, requested now and used after two
+    // barriers, so its latency overlaps steps A and B
+    T fq[7], sig = 0;
+    if (active) {
+      const T *fd = fdata + F * 7 * NQ2 + t;
 #pragma unroll
-      for (int k = 0; k < NM; ++k) {
-        const T w = U[base + k * sn];
-        dm += fs.dl[s * NM + k] * w;
-        if (k == (s ? NM - 1 : 0)) vm = w;
-      }
+      for (int c = 0; c < 7; ++c) fq[c] = fd[c * NQ2];
+      sig = fsig[F];
+    }
+
+    // A: neighbour trace and normal derivative on its nodal face grid, lane
+    // (alpha,beta); own value and normal derivative at face point (a,b) by a
+    // 1D contraction of the Gauss-point values along the normal
+    T um = 0, dnu = 0;
+    if (col) {
+      T vp = 0, dp = 0;
       if (role < 2) {
-        const int64_t nb = cnbr[cell * 6 + f];
-        const T *xn = x + nb * NM3 + base;
+        const T *xn = x + static_cast<int64_t>(cnbr[cell * 6 + f]) * NM3 + ta * t1n + tb * t2n;
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
-          const T w = xn[k * sn];
+          const T w = xn[k * snn];
           dp += fs.dl[(1 - s) * NM + k] * w;
           if (k == (s ? 0 : NM - 1)) vp = w;
         }
       }
-      const int id = tb * NM + ta;
-      NV[id] = vm;
-      NV[NM2 + id] = dm;
-      NV[2 * NM2 + id] = vp;
-      NV[3 * NM2 + id] = dp;
+      A[tb * NM + ta] = vp;
+      A[NQ2 + tb * NM + ta] = dp;
+    }
+    if (active) {
+      const T *Ul = P0 + ta * t1q + tb * t2q;
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        const T w = Ul[r * snq];
+        um += fs.L[s * NQ + r] * w;
+        dnu += fs.Ld[s * NQ + r] * w;
+      }
+      B[t] = um;
     }
     __syncthreads();
 
-    // F2: values and reference gradients at face q-point (a,b) = (ta,tb), flux
-    T cval = 0, cn = 0, c1 = 0, c2 = 0;
+    // B: neighbour value and normal derivative at face point (a,b)
+    T vpq = 0, dpq = 0;
     if (active) {
-      T um = 0, um1 = 0, um2 = 0, umn = 0, up = 0, up1 = 0, up2 = 0, upn = 0;
 #pragma unroll
       for (int be = 0; be < NM; ++be) {
-        T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+        T a0 = 0, a1 = 0;
 #pragma unroll
         for (int al = 0; al < NM; ++al) {
-          const T sa = sh.S[al * NQ + ta], da = fs.Sd[al * NQ + ta];
-          const T w0 = NV[be * NM + al], w1 = NV[NM2 + be * NM + al];
-          const T w2 = NV[2 * NM2 + be * NM + al], w3 = NV[3 * NM2 + be * NM + al];
-          a0 += sa * w0;
-          a1 += da * w0;
-          a2 += sa * w1;
-          b0 += sa * w2;
-          b1 += da * w2;
-          b2 += sa * w3;
+          const T sa = sh.S[al * NQ + ta];
+          a0 += sa * A[be * NM + al];
+          a1 += sa * A[NQ2 + be * NM + al];
         }
-        const T sb = sh.S[be * NQ + tb], db = fs.Sd[be * NQ + tb];
-        um += sb * a0;
-        um1 += sb * a1;
-        um2 += db * a0;
-        umn += sb * a2;
-        up += sb * b0;
-        up1 += sb * b1;
-        up2 += db * b0;
-        upn += sb * b2;
+        const T sb = sh.S[be * NQ + tb];
+        vpq += sb * a0;
+        dpq += sb * a1;
       }
-      const T *fd = fdata + F * 7 * NQ2 + t;
-      const T jxw = fd[6 * NQ2];
-      const T sig = fsig[F];
-      T mo0, mo1, mo2;
-      T dnm, dnp;
+      B[NQ2 + t] = vpq;
+    }
+    __syncthreads();
+
+    // C: tangential derivatives by collocation on the face Gauss grid, flux
+    T cval = 0, cn = 0;
+    if (active) {
+      T d1m = 0, d2m = 0, d1p = 0, d2p = 0;
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) {
+        const T da = sh.D[n * NQ + ta], db = sh.D[n * NQ + tb];
+        d1m += da * B[tb * NQ + n];
+        d2m += db * B[n * NQ + ta];
+        d1p += da * B[NQ2 + tb * NQ + n];
+        d2p += db * B[NQ2 + n * NQ + ta];
+      }
+      const T jxw = fq[6];
+      T mo0, mo1, mo2, dnm, dnp, up = vpq;
       if (role == 1) {
-        mo0 = -fd[3 * NQ2];
-        mo1 = -fd[4 * NQ2];
-        mo2 = -fd[5 * NQ2];
-        dnm = mo0 * umn + mo1 * um1 + mo2 * um2;
-        dnp = -(fd[0] * upn + fd[NQ2] * up1 + fd[2 * NQ2] * up2);
+        mo0 = -fq[3];
+        mo1 = -fq[4];
+        mo2 = -fq[5];
+        dnm = mo0 * dnu + mo1 * d1m + mo2 * d2m;
+        dnp = -(fq[0] * dpq + fq[1] * d1p + fq[2] * d2p);
       } else {
-        mo0 = fd[0];
-        mo1 = fd[NQ2];
-        mo2 = fd[2 * NQ2];
-        dnm = mo0 * umn + mo1 * um1 + mo2 * um2;
+        mo0 = fq[0];
+        mo1 = fq[1];
+        mo2 = fq[2];
+        dnm = mo0 * dnu + mo1 * d1m + mo2 * d2m;
         if (role == 0) {
-          dnp = fd[3 * NQ2] * upn + fd[4 * NQ2] * up1 + fd[5 * NQ2] * up2;
+          dnp = fq[3] * dpq + fq[4] * d1p + fq[5] * d2p;
         } else if (role == 2) {
           up = -um;
           dnp = dnm;
@@ -174,57 +193,36 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
         }
       }
       const T jump = um - up, avg = T(0.5) * (dnm + dnp);
-      const T tv = (sig * jump - avg) * jxw, td = T(-0.5) * jump * jxw;
-      cval = tv;
+      const T td = T(-0.5) * jump * jxw;
+      cval = (sig * jump - avg) * jxw;
       cn = td * mo0;
-      c1 = td * mo1;
-      c2 = td * mo2;
-    }
-    __syncthreads();
-    if (active) {
-      TA[t] = cval;
-      TA[NQ2 + t] = cn;
-      TA[2 * NQ2 + t] = c1;
-      TA[3 * NQ2 + t] = c2;
+      A[t] = td * mo1;        // tangential test coefficients
+      A[NQ2 + t] = td * mo2;
     }
     __syncthreads();
 
-    // F3: test with the own basis, lane (alpha,beta); add to the normal line
-    if (col) {
-      T W = 0, Z = 0;
+    // D: tangential test terms become value terms by the transposed
+    // collocation derivative; inject value and normal-derivative tests into V
+    if (active) {
+      T c = cval;
 #pragma unroll
-      for (int b = 0; b < NQ; ++b) {
-        T r1 = 0, r2 = 0, r3 = 0;
+      for (int n = 0; n < NQ; ++n)
+        c += sh.D[ta * NQ + n] * A[tb * NQ + n] + sh.D[tb * NQ + n] * A[NQ2 + n * NQ + ta];
+      T *Vl = V + ta * t1q + tb * t2q;
 #pragma unroll
-        for (int a = 0; a < NQ; ++a) {
-          const T sa = sh.S[ta * NQ + a], da = fs.Sd[ta * NQ + a];
-          r1 += TA[b * NQ + a] * sa + TA[2 * NQ2 + b * NQ + a] * da;
-          r2 += TA[3 * NQ2 + b * NQ + a] * sa;
-          r3 += TA[NQ2 + b * NQ + a] * sa;
-        }
-        const T sb = sh.S[tb * NQ + b], db = fs.Sd[tb * NQ + b];
-        W += r1 * sb + r2 * db;
-        Z += r3 * sb;
-      }
-      const int base = ta * st1 + tb * st2;
-#pragma unroll
-      for (int k = 0; k < NM; ++k) {
-        T v = fs.dl[s * NM + k] * Z;
-        if (k == (s ? NM - 1 : 0)) v += W;
-        Y[base + k * sn] += v;
-      }
+      for (int r = 0; r < NQ; ++r) Vl[r * snq] += c * fs.L[s * NQ + r] + cn * fs.Ld[s * NQ + r];
     }
     __syncthreads();
   }
 
-  // ---- write the cell's result: cell part (registers) + face part (Y)
+  T g[6][NQ];
+  bk3_backward<NM, NQ, T, kGLateNT, true>(out, g, Gc, sh, P0, P1, P2, V, active, ta, tb);
   if (col) {
     T *yc = y + cell * NM3 + tb * NM + ta;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
-      const T v = out[k] + Y[k * NM2 + tb * NM + ta];
-      if (ADD) yc[k * NM2] += v;
-      else yc[k * NM2] = v;
+      if (ADD) yc[k * NM2] += out[k];
+      else yc[k * NM2] = out[k];
     }
   }
 }

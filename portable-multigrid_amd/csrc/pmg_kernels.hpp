@@ -106,7 +106,9 @@ void launch_bk3_coarse_pcg(int degree, int nq, int64_t n_cells, const T *G, cons
 template <typename T>
 struct DgShape {
   T Sd[kMaxNm * kMaxNq];  // Sd[i*nq+q] = l_i'(xi_q)
-  T dl[2 * kMaxNm];       // dl[s*nm+i] = l_i'(s)
+  T dl[2 * kMaxNm];       // dl[s*nm+i] = l_i'(s), nodal (Gauss-Lobatto) basis
+  T L[2 * kMaxNq];        // L[s*nq+r] = L_r(s), Lagrange basis on the Gauss points
+  T Ld[2 * kMaxNq];       // Ld[s*nq+r] = L_r'(s)
 };
 
 // y (=, or += if add) = A_DG x; see pmg_dg.hip for the face data layout

@@ -180,6 +180,19 @@ LaplaceOperatorDG::LaplaceOperatorDG(int degree, int nq, int precision, int64_t 
     fs32_.dl[i] = static_cast<float>(d0[i]);
     fs32_.dl[nm + i] = static_cast<float>(d1[i]);
   }
+  // Lagrange basis on the Gauss points and its derivative at the cell ends
+  {
+    std::vector<double> lv(nq), ld(nq);
+    for (int end = 0; end < 2; ++end) {
+      lagrange_eval(shape_.quad.points, double(end), lv.data(), ld.data());
+      for (int r = 0; r < nq; ++r) {
+        fs64_.L[end * nq + r] = lv[r];
+        fs64_.Ld[end * nq + r] = ld[r];
+        fs32_.L[end * nq + r] = static_cast<float>(lv[r]);
+        fs32_.Ld[end * nq + r] = static_cast<float>(ld[r]);
+      }
+    }
+  }
 
   // device data
   cface_.upload(cface);
