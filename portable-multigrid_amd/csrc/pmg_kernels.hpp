@@ -133,10 +133,13 @@ struct Transfer1D {
   T M[kMaxNm * (2 * kMaxNm)];  // M[i*nmf + j]
 };
 
+// w == nullptr: owner mode (map_f bit 31 marks the owning entry of each fine
+// DoF, see pmg_transfer.hip); otherwise weighted, w pre-scaled for the
+// prolongation (scale is only used in owner mode)
 template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
-                       const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xc,
-                       T *yf, hipStream_t s);
+                       const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
+                       const T *xc, T *yf, hipStream_t s);
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,

@@ -505,8 +505,39 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     PMG_REQUIRE(int64_t(map_fine[e]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
                 "fine map entry out of range (fine maps carry no invalid entries)");
   map_c_.upload(map_coarse, static_cast<size_t>(n_cells * nc3));
-  map_f_.upload(map_fine, static_cast<size_t>(n_cells * nf3));
   const size_t nw = static_cast<size_t>(n_cells * nf3);
+  // Owner mode when the weights are 1/multiplicity (0 on constrained DoFs,
+  // consistently for every entry of a DoF), as the reference's setup_weights
+  // builds them: mark the first entry of each weighted fine DoF, drop weights.
+  {
+    bool owner_ok = n_dofs_fine < (int64_t(1) << 31);
+    std::vector<int32_t> mult(owner_ok ? static_cast<size_t>(n_dofs_fine) : 0, 0);
+    std::vector<signed char> zero(mult.size(), -1);  // -1 unseen, 0 weighted, 1 zero weight
+    for (size_t e = 0; e < nw && owner_ok; ++e) {
+      const uint32_t d = map_fine[e];
+      ++mult[d];
+      const signed char z = weights[e] == 0.0 ? 1 : 0;
+      if (zero[d] >= 0 && zero[d] != z) owner_ok = false;
+      zero[d] = z;
+    }
+    for (size_t e = 0; e < nw && owner_ok; ++e)
+      if (weights[e] != 0.0 && std::fabs(weights[e] * mult[map_fine[e]] - 1.0) > 1e-12)
+        owner_ok = false;
+    if (owner_ok) {
+      std::vector<uint32_t> mf(map_fine, map_fine + nw);
+      std::vector<char> taken(static_cast<size_t>(n_dofs_fine), 0);
+      for (size_t e = 0; e < nw; ++e)
+        if (weights[e] != 0.0 && !taken[mf[e]]) {
+          taken[mf[e]] = 1;
+          mf[e] |= 0x80000000u;
+        }
+      map_f_.upload(mf);
+      owner_ = true;
+      scale_ = scale;
+      return;
+    }
+  }
+  map_f_.upload(map_fine, nw);
   w_prolong_.reinit(precision, n_cells * nf3);
   w_restrict_.reinit(precision, n_cells * nf3);
   if (precision == PMG_FP64) {
@@ -539,12 +570,14 @@ void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) c
   }
   if (precision_ == PMG_FP64)
     launch_prolongate<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
-                              static_cast<const double *>(w_prolong_.data()),
-                              static_cast<const double *>(src), static_cast<double *>(dst), s);
+                              owner_ ? nullptr : static_cast<const double *>(w_prolong_.data()),
+                              scale_, static_cast<const double *>(src), static_cast<double *>(dst),
+                              s);
   else
     launch_prolongate<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
-                             static_cast<const float *>(w_prolong_.data()),
-                             static_cast<const float *>(src), static_cast<float *>(dst), s);
+                             owner_ ? nullptr : static_cast<const float *>(w_prolong_.data()),
+                             static_cast<float>(scale_), static_cast<const float *>(src),
+                             static_cast<float *>(dst), s);
 }
 
 void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) const {
@@ -561,11 +594,11 @@ void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) con
   }
   if (precision_ == PMG_FP64)
     launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
-                            static_cast<const double *>(w_restrict_.data()),
+                            owner_ ? nullptr : static_cast<const double *>(w_restrict_.data()),
                             static_cast<const double *>(src), static_cast<double *>(dst), s);
   else
     launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
-                           static_cast<const float *>(w_restrict_.data()),
+                           owner_ ? nullptr : static_cast<const float *>(w_restrict_.data()),
                            static_cast<const float *>(src), static_cast<float *>(dst), s);
 }
 

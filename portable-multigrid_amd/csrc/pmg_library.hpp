@@ -186,6 +186,8 @@ class MGTransfer {
  private:
   int kind_, precision_, nmc_, nmf_;
   int64_t n_cells_, n_dofs_c_, n_dofs_f_;
+  bool owner_ = false;  // owner-computes mode (weights 1/multiplicity), see pmg_transfer.hip
+  double scale_ = 1.0;
   Transfer1D<double> M64_{};
   Transfer1D<float> M32_{};
   DeviceArray<uint32_t> map_c_, map_f_;

@@ -24,10 +24,17 @@ struct XferTile {
   static constexpr int lanes = ((cells * NF2 + 63) / 64) * 64;
 };
 
-template <int NC, int NF, typename T>
+// OWNER: map_f carries bit 31 on the one (cell, local) entry that owns each
+// fine DoF (no weights): the owner adds scale * value with a plain
+// read-modify-write (no other cell of the launch touches that DoF), the
+// restriction gathers each fine value once through its owner.  This equals
+// the weighted form with weights 1/multiplicity because the prolongated value
+// of a node shared by several cells is the same from every cell, and a coarse
+// basis function of one cell vanishes on the faces it does not belong to.
+template <int NC, int NF, typename T, bool OWNER>
 __global__ void __launch_bounds__(256)
 prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__restrict__ map_c,
-                  const uint32_t *__restrict__ map_f, const T *__restrict__ w,
+                  const uint32_t *__restrict__ map_f, const T *__restrict__ w, const T scale,
                   const T *__restrict__ xc, T *__restrict__ yf)
 {
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
@@ -88,16 +95,21 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
       T v = 0;
 #pragma unroll
       for (int k = 0; k < NC; ++k) v += M.M[k * NF + c] * l[k];
-      const T wt = w[base + c * NF2];
-      if (wt != T(0)) atomic_add(yf + map_f[base + c * NF2], wt * v);
+      if (OWNER) {
+        const uint32_t m = map_f[base + c * NF2];
+        if (m & 0x80000000u) yf[m & 0x7FFFFFFFu] += scale * v;
+      } else {
+        const T wt = w[base + c * NF2];
+        if (wt != T(0)) atomic_add(yf + map_f[base + c * NF2], wt * v);
+      }
     }
   }
 }
 
-template <int NC, int NF, typename T>
+template <int NC, int NF, typename T, bool OWNER>
 __global__ void __launch_bounds__(256)
 restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__restrict__ map_c,
-                const uint32_t *__restrict__ map_f, const T *__restrict__ w,
+                const uint32_t *__restrict__ map_f, const T *__restrict__ w, const T scale,
                 const T *__restrict__ xf, T *__restrict__ yc)
 {
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
@@ -116,8 +128,13 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
     const int64_t base = cell * NF3 + tb * NF + ta;
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
-      const T wt = w[base + c * NF2];
-      f[c] = wt == T(0) ? T(0) : wt * xf[map_f[base + c * NF2]];
+      if (OWNER) {
+        const uint32_t m = map_f[base + c * NF2];
+        f[c] = (m & 0x80000000u) ? xf[m & 0x7FFFFFFFu] : T(0);
+      } else {
+        const T wt = w[base + c * NF2];
+        f[c] = wt == T(0) ? T(0) : wt * xf[map_f[base + c * NF2]];
+      }
     }
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -168,30 +185,37 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 
 template <int NC, int NF, typename T>
 static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
-                        const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *x,
-                        T *y, hipStream_t s) {
+                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
+                        const T *x, T *y, hipStream_t s) {
   if (n_cells <= 0) return;
   using Tile = XferTile<NF>;
   const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
-  if (prolong)
-    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T>), dim3(grid), dim3(Tile::lanes), 0, s,
-                       n_cells, M, map_c, map_f, w, x, y);
+  const dim3 g(grid), b(Tile::lanes);
+  if (prolong && w)
+    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, false>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, w, scale, x, y);
+  else if (prolong)
+    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, true>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, w, scale, x, y);
+  else if (w)
+    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, false>), g, b, 0, s, n_cells, M, map_c, map_f,
+                       w, scale, x, y);
   else
-    hipLaunchKernelGGL((restrict_kernel<NC, NF, T>), dim3(grid), dim3(Tile::lanes), 0, s,
-                       n_cells, M, map_c, map_f, w, x, y);
+    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, true>), g, b, 0, s, n_cells, M, map_c, map_f,
+                       w, scale, x, y);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 static void xfer_dispatch(bool prolong, int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
-                          const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *x,
-                          T *y, hipStream_t s) {
-#define PMG_XF(C, F)                                                            \
-  if (nmc == C && nmf == F) {                                                   \
-    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, x, y, s);        \
-    return;                                                                     \
+                          const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
+                          const T *x, T *y, hipStream_t s) {
+#define PMG_XF(C, F)                                                               \
+  if (nmc == C && nmf == F) {                                                      \
+    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, scale, x, y, s);    \
+    return;                                                                        \
   }
   // h-transfers (p+1 -> 2p+1)
   PMG_XF(2, 3) PMG_XF(3, 5) PMG_XF(4, 7) PMG_XF(5, 9) PMG_XF(6, 11) PMG_XF(7, 13)
@@ -208,22 +232,22 @@ static void xfer_dispatch(bool prolong, int nmc, int nmf, int64_t n_cells, const
 
 template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
-                       const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xc,
-                       T *yf, hipStream_t s) {
-  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, xc, yf, s);
+                       const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
+                       const T *xc, T *yf, hipStream_t s) {
+  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, scale, xc, yf, s);
 }
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
                      T *yc, hipStream_t s) {
-  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, xf, yc, s);
+  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, T(1), xf, yc, s);
 }
 
 template void launch_prolongate<double>(int, int, int64_t, const Transfer1D<double> &,
                                         const uint32_t *, const uint32_t *, const double *,
-                                        const double *, double *, hipStream_t);
+                                        double, const double *, double *, hipStream_t);
 template void launch_prolongate<float>(int, int, int64_t, const Transfer1D<float> &,
-                                       const uint32_t *, const uint32_t *, const float *,
+                                       const uint32_t *, const uint32_t *, const float *, float,
                                        const float *, float *, hipStream_t);
 template void launch_restrict<double>(int, int, int64_t, const Transfer1D<double> &,
                                       const uint32_t *, const uint32_t *, const double *,

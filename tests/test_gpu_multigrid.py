@@ -202,3 +202,26 @@ def test_vcycle_is_finite_over_poisoned_memory(pmg):
         for _ in range(3):
             mg.vmult(z, r)
             assert torch.isfinite(z).all().item()
+
+
+@pytest.mark.parametrize("kind", ["h", "p"])
+def test_weighted_transfer_path(pmg, orc, kind):
+    """Weights other than 1/multiplicity take the weighted (atomic) kernels;
+    1/multiplicity weights take the owner-computes kernels.  Both against the
+    oracle's weighted transfer with the same weights."""
+    m = orc.mesh(3, deform=1, amplitude=0.05)
+    ref = orc.Transfer(m, 2, 4 if kind == "p" else None, kind=kind)
+    b = pmg.box(3, deform=1, amplitude=0.05)
+    nc = pmg.box_n_dofs(b, 2)
+    nf = pmg.box_n_dofs(pmg.refine(b) if kind == "h" else b, 2 if kind == "h" else 4)
+    xc, xf = seeded_uniform(nc, 301), seeded_uniform(nf, 302)
+    for wscale in (1.0, 0.7):
+        ref.w = ref.w * wscale
+        tr = pmg._Transfer(pmg.TRANSFER_H if kind == "h" else pmg.TRANSFER_P, pmg.FP64, 2,
+                           2 if kind == "h" else 4, ref.map_c, ref.map_f, ref.w, nc, nf, 1.0)
+        yf = _t(np.zeros(nf))
+        tr.prolongate_and_add(yf, _t(xc))
+        assert rel_err(yf.cpu().numpy(), ref.prolongate_add(xc, np.zeros(nf))) < 1e-12
+        yc = _t(np.zeros(nc))
+        tr.restrict_and_add(yc, _t(xf))
+        assert rel_err(yc.cpu().numpy(), ref.restrict_add(xf, np.zeros(nc))) < 1e-12
