@@ -276,7 +276,13 @@ int pmg_multigrid_create(int n_levels, const pmg_operator *level_operators,
                          int coarse_kind, double coarse_tol, int outer_precision,
                          pmg_multigrid *mg);
 int pmg_multigrid_destroy(pmg_multigrid mg);
+/* The V-cycle body (everything between the conversion of src and of the
+ * result) is captured into a HIP graph on the first call and replayed on the
+ * caller's stream, unless the coarse solve needs host round trips (Jacobi-PCG
+ * on a coarse level with more than 4096 cells) or PMG_VCYCLE_GRAPH=0. */
 int pmg_multigrid_vmult(pmg_multigrid mg, void *dst, const void *src, pmg_stream stream);
+/* *uses_graph = 1 if vmult replays a captured HIP graph (see above) */
+int pmg_multigrid_uses_graph(pmg_multigrid mg, int *uses_graph);
 
 /* ------------------------------------------------------------ CG
  * Preconditioned CG of deal.II SolverCG as driven by MultigridSolver::solve_cg

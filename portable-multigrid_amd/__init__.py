@@ -141,6 +141,7 @@ def lib():
         "pmg_multigrid_create": (_i, [_i, _vp, _vp, _vp, _i, _d, _i, P(_vp)]),
         "pmg_multigrid_destroy": (_i, [_vp]),
         "pmg_multigrid_vmult": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_multigrid_uses_graph": (_i, [_vp, P(_i)]),
         "pmg_cg_solve": (_i, [_vp, _i, _vp, _vp, _vp, _d, _i, P(_i), P(_d), P(_d), _vp]),
         "pmg_vector_dot": (_i, [_i, _i64, _vp, _vp, P(_d), _vp]),
         "pmg_vector_convert": (_i, [_i64, _i, _vp, _i, _vp, _vp]),
@@ -536,6 +537,12 @@ class VCycleMultigrid:
         p = self.outer_precision
         _check(lib().pmg_multigrid_vmult(self.h, _dev_ptr(dst, p, self.n_fine),
                                          _dev_ptr(src, p, self.n_fine), _stream()))
+
+    def uses_graph(self):
+        """True if the V-cycle body replays a captured HIP graph."""
+        v = ctypes.c_int()
+        _check(lib().pmg_multigrid_uses_graph(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
 
 def solve_cg(op, b, x, precond=PRECOND_JACOBI, mg=None, rel_tol=1e-10, max_it=1000,

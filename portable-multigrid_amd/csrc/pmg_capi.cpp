@@ -333,6 +333,14 @@ int pmg_multigrid_destroy(pmg_multigrid mg) {
   return guard([&] { delete mg; });
 }
 
+int pmg_multigrid_uses_graph(pmg_multigrid mg, int *uses_graph) {
+  return guard([&] {
+    PMG_NOT_NULL(mg);
+    PMG_NOT_NULL(uses_graph);
+    *uses_graph = mg->impl->uses_graph() ? 1 : 0;
+  });
+}
+
 int pmg_multigrid_vmult(pmg_multigrid mg, void *dst, const void *src, pmg_stream s) {
   return guard([&] {
     PMG_NOT_NULL(mg);

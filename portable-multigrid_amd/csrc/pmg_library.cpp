@@ -864,6 +864,36 @@ VCycleMultigrid::VCycleMultigrid(std::vector<const Operator *> ops,
     t_[l].reinit(precision_, ops_[l]->m());
   }
   coarse_work_.reinit(precision_, ops_[0]->m());
+  // The V-cycle is a fixed launch sequence on device buffers of its own; it is
+  // captured once into a HIP graph and replayed, unless the coarse solve needs
+  // host round trips (CG on a large coarse level).  PMG_VCYCLE_GRAPH=0 turns
+  // graphs off.
+  const auto *bk3c = dynamic_cast<const LaplaceOperatorBK3 *>(ops_[0]);
+  const bool host_free = coarse_kind_ == PMG_COARSE_CHEBYSHEV ||
+                         (bk3c && bk3c->n_cells() <= kCoarseSingleWorkgroupCells);
+  const char *genv = std::getenv("PMG_VCYCLE_GRAPH");
+  use_graph_ = host_free && !(genv && std::string(genv) == "0");
+}
+
+VCycleMultigrid::~VCycleMultigrid() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+}
+
+void VCycleMultigrid::capture() const {
+  if (!cap_stream_) PMG_HIP_CHECK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+  PMG_HIP_CHECK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    v_cycle(levels() - 1, cap_stream_);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(cap_stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  PMG_HIP_CHECK(hipStreamEndCapture(cap_stream_, &graph_));
+  PMG_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
 }
 
 void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
@@ -915,7 +945,12 @@ void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {
   else if (in64) launch_convert<float, double>(n, static_cast<float *>(def_[top].data()), static_cast<const double *>(src), s);
   else if (lv64) launch_convert<double, float>(n, static_cast<double *>(def_[top].data()), static_cast<const float *>(src), s);
   else launch_copy<float>(n, static_cast<float *>(def_[top].data()), static_cast<const float *>(src), s);
-  v_cycle(top, s);
+  if (use_graph_) {
+    if (!graph_exec_) capture();
+    PMG_HIP_CHECK(hipGraphLaunch(graph_exec_, s));
+  } else {
+    v_cycle(top, s);
+  }
   if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(dst), static_cast<const double *>(sol_[top].data()), s);
   else if (in64) launch_convert<double, float>(n, static_cast<double *>(dst), static_cast<const float *>(sol_[top].data()), s);
   else if (lv64) launch_convert<float, double>(n, static_cast<float *>(dst), static_cast<const double *>(sol_[top].data()), s);

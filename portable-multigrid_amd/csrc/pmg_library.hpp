@@ -242,12 +242,18 @@ class VCycleMultigrid {
                   std::vector<const MGTransfer *> transfers,
                   std::vector<const PreconditionChebyshev *> smoothers, int coarse_kind,
                   double coarse_tol, int outer_precision);
+  ~VCycleMultigrid();
+  VCycleMultigrid(const VCycleMultigrid &) = delete;
+  VCycleMultigrid &operator=(const VCycleMultigrid &) = delete;
   void vmult(void *dst, const void *src, hipStream_t s) const;
   int outer_precision() const { return outer_precision_; }
   int levels() const { return static_cast<int>(ops_.size()); }
+  // the V-cycle body runs as a captured HIP graph (no host sync inside it)
+  bool uses_graph() const { return use_graph_; }
 
  private:
   void v_cycle(int level, hipStream_t s) const;
+  void capture() const;
   std::vector<const Operator *> ops_;
   std::vector<const MGTransfer *> xfer_;
   std::vector<const PreconditionChebyshev *> sm_;
@@ -256,6 +262,10 @@ class VCycleMultigrid {
   std::unique_ptr<PreconditionChebyshev> coarse_cheby_;
   mutable std::vector<DeviceVector> sol_, def_, t_;
   mutable CGWork coarse_work_;
+  bool use_graph_ = false;
+  mutable hipStream_t cap_stream_ = nullptr;
+  mutable hipGraph_t graph_ = nullptr;
+  mutable hipGraphExec_t graph_exec_ = nullptr;
 };
 
 }  // namespace pmg

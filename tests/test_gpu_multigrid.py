@@ -225,3 +225,28 @@ def test_weighted_transfer_path(pmg, orc, kind):
         yc = _t(np.zeros(nc))
         tr.restrict_and_add(yc, _t(xf))
         assert rel_err(yc.cpu().numpy(), ref.restrict_add(xf, np.zeros(nc))) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_vcycle_graph_replays_match_eager(pmg, monkeypatch, prec):
+    """The captured V-cycle graph, replayed several times with eager work and
+    synchronisations in between, gives the eager V-cycle's result each time."""
+    import torch
+    p = pmg.FP64 if prec == "fp64" else pmg.FP32
+    levels = pmg.ph_levels(pmg.box(8, deform=1, amplitude=0.05), 4, 3)
+    mg_g, ops_g, _, _ = pmg.build_hierarchy(levels, precision=p, outer_precision=pmg.FP64)
+    monkeypatch.setenv("PMG_VCYCLE_GRAPH", "0")
+    mg_e, ops_e, _, _ = pmg.build_hierarchy(levels, precision=p, outer_precision=pmg.FP64)
+    assert mg_g.uses_graph() and not mg_e.uses_graph()
+    n = ops_g[-1].m()
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for rep in range(4):
+        r = _t(seeded_uniform(n, 400 + rep))
+        zg, ze = torch.zeros_like(r), torch.zeros_like(r)
+        mg_g.vmult(zg, r)
+        junk = torch.randn(1 << 20, dtype=torch.float64, device="cuda").sum()  # eager work
+        torch.cuda.synchronize()
+        mg_e.vmult(ze, r)
+        torch.cuda.synchronize()
+        assert rel_err(zg.cpu().numpy(), ze.cpu().numpy()) < tol, rep
+        del junk
