@@ -18,7 +18,10 @@ Also reported on the same problem:
   cpu_baseline  the CPU oracle's vmult (plain-C restatement of the reference
                 kernel, OpenMP over 8 cell colours) on the host cores.
 
-  configs       the other workloads on their own sizes: BK3 Q4 16^3 Gauss-6
+  configs       rows: every hot-path kernel alone (setup, BK3, SIP-DG, h/p/DG-CG
+                transfers, Chebyshev step, V-cycle) with its time and HBM
+                fraction, FP64 and FP32; and the other workloads on their own
+                sizes: BK3 Q4 16^3 Gauss-6
                 vmult + Jacobi-PCG to 1e-8; SIP-DG Q3 on 32^3 deformed cells
                 (vmult roofline, DG MG-CG FP64 and mixed); ph-MG Q6 on 48^3
                 deformed cells (FP64 vs FP32 V-cycle); adjoint gradient 32^3 Q4
@@ -307,6 +310,112 @@ def bench_config5(pmg, n_dir=20):
             "fd_median_rel_err": float(np.median(errs))}
 
 
+def _roof(t, nbytes):
+    return {"time_us": t * 1e6, "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / t / 1e9,
+            "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS}
+
+
+def bench_rows(pmg):
+    """One line per kernel of the hot path, timed alone with HIP events (20
+    launches after a warm-up) on its own configuration's sizes, against its
+    algorithmic bytes (s = bytes per scalar; maps are listed apart, they are
+    not in the byte model): operator setup (geometry, diagonal), the BK3 and
+    SIP-DG operators, h-, p- and DG<->CG transfers, the Chebyshev step, the
+    V-cycle and one outer CG iteration, in FP64 and FP32."""
+    import torch
+    rows = {}
+    dt = {pmg.FP64: torch.float64, pmg.FP32: torch.float32}
+    sz = {pmg.FP64: 8, pmg.FP32: 4}
+    fine = pmg.box(64)
+    for prec in (pmg.FP64, pmg.FP32):
+        tag = "fp64" if prec == pmg.FP64 else "fp32"
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op = pmg.LaplaceOperatorBK3(fine, 4, 5, prec)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        op.compute_diagonal()
+        torch.cuda.synchronize()
+        t_diag = time.perf_counter() - t0
+        N, by = op.m(), op.algorithmic_bytes()[0]
+        x = torch.rand(N, dtype=dt[prec], device="cuda")
+        y = op.initialize_dof_vector()
+        rows[f"a1_a2_bk3_vmult_q4_64cube_{tag}"] = _roof(time_events(lambda: op.vmult(y, x), 20), by)
+        rows[f"a3_a4_operator_setup_q4_64cube_{tag}"] = {
+            "time_ms": t_setup * 1e3, "note": "host DoF map upload + G kernel (setup, not a hot op)"}
+        rows[f"a5_diagonal_q4_64cube_{tag}"] = {"time_ms": t_diag * 1e3, "note": "setup"}
+        sm = pmg.PreconditionChebyshev(op, 3, 15.0, 10)
+        b = torch.rand(N, dtype=dt[prec], device="cuda")
+        t_step = time_events(lambda: sm.step(y, b), 20)
+        t_mv = rows[f"a1_a2_bk3_vmult_q4_64cube_{tag}"]["time_us"] * 1e-6
+        rows[f"a16_chebyshev_step_deg3_q4_64cube_{tag}"] = {
+            "time_us": t_step * 1e6, "matvecs": 3, "matvec_share": 3 * t_mv / t_step,
+            "vector_update_us": (t_step - 3 * t_mv) / 3 * 1e6,
+            "vector_update_bytes": 7 * sz[prec] * N}
+        del sm, op, x, y, b
+        torch.cuda.empty_cache()
+
+        coarse = pmg.box(32)
+        tr = pmg.GeometricTransfer(coarse, 4, prec)
+        xc = torch.rand(tr.n_c, dtype=dt[prec], device="cuda")
+        yf = torch.zeros(tr.n_f, dtype=dt[prec], device="cuda")
+        s = sz[prec]
+        maps = 4 * 32 ** 3 * (5 ** 3 + 9 ** 3)
+        rows[f"a11_a13_h_prolongate_q4_32to64_{tag}"] = dict(
+            _roof(time_events(lambda: tr.prolongate_and_add(yf, xc), 20), s * (tr.n_c + 2 * tr.n_f)),
+            map_bytes=maps)
+        rows[f"a12_a13_h_restrict_q4_64to32_{tag}"] = dict(
+            _roof(time_events(lambda: tr.restrict_and_add(xc, yf), 20), s * (tr.n_f + 2 * tr.n_c)),
+            map_bytes=maps)
+        del tr, xc, yf
+        b48 = pmg.box(48, deform=1, amplitude=0.05)
+        tr = pmg.PolynomialTransfer(b48, 3, 6, prec)
+        xc = torch.rand(tr.n_c, dtype=dt[prec], device="cuda")
+        yf = torch.zeros(tr.n_f, dtype=dt[prec], device="cuda")
+        maps = 4 * 48 ** 3 * (4 ** 3 + 7 ** 3)
+        rows[f"a11_a14_p_prolongate_q3to6_48cube_{tag}"] = dict(
+            _roof(time_events(lambda: tr.prolongate_and_add(yf, xc), 20), s * (tr.n_c + 2 * tr.n_f)),
+            map_bytes=maps)
+        rows[f"a12_a14_p_restrict_q6to3_48cube_{tag}"] = dict(
+            _roof(time_events(lambda: tr.restrict_and_add(xc, yf), 20), s * (tr.n_f + 2 * tr.n_c)),
+            map_bytes=maps)
+        del tr, xc, yf
+        b32 = pmg.box(32, deform=1, amplitude=0.05)
+        tr = pmg.ContinuousTransfer(b32, 3, prec)
+        xc = torch.rand(tr.n_c, dtype=dt[prec], device="cuda")
+        yf = torch.zeros(tr.n_f, dtype=dt[prec], device="cuda")
+        maps = 4 * 32 ** 3 * 4 ** 3
+        rows[f"a15_cg_to_dg_q3_32cube_{tag}"] = dict(
+            _roof(time_events(lambda: tr.prolongate_and_add(yf, xc), 20), s * (tr.n_c + 2 * tr.n_f)),
+            map_bytes=maps)
+        rows[f"a15_dg_to_cg_q3_32cube_{tag}"] = dict(
+            _roof(time_events(lambda: tr.restrict_and_add(xc, yf), 20), s * (tr.n_f + 2 * tr.n_c)),
+            map_bytes=maps)
+        op = pmg.LaplaceOperatorDG(b32, 3, 4, prec, penalty_kind=1)
+        x = torch.rand(op.m(), dtype=dt[prec], device="cuda")
+        y = op.initialize_dof_vector()
+        rows[f"a6_a10_sipdg_vmult_q3_32cube_{tag}"] = _roof(
+            time_events(lambda: op.vmult(y, x), 20), op.algorithmic_bytes()[0])
+        del tr, xc, yf, op, x, y
+        torch.cuda.empty_cache()
+
+    for prec in (pmg.FP64, pmg.FP32):
+        tag = "fp64" if prec == pmg.FP64 else "fp32"
+        mg, ops, trs, sms, _ = pmg.geometric_hierarchy(
+            fine, 4, 6, precision=prec, smoother_degree=3, smoothing_range=15.0,
+            eig_cg_n_iterations=10, coarse=pmg.COARSE_PCG, coarse_tol=1e-12,
+            outer_precision=pmg.FP64)
+        r = torch.rand(ops[-1].m(), dtype=torch.float64, device="cuda")
+        z = torch.zeros_like(r)
+        rows[f"a17_vcycle_q4_64cube_6levels_{tag}"] = {
+            "time_us": time_events(lambda: mg.vmult(z, r), 10) * 1e6, "hip_graph": mg.uses_graph(),
+            "fine_level_matvecs": 6, "levels": "h 2^3..64^3 at Q4"}
+        del mg, ops, trs, sms, r, z
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +433,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (Jacobi-PCG 16^3, SIP-DG, ph-MG Q6, adjoint)")
     ap.add_argument("--config", action="append", default=None,
-                    help="run only these configuration legs (repeatable): bk3_q4_16cube_jacobi_pcg, "
+                    help="run only these configuration legs (repeatable): rows, bk3_q4_16cube_jacobi_pcg, "
                          "sipdg_q3_32cube_deformed, ph_mg_q6_48cube_deformed, adjoint_q4_32cube")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -412,7 +521,8 @@ def main():
     torch.cuda.synchronize()
     if not args.no_configs:
         extra = {}
-        for name, fn in [("bk3_q4_16cube_jacobi_pcg", bench_config1),
+        for name, fn in [("rows", bench_rows),
+                         ("bk3_q4_16cube_jacobi_pcg", bench_config1),
                          ("sipdg_q3_32cube_deformed", bench_config3),
                          ("ph_mg_q6_48cube_deformed", bench_config4),
                          ("adjoint_q4_32cube", bench_config5)]:

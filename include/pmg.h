@@ -172,20 +172,20 @@ int pmg_operator_assemble_rhs(pmg_operator op, int f_kind, int n_q_points_1d, vo
 int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_points_1d,
                           double *error, pmg_stream stream);
 /* Cell-loop schedule (BK3 operators):
- *  0 atomic (default): one launch, every cell scatter-adds its (p+1)^3 values
- *    with device FP atomics (as the reference's Kokkos::atomic_add,
+ *  0 atomic: one launch, every cell scatter-adds its (p+1)^3 values with
+ *    device FP atomics (as the reference's Kokkos::atomic_add,
  *    bk3_kokkos_kernel.h:732);
  *  1 colored: one launch per colour of a greedy cell colouring with a plain
  *    read-modify-write scatter (atomics-free);
  *  2 patch: one launch; each workgroup's batch of cells accumulates the union
  *    of its DoFs in LDS and writes it once, plain stores for DoFs no other
- *    batch touches, atomics for the rest.  Pays off when each batch's own
- *    DoFs are numbered contiguously (cell-blocked numberings); on
- *    lexicographic numberings the atomic schedule is faster.
+ *    batch touches, atomics for the rest (fewer, longer-run atomics).
+ * Default: patch for n_q_points_1d <= 4 (FP64) or <= 5 (FP32), atomic
+ * otherwise (the faster of the two on MI355X for each level type of the MG
+ * hierarchies); PMG_BK3_SCHEDULE=atomic|colored|patch overrides it.
  * g_load: 0 non-temporal G loads at kernel entry, 1 ordinary loads at entry,
- * 2 non-temporal loads at first use (variants 0, 1 of the atomic schedule
- * only for degree 4).  Defaults come from PMG_BK3_SCHEDULE=atomic|colored|patch
- * and PMG_BK3_GLOAD. */
+ * 2 (default, or PMG_BK3_GLOAD) non-temporal loads at first use (variants 0, 1
+ * of the atomic schedule only for degree 4). */
 int pmg_operator_set_schedule(pmg_operator op, int schedule, int g_load);
 /* Bytes one vmult must move (BK3: s(2N + 6 C nq^3); DG: adds the face data
  * and connectivity) and its flop count. */

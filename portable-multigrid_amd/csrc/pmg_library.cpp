@@ -155,11 +155,21 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   perm_.upload(perm);
   constrained_.upload(constrained, static_cast<size_t>(n_constrained));
   {
-    const char *env = std::getenv("PMG_BK3_SCHEDULE");  // atomic | colored | patch
+    // Default scatter schedule: the patch schedule where a batch holds enough
+    // cells for its LDS patch to fold many shared DoFs before the global
+    // update (E = 256/nq^2 cells: FP64 nq <= 4, i.e. E >= 16; FP32 nq <= 5),
+    // the per-cell atomic scatter otherwise.  From vmult timings on MI355X over
+    // the level types of the MG hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3
+    // cells), patch / atomic time: FP64 nq 2..4 0.65-0.94, nq 5 0.97-1.21,
+    // nq >= 6 0.96-1.6; FP32 nq 3..5 0.61-0.95, nq >= 6 1.0-1.4.
+    // PMG_BK3_SCHEDULE=atomic|colored|patch overrides.
+    const char *env = std::getenv("PMG_BK3_SCHEDULE");
     const char *genv = std::getenv("PMG_BK3_GLOAD");
-    const std::string kind = env ? env : "atomic";
-    set_schedule(kind == "colored" ? kColored : kind == "atomic" ? kAtomic : kPatch,
-                 genv ? std::atoi(genv) : 2);  // late non-temporal G loads measured fastest
+    int kind = nq <= (precision == PMG_FP32 ? 5 : 4) ? kPatch : kAtomic;
+    if (env)
+      kind = std::string(env) == "colored" ? kColored
+             : std::string(env) == "atomic" ? kAtomic : kPatch;
+    set_schedule(kind, genv ? std::atoi(genv) : 2);  // late non-temporal G loads measured fastest
   }
   G_.reinit(precision, g_size(nq, n_cells));
   if (cell_vertices) {

@@ -48,10 +48,14 @@ def test_vmult_fp32_matches_oracle(pmg, orc, n, p, nq, deform):
     print(f"[bk3 fp32] n={n} p={p} nq={nq}", flush=True)
     op, ref = _pair(pmg, orc, n, p, nq, deform, pmg.FP32)
     x = seeded_uniform(op.m(), 2000 + p)
-    dst = op.initialize_dof_vector()
-    op.vmult(dst, torch.tensor(x, dtype=torch.float32, device="cuda"))
     y = ref.vmult(x.astype(np.float32).astype(np.float64))
-    assert rel_err(dst.cpu().numpy(), y) < 2e-6
+    # the default schedule (patch for nq <= 5 in FP32), then the atomic one
+    for sched in (None, pmg.SCHEDULE_ATOMIC):
+        if sched is not None:
+            op.set_schedule(sched, 2)
+        dst = op.initialize_dof_vector()
+        op.vmult(dst, torch.tensor(x, dtype=torch.float32, device="cuda"))
+        assert rel_err(dst.cpu().numpy(), y) < 2e-6, sched
 
 
 def test_apply_add_and_constrained_rows(pmg, orc):
@@ -145,10 +149,11 @@ def test_bad_arguments_raise(pmg):
 
 @pytest.mark.parametrize("p,nq,sched,gload", [
     (4, 5, 1, 0), (4, 5, 0, 1), (4, 5, 0, 0), (4, 5, 2, 2), (4, 6, 1, 0), (4, 6, 2, 2),
-    (2, 3, 1, 0), (6, 7, 1, 0), (1, 2, 2, 2), (3, 4, 2, 2), (5, 7, 2, 2), (6, 8, 2, 2)])
+    (2, 3, 1, 0), (6, 7, 1, 0), (1, 2, 2, 2), (3, 4, 2, 2), (5, 7, 2, 2), (6, 8, 2, 2),
+    (1, 2, 0, 2), (3, 4, 0, 2)])
 def test_schedules_agree(pmg, orc, p, nq, sched, gload):
-    """Atomics-free colour-by-colour scatter, the batch-patch scatter and the
-    G-load variants give the oracle's result as the default atomic scatter."""
+    """Atomics-free colour-by-colour scatter, the batch-patch scatter, the
+    per-cell atomic scatter and the G-load variants all give the oracle's result."""
     import torch
     op, ref = _pair(pmg, orc, 3, p, nq, 1, pmg.FP64)
     op.set_schedule(sched, gload)
