@@ -121,6 +121,10 @@ template <typename T>
 void launch_dg_face_geometry(int nq, int64_t n_faces, const uint32_t *finfo, const double *cv,
                              const double *qp, const double *qw, int metric, T *fdata,
                              hipStream_t s);
+// DG -> CG restriction as a gather: dst[i] += sum of src[list[off[i] .. off[i+1])]
+template <typename T>
+void launch_dg_cg_restrict_gather(int64_t n_cg, const uint32_t *off, const uint32_t *list,
+                                  const T *src, T *dst, hipStream_t s);
 // DG(p) <-> CG(p): prolongate dst[dg(e)] += src[cg_map[e]], restrict
 // dst[cg_map[e]] += src[dg(e)]; dg(e) = dg_map ? dg_map[e] : e; invalid CG entries skipped
 template <typename T>

@@ -486,6 +486,18 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     }
     map_c_.upload(map_coarse, static_cast<size_t>(n_cells * n3));
     if (map_fine) map_f_.upload(map_fine, static_cast<size_t>(n_cells * n3));
+    if (n_cells * n3 < (int64_t(1) << 32)) {
+      std::vector<uint32_t> off(static_cast<size_t>(n_dofs_coarse + 1), 0u);
+      for (int64_t e = 0; e < n_cells * n3; ++e)
+        if (map_coarse[e] != kInvalid) ++off[map_coarse[e] + 1];
+      for (int64_t i = 0; i < n_dofs_coarse; ++i) off[i + 1] += off[i];
+      std::vector<uint32_t> list(off.back()), fill(off.begin(), off.end() - 1);
+      for (int64_t e = 0; e < n_cells * n3; ++e)
+        if (map_coarse[e] != kInvalid)
+          list[fill[map_coarse[e]]++] = map_fine ? map_fine[e] : static_cast<uint32_t>(e);
+      inv_off_.upload(off);
+      inv_list_.upload(list);
+    }
     return;
   }
   PMG_REQUIRE(map_coarse && map_fine && weights, PMG_ERR_INVALID_ARG, "NULL setup array");
@@ -594,7 +606,15 @@ void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) con
   PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
   if (kind_ == PMG_TRANSFER_DG_CG) {
     const int64_t n = n_cells_ * nmc_ * nmc_ * nmc_;
-    if (precision_ == PMG_FP64)
+    if (inv_off_.ptr && precision_ == PMG_FP64)
+      launch_dg_cg_restrict_gather<double>(n_dofs_c_, inv_off_.ptr, inv_list_.ptr,
+                                           static_cast<const double *>(src),
+                                           static_cast<double *>(dst), s);
+    else if (inv_off_.ptr)
+      launch_dg_cg_restrict_gather<float>(n_dofs_c_, inv_off_.ptr, inv_list_.ptr,
+                                          static_cast<const float *>(src),
+                                          static_cast<float *>(dst), s);
+    else if (precision_ == PMG_FP64)
       launch_dg_cg_transfer<double>(false, n, map_c_.ptr, map_f_.ptr, static_cast<const double *>(src),
                                     static_cast<double *>(dst), s);
     else

@@ -191,6 +191,9 @@ class MGTransfer {
   Transfer1D<double> M64_{};
   Transfer1D<float> M32_{};
   DeviceArray<uint32_t> map_c_, map_f_;
+  // DG-CG: inverse map (CSR) of the CG DoFs to their DG copies for the
+  // atomics-free restriction (empty if the entry count needs 64-bit offsets)
+  DeviceArray<uint32_t> inv_off_, inv_list_;
   DeviceVector w_prolong_, w_restrict_;
 };
 

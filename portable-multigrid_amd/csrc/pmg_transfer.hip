@@ -11,7 +11,13 @@
 // prolongation does z LAST and the restriction z FIRST, so the fine-side
 // scatter-add / gather (nmf^3 per cell, the dominant traffic) is issued by
 // lanes (a,b) over consecutive fine DoFs along x; M is a kernel argument held
-// in scalar registers.
+// in scalar registers.  The two LDS buffers of a cell hold at most
+// nmc nmf^2 and nmc^2 nmf values (the partially swept tensors), which keeps
+// the LDS per workgroup small enough for 8 workgroups per CU: both kernels are
+// bound by the latency of their dependent gathers, so occupancy is what hides
+// it.  The owner-mode prolongation loads its fine map entries and the fine
+// values it will update at kernel entry (it is their only writer), so those
+// two dependent loads overlap the coarse gather and the sweeps.
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
@@ -23,6 +29,11 @@ struct XferTile {
   static constexpr int cells = (256 / NF2) > 0 ? 256 / NF2 : 1;
   static constexpr int lanes = ((cells * NF2 + 63) / 64) * 64;
 };
+
+// LDS elements per cell: buffer 0 holds NC x NF x NF (or fewer) values,
+// buffer 1 NC x NC x NF
+template <int NC, int NF>
+constexpr int xfer_lds_cell() { return NC * NF * NF + NC * NC * NF; }
 
 // OWNER: map_f carries bit 31 on the one (cell, local) entry that owns each
 // fine DoF (no weights): the owner adds scale * value with a plain
@@ -38,13 +49,25 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
                   const T *__restrict__ xc, T *__restrict__ yf)
 {
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
-  constexpr int E = XferTile<NF>::cells;
-  __shared__ T lds[E * 2 * NF3];
+  constexpr int E = XferTile<NF>::cells, LC = xfer_lds_cell<NC, NF>();
+  __shared__ T lds[E * LC];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x, e = tid / NF2, t = tid - e * NF2, ta = t % NF, tb = t / NF;
   const int64_t cell = static_cast<int64_t>(lb) * E + e;
   const bool active = e < E && cell < n_cells;
-  T *L0 = lds + (active ? e : 0) * 2 * NF3, *L1 = L0 + NF3;
+  T *L0 = lds + (active ? e : 0) * LC, *L1 = L0 + NC * NF2;
+
+  // owner mode: this lane's fine column (map entries, current fine values)
+  // is fetched now; nothing else in the launch writes these DoFs
+  const int64_t base = cell * NF3 + tb * NF + ta;
+  uint32_t mf[NF];
+  T yv[NF];
+  if (OWNER && active) {
+#pragma unroll
+    for (int c = 0; c < NF; ++c) mf[c] = map_f[base + c * NF2];
+#pragma unroll
+    for (int c = 0; c < NF; ++c) yv[c] = (mf[c] & 0x80000000u) ? yf[mf[c] & 0x7FFFFFFFu] : T(0);
+  }
 
   // gather the coarse cell values, lane (i,j): L0[k][j][i]
   if (active && ta < NC && tb < NC) {
@@ -89,15 +112,13 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
     T l[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) l[k] = L0[(k * NF + tb) * NF + ta];
-    const int64_t base = cell * NF3 + tb * NF + ta;
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
       T v = 0;
 #pragma unroll
       for (int k = 0; k < NC; ++k) v += M.M[k * NF + c] * l[k];
       if (OWNER) {
-        const uint32_t m = map_f[base + c * NF2];
-        if (m & 0x80000000u) yf[m & 0x7FFFFFFFu] += scale * v;
+        if (mf[c] & 0x80000000u) yf[mf[c] & 0x7FFFFFFFu] = yv[c] + scale * v;
       } else {
         const T wt = w[base + c * NF2];
         if (wt != T(0)) atomic_add(yf + map_f[base + c * NF2], wt * v);
@@ -113,13 +134,13 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
                 const T *__restrict__ xf, T *__restrict__ yc)
 {
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
-  constexpr int E = XferTile<NF>::cells;
-  __shared__ T lds[E * 2 * NF3];
+  constexpr int E = XferTile<NF>::cells, LC = xfer_lds_cell<NC, NF>();
+  __shared__ T lds[E * LC];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x, e = tid / NF2, t = tid - e * NF2, ta = t % NF, tb = t / NF;
   const int64_t cell = static_cast<int64_t>(lb) * E + e;
   const bool active = e < E && cell < n_cells;
-  T *L0 = lds + (active ? e : 0) * 2 * NF3, *L1 = L0 + NF3;
+  T *L0 = lds + (active ? e : 0) * LC, *L1 = L0 + NC * NF2;
 
   // weighted gather of the fine column over c by lane (a,b) (consecutive fine
   // DoFs across lanes), z sweep in registers: L0[k][b][a]
