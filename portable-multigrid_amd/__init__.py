@@ -344,7 +344,7 @@ class LaplaceOperatorBK3:
         """Cell-loop schedule: SCHEDULE_ATOMIC (per-cell device atomics),
         SCHEDULE_COLORED (one launch per colour) or SCHEDULE_PATCH (batch
         patches in LDS).  The constructor's default is PATCH for
-        n_q_points_1d <= 4 (FP64) or <= 5 (FP32), ATOMIC otherwise."""
+        n_q_points_1d <= 4 (FP64) or <= 6 (FP32), ATOMIC otherwise."""
         _check(lib().pmg_operator_set_schedule(self.h, int(schedule), g_load))
 
     def assemble_rhs(self, f_kind=1, n_q_points_1d=0):

@@ -89,10 +89,9 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 // its source column from the patch, and the cells' results are summed into
 // the patch with LDS atomics; the patch is then written once: DoFs that no
 // other batch touches with plain (coalescing) stores, DoFs shared with other
-// batches with device-scope atomics.  On a structured mesh a batch is a
-// compact block (2^3 cells at nq = 5), so about 40-60 % of the device
-// atomics of the per-cell scatter disappear.  ASSIGN: y[d] = value for owned
-// DoFs (vmult; y must be zero on shared DoFs), else y[d] += value.
+// batches with device atomics, in sorted DoF order, so a wave's atomics cover
+// long runs of consecutive DoFs.  ASSIGN: y[d] = value for owned DoFs
+// (vmult; y must be zero on shared DoFs), else y[d] += value.
 template <int NM, int NQ, typename T, int GMODE, bool ASSIGN>
 __global__ void __launch_bounds__(256)
 bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
@@ -101,34 +100,48 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells, PMAX = E * NM3;
+  constexpr int LANES = Bk3Tile<NQ>::lanes, PL = (PMAX + LANES - 1) / LANES;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ3)];
   __shared__ T patch[PMAX];
   const int64_t batch = xcd_logical_block(blockIdx.x, gridDim.x);
   const int64_t cell_base = batch * E;
   if (cell_base >= n_cells) return;  // whole workgroup: padding blocks of the grid
   const int tid = threadIdx.x;
-  const int np = static_cast<int>(pcount[batch]);
-  const uint32_t *pd = pdof + batch * PMAX;
-  for (int i = tid; i < np; i += blockDim.x) patch[i] = x[pd[i] & 0x7FFFFFFFu];
-  __syncthreads();
-
   const int e = tid / NQ2, t = tid - e * NQ2, ta = t % NQ, tb = t / NQ;
   const int64_t cell = cell_base + e;
   const bool active = (e < E) && (cell < n_cells);
   const bool col = active && ta < NM && tb < NM;
+
+  // All index loads are issued up front, each lane's patch entries once into
+  // registers (they are reused for the write-out), so the workgroup pays one
+  // memory latency per dependent step instead of one per loop iteration.
+  const int np = static_cast<int>(pcount[batch]);
+  const uint32_t *pd = pdof + batch * PMAX;
+  uint16_t pi[NM];
+  if (col) {
+    const uint16_t *pc = pidx + cell * NM3 + tb * NM + ta;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) pi[k] = pc[k * NM2];
+  }
+  uint32_t pw[PL];
+#pragma unroll
+  for (int j = 0; j < PL; ++j) pw[j] = tid + j * LANES < np ? pd[tid + j * LANES] : 0u;
+  T xv[PL];
+#pragma unroll
+  for (int j = 0; j < PL; ++j) xv[j] = tid + j * LANES < np ? x[pw[j] & 0x7FFFFFFFu] : T(0);
+#pragma unroll
+  for (int j = 0; j < PL; ++j)
+    if (tid + j * LANES < np) patch[tid + j * LANES] = xv[j];
+  __syncthreads();
+
   T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
   const T *Gc = G + g_index(NQ, active ? cell : 0, 0, t);
   T g[6][NQ];
   if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
-  uint16_t pi[NM];
   T u[NM], v[NM];
   if (col) {
-    const uint16_t *pc = pidx + cell * NM3 + tb * NM + ta;
 #pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      pi[k] = pc[k * NM2];
-      u[k] = pi[k] == 0xFFFFu ? T(0) : patch[pi[k]];
-    }
+    for (int k = 0; k < NM; ++k) u[k] = pi[k] == 0xFFFFu ? T(0) : patch[pi[k]];
   }
   bk3_cell_apply<NM, NQ, T, GMODE>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3, active, ta, tb,
                                    patch, np);
@@ -138,12 +151,16 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
       if (pi[k] != 0xFFFFu) atomicAdd(patch + pi[k], v[k]);
   }
   __syncthreads();
-  for (int i = tid; i < np; i += blockDim.x) {
-    const uint32_t w = pd[i];
-    const uint32_t d = w & 0x7FFFFFFFu;
-    if (w & 0x80000000u) atomic_add(y + d, patch[i]);
-    else if (ASSIGN) y[d] = patch[i];
-    else y[d] += patch[i];
+#pragma unroll
+  for (int j = 0; j < PL; ++j) {
+    const int i = tid + j * LANES;
+    if (i < np) {
+      const uint32_t w = pw[j];
+      const uint32_t d = w & 0x7FFFFFFFu;
+      if (w & 0x80000000u) atomic_add(y + d, patch[i]);
+      else if (ASSIGN) y[d] = patch[i];
+      else y[d] += patch[i];
+    }
   }
 }
 

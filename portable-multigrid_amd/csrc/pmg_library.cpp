@@ -157,15 +157,15 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   {
     // Default scatter schedule: the patch schedule where a batch holds enough
     // cells for its LDS patch to fold many shared DoFs before the global
-    // update (E = 256/nq^2 cells: FP64 nq <= 4, i.e. E >= 16; FP32 nq <= 5),
+    // update (E = 256/nq^2 cells: FP64 nq <= 4, i.e. E >= 16; FP32 nq <= 6),
     // the per-cell atomic scatter otherwise.  From vmult timings on MI355X over
     // the level types of the MG hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3
-    // cells), patch / atomic time: FP64 nq 2..4 0.65-0.94, nq 5 0.97-1.21,
-    // nq >= 6 0.96-1.6; FP32 nq 3..5 0.61-0.95, nq >= 6 1.0-1.4.
+    // cells), patch / atomic time: FP64 nq 2..4 0.63-0.87, nq 5 0.87-1.07,
+    // nq 6 0.94-1.02, nq >= 7 1.27-1.35; FP32 nq 2..6 0.58-0.96, nq >= 7 1.10-1.13.
     // PMG_BK3_SCHEDULE=atomic|colored|patch overrides.
     const char *env = std::getenv("PMG_BK3_SCHEDULE");
     const char *genv = std::getenv("PMG_BK3_GLOAD");
-    int kind = nq <= (precision == PMG_FP32 ? 5 : 4) ? kPatch : kAtomic;
+    int kind = nq <= (precision == PMG_FP32 ? 6 : 4) ? kPatch : kAtomic;
     if (env)
       kind = std::string(env) == "colored" ? kColored
              : std::string(env) == "atomic" ? kAtomic : kPatch;

@@ -49,7 +49,7 @@ def test_vmult_fp32_matches_oracle(pmg, orc, n, p, nq, deform):
     op, ref = _pair(pmg, orc, n, p, nq, deform, pmg.FP32)
     x = seeded_uniform(op.m(), 2000 + p)
     y = ref.vmult(x.astype(np.float32).astype(np.float64))
-    # the default schedule (patch for nq <= 5 in FP32), then the atomic one
+    # the default schedule (patch for nq <= 6 in FP32), then the atomic one
     for sched in (None, pmg.SCHEDULE_ATOMIC):
         if sched is not None:
             op.set_schedule(sched, 2)
