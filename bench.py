@@ -436,7 +436,8 @@ def main():
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--schedule", choices=["atomic", "colored", "patch"], default=None,
                     help="BK3 cell-loop schedule (default: the library default)")
-    ap.add_argument("--gload", type=int, default=None, help="BK3 G-load policy 0/1/2")
+    ap.add_argument("--gload", type=int, default=None,
+                    help="BK3 G-load policy 0/1/2 (default: the schedule's own)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-configs", action="store_true",
@@ -459,7 +460,7 @@ def main():
     op = pmg.LaplaceOperatorBK3(fine_box, degree, nq, pmg.FP64)
     if args.schedule is not None or args.gload is not None:
         kinds = {"atomic": 0, "colored": 1, "patch": 2}
-        op.set_schedule(kinds[args.schedule or "atomic"], 2 if args.gload is None else args.gload)
+        op.set_schedule(kinds[args.schedule or "atomic"], -1 if args.gload is None else args.gload)
     N = op.m()
     g = torch.Generator(device="cuda").manual_seed(1234)
     x = torch.rand(N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
@@ -499,7 +500,7 @@ def main():
                                "ratio": gdofs / 14.16}
     if args.schedule is not None or args.gload is not None:
         out["config"]["schedule"] = {"scatter": args.schedule or "atomic",
-                                     "g_load": 2 if args.gload is None else args.gload}
+                                     "g_load": -1 if args.gload is None else args.gload}
     del y
     if args.calibrate_copy:
         # plain copy kernel of known size: 2^27 doubles read + written per launch

@@ -184,8 +184,10 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_po
  * otherwise (the faster of the two on MI355X for each level type of the MG
  * hierarchies); PMG_BK3_SCHEDULE=atomic|colored|patch overrides it.
  * g_load: 0 non-temporal G loads at kernel entry, 1 ordinary loads at entry,
- * 2 (default, or PMG_BK3_GLOAD) non-temporal loads at first use (variants 0, 1
- * of the atomic schedule only for degree 4). */
+ * 2 non-temporal loads at first use, -1 the schedule's default (0 for patch,
+ * 2 otherwise; PMG_BK3_GLOAD overrides the constructor's choice).  Variant 1,
+ * and variant 0 of the atomic schedule, exist for degree 4 only; elsewhere
+ * they fall back to 2. */
 int pmg_operator_set_schedule(pmg_operator op, int schedule, int g_load);
 /* Bytes one vmult must move (BK3: s(2N + 6 C nq^3); DG: adds the face data
  * and connectivity) and its flop count. */

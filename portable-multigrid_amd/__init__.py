@@ -340,11 +340,13 @@ class LaplaceOperatorBK3:
             self.h, _dev_ptr(u, self.precision, self._m), _dev_ptr(lam, self.precision, self._m),
             _dev_ptr(out, self.precision, self.n_cells * self.nq ** 3), _stream()))
 
-    def set_schedule(self, schedule=SCHEDULE_ATOMIC, g_load=2):
+    def set_schedule(self, schedule=SCHEDULE_ATOMIC, g_load=-1):
         """Cell-loop schedule: SCHEDULE_ATOMIC (per-cell device atomics),
         SCHEDULE_COLORED (one launch per colour) or SCHEDULE_PATCH (batch
         patches in LDS).  The constructor's default is PATCH for
-        n_q_points_1d <= 4 (FP64) or <= 6 (FP32), ATOMIC otherwise."""
+        n_q_points_1d <= 4 (FP64) or <= 6 (FP32), ATOMIC otherwise.  g_load:
+        G load policy (0 early non-temporal, 1 early, 2 late non-temporal,
+        -1 the schedule's default)."""
         _check(lib().pmg_operator_set_schedule(self.h, int(schedule), g_load))
 
     def assemble_rhs(self, f_kind=1, n_q_points_1d=0):

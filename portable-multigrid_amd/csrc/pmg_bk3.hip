@@ -322,7 +322,12 @@ template <int NM, int NQ, typename T, bool FULL>
 static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_t n_cells,
                        const Shape1D<T> &sh, hipStream_t s, const Bk3Schedule *sc, bool assign) {
   if (sc && sc->kind == kPatch && sc->pdof) {
-    bk3_launch_patch<NM, NQ, T, kGLateNT>(assign, x, y, G, *sc, n_cells, sh, s);
+    if (sc->gmode == kGEarlyNT)
+      bk3_launch_patch<NM, NQ, T, kGEarlyNT>(assign, x, y, G, *sc, n_cells, sh, s);
+    else if (FULL && sc->gmode == kGEarly)
+      bk3_launch_patch<NM, NQ, T, (FULL ? kGEarly : kGLateNT)>(assign, x, y, G, *sc, n_cells, sh, s);
+    else
+      bk3_launch_patch<NM, NQ, T, kGLateNT>(assign, x, y, G, *sc, n_cells, sh, s);
     return;
   }
   if (sc && sc->kind == kColored) {

@@ -217,7 +217,7 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int nq, do
 int pmg_operator_set_schedule(pmg_operator op, int schedule, int g_load) {
   return guard([&] {
     PMG_NOT_NULL(op);
-    PMG_REQUIRE(g_load >= 0 && g_load <= 2, PMG_ERR_INVALID_ARG, "g_load must be 0, 1 or 2");
+    PMG_REQUIRE(g_load >= -1 && g_load <= 2, PMG_ERR_INVALID_ARG, "g_load must be -1, 0, 1 or 2");
     PMG_REQUIRE(schedule >= 0 && schedule <= 2, PMG_ERR_INVALID_ARG,
                 "schedule must be 0 (atomic), 1 (colored) or 2 (patch)");
     bk3_of(op).set_schedule(schedule, g_load);

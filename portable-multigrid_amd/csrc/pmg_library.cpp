@@ -155,21 +155,22 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   perm_.upload(perm);
   constrained_.upload(constrained, static_cast<size_t>(n_constrained));
   {
-    // Default scatter schedule: the patch schedule where a batch holds enough
-    // cells for its LDS patch to fold many shared DoFs before the global
-    // update (E = 256/nq^2 cells: FP64 nq <= 4, i.e. E >= 16; FP32 nq <= 6),
-    // the per-cell atomic scatter otherwise.  From vmult timings on MI355X over
-    // the level types of the MG hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3
-    // cells), patch / atomic time: FP64 nq 2..4 0.63-0.87, nq 5 0.87-1.07,
-    // nq 6 0.94-1.02, nq >= 7 1.27-1.35; FP32 nq 2..6 0.58-0.96, nq >= 7 1.10-1.13.
-    // PMG_BK3_SCHEDULE=atomic|colored|patch overrides.
+    // Default scatter schedule: the patch schedule (with early non-temporal G
+    // loads) where a batch holds enough cells for its LDS patch to fold many
+    // shared DoFs before the global update (E = 256/nq^2 cells: FP64 nq <= 4,
+    // i.e. E >= 16; FP32 nq <= 6), the per-cell atomic scatter (late G loads)
+    // otherwise.  From vmult timings on MI355X over the level types of the MG
+    // hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3 cells), patch / atomic
+    // time: FP64 nq 2..4 0.66-0.83, nq 5 0.87-1.06, nq 6 0.95-1.02, nq >= 7
+    // 1.22-1.26; FP32 nq 2..6 0.57-0.96, nq >= 7 1.07-1.13.
+    // PMG_BK3_SCHEDULE=atomic|colored|patch and PMG_BK3_GLOAD=0|1|2 override.
     const char *env = std::getenv("PMG_BK3_SCHEDULE");
     const char *genv = std::getenv("PMG_BK3_GLOAD");
     int kind = nq <= (precision == PMG_FP32 ? 6 : 4) ? kPatch : kAtomic;
     if (env)
       kind = std::string(env) == "colored" ? kColored
              : std::string(env) == "atomic" ? kAtomic : kPatch;
-    set_schedule(kind, genv ? std::atoi(genv) : 2);  // late non-temporal G loads measured fastest
+    set_schedule(kind, genv ? std::atoi(genv) : -1);
   }
   G_.reinit(precision, g_size(nq, n_cells));
   if (cell_vertices) {
@@ -270,7 +271,9 @@ void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {
   if (kind == kColored && sched_.color_offsets.empty()) kind = kAtomic;
   if (kind == kPatch && sched_.pdof == nullptr) kind = kAtomic;
   sched_.kind = kind;
-  sched_.gmode = gmode;
+  // -1: the schedule's measured best G-load policy (the patch kernel's first
+  // phase only gathers indices and x, which the G stream then overlaps)
+  sched_.gmode = gmode >= 0 ? gmode : (kind == kPatch ? 0 : 2);  // 0 early NT, 2 late NT
 }
 
 template <typename T>

@@ -150,7 +150,8 @@ def test_bad_arguments_raise(pmg):
 @pytest.mark.parametrize("p,nq,sched,gload", [
     (4, 5, 1, 0), (4, 5, 0, 1), (4, 5, 0, 0), (4, 5, 2, 2), (4, 6, 1, 0), (4, 6, 2, 2),
     (2, 3, 1, 0), (6, 7, 1, 0), (1, 2, 2, 2), (3, 4, 2, 2), (5, 7, 2, 2), (6, 8, 2, 2),
-    (1, 2, 0, 2), (3, 4, 0, 2)])
+    (1, 2, 0, 2), (3, 4, 0, 2), (4, 5, 2, 0), (4, 5, 2, 1), (3, 4, 2, 0), (6, 7, 2, 0),
+    (1, 2, 2, -1), (5, 6, 0, -1)])
 def test_schedules_agree(pmg, orc, p, nq, sched, gload):
     """Atomics-free colour-by-colour scatter, the batch-patch scatter, the
     per-cell atomic scatter and the G-load variants all give the oracle's result."""
