@@ -27,10 +27,15 @@ namespace pmg {
 // processed by the first E*NQ^2 lanes of the workgroup (all lanes must call it:
 // it holds workgroup barriers).  lds: E * 3 * NQ^3 elements.  ATOMIC = false is
 // only valid when no two cells of the range share a DoF (one colour).
+// Compressed indices (optional, lines != nullptr): for a cell with lflag set,
+// every x-line (j,k) of its DoF map is a run of consecutive DoFs starting at
+// lines[cell][k][j]; other cells read the full map.  The line starts and the
+// flag are loaded together, so compressed cells need no dependent step.
 template <int NM, int NQ, typename T, bool ATOMIC = true, int GMODE = kGEarlyNT>
 __device__ __forceinline__ void
 bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
-          const uint32_t *__restrict__ map, const uint32_t *__restrict__ cell_ids,
+          const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
+          const uint8_t *__restrict__ lflag, const uint32_t *__restrict__ cell_ids,
           const int64_t n_range, const Shape1D<T> &sh, const int64_t cell_base, T *lds)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
@@ -51,12 +56,20 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   uint32_t idx[NM];
   T u[NM], v[NM];
   if (active && ta < NM && tb < NM) {
-    const uint32_t *mc = map + cell * NM3 + tb * NM + ta;
+    bool full = true;
+    if (lines) {
+      const uint32_t *lc = lines + cell * NM2 + tb;
 #pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      idx[k] = mc[k * NM2];
-      u[k] = (idx[k] == kInvalid) ? T(0) : x[idx[k]];
+      for (int k = 0; k < NM; ++k) idx[k] = lc[k * NM] + ta;
+      full = lflag[cell] == 0;
     }
+    if (full) {
+      const uint32_t *mc = map + cell * NM3 + tb * NM + ta;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) idx[k] = mc[k * NM2];
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) u[k] = (idx[k] == kInvalid) ? T(0) : x[idx[k]];
   }
   bk3_cell_apply<NM, NQ, T, GMODE>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3, active, ta, tb);
   // scatter-add
@@ -73,13 +86,14 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
 template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
 __global__ void __launch_bounds__(256)
 bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
-                 const uint32_t *__restrict__ map, const uint32_t *__restrict__ cell_ids,
+                 const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
+                 const uint8_t *__restrict__ lflag, const uint32_t *__restrict__ cell_ids,
                  const int64_t n_range, const Shape1D<T> sh)
 {
   constexpr int E = Bk3Tile<NQ>::cells;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  bk3_batch<NM, NQ, T, ATOMIC, GMODE>(x, y, G, map, cell_ids, n_range, sh,
+  bk3_batch<NM, NQ, T, ATOMIC, GMODE>(x, y, G, map, lines, lflag, cell_ids, n_range, sh,
                                       static_cast<int64_t>(lb) * E, lds);
 }
 
@@ -220,7 +234,7 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
     for (int64_t i = tid; i < n; i += nt) out[i] = T(0);
     __syncthreads();
     for (int64_t base = 0; base < n_cells; base += E) {
-      bk3_batch<NM, NQ, T>(in, out, G, map, nullptr, n_cells, sh, base, lds);
+      bk3_batch<NM, NQ, T>(in, out, G, map, nullptr, nullptr, nullptr, n_cells, sh, base, lds);
       __syncthreads();
     }
     for (int64_t i = tid; i < n_cons; i += nt) out[cons[i]] = in[cons[i]];
@@ -305,6 +319,7 @@ template void launch_bk3_coarse_pcg<float>(int, int, int64_t, const float *, con
 
 template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
 static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
+                             const uint32_t *lines, const uint8_t *lflag,
                              const uint32_t *cell_ids, int64_t n_range, const Shape1D<T> &sh,
                              hipStream_t s) {
   if (n_range <= 0) return;
@@ -313,7 +328,7 @@ static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
   hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, ATOMIC, GMODE>), dim3(grid), dim3(Tile::lanes), 0,
-                     s, x, y, G, map, cell_ids, n_range, sh);
+                     s, x, y, G, map, lines, lflag, cell_ids, n_range, sh);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -335,17 +350,21 @@ static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_
     // colour sees the updates of the previous ones
     for (size_t c = 0; c + 1 < sc->color_offsets.size(); ++c) {
       const int64_t b = sc->color_offsets[c], n = sc->color_offsets[c + 1] - b;
-      bk3_launch_range<NM, NQ, T, false, kGEarlyNT>(x, y, G, map, sc->cell_ids + b, n, sh, s);
+      bk3_launch_range<NM, NQ, T, false, kGEarlyNT>(x, y, G, map, sc->lines, sc->lflag,
+                                                    sc->cell_ids + b, n, sh, s);
     }
     return;
   }
   const int gmode = sc ? sc->gmode : kGLateNT;
+  const uint32_t *lines = sc ? sc->lines : nullptr;
+  const uint8_t *lflag = sc ? sc->lflag : nullptr;
   if (FULL && gmode == kGEarly)
-    bk3_launch_range<NM, NQ, T, true, (FULL ? kGEarly : kGLateNT)>(x, y, G, map, nullptr, n_cells, sh, s);
+    bk3_launch_range<NM, NQ, T, true, (FULL ? kGEarly : kGLateNT)>(x, y, G, map, lines, lflag,
+                                                                   nullptr, n_cells, sh, s);
   else if (FULL && gmode == kGEarlyNT)
-    bk3_launch_range<NM, NQ, T, true, kGEarlyNT>(x, y, G, map, nullptr, n_cells, sh, s);
+    bk3_launch_range<NM, NQ, T, true, kGEarlyNT>(x, y, G, map, lines, lflag, nullptr, n_cells, sh, s);
   else
-    bk3_launch_range<NM, NQ, T, true, kGLateNT>(x, y, G, map, nullptr, n_cells, sh, s);
+    bk3_launch_range<NM, NQ, T, true, kGLateNT>(x, y, G, map, lines, lflag, nullptr, n_cells, sh, s);
 }
 
 template <typename T>

@@ -68,6 +68,12 @@ struct Bk3Schedule {
   // cell-local DoF its patch index pidx[cell][i] (0xFFFF: constrained)
   const uint32_t *pdof = nullptr, *pcount = nullptr;
   const uint16_t *pidx = nullptr;
+  // compressed DoF indices (atomic and colored schedules): per cell
+  // lines[cell][k][j] = first DoF of x-line (j,k) when lflag[cell] says every
+  // x-line of the cell is a run of consecutive DoFs (no constrained entry);
+  // null when too few cells qualify
+  const uint32_t *lines = nullptr;
+  const uint8_t *lflag = nullptr;
 };
 
 // y += sum_cells A_e x.  assign (patch schedule only): DoFs owned by one

@@ -108,6 +108,9 @@ class LaplaceOperatorBK3 : public Operator {
   // cell colouring for the atomics-free schedule, DoF patches for the patch schedule
   DeviceArray<uint32_t> cell_ids_, pdof_, pcount_;
   DeviceArray<uint16_t> pidx_;
+  // compressed DoF indices (x-line starts) and their per-cell flag
+  DeviceArray<uint32_t> lines_;
+  DeviceArray<uint8_t> lflag_;
   Bk3Schedule sched_;
 
  public:
