@@ -82,20 +82,43 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
   }
   if (active)
     for (int i = t; i < NQ3; i += NQ2) V[i] = T(0);
+
+  // face connectivity of all six faces, and the neighbour's nodal line for
+  // the first face, are requested before the cell sweeps so their latency
+  // overlaps them; inside the face loop the next face's neighbour line is
+  // requested while the current face is computed
+  uint32_t cfs[6], nbs[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    cfs[f] = active ? cface[cell * 6 + f] : 0u;
+    nbs[f] = col ? cnbr[cell * 6 + f] : 0u;
+  }
+  // neighbour nodal line for face f (normal d, side s): entries k along the
+  // normal at this lane's tangential position; zero where there is no neighbour
+  auto load_line = [&](int f, T (&w)[NM]) {
+    const int d = f >> 1;
+    const int snn = d == 0 ? 1 : (d == 1 ? NM : NM2);
+    const int t1n = d == 0 ? NM : 1, t2n = d == 2 ? NM : NM2;
+    const bool has = col && (cfs[f] >> 30) < 2u;
+    const T *xn = x + (has ? static_cast<int64_t>(nbs[f]) * NM3 + ta * t1n + tb * t2n : 0);
+#pragma unroll
+    for (int k = 0; k < NM; ++k) w[k] = has ? xn[k * snn] : T(0);
+  };
+  T wcur[NM], wnext[NM];
+  load_line(0, wcur);
   bk3_forward<NM, NQ, T>(u, sh, P0, P1, P2, active, ta, tb);  // values at Gauss points in P0
 
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
     const int d = f >> 1, s = f & 1;
-    // strides of the normal and the tangential axes (ascending) on the nodal
-    // grid [k][j][i] and on the Gauss-point grid [r][q][p]
-    const int snn = d == 0 ? 1 : (d == 1 ? NM : NM2);
-    const int t1n = d == 0 ? NM : 1, t2n = d == 2 ? NM : NM2;
+    // strides of the tangential axes (ascending) on the Gauss-point grid
+    // [r][q][p] and of the normal
     const int snq = d == 0 ? 1 : (d == 1 ? NQ : NQ2);
     const int t1q = d == 0 ? NQ : 1, t2q = d == 2 ? NQ : NQ2;
-    const uint32_t cf = active ? cface[cell * 6 + f] : 0u;
+    const uint32_t cf = cfs[f];
     const int role = static_cast<int>(cf >> 30);
     const int64_t F = cf & 0x3FFFFFFFu;
+    if (f < 5) load_line(f + 1, wnext);
 
     // face geometry of this face point, requested now and used after two
     // barriers, so its latency overlaps steps A and B
@@ -112,17 +135,10 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
     // 1D contraction of the Gauss-point values along the normal
     T um = 0, dnu = 0;
     if (col) {
-      T vp = 0, dp = 0;
-      if (role < 2) {
-        const T *xn = x + static_cast<int64_t>(cnbr[cell * 6 + f]) * NM3 + ta * t1n + tb * t2n;
+      T dp = 0;
 #pragma unroll
-        for (int k = 0; k < NM; ++k) {
-          const T w = xn[k * snn];
-          dp += fs.dl[(1 - s) * NM + k] * w;
-          if (k == (s ? 0 : NM - 1)) vp = w;
-        }
-      }
-      A[tb * NM + ta] = vp;
+      for (int k = 0; k < NM; ++k) dp += fs.dl[(1 - s) * NM + k] * wcur[k];
+      A[tb * NM + ta] = s ? wcur[0] : wcur[NM - 1];
       A[NQ2 + tb * NM + ta] = dp;
     }
     if (active) {
@@ -213,6 +229,8 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
       for (int r = 0; r < NQ; ++r) Vl[r * snq] += c * fs.L[s * NQ + r] + cn * fs.Ld[s * NQ + r];
     }
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NM; ++k) wcur[k] = wnext[k];
   }
 
   T g[6][NQ];
