@@ -16,7 +16,9 @@ Also reported on the same problem:
                 degree 3 over Jacobi, coarse Jacobi-PCG to 1e-12), time,
                 iterations, L2 error;
   cpu_baseline  the CPU oracle's vmult (plain-C restatement of the reference
-                kernel, OpenMP over 8 cell colours) on the host cores.
+                kernel, OpenMP over 8 cell colours) on the host cores, and the
+                oracle's MG-CG FP64 solve of the 32^3 Q4 problem (a bounded
+                sample of the solver path) beside the GPU's time for it.
 
   configs       rows: every hot-path kernel alone (setup, BK3, SIP-DG, h/p/DG-CG
                 transfers, Chebyshev step, V-cycle) with its time and HBM
@@ -150,11 +152,31 @@ def cpu_baseline(n_cells_dir, degree, nq, seconds):
     single, n1, t1 = rate(1, seconds / 3)
     multi, n2, t2 = rate(threads, seconds / 3)
     log(f"[cpu] 1 thread {single:.4f} GDoF/s ({n1} vmults), {threads} threads {multi:.4f} GDoF/s")
+    del op
+    # the solver rows on a bounded sample: MG-CG FP64 to 1e-10 on the 32^3 Q4
+    # problem (5 h-levels, same smoother and coarse solve as the GPU legs)
+    n_s = n_cells_dir // 2
+    meshes = [oracle.mesh(n_s >> l) for l in reversed(range(5))]
+    t0 = time.perf_counter()
+    mg = oracle.Multigrid(meshes, [degree] * 5, coarse_kind=1, coarse_tol=1e-12,
+                          n_threads=threads)
+    t_setup = time.perf_counter() - t0
+    fine = oracle.LaplaceBK3(meshes[-1], degree, nq, n_threads=threads)
+    b = fine.rhs(1)
+    t0 = time.perf_counter()
+    xs, its, r0, r = mg.pcg(b, precond=2, rel_tol=1e-10, max_it=100)
+    t_solve = time.perf_counter() - t0
+    log(f"[cpu] oracle MG-CG {n_s}^3 Q{degree}: setup {t_setup:.1f}s, solve {t_solve:.2f}s, {its} its")
     return {"value": multi, "unit": "GDoF/s", "cores": threads, "kind": "port",
             "sample": f"{n2} FP64 vmults of the same 64^3 Q4 operator with {threads} OpenMP "
                       f"threads ({t2:.1f}s); oracle = plain-C restatement of the reference "
                       f"BK3 kernel",
-            "single_thread_value": single, "single_thread_sample": f"{n1} vmults, {t1:.1f}s"}
+            "single_thread_value": single, "single_thread_sample": f"{n1} vmults, {t1:.1f}s",
+            "mgcg_fp64": {"workload": f"MG-CG FP64 to 1e-10, {n_s}^3 Q{degree}, 5 h-levels "
+                                      f"(bounded sample of the 64^3 solve)",
+                          "dofs": int(fine.N), "setup_s": t_setup, "solve_s": t_solve,
+                          "iterations": its, "rel_residual": r / r0,
+                          "l2_error": fine.l2_error(xs, 1), "threads": threads}}
 
 
 def solve_timed(pmg, outer, b, mg, precond, tol, reps=2, max_it=500):
@@ -556,6 +578,14 @@ def main():
         out["configs"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, degree, nq, args.cpu_seconds)
+        # the same bounded solve on the GPU, for the side-by-side figure
+        if not args.no_solve:
+            bs = pmg.box(n // 2)
+            ops = pmg.LaplaceOperatorBK3(bs, degree, nq, pmg.FP64)
+            r_gpu, _ = mg_solve(pmg, bs, degree, 5, pmg.FP64, ops, ops.assemble_rhs(1), 1e-10)
+            out["cpu_baseline"]["mgcg_fp64"]["gpu_same_problem"] = r_gpu
+            out["cpu_baseline"]["mgcg_fp64"]["gpu_speedup"] = (
+                out["cpu_baseline"]["mgcg_fp64"]["solve_s"] / r_gpu["time_s"])
     print(json.dumps(out), flush=True)
 
 
