@@ -144,6 +144,10 @@ class LaplaceOperatorBK3 : public LaplaceOperatorBase {
                             pmg_stream s = nullptr) const {
     check(pmg_operator_gradient_coefficient(h_.get(), u, lambda, out, s));
   }
+  // scatter schedule 0 atomic, 1 colored, 2 patch; g_load -1 = the schedule's default
+  void set_schedule(int schedule, int g_load = -1) {
+    check(pmg_operator_set_schedule(h_.get(), schedule, g_load));
+  }
 };
 
 class LaplaceOperatorDG : public LaplaceOperatorBase {
@@ -286,6 +290,12 @@ class VCycleMultigrid {
   }
   void vmult(void *dst, const void *src, pmg_stream s = nullptr) const {
     check(pmg_multigrid_vmult(h_.get(), dst, src, s));
+  }
+  // true if vmult replays a captured HIP graph of the V-cycle body
+  bool uses_graph() const {
+    int g = 0;
+    check(pmg_multigrid_uses_graph(h_.get(), &g));
+    return g != 0;
   }
   pmg_multigrid handle() const { return h_.get(); }
 
