@@ -936,24 +936,49 @@ VCycleMultigrid::VCycleMultigrid(std::vector<const Operator *> ops,
 }
 
 VCycleMultigrid::~VCycleMultigrid() {
+  // a replay may still run on the caller's stream: let it finish before the
+  // graph and the level vectors it uses go away
+  if (graph_exec_) (void)hipDeviceSynchronize();
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
 }
 
+// Captures the V-cycle body once.  The graph is an optimisation: if capturing
+// or instantiating fails, everything built so far is released and the eager
+// path is used from then on (a genuine launch error then shows there).
 void VCycleMultigrid::capture() const {
-  if (!cap_stream_) PMG_HIP_CHECK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
-  PMG_HIP_CHECK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+  auto give_up = [this] {
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_ = nullptr;
+    graph_exec_ = nullptr;
+    use_graph_ = false;
+    (void)hipGetLastError();  // the failed capture's error is not the eager path's
+  };
+  if (!cap_stream_ && hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess) {
+    cap_stream_ = nullptr;
+    give_up();
+    return;
+  }
+  if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    give_up();
+    return;
+  }
+  bool recorded = true;
   try {
     v_cycle(levels() - 1, cap_stream_);
   } catch (...) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(cap_stream_, &g);
-    if (g) (void)hipGraphDestroy(g);
-    throw;
+    recorded = false;
   }
-  PMG_HIP_CHECK(hipStreamEndCapture(cap_stream_, &graph_));
-  PMG_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(cap_stream_, &g);
+  if (!recorded || end != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    give_up();
+    return;
+  }
+  graph_ = g;
+  if (hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0) != hipSuccess) give_up();
 }
 
 void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
@@ -1005,12 +1030,9 @@ void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {
   else if (in64) launch_convert<float, double>(n, static_cast<float *>(def_[top].data()), static_cast<const double *>(src), s);
   else if (lv64) launch_convert<double, float>(n, static_cast<double *>(def_[top].data()), static_cast<const float *>(src), s);
   else launch_copy<float>(n, static_cast<float *>(def_[top].data()), static_cast<const float *>(src), s);
-  if (use_graph_) {
-    if (!graph_exec_) capture();
-    PMG_HIP_CHECK(hipGraphLaunch(graph_exec_, s));
-  } else {
-    v_cycle(top, s);
-  }
+  if (use_graph_ && !graph_exec_) capture();  // may fall back to the eager path
+  if (use_graph_) PMG_HIP_CHECK(hipGraphLaunch(graph_exec_, s));
+  else v_cycle(top, s);
   if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(dst), static_cast<const double *>(sol_[top].data()), s);
   else if (in64) launch_convert<double, float>(n, static_cast<double *>(dst), static_cast<const float *>(sol_[top].data()), s);
   else if (lv64) launch_convert<float, double>(n, static_cast<float *>(dst), static_cast<const double *>(sol_[top].data()), s);

@@ -268,7 +268,7 @@ class VCycleMultigrid {
   std::unique_ptr<PreconditionChebyshev> coarse_cheby_;
   mutable std::vector<DeviceVector> sol_, def_, t_;
   mutable CGWork coarse_work_;
-  bool use_graph_ = false;
+  mutable bool use_graph_ = false;  // cleared for good if capturing fails
   mutable hipStream_t cap_stream_ = nullptr;
   mutable hipGraph_t graph_ = nullptr;
   mutable hipGraphExec_t graph_exec_ = nullptr;
