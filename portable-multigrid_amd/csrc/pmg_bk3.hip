@@ -287,6 +287,125 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
   }
 }
 
+// The same Jacobi-PCG with the vectors the operator touches held in LDS (the
+// search direction p it gathers from and q = A p it scatters into, with LDS
+// atomics) and the others in registers (lane entries i = tid + j 256, j <
+// NPL): x, r, z and the inverse diagonal.  An iteration then goes to memory
+// only for G and the map (L2-resident), instead of a global round trip per
+// vector access.  Same arithmetic in the same order as coarse_pcg_kernel.
+// Requires n <= 256 NPL.
+template <int NM, int NQ, typename T, int NPL>
+__global__ void __launch_bounds__(256)
+coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
+                      const int64_t n_cells, const Shape1D<T> sh, const uint32_t *__restrict__ cons,
+                      const int64_t n_cons, const T *__restrict__ pinv, const int64_t n,
+                      const T *__restrict__ b, T *x, const double rel_tol, const int max_it,
+                      double *info)
+{
+  constexpr int E = Bk3Tile<NQ>::cells, LANES = 256;
+  __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
+  __shared__ T pv[NPL * LANES], qv[NPL * LANES];
+  __shared__ double red[9];
+  const int tid = threadIdx.x;
+  T xr[NPL], rr[NPL], zr[NPL], dr[NPL];
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    const int64_t i = tid + j * LANES;
+    xr[j] = i < n ? x[i] : T(0);
+    dr[j] = i < n ? pinv[i] : T(0);
+    if (i < n) pv[i] = xr[j];
+  }
+  __syncthreads();
+
+  auto apply = [&]() {  // qv = A pv, constrained rows copied
+#pragma unroll
+    for (int j = 0; j < NPL; ++j)
+      if (tid + j * LANES < n) qv[tid + j * LANES] = T(0);
+    __syncthreads();
+    for (int64_t base = 0; base < n_cells; base += E) {
+      bk3_batch<NM, NQ, T>(pv, qv, G, map, nullptr, nullptr, nullptr, n_cells, sh, base, lds);
+      __syncthreads();
+    }
+    for (int64_t i = tid; i < n_cons; i += LANES) qv[cons[i]] = pv[cons[i]];
+    __syncthreads();
+  };
+
+  apply();
+  double s = 0;
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    const int64_t i = tid + j * LANES;
+    const T ri = i < n ? b[i] - qv[i] : T(0);
+    rr[j] = ri;
+    s += double(ri) * double(ri);
+  }
+  const double r0 = sqrt(wg_sum_bcast(s, red));
+  double res = r0, rho_old = 1.0;
+  int it = 0;
+  while (it < max_it && res > rel_tol * r0) {
+    s = 0;
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+      zr[j] = dr[j] * rr[j];
+      s += double(zr[j]) * double(rr[j]);
+    }
+    const double rho = wg_sum_bcast(s, red);
+    // first iteration: p = z exactly
+    const T beta = it == 0 ? T(0) : static_cast<T>(rho / rho_old);
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+      const int64_t i = tid + j * LANES;
+      if (i < n) pv[i] = it == 0 ? zr[j] : zr[j] + beta * pv[i];
+    }
+    __syncthreads();
+    apply();
+    s = 0;
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+      const int64_t i = tid + j * LANES;
+      if (i < n) s += double(pv[i]) * double(qv[i]);
+    }
+    const double pq = wg_sum_bcast(s, red);
+    const T alpha = pq != 0.0 ? static_cast<T>(rho / pq) : T(0);
+    s = 0;
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+      const int64_t i = tid + j * LANES;
+      if (i < n) {
+        xr[j] += alpha * pv[i];
+        rr[j] -= alpha * qv[i];
+      }
+      s += double(rr[j]) * double(rr[j]);
+    }
+    res = sqrt(wg_sum_bcast(s, red));
+    rho_old = rho;
+    ++it;
+    if (pq == 0.0) break;
+  }
+#pragma unroll
+  for (int j = 0; j < NPL; ++j)
+    if (tid + j * LANES < n) x[tid + j * LANES] = xr[j];
+  if (tid == 0) {
+    info[0] = it;
+    info[1] = r0;
+    info[2] = res;
+  }
+}
+
+template <int NM, int NQ, typename T, int NPL>
+static bool coarse_pcg_lds(int64_t n_cells, const T *G, const uint32_t *map, const Shape1D<T> &sh,
+                           const uint32_t *cons, int64_t n_cons, const T *pinv, int64_t n,
+                           const T *b, T *x, double rel_tol, int max_it, double *info,
+                           hipStream_t s) {
+  constexpr size_t lds = sizeof(T) * (Bk3Tile<NQ>::cells * lds_cell_stride<T>(3 * NQ * NQ * NQ) +
+                                      2 * NPL * 256) + 9 * sizeof(double);
+  if (n > 256 * NPL || lds > 64 * 1024) return false;
+  hipLaunchKernelGGL((coarse_pcg_lds_kernel<NM, NQ, T, NPL>), dim3(1), dim3(256), 0, s, G, map,
+                     n_cells, sh, cons, n_cons, pinv, n, b, x, rel_tol, max_it, info);
+  PMG_HIP_CHECK(hipGetLastError());
+  return true;
+}
+
 template <typename T>
 void launch_bk3_coarse_pcg(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
                            const Shape1D<T> &sh, const uint32_t *cons, int64_t n_cons,
@@ -294,6 +413,13 @@ void launch_bk3_coarse_pcg(int degree, int nq, int64_t n_cells, const T *G, cons
                            double rel_tol, int max_it, double *info, hipStream_t s) {
 #define PMG_CPCG_CASE(P, Q)                                                                     \
   if (degree == P && nq == Q) {                                                                 \
+    if (coarse_pcg_lds<P + 1, Q, T, 1>(n_cells, G, map, sh, cons, n_cons, pinv, n, b, x, rel_tol, \
+                                       max_it, info, s) ||                                      \
+        coarse_pcg_lds<P + 1, Q, T, 4>(n_cells, G, map, sh, cons, n_cons, pinv, n, b, x, rel_tol, \
+                                       max_it, info, s) ||                                      \
+        coarse_pcg_lds<P + 1, Q, T, 8>(n_cells, G, map, sh, cons, n_cons, pinv, n, b, x, rel_tol, \
+                                       max_it, info, s))                                        \
+      return;                                                                                   \
     hipLaunchKernelGGL((coarse_pcg_kernel<P + 1, Q, T>), dim3(1), dim3(Bk3Tile<Q>::lanes), 0, s, \
                        G, map, n_cells, sh, cons, n_cons, pinv, n, b, x, r, z, p, q, rel_tol,    \
                        max_it, info);                                                           \
