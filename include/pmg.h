@@ -119,7 +119,10 @@ int pmg_box_dg_faces(const pmg_box *box, uint32_t *inner_faces, int64_t *n_inner
  * (the reference, portable_laplace_operator_bk3.h:120) or degree+2.
  * Geometry: the trilinear map of each cell's 8 vertices (what MappingQ(p)
  * reproduces on straight-sided hexahedra), G computed on the device
- * (compute_G_tensors, portable_laplace_operator_bk3.h:456-515). */
+ * (compute_G_tensors, portable_laplace_operator_bk3.h:456-515).
+ * Constrained DoFs are eliminated: a map entry naming a DoF of the
+ * constrained list acts as 0xFFFFFFFF, and vmult returns dst = src on those
+ * rows (setup_dof_indices_per_color, portable_laplace_operator_bk3.h:360-437). */
 int pmg_operator_create(int dim, int degree, int n_q_points_1d, int precision,
                         int64_t n_cells, const uint32_t *dofmap, int64_t n_dofs,
                         const uint32_t *constrained_dofs, int64_t n_constrained,

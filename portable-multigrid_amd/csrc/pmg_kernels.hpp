@@ -161,6 +161,10 @@ template <typename T> void launch_copy(int64_t n, T *dst, const T *src, hipStrea
 // dst[idx[i]] = src[idx[i]]
 template <typename T>
 void launch_copy_indexed(int64_t n, const uint32_t *idx, T *dst, const T *src, hipStream_t s);
+// dst[i] = mask[i] ? src[i] : 0 (one pass: the zero fill and the copy of the
+// constrained rows of a vmult)
+template <typename T>
+void launch_prep_constrained(int64_t n, const uint8_t *mask, T *dst, const T *src, hipStream_t s);
 template <typename T>
 void launch_set_indexed(int64_t n, const uint32_t *idx, T *dst, T v, hipStream_t s);
 // x[i] = 1/x[i]

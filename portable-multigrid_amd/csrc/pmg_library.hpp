@@ -13,6 +13,10 @@
 
 namespace pmg {
 
+// vmult of BK3 operators up to this many DoFs: fused zero fill + constrained copy
+constexpr int64_t kFusedPrepMaxDofs = int64_t(1) << 20;
+
+
 // Untyped owning device buffer of either precision.
 struct DeviceVector {
   DeviceArray<char> raw;
@@ -111,6 +115,7 @@ class LaplaceOperatorBK3 : public Operator {
   // compressed DoF indices (x-line starts) and their per-cell flag
   DeviceArray<uint32_t> lines_;
   DeviceArray<uint8_t> lflag_;
+  DeviceArray<uint8_t> cmask_;  // 1 on constrained DoFs
   Bk3Schedule sched_;
 
  public:

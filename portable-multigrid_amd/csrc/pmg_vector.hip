@@ -33,6 +33,12 @@ __global__ void copy_idx_k(int64_t n, const uint32_t *idx, T *d, const T *s) {
     d[j] = s[j];
   }
 }
+// d = 0 on free DoFs, d = s on constrained ones (mask[i] != 0)
+template <typename T>
+__global__ void prep_constrained_k(int64_t n, const uint8_t *mask, T *d, const T *s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    d[i] = mask[i] ? s[i] : T(0);
+}
 template <typename T>
 __global__ void set_idx_k(int64_t n, const uint32_t *idx, T *d, T v) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
@@ -170,6 +176,13 @@ void launch_copy_indexed(int64_t n, const uint32_t *idx, T *d, const T *src, hip
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T>
+void launch_prep_constrained(int64_t n, const uint8_t *mask, T *d, const T *src, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(prep_constrained_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, mask, d,
+                     src);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
 void launch_set_indexed(int64_t n, const uint32_t *idx, T *d, T v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(set_idx_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, idx, d, v);
@@ -240,6 +253,7 @@ void launch_cg_scalars(double *sc, int op, hipStream_t s) {
   template void launch_fill<T>(int64_t, T *, T, hipStream_t);                                 \
   template void launch_copy<T>(int64_t, T *, const T *, hipStream_t);                         \
   template void launch_copy_indexed<T>(int64_t, const uint32_t *, T *, const T *, hipStream_t); \
+  template void launch_prep_constrained<T>(int64_t, const uint8_t *, T *, const T *, hipStream_t); \
   template void launch_set_indexed<T>(int64_t, const uint32_t *, T *, T, hipStream_t);        \
   template void launch_invert<T>(int64_t, T *, hipStream_t);                                  \
   template void launch_residual<T>(int64_t, T *, const T *, hipStream_t);                     \

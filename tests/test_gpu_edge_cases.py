@@ -99,3 +99,23 @@ def test_geometry_beyond_2e31_entries(pmg):
     M = 4 * n + 1
     inner = a1.reshape(M, M, M)[5:-5, 5:-5, 5:-5]
     assert inner.abs().max().item() < 1e-11 * a1.abs().max().item()
+
+
+def test_constrained_dofs_are_eliminated_from_the_map(pmg):
+    """A map that still names the constrained DoFs (no 0xFFFFFFFF entries)
+    gives the operator of the map that marks them invalid: constrained DoFs
+    are eliminated from rows and columns, and their rows are the identity."""
+    import torch
+    b, p = pmg.box(3, deform=1, amplitude=0.05), 2
+    n = pmg.box_n_dofs(b, p)
+    cons = pmg.box_boundary_dofs(b, p)
+    op_plain = pmg.LaplaceOperatorBK3.from_arrays(p, pmg.box_dofmap(b, p, False), n, cons,
+                                                  cell_vertices=pmg.box_cell_vertices(b))
+    op = pmg.LaplaceOperatorBK3(b, p)
+    x = torch.tensor(seeded_uniform(n, 608), device="cuda")
+    y1, y2 = op.initialize_dof_vector(), op_plain.initialize_dof_vector()
+    op.vmult(y1, x)
+    op_plain.vmult(y2, x)
+    assert rel_err(y2.cpu().numpy(), y1.cpu().numpy()) < 1e-14
+    assert torch.equal(y2[torch.tensor(cons.astype(np.int64), device="cuda")],
+                       x[torch.tensor(cons.astype(np.int64), device="cuda")])
