@@ -172,7 +172,7 @@ template <typename T> void launch_invert(int64_t n, T *x, hipStream_t s);
 template <typename TD, typename TS>
 void launch_convert(int64_t n, TD *dst, const TS *src, hipStream_t s);
 // partial sums of a.b into `partials` (kDotBlocks entries), then final sum into out[0]
-constexpr int kDotBlocks = 1024;
+constexpr int kDotBlocks = 4096;
 template <typename T>
 void launch_dot(int64_t n, const T *a, const T *b, double *partials, double *out, hipStream_t s);
 // t = b - t   (residual)
