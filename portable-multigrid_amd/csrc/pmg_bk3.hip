@@ -110,7 +110,8 @@ template <int NM, int NQ, typename T, int GMODE, bool ASSIGN>
 __global__ void __launch_bounds__(256)
 bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ pcount,
-                 const uint16_t *__restrict__ pidx, const int64_t n_cells, const Shape1D<T> sh)
+                 const uint16_t *__restrict__ pidx, const uint16_t *__restrict__ pline,
+                 const uint8_t *__restrict__ pflag, const int64_t n_cells, const Shape1D<T> sh)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells, PMAX = E * NM3;
@@ -133,9 +134,18 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   const uint32_t *pd = pdof + batch * PMAX;
   uint16_t pi[NM];
   if (col) {
-    const uint16_t *pc = pidx + cell * NM3 + tb * NM + ta;
+    bool full = true;
+    if (pline) {  // x-line slot starts and the cell's flag, loaded together
+      const uint16_t *pl = pline + cell * NM2 + tb;
 #pragma unroll
-    for (int k = 0; k < NM; ++k) pi[k] = pc[k * NM2];
+      for (int k = 0; k < NM; ++k) pi[k] = static_cast<uint16_t>(pl[k * NM] + ta);
+      full = pflag[cell] == 0;
+    }
+    if (full) {
+      const uint16_t *pc = pidx + cell * NM3 + tb * NM + ta;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) pi[k] = pc[k * NM2];
+    }
   }
   uint32_t pw[PL];
 #pragma unroll
@@ -188,10 +198,10 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
   if (assign)
     hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true>), dim3(grid), dim3(Tile::lanes),
-                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, n_cells, sh);
+                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, n_cells, sh);
   else
     hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, false>), dim3(grid), dim3(Tile::lanes),
-                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, n_cells, sh);
+                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, n_cells, sh);
   PMG_HIP_CHECK(hipGetLastError());
 }
 

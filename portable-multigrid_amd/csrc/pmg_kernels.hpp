@@ -74,6 +74,11 @@ struct Bk3Schedule {
   // null when too few cells qualify
   const uint32_t *lines = nullptr;
   const uint8_t *lflag = nullptr;
+  // compressed patch indices (patch schedule): pline[cell][k][j] = patch slot
+  // of the first DoF of x-line (j,k) when pflag[cell] says the slots of every
+  // x-line are consecutive; null when too few cells qualify
+  const uint16_t *pline = nullptr;
+  const uint8_t *pflag = nullptr;
 };
 
 // y += sum_cells A_e x.  assign (patch schedule only): DoFs owned by one

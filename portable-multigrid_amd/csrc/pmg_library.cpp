@@ -299,6 +299,30 @@ void LaplaceOperatorBK3::build_patches_once() {
   sched_.pdof = pdof_.ptr;
   sched_.pcount = pcount_.ptr;
   sched_.pidx = pidx_.ptr;
+  // x-lines whose patch slots are consecutive need only their first slot
+  // (nm^2 instead of nm^3 16-bit loads per cell); used when at least half of
+  // the cells qualify
+  const int nm = degree_ + 1;
+  std::vector<uint16_t> pline(static_cast<size_t>(n_cells_ * nm * nm), 0);
+  std::vector<uint8_t> pflag(static_cast<size_t>(n_cells_), 0);
+  int64_t n_comp = 0;
+  for (int64_t c = 0; c < n_cells_; ++c) {
+    bool ok = true;
+    for (int kj = 0; kj < nm * nm && ok; ++kj) {
+      const uint16_t *ln = P.pidx.data() + c * nm3 + int64_t(kj) * nm;
+      ok = ln[0] != 0xFFFFu;
+      for (int i = 1; i < nm && ok; ++i) ok = ln[i] == uint16_t(ln[0] + i);
+      pline[c * nm * nm + kj] = ok ? ln[0] : 0;
+    }
+    pflag[c] = ok ? 1 : 0;
+    n_comp += ok;
+  }
+  if (2 * n_comp >= n_cells_) {
+    pline_.upload(pline);
+    pflag_.upload(pflag);
+    sched_.pline = pline_.ptr;
+    sched_.pflag = pflag_.ptr;
+  }
 }
 
 void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {

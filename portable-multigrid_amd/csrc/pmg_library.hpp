@@ -116,6 +116,8 @@ class LaplaceOperatorBK3 : public Operator {
   DeviceArray<uint32_t> lines_;
   DeviceArray<uint8_t> lflag_;
   DeviceArray<uint8_t> cmask_;  // 1 on constrained DoFs
+  DeviceArray<uint16_t> pline_;  // compressed patch indices (x-line slot starts)
+  DeviceArray<uint8_t> pflag_;
   Bk3Schedule sched_;
 
  public:
