@@ -81,6 +81,42 @@ def test_bk3_equivariant_under_renumbering(pmg, p, nq, prec):
         assert abs(e1 - e2) <= 1e-13 * abs(e1)
 
 
+@pytest.mark.parametrize("p,nq,prec", [(3, 4, 0), (4, 5, 0), (4, 5, 1), (2, 3, 1)])
+def test_patch_schedule_with_partly_scattered_numbering(pmg, p, nq, prec):
+    """DoFs of the bottom quarter of the box shuffled, the rest lexicographic:
+    batches there have hundreds of runs of consecutive DoFs, the others a few
+    dozen, so one patch launch mixes batches read through run tables with
+    batches read through their plain DoF lists."""
+    import torch
+    print(f"[numbering patch mixed] p={p} nq={nq} prec={prec}", flush=True)
+    b = pmg.box(8, deform=1, amplitude=0.05)
+    N, nm3 = pmg.box_n_dofs(b, p), (p + 1) ** 3
+    pi = np.arange(N, dtype=np.int64)
+    q = N // 4
+    pi[:q] = _perm(q, 520 + p)
+    m = _permute_map(pmg.box_dofmap(b, p), nm3, pi, np.arange(pmg.box_n_cells(b)))
+    cons = pi[pmg.box_boundary_dofs(b, p).astype(np.int64)].astype(np.uint32)
+    op1 = pmg.LaplaceOperatorBK3(b, p, nq, prec)
+    op2 = pmg.LaplaceOperatorBK3.from_arrays(p, m, N, cons, cell_vertices=pmg.box_cell_vertices(b),
+                                             n_q_points_1d=nq, precision=prec)
+    op1.set_schedule(pmg.SCHEDULE_ATOMIC)
+    op2.set_schedule(pmg.SCHEDULE_PATCH)
+    dt = torch.float64 if prec == pmg.FP64 else torch.float32
+    tol = 1e-12 if prec == pmg.FP64 else 2e-6
+    x = seeded_uniform(N, 530 + p)
+    y1, y2 = op1.initialize_dof_vector(), op2.initialize_dof_vector()
+    op1.vmult(y1, torch.tensor(x, dtype=dt, device="cuda"))
+    op2.vmult(y2, torch.tensor(_to(pi, x), dtype=dt, device="cuda"))
+    y1 = y1.cpu().numpy().astype(np.float64)
+    assert rel_err(y2.cpu().numpy().astype(np.float64)[pi], y1) < tol
+    base = seeded_uniform(N, 540 + p)
+    z2 = torch.tensor(_to(pi, base), dtype=dt, device="cuda")
+    op2.apply_add(z2, torch.tensor(_to(pi, x), dtype=dt, device="cuda"))
+    free = np.ones(N, dtype=bool)
+    free[pmg.box_boundary_dofs(b, p).astype(np.int64)] = False
+    assert rel_err(z2.cpu().numpy().astype(np.float64)[pi][free], (base + y1)[free]) < tol
+
+
 @pytest.mark.parametrize("kind", ["h", "p"])
 def test_transfers_equivariant_under_renumbering(pmg, kind):
     import torch
