@@ -187,9 +187,12 @@ template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStrea
 // mode 0: x = f2 * P b                                     (vmult start)
 // mode 1: old = x; x = x + f2 * P (b - t)                   (step start)
 // mode 2: xn = (1+f1) x - f1*old + f2 P (b - t); old = x; x = xn   (old = 0 if first)
+// prep (optional, may equal t): prep = mask ? x_new : 0, the prepared output
+// of the matvec that follows (mask null: no constrained rows)
 template <typename T>
 void launch_cheby_update(int mode, int64_t n, T *x, T *old, const T *b, const T *t,
-                         const T *pinv, double f1, double f2, bool old_is_zero, hipStream_t s);
+                         const T *pinv, double f1, double f2, bool old_is_zero, hipStream_t s,
+                         T *prep = nullptr, const uint8_t *mask = nullptr);
 
 // CG kernels with device-resident scalars (sc = array of doubles, see pmg_solver.cpp)
 template <typename T>

@@ -366,6 +366,24 @@ void LaplaceOperatorBK3::vmult(void *dst, const void *src, hipStream_t s) const 
   else vmult_t(static_cast<float *>(dst), static_cast<const float *>(src), s);
 }
 
+bool LaplaceOperatorBK3::supports_prepared_vmult() const {
+  const int64_t elem = precision_ == PMG_FP64 ? 8 : 4;
+  return n_dofs_ * elem <= kPreparedOutputMaxBytes;
+}
+
+// dst already holds src on constrained rows and 0 elsewhere
+void LaplaceOperatorBK3::vmult_prepared(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
+  if (precision_ == PMG_FP64)
+    launch_bk3_cells<double>(degree_, nq_, static_cast<const double *>(src), static_cast<double *>(dst),
+                             static_cast<const double *>(G_.data()), map_.ptr, n_cells_, sh64_, s,
+                             &sched_, /*assign=*/true);
+  else
+    launch_bk3_cells<float>(degree_, nq_, static_cast<const float *>(src), static_cast<float *>(dst),
+                            static_cast<const float *>(G_.data()), map_.ptr, n_cells_, sh32_, s,
+                            &sched_, /*assign=*/true);
+}
+
 void LaplaceOperatorBK3::apply_add(void *dst, const void *src, hipStream_t s) const {
   PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
   if (precision_ == PMG_FP64)
@@ -883,43 +901,60 @@ PreconditionChebyshev::PreconditionChebyshev(const Operator &op, int degree,
   old_.reinit(op.precision(), op.m());
 }
 
-void PreconditionChebyshev::run(void *dst, const void *src, bool step, hipStream_t s) const {
+void PreconditionChebyshev::run(void *dst, const void *src, bool step, void *next_out,
+                                hipStream_t s) const {
   PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
   const int64_t n = op_.m();
   const bool f64 = op_.precision() == PMG_FP64;
-  auto upd = [&](int mode, double f1, double f2, bool old_zero) {
+  const bool prep = op_.supports_prepared_vmult();
+  const uint8_t *mask = op_.constrained_mask();
+  // Each update also writes the prepared output of the matvec that follows
+  // it (tmp_ inside the recurrence, next_out after the last update), which
+  // saves the operator's own zero fill and constrained copy.
+  auto upd = [&](int mode, double f1, double f2, bool old_zero, void *out) {
+    void *pp = prep ? out : nullptr;
     if (f64)
       launch_cheby_update<double>(mode, n, static_cast<double *>(dst), static_cast<double *>(old_.data()),
                                   static_cast<const double *>(src), static_cast<const double *>(tmp_.data()),
-                                  static_cast<const double *>(op_.inverse_diagonal()), f1, f2, old_zero, s);
+                                  static_cast<const double *>(op_.inverse_diagonal()), f1, f2, old_zero, s,
+                                  static_cast<double *>(pp), mask);
     else
       launch_cheby_update<float>(mode, n, static_cast<float *>(dst), static_cast<float *>(old_.data()),
                                  static_cast<const float *>(src), static_cast<const float *>(tmp_.data()),
-                                 static_cast<const float *>(op_.inverse_diagonal()), f1, f2, old_zero, s);
+                                 static_cast<const float *>(op_.inverse_diagonal()), f1, f2, old_zero, s,
+                                 static_cast<float *>(pp), mask);
   };
+  const bool more = degree_ >= 2 && std::fabs(delta_) >= 1e-40;
   if (!step) {
-    upd(0, 0.0, 1.0 / theta_, false);
+    upd(0, 0.0, 1.0 / theta_, false, more ? tmp_.data() : next_out);
   } else {
     op_.vmult(tmp_.data(), dst, s);
-    upd(1, 0.0, 1.0 / theta_, false);
+    upd(1, 0.0, 1.0 / theta_, false, more ? tmp_.data() : next_out);
   }
-  if (degree_ < 2 || std::fabs(delta_) < 1e-40) return;
+  if (!more) return;
   double rhok = delta_ / theta_;
   const double sigma = theta_ / delta_;
   for (int k = 0; k < degree_ - 1; ++k) {
-    op_.vmult(tmp_.data(), dst, s);
+    if (prep) op_.vmult_prepared(tmp_.data(), dst, s);
+    else op_.vmult(tmp_.data(), dst, s);
     const double rhokp = 1.0 / (2.0 * sigma - rhok);
     const double f1 = rhokp * rhok, f2 = 2.0 * rhokp / delta_;
     rhok = rhokp;
-    upd(2, f1, f2, !step && k == 0);
+    upd(2, f1, f2, !step && k == 0, k + 2 < degree_ ? tmp_.data() : next_out);
   }
 }
 
 void PreconditionChebyshev::vmult(void *dst, const void *src, hipStream_t s) const {
-  run(dst, src, false, s);
+  run(dst, src, false, nullptr, s);
 }
 void PreconditionChebyshev::step(void *dst, const void *src, hipStream_t s) const {
-  run(dst, src, true, s);
+  run(dst, src, true, nullptr, s);
+}
+bool PreconditionChebyshev::vmult_and_prepare(void *dst, const void *src, void *next_out,
+                                              hipStream_t s) const {
+  PMG_REQUIRE(next_out && next_out != dst && next_out != src, PMG_ERR_INVALID_ARG, "bad vectors");
+  run(dst, src, false, next_out, s);
+  return op_.supports_prepared_vmult();
 }
 
 // ============================================================ V-cycle
@@ -1045,8 +1080,11 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
     }
     return;
   }
-  sm_[l]->vmult(sol_[l].data(), def_[l].data(), s);
-  ops_[l]->vmult(t_[l].data(), sol_[l].data(), s);
+  // pre-smoothing; its last update prepares t for the residual's matvec
+  if (sm_[l]->vmult_and_prepare(sol_[l].data(), def_[l].data(), t_[l].data(), s))
+    ops_[l]->vmult_prepared(t_[l].data(), sol_[l].data(), s);
+  else
+    ops_[l]->vmult(t_[l].data(), sol_[l].data(), s);
   const int64_t nc = ops_[l - 1]->m();
   if (f64) {
     launch_residual<double>(n, static_cast<double *>(t_[l].data()), static_cast<const double *>(def_[l].data()), s);

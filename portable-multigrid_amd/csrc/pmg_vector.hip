@@ -91,22 +91,27 @@ reduce_k(const double *partials, int n_partials, double *sc, int slot, int n_sum
   }
 }
 
+// prep (optional, may be t itself: each element is read before it is
+// written): the output of the matvec that follows, prep = mask ? x_new : 0
 template <typename T>
 __global__ void cheby_k(int mode, int64_t n, T *x, T *old, const T *b, const T *t, const T *pinv,
-                        T f1, T f2, bool old_zero) {
+                        T f1, T f2, bool old_zero, T *prep, const uint8_t *mask) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    T xn;
     if (mode == 0) {
-      x[i] = f2 * pinv[i] * b[i];
+      xn = f2 * pinv[i] * b[i];
     } else if (mode == 1) {
       const T xi = x[i];
       old[i] = xi;
-      x[i] = xi + f2 * pinv[i] * (b[i] - t[i]);
+      xn = xi + f2 * pinv[i] * (b[i] - t[i]);
     } else {
       const T xi = x[i];
       const T xo = old_zero ? T(0) : old[i];
       old[i] = xi;
-      x[i] = (T(1) + f1) * xi - f1 * xo + f2 * pinv[i] * (b[i] - t[i]);
+      xn = (T(1) + f1) * xi - f1 * xo + f2 * pinv[i] * (b[i] - t[i]);
     }
+    x[i] = xn;
+    if (prep) prep[i] = (mask && mask[i]) ? xn : T(0);
   }
 }
 
@@ -219,10 +224,11 @@ void launch_reduce_partials(const double *partials, int n_partials, double *sc, 
 }
 template <typename T>
 void launch_cheby_update(int mode, int64_t n, T *x, T *old, const T *b, const T *t,
-                         const T *pinv, double f1, double f2, bool old_is_zero, hipStream_t s) {
+                         const T *pinv, double f1, double f2, bool old_is_zero, hipStream_t s,
+                         T *prep, const uint8_t *mask) {
   if (n <= 0) return;
   hipLaunchKernelGGL(cheby_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, mode, n, x, old, b, t,
-                     pinv, static_cast<T>(f1), static_cast<T>(f2), old_is_zero);
+                     pinv, static_cast<T>(f1), static_cast<T>(f2), old_is_zero, prep, mask);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T>
@@ -259,7 +265,8 @@ void launch_cg_scalars(double *sc, int op, hipStream_t s) {
   template void launch_residual<T>(int64_t, T *, const T *, hipStream_t);                     \
   template void launch_dot<T>(int64_t, const T *, const T *, double *, double *, hipStream_t); \
   template void launch_cheby_update<T>(int, int64_t, T *, T *, const T *, const T *,          \
-                                       const T *, double, double, bool, hipStream_t);         \
+                                       const T *, double, double, bool, hipStream_t, T *,     \
+                                       const uint8_t *);                                      \
   template void launch_cg_update_xr<T>(int64_t, T *, T *, const T *, const T *,               \
                                        const double *, double *, hipStream_t);                \
   template void launch_cg_precond_dot<T>(int64_t, const T *, const T *, T *, double *,        \
