@@ -14,9 +14,12 @@
  *    handle's precision (double for PMG_FP64, float for PMG_FP32), length m().
  *  - Setup arrays (DoF maps, vertices, weights, G tensors) are HOST pointers;
  *    the library copies them to device memory it owns.  Operator, transfer,
- *    smoother and V-cycle applications allocate nothing; pmg_cg_solve,
- *    pmg_vector_dot and the set-up/post-processing calls (create,
- *    compute_diagonal, assemble_rhs, l2_error) allocate temporaries.
+ *    smoother and V-cycle applications allocate nothing (the first V-cycle
+ *    application instantiates its graph); the first pmg_cg_solve on an
+ *    operator allocates the CG vectors, which the operator handle keeps for
+ *    later solves of the same size; pmg_vector_dot and the set-up/
+ *    post-processing calls (create, compute_diagonal, assemble_rhs,
+ *    l2_error) allocate temporaries.
  *  - stream: a hipStream_t (NULL = the default stream).  All work is
  *    stream-ordered and asynchronous unless stated otherwise.  A handle must
  *    not be used from two host threads at once.
@@ -294,7 +297,9 @@ int pmg_multigrid_uses_graph(pmg_multigrid mg, int *uses_graph);
  * (tests/common/portable_multigrid_solver.h:147-169): x is the initial guess
  * (device, op precision), stop when ||r|| <= rel_tol ||r_0|| or max_it.
  * precond: NONE, JACOBI (op's inverse diagonal) or MG (mg, outer precision
- * = op precision).  Returns PMG_ERR_NOT_CONVERGED if max_it was reached. */
+ * = op precision).  Returns PMG_ERR_NOT_CONVERGED if max_it was reached.
+ * The CG work vectors (4 of length m()) are allocated on the first call and
+ * kept with op until pmg_operator_destroy. */
 int pmg_cg_solve(pmg_operator op, int precond, pmg_multigrid mg, const void *b, void *x,
                  double rel_tol, int max_it, int *iterations, double *res0, double *res,
                  pmg_stream stream);

@@ -9,6 +9,7 @@
 
 struct pmg_operator_s {
   std::unique_ptr<pmg::Operator> impl;
+  pmg::CGWork cg_work;  // CG vectors of pmg_cg_solve on this operator, kept between solves
 };
 struct pmg_transfer_s {
   std::unique_ptr<pmg::MGTransfer> impl;
@@ -355,9 +356,8 @@ int pmg_cg_solve(pmg_operator op, int precond, pmg_multigrid mg, const void *b, 
   int status = PMG_OK;
   const int g = guard([&] {
     PMG_NOT_NULL(op);
-    pmg::CGWork work;
     const auto r = pmg::solve_cg(*op->impl, precond, mg ? mg->impl.get() : nullptr, b, x, rel_tol,
-                                 max_it, work, as_stream(s));
+                                 max_it, op->cg_work, as_stream(s));
     if (iterations) *iterations = r.iterations;
     if (res0) *res0 = r.res0;
     if (res) *res = r.res;

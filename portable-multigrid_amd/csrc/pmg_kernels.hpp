@@ -178,8 +178,8 @@ template <typename TD, typename TS>
 void launch_convert(int64_t n, TD *dst, const TS *src, hipStream_t s);
 // partial sums of a.b into `partials` (kDotBlocks entries), then final sum into out[0]
 constexpr int kDotBlocks = 4096;
-template <typename T>
-void launch_dot(int64_t n, const T *a, const T *b, double *partials, double *out, hipStream_t s);
+template <typename TA, typename TB = TA>
+void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s);
 // t = b - t   (residual)
 template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStream_t s);
 
@@ -201,8 +201,9 @@ void launch_cg_update_xr(int64_t n, T *x, T *r, const T *p, const T *q, const do
 template <typename T>
 void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *partials,
                            hipStream_t s);
-template <typename T>
-void launch_cg_update_p(int64_t n, T *p, const T *z, const double *sc, int first, hipStream_t s);
+// p = z (first) or z + beta p; z may be of the other precision
+template <typename T, typename TZ = T>
+void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int first, hipStream_t s);
 // final reduction of kDotBlocks partial sums into sc[slot] (optionally sqrt)
 void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
                             int n_sums, hipStream_t s);

@@ -764,13 +764,21 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
   static const bool verbose = std::getenv("PMG_CG_VERBOSE") != nullptr;
   while (it < max_it) {
     if (precond == PMG_PRECOND_MG) {
-      mg->vmult(z, r, s);
-      launch_dot<T>(n, r, z, w.partials.ptr, sc + 0, s);
+      // z = the V-cycle's own finest-level solution, read in its precision
+      // (the same values as after a conversion, one pass less)
+      const void *zl = mg->apply(r, s);
+      if (mg->level_precision() == PMG_FP64) {
+        launch_dot<T, double>(n, r, static_cast<const double *>(zl), w.partials.ptr, sc + 0, s);
+        launch_cg_update_p<T, double>(n, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
+      } else {
+        launch_dot<T, float>(n, r, static_cast<const float *>(zl), w.partials.ptr, sc + 0, s);
+        launch_cg_update_p<T, float>(n, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s);
+      }
     } else {
       launch_cg_precond_dot<T>(n, r, pinv, z, w.partials.ptr, s);
       launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 0, 1, s);
+      launch_cg_update_p<T>(n, p, z, sc, it == 0 ? 1 : 0, s);
     }
-    launch_cg_update_p<T>(n, p, z, sc, it == 0 ? 1 : 0, s);
     op.vmult(q, p, s);
     launch_dot<T>(n, p, q, w.partials.ptr, sc + 1, s);
     launch_cg_update_xr<T>(n, x, r, p, q, sc, w.partials.ptr, s);
@@ -800,6 +808,8 @@ CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
   PMG_REQUIRE(precond >= 0 && precond <= 2, PMG_ERR_INVALID_ARG, "unknown preconditioner");
   PMG_REQUIRE(precond != PMG_PRECOND_MG || (mg && mg->outer_precision() == op.precision()),
               PMG_ERR_INVALID_ARG, "MG preconditioner missing or of the wrong outer precision");
+  PMG_REQUIRE(precond != PMG_PRECOND_MG || mg->m() == op.m(), PMG_ERR_SIZE,
+              "the MG preconditioner's finest level does not match the operator size");
   PMG_REQUIRE(rel_tol > 0 && max_it >= 0, PMG_ERR_INVALID_ARG, "bad tolerance or max_it");
   if (op.precision() == PMG_FP64)
     return solve_cg_t<double>(op, precond, mg, static_cast<const double *>(b),
@@ -1099,8 +1109,8 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
   sm_[l]->step(sol_[l].data(), def_[l].data(), s);
 }
 
-void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {
-  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+const void *VCycleMultigrid::apply(const void *src, hipStream_t s) const {
+  PMG_REQUIRE(src, PMG_ERR_INVALID_ARG, "NULL vector");
   const int top = levels() - 1;
   const int64_t n = ops_[top]->m();
   const bool in64 = outer_precision_ == PMG_FP64, lv64 = precision_ == PMG_FP64;
@@ -1111,10 +1121,18 @@ void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {
   if (use_graph_ && !graph_exec_) capture();  // may fall back to the eager path
   if (use_graph_) PMG_HIP_CHECK(hipGraphLaunch(graph_exec_, s));
   else v_cycle(top, s);
-  if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(dst), static_cast<const double *>(sol_[top].data()), s);
-  else if (in64) launch_convert<double, float>(n, static_cast<double *>(dst), static_cast<const float *>(sol_[top].data()), s);
-  else if (lv64) launch_convert<float, double>(n, static_cast<float *>(dst), static_cast<const double *>(sol_[top].data()), s);
-  else launch_copy<float>(n, static_cast<float *>(dst), static_cast<const float *>(sol_[top].data()), s);
+  return sol_[top].data();
+}
+
+void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  const int64_t n = ops_.back()->m();
+  const bool in64 = outer_precision_ == PMG_FP64, lv64 = precision_ == PMG_FP64;
+  const void *sol = apply(src, s);
+  if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(dst), static_cast<const double *>(sol), s);
+  else if (in64) launch_convert<double, float>(n, static_cast<double *>(dst), static_cast<const float *>(sol), s);
+  else if (lv64) launch_convert<float, double>(n, static_cast<float *>(dst), static_cast<const double *>(sol), s);
+  else launch_copy<float>(n, static_cast<float *>(dst), static_cast<const float *>(sol), s);
 }
 
 }  // namespace pmg

@@ -280,7 +280,12 @@ class VCycleMultigrid {
   VCycleMultigrid(const VCycleMultigrid &) = delete;
   VCycleMultigrid &operator=(const VCycleMultigrid &) = delete;
   void vmult(void *dst, const void *src, hipStream_t s) const;
+  // The V-cycle without the conversion of its result: returns the finest
+  // level's solution (level precision, valid until the next application).
+  const void *apply(const void *src, hipStream_t s) const;
   int outer_precision() const { return outer_precision_; }
+  int level_precision() const { return precision_; }
+  int64_t m() const { return ops_.back()->m(); }  // size of the finest level
   int levels() const { return static_cast<int>(ops_.size()); }
   // the V-cycle body runs as a captured HIP graph (no host sync inside it)
   bool uses_graph() const { return use_graph_; }

@@ -71,8 +71,8 @@ __device__ inline double block_sum(double v) {
   return s;  // valid in thread 0
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads) dot_k(int64_t n, const T *a, const T *b, double *partials) {
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(kThreads) dot_k(int64_t n, const TA *a, const TB *b, double *partials) {
   double s = 0;
   for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
     s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
@@ -149,12 +149,14 @@ cg_pz_k(int64_t n, const T *r, const T *pinv, T *z, double *partials) {
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-template <typename T>
-__global__ void cg_p_k(int64_t n, T *p, const T *z, const double *sc, int first) {
+template <typename T, typename TZ>
+__global__ void cg_p_k(int64_t n, T *p, const TZ *z, const double *sc, int first) {
   const double ro = sc[kRhoOld];
   const T beta = (first || ro == 0.0) ? T(0) : static_cast<T>(sc[kRho] / ro);
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    p[i] = first ? z[i] : z[i] + beta * p[i];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const T zi = static_cast<T>(z[i]);
+    p[i] = first ? zi : zi + beta * p[i];
+  }
 }
 
 __global__ void cg_scalars_k(double *sc, int op) {
@@ -209,9 +211,9 @@ template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStrea
   hipLaunchKernelGGL(residual_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, t, b);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template <typename T>
-void launch_dot(int64_t n, const T *a, const T *b, double *partials, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(dot_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, a, b, partials);
+template <typename TA, typename TB>
+void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s) {
+  hipLaunchKernelGGL((dot_k<TA, TB>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, a, b, partials);
   PMG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, kDotBlocks, out, 0, 1);
   PMG_HIP_CHECK(hipGetLastError());
@@ -244,10 +246,10 @@ void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *p
   hipLaunchKernelGGL(cg_pz_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, pinv, z, partials);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template <typename T>
-void launch_cg_update_p(int64_t n, T *p, const T *z, const double *sc, int first, hipStream_t s) {
+template <typename T, typename TZ>
+void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int first, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(cg_p_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, p, z, sc, first);
+  hipLaunchKernelGGL((cg_p_k<T, TZ>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, p, z, sc, first);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_cg_scalars(double *sc, int op, hipStream_t s) {
@@ -274,6 +276,15 @@ void launch_cg_scalars(double *sc, int op, hipStream_t s) {
   template void launch_cg_update_p<T>(int64_t, T *, const T *, const double *, int, hipStream_t);
 PMG_INST(double)
 PMG_INST(float)
+// CG vectors of one precision with a preconditioned vector z of the other
+template void launch_dot<double, float>(int64_t, const double *, const float *, double *, double *,
+                                        hipStream_t);
+template void launch_dot<float, double>(int64_t, const float *, const double *, double *, double *,
+                                        hipStream_t);
+template void launch_cg_update_p<double, float>(int64_t, double *, const float *, const double *,
+                                                int, hipStream_t);
+template void launch_cg_update_p<float, double>(int64_t, float *, const double *, const double *,
+                                                int, hipStream_t);
 #undef PMG_INST
 template void launch_convert<double, float>(int64_t, double *, const float *, hipStream_t);
 template void launch_convert<float, double>(int64_t, float *, const double *, hipStream_t);

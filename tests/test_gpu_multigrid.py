@@ -238,6 +238,33 @@ def test_cg_reports_non_convergence(pmg):
     assert e.value.status == pmg.ERR_NOT_CONVERGED
 
 
+def test_cg_rejects_mismatched_mg_and_reuses_its_vectors(pmg):
+    """An MG preconditioner whose finest level is not the operator's size is
+    an error; repeated solves on one operator (whose handle keeps the CG
+    vectors) give the same result, also after a solve of another size (to
+    rounding: the atomic scatter's summation order varies between runs)."""
+    import torch
+    mg, ops, _, _ = pmg.build_hierarchy([(pmg.box(2), 2), (pmg.box(4), 2)])
+    other = pmg.LaplaceOperatorBK3(pmg.box(3), 2)
+    x = other.initialize_dof_vector()
+    with pytest.raises(pmg.PMGError) as e:
+        pmg.solve_cg(other, other.assemble_rhs(1), x, pmg.PRECOND_MG, mg, 1e-10, 50)
+    assert e.value.status == pmg.ERR_SIZE
+    fine = ops[-1]
+    b = fine.assemble_rhs(1)
+    runs = []
+    for _ in range(2):
+        x = fine.initialize_dof_vector()
+        runs.append((pmg.solve_cg(fine, b, x, pmg.PRECOND_MG, mg, 1e-10, 50), x))
+        other.compute_diagonal()
+        y = other.initialize_dof_vector()
+        pmg.solve_cg(other, other.assemble_rhs(1), y, pmg.PRECOND_JACOBI, None, 1e-8, 500)
+    (it0, r00, r0), x0 = runs[0]
+    (it1, r01, r1), x1 = runs[1]
+    assert it0 == it1 and abs(r00 - r01) <= 1e-12 * r00 and r1 <= 1e-10 * r01
+    assert rel_err(x1.cpu().numpy(), x0.cpu().numpy()) < 1e-12
+
+
 def test_vcycle_is_finite_over_poisoned_memory(pmg):
     """Work vectors come from fresh device allocations whose contents are
     arbitrary: the V-cycle (including the one-workgroup coarse PCG) must never
