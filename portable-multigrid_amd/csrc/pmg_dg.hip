@@ -420,14 +420,14 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 dg_cg_restrict_kernel(const int64_t n, const uint32_t *__restrict__ cg_map,
                       const uint32_t *__restrict__ dg_map, const T *__restrict__ src,
-                      T *__restrict__ dst)
+                      const T *__restrict__ sub, T *__restrict__ dst)
 {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const uint32_t c = cg_map[e];
   if (c == kInvalid) return;
   const int64_t fdof = dg_map ? static_cast<int64_t>(dg_map[e]) : e;
-  atomic_add(dst + c, src[fdof]);
+  atomic_add(dst + c, sub ? src[fdof] - sub[fdof] : src[fdof]);
 }
 
 // Restriction as a gather over the inverse map: CG DoF i sums the DG values
@@ -437,33 +437,39 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 dg_cg_restrict_gather_kernel(const int64_t n_cg, const uint32_t *__restrict__ off,
                              const uint32_t *__restrict__ list, const T *__restrict__ src,
-                             T *__restrict__ dst)
+                             const T *__restrict__ sub, T *__restrict__ dst)
 {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n_cg) return;
   const uint32_t k0 = off[i], k1 = off[i + 1];
   if (k0 == k1) return;
   T acc = T(0);
-  for (uint32_t k = k0; k < k1; ++k) acc += src[list[k]];
+  if (sub)
+    for (uint32_t k = k0; k < k1; ++k) acc += src[list[k]] - sub[list[k]];
+  else
+    for (uint32_t k = k0; k < k1; ++k) acc += src[list[k]];
   dst[i] += acc;
 }
 
 template <typename T>
 void launch_dg_cg_restrict_gather(int64_t n_cg, const uint32_t *off, const uint32_t *list,
-                                  const T *src, T *dst, hipStream_t s) {
+                                  const T *src, T *dst, hipStream_t s, const T *sub) {
   if (n_cg == 0) return;
   hipLaunchKernelGGL(dg_cg_restrict_gather_kernel<T>, dim3(static_cast<unsigned>((n_cg + 255) / 256)),
-                     dim3(256), 0, s, n_cg, off, list, src, dst);
+                     dim3(256), 0, s, n_cg, off, list, src, sub, dst);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_dg_cg_restrict_gather<double>(int64_t, const uint32_t *, const uint32_t *,
-                                                   const double *, double *, hipStream_t);
+                                                   const double *, double *, hipStream_t,
+                                                   const double *);
 template void launch_dg_cg_restrict_gather<float>(int64_t, const uint32_t *, const uint32_t *,
-                                                  const float *, float *, hipStream_t);
+                                                  const float *, float *, hipStream_t,
+                                                  const float *);
 
 template <typename T>
 void launch_dg_cg_transfer(bool prolongate, int64_t n_entries, const uint32_t *cg_map,
-                           const uint32_t *dg_map, const T *src, T *dst, hipStream_t s) {
+                           const uint32_t *dg_map, const T *src, T *dst, hipStream_t s,
+                           const T *sub) {
   if (n_entries == 0) return;
   const dim3 grid(static_cast<unsigned>((n_entries + 255) / 256));
   if (prolongate)
@@ -471,12 +477,12 @@ void launch_dg_cg_transfer(bool prolongate, int64_t n_entries, const uint32_t *c
                        dg_map, src, dst);
   else
     hipLaunchKernelGGL(dg_cg_restrict_kernel<T>, grid, dim3(256), 0, s, n_entries, cg_map, dg_map,
-                       src, dst);
+                       src, sub, dst);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_dg_cg_transfer<double>(bool, int64_t, const uint32_t *, const uint32_t *,
-                                            const double *, double *, hipStream_t);
+                                            const double *, double *, hipStream_t, const double *);
 template void launch_dg_cg_transfer<float>(bool, int64_t, const uint32_t *, const uint32_t *,
-                                           const float *, float *, hipStream_t);
+                                           const float *, float *, hipStream_t, const float *);
 
 }  // namespace pmg

@@ -132,15 +132,18 @@ template <typename T>
 void launch_dg_face_geometry(int nq, int64_t n_faces, const uint32_t *finfo, const double *cv,
                              const double *qp, const double *qw, int metric, T *fdata,
                              hipStream_t s);
-// DG -> CG restriction as a gather: dst[i] += sum of src[list[off[i] .. off[i+1])]
+// DG -> CG restriction as a gather: dst[i] += sum of (src - sub)[list[off[i] .. off[i+1])]
+// (sub optional)
 template <typename T>
 void launch_dg_cg_restrict_gather(int64_t n_cg, const uint32_t *off, const uint32_t *list,
-                                  const T *src, T *dst, hipStream_t s);
+                                  const T *src, T *dst, hipStream_t s, const T *sub = nullptr);
 // DG(p) <-> CG(p): prolongate dst[dg(e)] += src[cg_map[e]], restrict
-// dst[cg_map[e]] += src[dg(e)]; dg(e) = dg_map ? dg_map[e] : e; invalid CG entries skipped
+// dst[cg_map[e]] += (src - sub)[dg(e)] (sub optional); dg(e) = dg_map ? dg_map[e] : e;
+// invalid CG entries skipped
 template <typename T>
 void launch_dg_cg_transfer(bool prolongate, int64_t n_entries, const uint32_t *cg_map,
-                           const uint32_t *dg_map, const T *src, T *dst, hipStream_t s);
+                           const uint32_t *dg_map, const T *src, T *dst, hipStream_t s,
+                           const T *sub = nullptr);
 
 // ---------------------------------------------------------------- transfers
 template <typename T>
@@ -155,10 +158,11 @@ template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
                        const T *xc, T *yf, hipStream_t s);
+// yc += R (xf - xf_sub); xf_sub optional
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
-                     T *yc, hipStream_t s);
+                     T *yc, hipStream_t s, const T *xf_sub = nullptr);
 
 // ---------------------------------------------------------------- vectors
 template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s);

@@ -692,32 +692,44 @@ void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) c
 
 void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) const {
   PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  restrict_sub(dst, src, nullptr, s);
+}
+
+void MGTransfer::restrict_residual_and_add(void *dst, const void *b, const void *t,
+                                           hipStream_t s) const {
+  PMG_REQUIRE(dst && b && t, PMG_ERR_INVALID_ARG, "NULL vector");
+  restrict_sub(dst, b, t, s);
+}
+
+// dst += R (src - sub), sub optional
+void MGTransfer::restrict_sub(void *dst, const void *src, const void *sub, hipStream_t s) const {
+  const bool f64 = precision_ == PMG_FP64;
+  auto D = [](const void *p) { return static_cast<const double *>(p); };
+  auto F = [](const void *p) { return static_cast<const float *>(p); };
   if (kind_ == PMG_TRANSFER_DG_CG) {
     const int64_t n = n_cells_ * nmc_ * nmc_ * nmc_;
-    if (inv_off_.ptr && precision_ == PMG_FP64)
-      launch_dg_cg_restrict_gather<double>(n_dofs_c_, inv_off_.ptr, inv_list_.ptr,
-                                           static_cast<const double *>(src),
-                                           static_cast<double *>(dst), s);
+    if (inv_off_.ptr && f64)
+      launch_dg_cg_restrict_gather<double>(n_dofs_c_, inv_off_.ptr, inv_list_.ptr, D(src),
+                                           static_cast<double *>(dst), s, D(sub));
     else if (inv_off_.ptr)
-      launch_dg_cg_restrict_gather<float>(n_dofs_c_, inv_off_.ptr, inv_list_.ptr,
-                                          static_cast<const float *>(src),
-                                          static_cast<float *>(dst), s);
-    else if (precision_ == PMG_FP64)
-      launch_dg_cg_transfer<double>(false, n, map_c_.ptr, map_f_.ptr, static_cast<const double *>(src),
-                                    static_cast<double *>(dst), s);
+      launch_dg_cg_restrict_gather<float>(n_dofs_c_, inv_off_.ptr, inv_list_.ptr, F(src),
+                                          static_cast<float *>(dst), s, F(sub));
+    else if (f64)
+      launch_dg_cg_transfer<double>(false, n, map_c_.ptr, map_f_.ptr, D(src),
+                                    static_cast<double *>(dst), s, D(sub));
     else
-      launch_dg_cg_transfer<float>(false, n, map_c_.ptr, map_f_.ptr, static_cast<const float *>(src),
-                                   static_cast<float *>(dst), s);
+      launch_dg_cg_transfer<float>(false, n, map_c_.ptr, map_f_.ptr, F(src),
+                                   static_cast<float *>(dst), s, F(sub));
     return;
   }
-  if (precision_ == PMG_FP64)
+  if (f64)
     launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
-                            owner_ ? nullptr : static_cast<const double *>(w_restrict_.data()),
-                            static_cast<const double *>(src), static_cast<double *>(dst), s);
+                            owner_ ? nullptr : D(w_restrict_.data()), D(src),
+                            static_cast<double *>(dst), s, D(sub));
   else
     launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
-                           owner_ ? nullptr : static_cast<const float *>(w_restrict_.data()),
-                           static_cast<const float *>(src), static_cast<float *>(dst), s);
+                           owner_ ? nullptr : F(w_restrict_.data()), F(src),
+                           static_cast<float *>(dst), s, F(sub));
 }
 
 // ============================================================ CG
@@ -1096,14 +1108,10 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
   else
     ops_[l]->vmult(t_[l].data(), sol_[l].data(), s);
   const int64_t nc = ops_[l - 1]->m();
-  if (f64) {
-    launch_residual<double>(n, static_cast<double *>(t_[l].data()), static_cast<const double *>(def_[l].data()), s);
-    launch_fill<double>(nc, static_cast<double *>(def_[l - 1].data()), 0.0, s);
-  } else {
-    launch_residual<float>(n, static_cast<float *>(t_[l].data()), static_cast<const float *>(def_[l].data()), s);
-    launch_fill<float>(nc, static_cast<float *>(def_[l - 1].data()), 0.0f, s);
-  }
-  xfer_[l]->restrict_and_add(def_[l - 1].data(), t_[l].data(), s);
+  if (f64) launch_fill<double>(nc, static_cast<double *>(def_[l - 1].data()), 0.0, s);
+  else launch_fill<float>(nc, static_cast<float *>(def_[l - 1].data()), 0.0f, s);
+  // coarse defect = R (d - A s): the residual is formed inside the restriction
+  xfer_[l]->restrict_residual_and_add(def_[l - 1].data(), def_[l].data(), t_[l].data(), s);
   v_cycle(l - 1, s);
   xfer_[l]->prolongate_and_add(sol_[l].data(), sol_[l - 1].data(), s);
   sm_[l]->step(sol_[l].data(), def_[l].data(), s);

@@ -207,11 +207,15 @@ class MGTransfer {
              int64_t n_dofs_coarse, int64_t n_dofs_fine, double prolongation_scale);
   void prolongate_and_add(void *dst_fine, const void *src_coarse, hipStream_t s) const;
   void restrict_and_add(void *dst_coarse, const void *src_fine, hipStream_t s) const;
+  // dst_coarse += R (b - t) (the V-cycle's defect restriction, one pass)
+  void restrict_residual_and_add(void *dst_coarse, const void *b, const void *t,
+                                 hipStream_t s) const;
   int precision() const { return precision_; }
   int64_t n_dofs_coarse() const { return n_dofs_c_; }
   int64_t n_dofs_fine() const { return n_dofs_f_; }
 
  private:
+  void restrict_sub(void *dst, const void *src, const void *sub, hipStream_t s) const;
   int kind_, precision_, nmc_, nmf_;
   int64_t n_cells_, n_dofs_c_, n_dofs_f_;
   bool owner_ = false;  // owner-computes mode (weights 1/multiplicity), see pmg_transfer.hip

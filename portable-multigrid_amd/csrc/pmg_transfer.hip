@@ -131,8 +131,10 @@ template <int NC, int NF, typename T, bool OWNER>
 __global__ void __launch_bounds__(256)
 restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__restrict__ map_c,
                 const uint32_t *__restrict__ map_f, const T *__restrict__ w, const T scale,
-                const T *__restrict__ xf, T *__restrict__ yc)
+                const T *__restrict__ xf, const T *__restrict__ xs, T *__restrict__ yc)
 {
+  // restricts xf - xs (xs optional: the V-cycle's residual d - A s, computed
+  // here instead of in a pass of its own)
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
   constexpr int E = XferTile<NF>::cells, LC = xfer_lds_cell<NC, NF>();
   __shared__ T lds[E * LC];
@@ -151,10 +153,12 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
     for (int c = 0; c < NF; ++c) {
       if (OWNER) {
         const uint32_t m = map_f[base + c * NF2];
-        f[c] = (m & 0x80000000u) ? xf[m & 0x7FFFFFFFu] : T(0);
+        const uint32_t d = m & 0x7FFFFFFFu;
+        f[c] = (m & 0x80000000u) ? (xs ? xf[d] - xs[d] : xf[d]) : T(0);
       } else {
         const T wt = w[base + c * NF2];
-        f[c] = wt == T(0) ? T(0) : wt * xf[map_f[base + c * NF2]];
+        const uint32_t d = map_f[base + c * NF2];
+        f[c] = wt == T(0) ? T(0) : wt * (xs ? xf[d] - xs[d] : xf[d]);
       }
     }
 #pragma unroll
@@ -207,7 +211,7 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 template <int NC, int NF, typename T>
 static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
                         const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                        const T *x, T *y, hipStream_t s) {
+                        const T *x, const T *xs, T *y, hipStream_t s) {
   if (n_cells <= 0) return;
   using Tile = XferTile<NF>;
   const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
@@ -222,21 +226,21 @@ static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
                        map_f, w, scale, x, y);
   else if (w)
     hipLaunchKernelGGL((restrict_kernel<NC, NF, T, false>), g, b, 0, s, n_cells, M, map_c, map_f,
-                       w, scale, x, y);
+                       w, scale, x, xs, y);
   else
     hipLaunchKernelGGL((restrict_kernel<NC, NF, T, true>), g, b, 0, s, n_cells, M, map_c, map_f,
-                       w, scale, x, y);
+                       w, scale, x, xs, y);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 static void xfer_dispatch(bool prolong, int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                           const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                          const T *x, T *y, hipStream_t s) {
-#define PMG_XF(C, F)                                                               \
-  if (nmc == C && nmf == F) {                                                      \
-    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, scale, x, y, s);    \
-    return;                                                                        \
+                          const T *x, const T *xs, T *y, hipStream_t s) {
+#define PMG_XF(C, F)                                                                 \
+  if (nmc == C && nmf == F) {                                                        \
+    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, scale, x, xs, y, s);  \
+    return;                                                                          \
   }
   // h-transfers (p+1 -> 2p+1)
   PMG_XF(2, 3) PMG_XF(3, 5) PMG_XF(4, 7) PMG_XF(5, 9) PMG_XF(6, 11) PMG_XF(7, 13)
@@ -255,13 +259,13 @@ template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
                        const T *xc, T *yf, hipStream_t s) {
-  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, scale, xc, yf, s);
+  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, scale, xc, nullptr, yf, s);
 }
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
-                     T *yc, hipStream_t s) {
-  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, T(1), xf, yc, s);
+                     T *yc, hipStream_t s, const T *xf_sub) {
+  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, T(1), xf, xf_sub, yc, s);
 }
 
 template void launch_prolongate<double>(int, int, int64_t, const Transfer1D<double> &,
@@ -272,9 +276,9 @@ template void launch_prolongate<float>(int, int, int64_t, const Transfer1D<float
                                        const float *, float *, hipStream_t);
 template void launch_restrict<double>(int, int, int64_t, const Transfer1D<double> &,
                                       const uint32_t *, const uint32_t *, const double *,
-                                      const double *, double *, hipStream_t);
+                                      const double *, double *, hipStream_t, const double *);
 template void launch_restrict<float>(int, int, int64_t, const Transfer1D<float> &,
                                      const uint32_t *, const uint32_t *, const float *,
-                                     const float *, float *, hipStream_t);
+                                     const float *, float *, hipStream_t, const float *);
 
 }  // namespace pmg
