@@ -327,14 +327,47 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
   }
   __syncthreads();
 
+  // A level of at most E cells is one batch: its geometric factors and DoF
+  // indices are loaded once into registers instead of once per iteration.
+  constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
+  const bool one = n_cells <= E;
+  const int e1 = tid / NQ2, t1 = tid - e1 * NQ2, ta = t1 % NQ, tb = t1 / NQ;
+  const bool act1 = one && e1 < E && e1 < n_cells;
+  const bool col1 = act1 && ta < NM && tb < NM;
+  T *P1 = lds + (act1 ? e1 : 0) * lds_cell_stride<T>(3 * NQ3);
+  const T *G1 = G + g_index(NQ, act1 ? e1 : 0, 0, t1);
+  T g1[6][NQ];
+  uint32_t idx1[NM];
+  if (act1) bk3_load_g<NQ, T, kGEarly>(G1, g1);
+  if (col1) {
+#pragma unroll
+    for (int k = 0; k < NM; ++k) idx1[k] = map[e1 * NM3 + k * NM2 + tb * NM + ta];
+  }
+
   auto apply = [&]() {  // qv = A pv, constrained rows copied
 #pragma unroll
     for (int j = 0; j < NPL; ++j)
       if (tid + j * LANES < n) qv[tid + j * LANES] = T(0);
     __syncthreads();
-    for (int64_t base = 0; base < n_cells; base += E) {
-      bk3_batch<NM, NQ, T>(pv, qv, G, map, nullptr, nullptr, nullptr, n_cells, sh, base, lds);
+    if (one) {
+      T u[NM], v[NM];
+      if (col1) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) u[k] = idx1[k] == kInvalid ? T(0) : pv[idx1[k]];
+      }
+      bk3_cell_apply<NM, NQ, T, kGEarly>(u, v, g1, G1, sh, P1, P1 + NQ3, P1 + 2 * NQ3, act1, ta,
+                                         tb);
+      if (col1) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k)
+          if (idx1[k] != kInvalid) atomicAdd(qv + idx1[k], v[k]);
+      }
       __syncthreads();
+    } else {
+      for (int64_t base = 0; base < n_cells; base += E) {
+        bk3_batch<NM, NQ, T>(pv, qv, G, map, nullptr, nullptr, nullptr, n_cells, sh, base, lds);
+        __syncthreads();
+      }
     }
     for (int64_t i = tid; i < n_cons; i += LANES) qv[cons[i]] = pv[cons[i]];
     __syncthreads();
