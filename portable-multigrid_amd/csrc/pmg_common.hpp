@@ -77,6 +77,28 @@ struct DeviceArray {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned, host-coherent doubles: kernels store into them directly and the
+// host reads them after synchronising, without a device-to-host copy.
+struct PinnedDoubles {
+  double *ptr = nullptr;
+  PinnedDoubles() = default;
+  PinnedDoubles(const PinnedDoubles &) = delete;
+  PinnedDoubles &operator=(const PinnedDoubles &) = delete;
+  ~PinnedDoubles() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  void allocate(size_t count) {
+    if (ptr) return;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(double),
+                                 hipHostMallocCoherent);
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      throw Error(PMG_ERR_ALLOC, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+    }
+    for (size_t i = 0; i < count; ++i) ptr[i] = 0.0;
+  }
+};
+
 inline size_t element_size(int precision) { return precision == PMG_FP32 ? 4 : 8; }
 
 inline void check_precision(int precision) {

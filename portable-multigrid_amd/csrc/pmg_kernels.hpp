@@ -180,6 +180,9 @@ void launch_set_indexed(int64_t n, const uint32_t *idx, T *dst, T v, hipStream_t
 template <typename T> void launch_invert(int64_t n, T *x, hipStream_t s);
 template <typename TD, typename TS>
 void launch_convert(int64_t n, TD *dst, const TS *src, hipStream_t s);
+// the same, unless *skip != 0 on the device (skip may be null)
+template <typename TD, typename TS>
+void launch_convert_unless(int64_t n, TD *dst, const TS *src, const double *skip, hipStream_t s);
 // partial sums of a.b into `partials` (kDotBlocks entries), then final sum into out[0]
 constexpr int kDotBlocks = 4096;
 template <typename TA, typename TB = TA>
@@ -212,6 +215,8 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
 void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
                             int n_sums, hipStream_t s);
 // scalar bookkeeping for CG on the device: see kernel for slot layout
-void launch_cg_scalars(double *sc, int op, hipStream_t s);
+// also: sc[7] = 1 once sc[2] = r.r <= target2 (or NaN); host[0] = r.r (pinned, optional)
+void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host = nullptr,
+                       double target2 = -1.0);
 
 }  // namespace pmg

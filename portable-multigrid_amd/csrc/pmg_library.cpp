@@ -774,11 +774,14 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
   }
   int it = 0;
   static const bool verbose = std::getenv("PMG_CG_VERBOSE") != nullptr;
+  w.host.allocate(1);
+  const double target2 = target * target;
+  if (precond == PMG_PRECOND_MG) mg->stage(r, s, nullptr);
   while (it < max_it) {
     if (precond == PMG_PRECOND_MG) {
       // z = the V-cycle's own finest-level solution, read in its precision
       // (the same values as after a conversion, one pass less)
-      const void *zl = mg->apply(r, s);
+      const void *zl = mg->run(s);
       if (mg->level_precision() == PMG_FP64) {
         launch_dot<T, double>(n, r, static_cast<const double *>(zl), w.partials.ptr, sc + 0, s);
         launch_cg_update_p<T, double>(n, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
@@ -795,11 +798,15 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
     launch_dot<T>(n, p, q, w.partials.ptr, sc + 1, s);
     launch_cg_update_xr<T>(n, x, r, p, q, sc, w.partials.ptr, s);
     launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 2, 1, s);
-    launch_cg_scalars(sc, 0, s);
+    launch_cg_scalars(sc, 0, s, w.host.ptr, target2);
     ++it;
-    if (it % check_every == 0 || it == max_it) {
-      PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    const bool check = it % check_every == 0 || it == max_it;
+    // the next iteration's V-cycle input is queued before the host waits for
+    // r.r, so the GPU stays busy; it skips itself once r.r has converged
+    if (precond == PMG_PRECOND_MG && it < max_it) mg->stage(r, s, check ? sc + 7 : nullptr);
+    if (check) {
       PMG_HIP_CHECK(hipStreamSynchronize(s));
+      rr = w.host.ptr[0];
       res.res = std::sqrt(rr);
       if (verbose) std::fprintf(stderr, "[pmg cg] it %d  |r|/|r0| = %.3e\n", it, res.res / res.res0);
       if (!(res.res > target)) {  // also stops on NaN
@@ -1117,19 +1124,28 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
   sm_[l]->step(sol_[l].data(), def_[l].data(), s);
 }
 
-const void *VCycleMultigrid::apply(const void *src, hipStream_t s) const {
+void VCycleMultigrid::stage(const void *src, hipStream_t s, const double *skip) const {
   PMG_REQUIRE(src, PMG_ERR_INVALID_ARG, "NULL vector");
   const int top = levels() - 1;
   const int64_t n = ops_[top]->m();
   const bool in64 = outer_precision_ == PMG_FP64, lv64 = precision_ == PMG_FP64;
-  if (in64 && lv64) launch_copy<double>(n, static_cast<double *>(def_[top].data()), static_cast<const double *>(src), s);
-  else if (in64) launch_convert<float, double>(n, static_cast<float *>(def_[top].data()), static_cast<const double *>(src), s);
-  else if (lv64) launch_convert<double, float>(n, static_cast<double *>(def_[top].data()), static_cast<const float *>(src), s);
-  else launch_copy<float>(n, static_cast<float *>(def_[top].data()), static_cast<const float *>(src), s);
+  void *d = def_[top].data();
+  if (in64 && lv64) launch_convert_unless<double, double>(n, static_cast<double *>(d), static_cast<const double *>(src), skip, s);
+  else if (in64) launch_convert_unless<float, double>(n, static_cast<float *>(d), static_cast<const double *>(src), skip, s);
+  else if (lv64) launch_convert_unless<double, float>(n, static_cast<double *>(d), static_cast<const float *>(src), skip, s);
+  else launch_convert_unless<float, float>(n, static_cast<float *>(d), static_cast<const float *>(src), skip, s);
+}
+
+const void *VCycleMultigrid::run(hipStream_t s) const {
   if (use_graph_ && !graph_exec_) capture();  // may fall back to the eager path
   if (use_graph_) PMG_HIP_CHECK(hipGraphLaunch(graph_exec_, s));
-  else v_cycle(top, s);
-  return sol_[top].data();
+  else v_cycle(levels() - 1, s);
+  return sol_[levels() - 1].data();
+}
+
+const void *VCycleMultigrid::apply(const void *src, hipStream_t s) const {
+  stage(src, s, nullptr);
+  return run(s);
 }
 
 void VCycleMultigrid::vmult(void *dst, const void *src, hipStream_t s) const {

@@ -255,6 +255,7 @@ class PreconditionChebyshev {
 struct CGWork {
   DeviceArray<double> sc, partials;
   DeviceVector r, z, p, q;
+  PinnedDoubles host;  // r.r of the last iteration, stored by the device
   void reinit(int precision, int64_t n);
 };
 
@@ -287,6 +288,11 @@ class VCycleMultigrid {
   // The V-cycle without the conversion of its result: returns the finest
   // level's solution (level precision, valid until the next application).
   const void *apply(const void *src, hipStream_t s) const;
+  // apply() in two parts: stage converts src into the finest level's defect
+  // (on the device: not if *skip != 0; skip may be null), run executes the
+  // V-cycle on it and returns the finest level's solution
+  void stage(const void *src, hipStream_t s, const double *skip) const;
+  const void *run(hipStream_t s) const;
   int outer_precision() const { return outer_precision_; }
   int level_precision() const { return precision_; }
   int64_t m() const { return ops_.back()->m(); }  // size of the finest level

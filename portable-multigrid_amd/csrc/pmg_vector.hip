@@ -117,7 +117,7 @@ __global__ void cheby_k(int mode, int64_t n, T *x, T *old, const T *b, const T *
 
 // CG scalar slots (device doubles):
 //  0 rho = r.z      1 p.q       2 r.r       3 rho_old   4 alpha   5 beta
-enum { kRho = 0, kPq = 1, kRr = 2, kRhoOld = 3, kAlpha = 4, kBeta = 5 };
+enum { kRho = 0, kPq = 1, kRr = 2, kRhoOld = 3, kAlpha = 4, kBeta = 5, kConverged = 7 };
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -159,10 +159,22 @@ __global__ void cg_p_k(int64_t n, T *p, const TZ *z, const double *sc, int first
   }
 }
 
-__global__ void cg_scalars_k(double *sc, int op) {
+// end of an iteration: rho_old = rho; host[0] = r.r (optional, pinned);
+// sc[kConverged] = 1 once r.r <= target2 (or is NaN), which skips the
+// speculatively queued input pass of the next iteration
+__global__ void cg_scalars_k(double *sc, int op, double *host, double target2) {
   if (threadIdx.x == 0) {
-    if (op == 0) sc[kRhoOld] = sc[kRho];  // end of iteration bookkeeping
+    if (op == 0) sc[kRhoOld] = sc[kRho];
+    const double rr = sc[kRr];
+    sc[kConverged] = !(rr > target2) ? 1.0 : 0.0;
+    if (host) host[0] = rr;
   }
+}
+template <typename TD, typename TS>
+__global__ void convert_unless_k(int64_t n, TD *d, const TS *s, const double *skip) {
+  if (skip && *skip != 0.0) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    d[i] = static_cast<TD>(s[i]);
 }
 }  // namespace
 
@@ -204,6 +216,13 @@ template <typename TD, typename TS>
 void launch_convert(int64_t n, TD *d, const TS *src, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL((convert_k<TD, TS>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, d, src);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename TD, typename TS>
+void launch_convert_unless(int64_t n, TD *d, const TS *src, const double *skip, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((convert_unless_k<TD, TS>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, d,
+                     src, skip);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStream_t s) {
@@ -252,8 +271,8 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
   hipLaunchKernelGGL((cg_p_k<T, TZ>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, p, z, sc, first);
   PMG_HIP_CHECK(hipGetLastError());
 }
-void launch_cg_scalars(double *sc, int op, hipStream_t s) {
-  hipLaunchKernelGGL(cg_scalars_k, dim3(1), dim3(64), 0, s, sc, op);
+void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double target2) {
+  hipLaunchKernelGGL(cg_scalars_k, dim3(1), dim3(64), 0, s, sc, op, host, target2);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -290,5 +309,12 @@ template void launch_convert<double, float>(int64_t, double *, const float *, hi
 template void launch_convert<float, double>(int64_t, float *, const double *, hipStream_t);
 template void launch_convert<double, double>(int64_t, double *, const double *, hipStream_t);
 template void launch_convert<float, float>(int64_t, float *, const float *, hipStream_t);
+#define PMG_CU(TD, TS) \
+  template void launch_convert_unless<TD, TS>(int64_t, TD *, const TS *, const double *, hipStream_t);
+PMG_CU(double, float)
+PMG_CU(float, double)
+PMG_CU(double, double)
+PMG_CU(float, float)
+#undef PMG_CU
 
 }  // namespace pmg
