@@ -615,6 +615,29 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     PMG_REQUIRE(int64_t(map_fine[e]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
                 "fine map entry out of range (fine maps carry no invalid entries)");
   map_c_.upload(map_coarse, static_cast<size_t>(n_cells * nc3));
+  // Restriction without coarse atomics: cells store their coarse values to
+  // a buffer [cell][nc^3], and each coarse DoF gathers its entries through
+  // the inverse map (CSR over buffer positions; constrained entries in none).
+  // It costs a second launch and the buffer's bytes, so it is used where it
+  // measured faster: large levels (>= 2^20 entries) in FP32 or of a
+  // p-transfer (Q4 h 64^3 -> 32^3: FP32 76 -> 58 us, FP64 83 -> 87 us;
+  // Q6 -> Q3 on 48^3: FP32 134 -> 114 us, FP64 148 -> 139 us).
+  // PMG_RESTRICT_GATHER=0|1 forces either path.
+  const bool big = n_cells * nc3 >= (int64_t(1) << 20);
+  bool use_gather = big && (precision == PMG_FP32 || kind == PMG_TRANSFER_P);
+  if (const char *genv = std::getenv("PMG_RESTRICT_GATHER")) use_gather = std::string(genv) == "1";
+  if (use_gather && n_cells * nc3 < (int64_t(1) << 32) && n_dofs_coarse > 0) {
+    std::vector<uint32_t> off(static_cast<size_t>(n_dofs_coarse + 1), 0u);
+    for (int64_t e = 0; e < n_cells * nc3; ++e)
+      if (map_coarse[e] != kInvalid) ++off[map_coarse[e] + 1];
+    for (int64_t i = 0; i < n_dofs_coarse; ++i) off[i + 1] += off[i];
+    std::vector<uint32_t> list(off.back()), fill(off.begin(), off.end() - 1);
+    for (int64_t e = 0; e < n_cells * nc3; ++e)
+      if (map_coarse[e] != kInvalid) list[fill[map_coarse[e]]++] = static_cast<uint32_t>(e);
+    cinv_off_.upload(off);
+    cinv_list_.upload(list);
+    cell_buf_.reinit(precision, n_cells * nc3);
+  }
   const size_t nw = static_cast<size_t>(n_cells * nf3);
   // Owner mode when the weights are 1/multiplicity (0 on constrained DoFs,
   // consistently for every entry of a DoF), as the reference's setup_weights
@@ -722,14 +745,24 @@ void MGTransfer::restrict_sub(void *dst, const void *src, const void *sub, hipSt
                                    static_cast<float *>(dst), s, F(sub));
     return;
   }
-  if (f64)
+  const bool gather = cinv_off_.ptr != nullptr;
+  if (f64) {
     launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
                             owner_ ? nullptr : D(w_restrict_.data()), D(src),
-                            static_cast<double *>(dst), s, D(sub));
-  else
+                            static_cast<double *>(dst), s, D(sub),
+                            gather ? static_cast<double *>(cell_buf_.data()) : nullptr);
+    if (gather)
+      launch_dg_cg_restrict_gather<double>(n_dofs_c_, cinv_off_.ptr, cinv_list_.ptr,
+                                           D(cell_buf_.data()), static_cast<double *>(dst), s);
+  } else {
     launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
                            owner_ ? nullptr : F(w_restrict_.data()), F(src),
-                           static_cast<float *>(dst), s, F(sub));
+                           static_cast<float *>(dst), s, F(sub),
+                           gather ? static_cast<float *>(cell_buf_.data()) : nullptr);
+    if (gather)
+      launch_dg_cg_restrict_gather<float>(n_dofs_c_, cinv_off_.ptr, cinv_list_.ptr,
+                                          F(cell_buf_.data()), static_cast<float *>(dst), s);
+  }
 }
 
 // ============================================================ CG

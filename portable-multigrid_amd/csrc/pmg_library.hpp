@@ -226,6 +226,11 @@ class MGTransfer {
   // DG-CG: inverse map (CSR) of the CG DoFs to their DG copies for the
   // atomics-free restriction (empty if the entry count needs 64-bit offsets)
   DeviceArray<uint32_t> inv_off_, inv_list_;
+  // h/p restriction through a cell buffer: inverse coarse map (CSR over
+  // positions cell * nc^3 + i) and the buffer; a transfer's restrictions
+  // must therefore be ordered (they share the buffer)
+  DeviceArray<uint32_t> cinv_off_, cinv_list_;
+  mutable DeviceVector cell_buf_;
   DeviceVector w_prolong_, w_restrict_;
 };
 

@@ -131,10 +131,13 @@ template <int NC, int NF, typename T, bool OWNER>
 __global__ void __launch_bounds__(256)
 restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__restrict__ map_c,
                 const uint32_t *__restrict__ map_f, const T *__restrict__ w, const T scale,
-                const T *__restrict__ xf, const T *__restrict__ xs, T *__restrict__ yc)
+                const T *__restrict__ xf, const T *__restrict__ xs, T *__restrict__ yc,
+                T *__restrict__ buf)
 {
   // restricts xf - xs (xs optional: the V-cycle's residual d - A s, computed
-  // here instead of in a pass of its own)
+  // here instead of in a pass of its own).  buf (optional): the cell's coarse
+  // values are stored to buf[cell][nc^3] instead of added to yc with atomics;
+  // a gather over the inverse coarse map then sums them.
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
   constexpr int E = XferTile<NF>::cells, LC = xfer_lds_cell<NC, NF>();
   __shared__ T lds[E * LC];
@@ -198,12 +201,18 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
     }
   }
   __syncthreads();
-  // scatter-add to the coarse DoFs by lane (i,j) over k (skip constrained)
+  // scatter-add to the coarse DoFs by lane (i,j) over k (skip constrained),
+  // or store the cell's values contiguously
   if (active && ta < NC && tb < NC) {
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      const uint32_t d = map_c[cell * NC3 + k * NC2 + tb * NC + ta];
-      if (d != kInvalid) atomic_add(yc + d, L0[k * NC2 + tb * NC + ta]);
+      const int l = k * NC2 + tb * NC + ta;
+      if (buf) {
+        buf[cell * NC3 + l] = L0[l];
+      } else {
+        const uint32_t d = map_c[cell * NC3 + l];
+        if (d != kInvalid) atomic_add(yc + d, L0[l]);
+      }
     }
   }
 }
@@ -211,7 +220,7 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 template <int NC, int NF, typename T>
 static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
                         const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                        const T *x, const T *xs, T *y, hipStream_t s) {
+                        const T *x, const T *xs, T *y, T *buf, hipStream_t s) {
   if (n_cells <= 0) return;
   using Tile = XferTile<NF>;
   const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
@@ -226,21 +235,21 @@ static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
                        map_f, w, scale, x, y);
   else if (w)
     hipLaunchKernelGGL((restrict_kernel<NC, NF, T, false>), g, b, 0, s, n_cells, M, map_c, map_f,
-                       w, scale, x, xs, y);
+                       w, scale, x, xs, y, buf);
   else
     hipLaunchKernelGGL((restrict_kernel<NC, NF, T, true>), g, b, 0, s, n_cells, M, map_c, map_f,
-                       w, scale, x, xs, y);
+                       w, scale, x, xs, y, buf);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 static void xfer_dispatch(bool prolong, int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                           const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                          const T *x, const T *xs, T *y, hipStream_t s) {
-#define PMG_XF(C, F)                                                                 \
-  if (nmc == C && nmf == F) {                                                        \
-    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, scale, x, xs, y, s);  \
-    return;                                                                          \
+                          const T *x, const T *xs, T *y, T *buf, hipStream_t s) {
+#define PMG_XF(C, F)                                                                      \
+  if (nmc == C && nmf == F) {                                                             \
+    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, scale, x, xs, y, buf, s);  \
+    return;                                                                               \
   }
   // h-transfers (p+1 -> 2p+1)
   PMG_XF(2, 3) PMG_XF(3, 5) PMG_XF(4, 7) PMG_XF(5, 9) PMG_XF(6, 11) PMG_XF(7, 13)
@@ -259,13 +268,15 @@ template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
                        const T *xc, T *yf, hipStream_t s) {
-  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, scale, xc, nullptr, yf, s);
+  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, scale, xc, nullptr, yf, nullptr,
+                   s);
 }
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
-                     T *yc, hipStream_t s, const T *xf_sub) {
-  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, T(1), xf, xf_sub, yc, s);
+                     T *yc, hipStream_t s, const T *xf_sub, T *cell_buf) {
+  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, T(1), xf, xf_sub, yc, cell_buf,
+                   s);
 }
 
 template void launch_prolongate<double>(int, int, int64_t, const Transfer1D<double> &,
@@ -276,9 +287,10 @@ template void launch_prolongate<float>(int, int, int64_t, const Transfer1D<float
                                        const float *, float *, hipStream_t);
 template void launch_restrict<double>(int, int, int64_t, const Transfer1D<double> &,
                                       const uint32_t *, const uint32_t *, const double *,
-                                      const double *, double *, hipStream_t, const double *);
+                                      const double *, double *, hipStream_t, const double *,
+                                      double *);
 template void launch_restrict<float>(int, int, int64_t, const Transfer1D<float> &,
                                      const uint32_t *, const uint32_t *, const float *,
-                                     const float *, float *, hipStream_t, const float *);
+                                     const float *, float *, hipStream_t, const float *, float *);
 
 }  // namespace pmg

@@ -54,6 +54,32 @@ def test_p_transfer_matches_oracle(pmg, orc, pc, pf):
     assert rel_err(yc.cpu().numpy(), ref.restrict_add(xf, np.zeros(nc))) < 1e-12
 
 
+@pytest.mark.parametrize("gather", ["0", "1"])
+@pytest.mark.parametrize("kind,prec", [("h", "fp64"), ("h", "fp32"), ("p", "fp64"), ("p", "fp32")])
+def test_restriction_paths_match_oracle(pmg, orc, monkeypatch, gather, kind, prec):
+    """Both restriction paths: coarse FP atomics, and cell values stored to a
+    buffer then gathered through the inverse coarse map (used on large levels)."""
+    import torch
+    monkeypatch.setenv("PMG_RESTRICT_GATHER", gather)
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    b = pmg.box(3, deform=1, amplitude=0.05)
+    if kind == "h":
+        tr = pmg.GeometricTransfer(b, 2, precision=precision)
+        ref = orc.Transfer(orc.mesh(3, deform=1, amplitude=0.05), 2, kind="h")
+        nc, nf = pmg.box_n_dofs(b, 2), pmg.box_n_dofs(pmg.refine(b), 2)
+    else:
+        tr = pmg.PolynomialTransfer(b, 2, 4, precision=precision)
+        ref = orc.Transfer(orc.mesh(3, deform=1, amplitude=0.05), 2, 4, kind="p")
+        nc, nf = pmg.box_n_dofs(b, 2), pmg.box_n_dofs(b, 4)
+    xf, yc0 = seeded_uniform(nf, 45), seeded_uniform(nc, 46)
+    for rep in range(2):  # the buffer is reused by the second call
+        yc = _t(yc0, dt)
+        tr.restrict_and_add(yc, _t(xf, dt))
+        assert rel_err(yc.double().cpu().numpy(), ref.restrict_add(xf, yc0.copy())) < tol
+
+
 def test_transfer_fp32_and_doubled_prolongation(pmg, orc):
     import torch
     bc = pmg.box(2)
@@ -198,6 +224,28 @@ def test_vcycle_and_mgcg_match_oracle(pmg, orc, levels_spec):
     its32, _, res32 = pmg.solve_cg(fine, b, x32, pmg.PRECOND_MG, mg32, 1e-10, 200)
     assert res32 <= 1e-10 * r0 and its32 <= its + 2
     assert rel_err(x32.cpu().numpy(), x.cpu().numpy()) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_vcycle_with_gathered_restriction_matches_oracle(pmg, orc, monkeypatch, prec):
+    """The V-cycle with every restriction through the cell buffer and the
+    inverse-map gather, forming the defect d - A s inside the restriction."""
+    import torch
+    monkeypatch.setenv("PMG_RESTRICT_GATHER", "1")
+    spec = [(2, 2), (4, 2), (4, 3)]
+    boxes = [pmg.box(n, deform=1, amplitude=0.05) for n, _ in spec]
+    degs = [p for _, p in spec]
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    mg, ops, _, _ = pmg.build_hierarchy(list(zip(boxes, degs)), precision=precision,
+                                        outer_precision=pmg.FP64)
+    ref = orc.Multigrid([orc.mesh(n, deform=1, amplitude=0.05) for n, _ in spec], degs,
+                        coarse_kind=1, coarse_tol=1e-12)
+    r = seeded_uniform(pmg.box_n_dofs(boxes[-1], degs[-1]), 72)
+    r[orc.boundary_dofs(orc.mesh(spec[-1][0]), degs[-1])] = 0
+    z = torch.zeros(len(r), dtype=torch.float64, device="cuda")
+    mg.vmult(z, _t(r))
+    zr = ref.vmult(r)
+    assert rel_err(z.cpu().numpy(), zr) < (1e-9 if prec == "fp64" else 1e-4)
 
 
 def test_jacobi_pcg_matches_oracle(pmg, orc):
