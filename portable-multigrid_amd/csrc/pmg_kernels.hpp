@@ -153,11 +153,13 @@ struct Transfer1D {
 
 // w == nullptr: owner mode (map_f bit 31 marks the owning entry of each fine
 // DoF, see pmg_transfer.hip); otherwise weighted, w pre-scaled for the
-// prolongation (scale is only used in owner mode)
+// prolongation (scale is only used in owner mode).  fine_lines (owner mode,
+// optional): the fine map as x-line words [cell][c][b] = {first DoF, owner
+// bits over a}, used instead of map_f when every fine x-line is consecutive
 template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                       const T *xc, T *yf, hipStream_t s);
+                       const T *xc, T *yf, hipStream_t s, const uint2 *fine_lines = nullptr);
 // yc += R (xf - xf_sub); xf_sub optional.  cell_buf (optional, n_cells nmc^3
 // entries): the per-cell coarse values are stored there instead of being
 // added to yc; the caller sums them with a gather (launch_dg_cg_restrict_gather
@@ -165,7 +167,8 @@ void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
-                     T *yc, hipStream_t s, const T *xf_sub = nullptr, T *cell_buf = nullptr);
+                     T *yc, hipStream_t s, const T *xf_sub = nullptr, T *cell_buf = nullptr,
+                     const uint2 *fine_lines = nullptr);
 
 // ---------------------------------------------------------------- vectors
 template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s);

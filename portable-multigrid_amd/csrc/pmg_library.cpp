@@ -667,6 +667,25 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
       map_f_.upload(mf);
       owner_ = true;
       scale_ = scale;
+      // every fine x-line a run of consecutive DoFs (box numberings): one
+      // word per line {first DoF, owner bits} replaces nmf map entries
+      if (nmf_ <= 32) {
+        const int64_t nl = n_cells * nmf_ * nmf_;
+        std::vector<uint32_t> fl(static_cast<size_t>(2 * nl));
+        bool lines_ok = true;
+        for (int64_t l = 0; l < nl && lines_ok; ++l) {
+          const uint32_t *e = mf.data() + l * nmf_;
+          const uint32_t first = e[0] & 0x7FFFFFFFu;
+          uint32_t bits = 0;
+          for (int a = 0; a < nmf_; ++a) {
+            lines_ok = lines_ok && (e[a] & 0x7FFFFFFFu) == first + static_cast<uint32_t>(a);
+            bits |= (e[a] >> 31) << a;
+          }
+          fl[2 * l] = first;
+          fl[2 * l + 1] = bits;
+        }
+        if (lines_ok) fine_lines_.upload(fl);
+      }
       return;
     }
   }
@@ -701,16 +720,17 @@ void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) c
                                    static_cast<float *>(dst), s);
     return;
   }
+  const uint2 *fl = reinterpret_cast<const uint2 *>(fine_lines_.ptr);
   if (precision_ == PMG_FP64)
     launch_prolongate<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
                               owner_ ? nullptr : static_cast<const double *>(w_prolong_.data()),
                               scale_, static_cast<const double *>(src), static_cast<double *>(dst),
-                              s);
+                              s, fl);
   else
     launch_prolongate<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
                              owner_ ? nullptr : static_cast<const float *>(w_prolong_.data()),
                              static_cast<float>(scale_), static_cast<const float *>(src),
-                             static_cast<float *>(dst), s);
+                             static_cast<float *>(dst), s, fl);
 }
 
 void MGTransfer::restrict_and_add(void *dst, const void *src, hipStream_t s) const {
@@ -746,11 +766,12 @@ void MGTransfer::restrict_sub(void *dst, const void *src, const void *sub, hipSt
     return;
   }
   const bool gather = cinv_off_.ptr != nullptr;
+  const uint2 *fl = reinterpret_cast<const uint2 *>(fine_lines_.ptr);
   if (f64) {
     launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
                             owner_ ? nullptr : D(w_restrict_.data()), D(src),
                             static_cast<double *>(dst), s, D(sub),
-                            gather ? static_cast<double *>(cell_buf_.data()) : nullptr);
+                            gather ? static_cast<double *>(cell_buf_.data()) : nullptr, fl);
     if (gather)
       launch_dg_cg_restrict_gather<double>(n_dofs_c_, cinv_off_.ptr, cinv_list_.ptr,
                                            D(cell_buf_.data()), static_cast<double *>(dst), s);
@@ -758,7 +779,7 @@ void MGTransfer::restrict_sub(void *dst, const void *src, const void *sub, hipSt
     launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
                            owner_ ? nullptr : F(w_restrict_.data()), F(src),
                            static_cast<float *>(dst), s, F(sub),
-                           gather ? static_cast<float *>(cell_buf_.data()) : nullptr);
+                           gather ? static_cast<float *>(cell_buf_.data()) : nullptr, fl);
     if (gather)
       launch_dg_cg_restrict_gather<float>(n_dofs_c_, cinv_off_.ptr, cinv_list_.ptr,
                                           F(cell_buf_.data()), static_cast<float *>(dst), s);

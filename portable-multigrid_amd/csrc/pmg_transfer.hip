@@ -42,11 +42,23 @@ constexpr int xfer_lds_cell() { return NC * NF * NF + NC * NC * NF; }
 // the weighted form with weights 1/multiplicity because the prolongated value
 // of a node shared by several cells is the same from every cell, and a coarse
 // basis function of one cell vanishes on the faces it does not belong to.
-template <int NC, int NF, typename T, bool OWNER>
+// LINES (owner mode): the fine map is given per x-line, fl[cell][c][b] =
+// {first DoF of the line, bit a set if entry (a,b,c) is the owner}, every
+// x-line of every cell being a run of consecutive DoFs: the same owner-mode
+// entries in 8 bytes per NF entries instead of 4 bytes per entry.
+template <int NF>
+__device__ __forceinline__ uint32_t owner_entry(const uint2 *__restrict__ fl, int64_t cell, int c,
+                                                int tb, int ta) {
+  const uint2 w = fl[(cell * NF + c) * NF + tb];
+  return (w.x + ta) | (((w.y >> ta) & 1u) << 31);
+}
+
+template <int NC, int NF, typename T, bool OWNER, bool LINES>
 __global__ void __launch_bounds__(256)
 prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__restrict__ map_c,
-                  const uint32_t *__restrict__ map_f, const T *__restrict__ w, const T scale,
-                  const T *__restrict__ xc, T *__restrict__ yf)
+                  const uint32_t *__restrict__ map_f, const uint2 *__restrict__ fl,
+                  const T *__restrict__ w, const T scale, const T *__restrict__ xc,
+                  T *__restrict__ yf)
 {
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
   constexpr int E = XferTile<NF>::cells, LC = xfer_lds_cell<NC, NF>();
@@ -64,7 +76,8 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
   T yv[NF];
   if (OWNER && active) {
 #pragma unroll
-    for (int c = 0; c < NF; ++c) mf[c] = map_f[base + c * NF2];
+    for (int c = 0; c < NF; ++c)
+      mf[c] = LINES ? owner_entry<NF>(fl, cell, c, tb, ta) : map_f[base + c * NF2];
 #pragma unroll
     for (int c = 0; c < NF; ++c) yv[c] = (mf[c] & 0x80000000u) ? yf[mf[c] & 0x7FFFFFFFu] : T(0);
   }
@@ -127,10 +140,11 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
   }
 }
 
-template <int NC, int NF, typename T, bool OWNER>
+template <int NC, int NF, typename T, bool OWNER, bool LINES>
 __global__ void __launch_bounds__(256)
 restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__restrict__ map_c,
-                const uint32_t *__restrict__ map_f, const T *__restrict__ w, const T scale,
+                const uint32_t *__restrict__ map_f, const uint2 *__restrict__ fl,
+                const T *__restrict__ w, const T scale,
                 const T *__restrict__ xf, const T *__restrict__ xs, T *__restrict__ yc,
                 T *__restrict__ buf)
 {
@@ -155,7 +169,7 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
       if (OWNER) {
-        const uint32_t m = map_f[base + c * NF2];
+        const uint32_t m = LINES ? owner_entry<NF>(fl, cell, c, tb, ta) : map_f[base + c * NF2];
         const uint32_t d = m & 0x7FFFFFFFu;
         f[c] = (m & 0x80000000u) ? (xs ? xf[d] - xs[d] : xf[d]) : T(0);
       } else {
@@ -219,8 +233,8 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 
 template <int NC, int NF, typename T>
 static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
-                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                        const T *x, const T *xs, T *y, T *buf, hipStream_t s) {
+                        const uint32_t *map_c, const uint32_t *map_f, const uint2 *fl, const T *w,
+                        T scale, const T *x, const T *xs, T *y, T *buf, hipStream_t s) {
   if (n_cells <= 0) return;
   using Tile = XferTile<NF>;
   const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
@@ -228,28 +242,35 @@ static void xfer_launch(bool prolong, int64_t n_cells, const Transfer1D<T> &M,
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
   const dim3 g(grid), b(Tile::lanes);
   if (prolong && w)
-    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, false>), g, b, 0, s, n_cells, M, map_c,
-                       map_f, w, scale, x, y);
+    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, false, false>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, fl, w, scale, x, y);
+  else if (prolong && fl)
+    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, true, true>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, fl, w, scale, x, y);
   else if (prolong)
-    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, true>), g, b, 0, s, n_cells, M, map_c,
-                       map_f, w, scale, x, y);
+    hipLaunchKernelGGL((prolongate_kernel<NC, NF, T, true, false>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, fl, w, scale, x, y);
   else if (w)
-    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, false>), g, b, 0, s, n_cells, M, map_c, map_f,
-                       w, scale, x, xs, y, buf);
+    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, false, false>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, fl, w, scale, x, xs, y, buf);
+  else if (fl)
+    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, true, true>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, fl, w, scale, x, xs, y, buf);
   else
-    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, true>), g, b, 0, s, n_cells, M, map_c, map_f,
-                       w, scale, x, xs, y, buf);
+    hipLaunchKernelGGL((restrict_kernel<NC, NF, T, true, false>), g, b, 0, s, n_cells, M, map_c,
+                       map_f, fl, w, scale, x, xs, y, buf);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 static void xfer_dispatch(bool prolong, int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
-                          const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                          const T *x, const T *xs, T *y, T *buf, hipStream_t s) {
-#define PMG_XF(C, F)                                                                      \
-  if (nmc == C && nmf == F) {                                                             \
-    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, w, scale, x, xs, y, buf, s);  \
-    return;                                                                               \
+                          const uint32_t *map_c, const uint32_t *map_f, const uint2 *fl,
+                          const T *w, T scale, const T *x, const T *xs, T *y, T *buf,
+                          hipStream_t s) {
+#define PMG_XF(C, F)                                                                          \
+  if (nmc == C && nmf == F) {                                                                 \
+    xfer_launch<C, F, T>(prolong, n_cells, M, map_c, map_f, fl, w, scale, x, xs, y, buf, s);  \
+    return;                                                                                   \
   }
   // h-transfers (p+1 -> 2p+1)
   PMG_XF(2, 3) PMG_XF(3, 5) PMG_XF(4, 7) PMG_XF(5, 9) PMG_XF(6, 11) PMG_XF(7, 13)
@@ -267,30 +288,33 @@ static void xfer_dispatch(bool prolong, int nmc, int nmf, int64_t n_cells, const
 template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
-                       const T *xc, T *yf, hipStream_t s) {
-  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w, scale, xc, nullptr, yf, nullptr,
-                   s);
+                       const T *xc, T *yf, hipStream_t s, const uint2 *fine_lines) {
+  xfer_dispatch<T>(true, nmc, nmf, n_cells, M, map_c, map_f, w ? nullptr : fine_lines, w, scale,
+                   xc, nullptr, yf, nullptr, s);
 }
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
-                     T *yc, hipStream_t s, const T *xf_sub, T *cell_buf) {
-  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w, T(1), xf, xf_sub, yc, cell_buf,
-                   s);
+                     T *yc, hipStream_t s, const T *xf_sub, T *cell_buf,
+                     const uint2 *fine_lines) {
+  xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w ? nullptr : fine_lines, w, T(1),
+                   xf, xf_sub, yc, cell_buf, s);
 }
 
 template void launch_prolongate<double>(int, int, int64_t, const Transfer1D<double> &,
                                         const uint32_t *, const uint32_t *, const double *,
-                                        double, const double *, double *, hipStream_t);
+                                        double, const double *, double *, hipStream_t,
+                                        const uint2 *);
 template void launch_prolongate<float>(int, int, int64_t, const Transfer1D<float> &,
                                        const uint32_t *, const uint32_t *, const float *, float,
-                                       const float *, float *, hipStream_t);
+                                       const float *, float *, hipStream_t, const uint2 *);
 template void launch_restrict<double>(int, int, int64_t, const Transfer1D<double> &,
                                       const uint32_t *, const uint32_t *, const double *,
                                       const double *, double *, hipStream_t, const double *,
-                                      double *);
+                                      double *, const uint2 *);
 template void launch_restrict<float>(int, int, int64_t, const Transfer1D<float> &,
                                      const uint32_t *, const uint32_t *, const float *,
-                                     const float *, float *, hipStream_t, const float *, float *);
+                                     const float *, float *, hipStream_t, const float *, float *,
+                                     const uint2 *);
 
 }  // namespace pmg
