@@ -97,18 +97,22 @@ template <typename T>
 __global__ void cheby_k(int mode, int64_t n, T *x, T *old, const T *b, const T *t, const T *pinv,
                         T f1, T f2, bool old_zero, T *prep, const uint8_t *mask) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    // b, the inverse diagonal, t and old are streamed (non-temporal): only
+    // x (the next matvec's input) and prep (its output) should stay in the
+    // last-level cache for the matvec that follows
     T xn;
+    const T pi = __builtin_nontemporal_load(pinv + i), bi = __builtin_nontemporal_load(b + i);
     if (mode == 0) {
-      xn = f2 * pinv[i] * b[i];
+      xn = f2 * pi * bi;
     } else if (mode == 1) {
       const T xi = x[i];
-      old[i] = xi;
-      xn = xi + f2 * pinv[i] * (b[i] - t[i]);
+      __builtin_nontemporal_store(xi, old + i);
+      xn = xi + f2 * pi * (bi - __builtin_nontemporal_load(t + i));
     } else {
       const T xi = x[i];
-      const T xo = old_zero ? T(0) : old[i];
-      old[i] = xi;
-      xn = (T(1) + f1) * xi - f1 * xo + f2 * pinv[i] * (b[i] - t[i]);
+      const T xo = old_zero ? T(0) : __builtin_nontemporal_load(old + i);
+      __builtin_nontemporal_store(xi, old + i);
+      xn = (T(1) + f1) * xi - f1 * xo + f2 * pi * (bi - __builtin_nontemporal_load(t + i));
     }
     x[i] = xn;
     if (prep) prep[i] = (mask && mask[i]) ? xn : T(0);
