@@ -191,8 +191,10 @@ template <typename TD, typename TS>
 void launch_convert_unless(int64_t n, TD *dst, const TS *src, const double *skip, hipStream_t s);
 // partial sums of a.b into `partials` (kDotBlocks entries), then final sum into out[0]
 constexpr int kDotBlocks = 4096;
+// stream_a: read a non-temporally (its last use for a while)
 template <typename TA, typename TB = TA>
-void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s);
+void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s,
+                bool stream_a = false);
 // t = b - t   (residual)
 template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStream_t s);
 

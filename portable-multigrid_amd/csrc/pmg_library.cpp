@@ -837,10 +837,10 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
       // (the same values as after a conversion, one pass less)
       const void *zl = mg->run(s);
       if (mg->level_precision() == PMG_FP64) {
-        launch_dot<T, double>(n, r, static_cast<const double *>(zl), w.partials.ptr, sc + 0, s);
+        launch_dot<T, double>(n, r, static_cast<const double *>(zl), w.partials.ptr, sc + 0, s, true);
         launch_cg_update_p<T, double>(n, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
       } else {
-        launch_dot<T, float>(n, r, static_cast<const float *>(zl), w.partials.ptr, sc + 0, s);
+        launch_dot<T, float>(n, r, static_cast<const float *>(zl), w.partials.ptr, sc + 0, s, true);
         launch_cg_update_p<T, float>(n, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s);
       }
     } else {

@@ -71,11 +71,14 @@ __device__ inline double block_sum(double v) {
   return s;  // valid in thread 0
 }
 
+// stream_a: a is read non-temporally (its last use for a while)
 template <typename TA, typename TB>
-__global__ void __launch_bounds__(kThreads) dot_k(int64_t n, const TA *a, const TB *b, double *partials) {
+__global__ void __launch_bounds__(kThreads)
+dot_k(int64_t n, const TA *a, const TB *b, double *partials, bool stream_a) {
   double s = 0;
   for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
-    s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+    s += static_cast<double>(stream_a ? __builtin_nontemporal_load(a + i) : a[i]) *
+         static_cast<double>(b[i]);
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -130,9 +133,12 @@ cg_xr_k(int64_t n, T *x, T *r, const T *p, const T *q, const double *sc, double 
   const double alpha = pq != 0.0 ? sc[kRho] / pq : 0.0;
   const T a = static_cast<T>(alpha);
   double s = 0;
+  // x, p and q are streamed (not needed again before the next iteration's
+  // update); r stays cached for the preconditioner that reads it next
   for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
-    x[i] += a * p[i];
-    const T ri = r[i] - a * q[i];
+    __builtin_nontemporal_store(__builtin_nontemporal_load(x + i) + a * __builtin_nontemporal_load(p + i),
+                                x + i);
+    const T ri = r[i] - a * __builtin_nontemporal_load(q + i);
     r[i] = ri;
     s += static_cast<double>(ri) * static_cast<double>(ri);
   }
@@ -158,7 +164,7 @@ __global__ void cg_p_k(int64_t n, T *p, const TZ *z, const double *sc, int first
   const double ro = sc[kRhoOld];
   const T beta = (first || ro == 0.0) ? T(0) : static_cast<T>(sc[kRho] / ro);
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const T zi = static_cast<T>(z[i]);
+    const T zi = static_cast<T>(__builtin_nontemporal_load(z + i));
     p[i] = first ? zi : zi + beta * p[i];
   }
 }
@@ -178,7 +184,7 @@ template <typename TD, typename TS>
 __global__ void convert_unless_k(int64_t n, TD *d, const TS *s, const double *skip) {
   if (skip && *skip != 0.0) return;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    d[i] = static_cast<TD>(s[i]);
+    d[i] = static_cast<TD>(__builtin_nontemporal_load(s + i));
 }
 }  // namespace
 
@@ -235,8 +241,10 @@ template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStrea
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename TA, typename TB>
-void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s) {
-  hipLaunchKernelGGL((dot_k<TA, TB>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, a, b, partials);
+void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s,
+                bool stream_a) {
+  hipLaunchKernelGGL((dot_k<TA, TB>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, a, b, partials,
+                     stream_a);
   PMG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, kDotBlocks, out, 0, 1);
   PMG_HIP_CHECK(hipGetLastError());
@@ -288,7 +296,7 @@ void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double t
   template void launch_set_indexed<T>(int64_t, const uint32_t *, T *, T, hipStream_t);        \
   template void launch_invert<T>(int64_t, T *, hipStream_t);                                  \
   template void launch_residual<T>(int64_t, T *, const T *, hipStream_t);                     \
-  template void launch_dot<T>(int64_t, const T *, const T *, double *, double *, hipStream_t); \
+  template void launch_dot<T>(int64_t, const T *, const T *, double *, double *, hipStream_t, bool); \
   template void launch_cheby_update<T>(int, int64_t, T *, T *, const T *, const T *,          \
                                        const T *, double, double, bool, hipStream_t, T *,     \
                                        const uint8_t *);                                      \
@@ -301,9 +309,9 @@ PMG_INST(double)
 PMG_INST(float)
 // CG vectors of one precision with a preconditioned vector z of the other
 template void launch_dot<double, float>(int64_t, const double *, const float *, double *, double *,
-                                        hipStream_t);
+                                        hipStream_t, bool);
 template void launch_dot<float, double>(int64_t, const float *, const double *, double *, double *,
-                                        hipStream_t);
+                                        hipStream_t, bool);
 template void launch_cg_update_p<double, float>(int64_t, double *, const float *, const double *,
                                                 int, hipStream_t);
 template void launch_cg_update_p<float, double>(int64_t, float *, const double *, const double *,
