@@ -216,9 +216,21 @@ void launch_cg_update_xr(int64_t n, T *x, T *r, const T *p, const T *q, const do
 template <typename T>
 void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *partials,
                            hipStream_t s);
-// p = z (first) or z + beta p; z may be of the other precision
+// p = z (first) or z + beta p
 template <typename T, typename TZ = T>
 void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int first, hipStream_t s);
+// CG with the x update deferred by one iteration (same arithmetic): r -= alpha q
+// with r.r partials; x += alpha_prev p_prev (not if first) and p = z + beta p;
+// the last x += alpha p after the loop.  alpha is kept in sc[4] by
+// launch_cg_scalars(op 0).
+template <typename T>
+void launch_cg_update_r(int64_t n, T *r, const T *q, const double *sc, double *partials,
+                        hipStream_t s);
+template <typename T, typename TZ = T>
+void launch_cg_update_xp(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first,
+                         hipStream_t s);
+template <typename T>
+void launch_cg_update_x(int64_t n, T *x, const T *p, const double *sc, hipStream_t s);
 // final reduction of kDotBlocks partial sums into sc[slot] (optionally sqrt)
 void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
                             int n_sums, hipStream_t s);

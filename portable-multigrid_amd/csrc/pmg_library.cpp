@@ -838,19 +838,19 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
       const void *zl = mg->run(s);
       if (mg->level_precision() == PMG_FP64) {
         launch_dot<T, double>(n, r, static_cast<const double *>(zl), w.partials.ptr, sc + 0, s, true);
-        launch_cg_update_p<T, double>(n, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
+        launch_cg_update_xp<T, double>(n, x, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
       } else {
         launch_dot<T, float>(n, r, static_cast<const float *>(zl), w.partials.ptr, sc + 0, s, true);
-        launch_cg_update_p<T, float>(n, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s);
+        launch_cg_update_xp<T, float>(n, x, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s);
       }
     } else {
       launch_cg_precond_dot<T>(n, r, pinv, z, w.partials.ptr, s);
       launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 0, 1, s);
-      launch_cg_update_p<T>(n, p, z, sc, it == 0 ? 1 : 0, s);
+      launch_cg_update_xp<T>(n, x, p, z, sc, it == 0 ? 1 : 0, s);
     }
     op.vmult(q, p, s);
     launch_dot<T>(n, p, q, w.partials.ptr, sc + 1, s);
-    launch_cg_update_xr<T>(n, x, r, p, q, sc, w.partials.ptr, s);
+    launch_cg_update_r<T>(n, r, q, sc, w.partials.ptr, s);
     launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 2, 1, s);
     launch_cg_scalars(sc, 0, s, w.host.ptr, target2);
     ++it;
@@ -869,6 +869,7 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
       }
     }
   }
+  if (it > 0) launch_cg_update_x<T>(n, x, p, sc, s);  // the deferred last x update
   res.iterations = it;
   return res;
 }

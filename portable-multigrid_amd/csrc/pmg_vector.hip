@@ -169,12 +169,58 @@ __global__ void cg_p_k(int64_t n, T *p, const TZ *z, const double *sc, int first
   }
 }
 
-// end of an iteration: rho_old = rho; host[0] = r.r (optional, pinned);
+// The CG update with x deferred: iteration k updates r with alpha_k, and the
+// p update of iteration k+1, which reads p_k anyway, also applies
+// x += alpha_k p_k (cg_xp_k); a last cg_x_k after the loop.  The same
+// arithmetic as updating x and r together, one pass over p less.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+cg_r_k(int64_t n, T *r, const T *q, const double *sc, double *partials) {
+  const double pq = sc[kPq];
+  const T a = static_cast<T>(pq != 0.0 ? sc[kRho] / pq : 0.0);
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const T ri = r[i] - a * __builtin_nontemporal_load(q + i);
+    r[i] = ri;
+    s += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+template <typename T, typename TZ>
+__global__ void cg_xp_k(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first) {
+  const double ro = sc[kRhoOld];
+  const T beta = (first || ro == 0.0) ? T(0) : static_cast<T>(sc[kRho] / ro);
+  const T a = first ? T(0) : static_cast<T>(sc[kAlpha]);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const T zi = static_cast<T>(__builtin_nontemporal_load(z + i));
+    if (first) {
+      p[i] = zi;
+    } else {
+      const T pi = p[i];
+      __builtin_nontemporal_store(__builtin_nontemporal_load(x + i) + a * pi, x + i);
+      p[i] = zi + beta * pi;
+    }
+  }
+}
+template <typename T>
+__global__ void cg_x_k(int64_t n, T *x, const T *p, const double *sc) {
+  const T a = static_cast<T>(sc[kAlpha]);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    x[i] += a * p[i];
+}
+
+// end of an iteration: alpha = rho / p.q kept for the deferred x update,
+// rho_old = rho; host[0] = r.r (optional, pinned);
 // sc[kConverged] = 1 once r.r <= target2 (or is NaN), which skips the
 // speculatively queued input pass of the next iteration
 __global__ void cg_scalars_k(double *sc, int op, double *host, double target2) {
   if (threadIdx.x == 0) {
-    if (op == 0) sc[kRhoOld] = sc[kRho];
+    if (op == 0) {
+      const double pq = sc[kPq];
+      sc[kAlpha] = pq != 0.0 ? sc[kRho] / pq : 0.0;
+      sc[kRhoOld] = sc[kRho];
+    }
     const double rr = sc[kRr];
     sc[kConverged] = !(rr > target2) ? 1.0 : 0.0;
     if (host) host[0] = rr;
@@ -283,6 +329,26 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
   hipLaunchKernelGGL((cg_p_k<T, TZ>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, p, z, sc, first);
   PMG_HIP_CHECK(hipGetLastError());
 }
+template <typename T>
+void launch_cg_update_r(int64_t n, T *r, const T *q, const double *sc, double *partials,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(cg_r_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, q, sc, partials);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T, typename TZ>
+void launch_cg_update_xp(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((cg_xp_k<T, TZ>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, p, z, sc,
+                     first);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cg_update_x(int64_t n, T *x, const T *p, const double *sc, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cg_x_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, p, sc);
+  PMG_HIP_CHECK(hipGetLastError());
+}
 void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double target2) {
   hipLaunchKernelGGL(cg_scalars_k, dim3(1), dim3(64), 0, s, sc, op, host, target2);
   PMG_HIP_CHECK(hipGetLastError());
@@ -312,10 +378,23 @@ template void launch_dot<double, float>(int64_t, const double *, const float *, 
                                         hipStream_t, bool);
 template void launch_dot<float, double>(int64_t, const float *, const double *, double *, double *,
                                         hipStream_t, bool);
-template void launch_cg_update_p<double, float>(int64_t, double *, const float *, const double *,
-                                                int, hipStream_t);
-template void launch_cg_update_p<float, double>(int64_t, float *, const double *, const double *,
-                                                int, hipStream_t);
+#define PMG_CGX(T, TZ)                                                                             \
+  template void launch_cg_update_xp<T, TZ>(int64_t, T *, T *, const TZ *, const double *, int,    \
+                                           hipStream_t);
+PMG_CGX(double, double)
+PMG_CGX(double, float)
+PMG_CGX(float, float)
+PMG_CGX(float, double)
+#undef PMG_CGX
+template void launch_cg_update_r<double>(int64_t, double *, const double *, const double *,
+                                         double *, hipStream_t);
+template void launch_cg_update_r<float>(int64_t, float *, const float *, const double *, double *,
+                                        hipStream_t);
+template void launch_cg_update_x<double>(int64_t, double *, const double *, const double *,
+                                         hipStream_t);
+template void launch_cg_update_x<float>(int64_t, float *, const float *, const double *,
+                                        hipStream_t);
+
 #undef PMG_INST
 template void launch_convert<double, float>(int64_t, double *, const float *, hipStream_t);
 template void launch_convert<float, double>(int64_t, float *, const double *, hipStream_t);
