@@ -636,48 +636,79 @@ template void launch_geometry<float>(int, int64_t, const double *, const double 
                                      hipStream_t);
 
 // ---------------------------------------------------------------- diagonal
-// One lane per (cell, local DoF i): (A_e)_ii = sum_q g_i(q)^T G_q g_i(q) with
-// g_i(q) = (phi'phi phi, phi phi' phi, phi phi phi') from the exact 1D
-// derivatives (sh.D holds Sd here).  Setup only (compute_diagonal,
+// (A_e)_ii = sum_q g_i(q)^T G_q g_i(q), g_i(q) = (phi'phi phi, phi phi' phi,
+// phi phi phi') from the exact 1D derivatives (sh.D holds Sd here).  Each of
+// the six G components contributes a separable sum
+//   sum_{q0,q1,q2} G_c(q) X(i0,q0) Y(i1,q1) Z(i2,q2),  X,Y,Z in {S^2, D^2, DS},
+// evaluated by sum factorisation (three sweeps of nm nq^3, nm^2 nq^2, nm^3 nq
+// products) instead of a quadrature loop per DoF.  One wave per cell, double
+// accumulation.  Setup only (compute_diagonal,
 // portable_laplace_operator_bk3.h:533-561).
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const T *__restrict__ G,
                     const uint32_t *__restrict__ map, const Shape1D<T> sh, T *__restrict__ diag)
 {
-  const int nm3 = nm * nm * nm, nq3 = nq * nq * nq;
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= n_cells * nm3) return;
-  const int64_t c = gid / nm3;
-  const int i = static_cast<int>(gid - c * nm3);
-  const uint32_t dof = map[gid];
-  if (dof == kInvalid) return;
-  const int i0 = i % nm, i1 = (i / nm) % nm, i2 = i / (nm * nm);
-  (void)nq3;
-  double acc = 0;
-  for (int q2 = 0; q2 < nq; ++q2)
-    for (int q1 = 0; q1 < nq; ++q1)
-      for (int q0 = 0; q0 < nq; ++q0) {
-        const int q = (q2 * nq + q1) * nq + q0;
-        const double s0 = sh.S[i0 * nq + q0], s1 = sh.S[i1 * nq + q1], s2 = sh.S[i2 * nq + q2];
-        const double d0 = sh.D[i0 * nq + q0], d1 = sh.D[i1 * nq + q1], d2 = sh.D[i2 * nq + q2];
-        const double g0 = d0 * s1 * s2, g1 = s0 * d1 * s2, g2 = s0 * s1 * d2;
-        const T *Gq = G + g_index(nq, c, 0, q);
-        const int64_t cs = static_cast<int64_t>(nq) * g_batch_lanes(nq);  // component stride
-        acc += Gq[0] * g0 * g0 + Gq[3 * cs] * g1 * g1 + Gq[5 * cs] * g2 * g2 +
-               2.0 * (Gq[cs] * g0 * g1 + Gq[2 * cs] * g0 * g2 + Gq[4 * cs] * g1 * g2);
-      }
-  atomic_add(diag + dof, static_cast<T>(acc));
+  __shared__ double f[3][kMaxNm * kMaxNq];  // S^2, D^2, DS
+  __shared__ double g[kMaxNq * kMaxNq * kMaxNq];
+  __shared__ double t1[kMaxNm * kMaxNq * kMaxNq];
+  __shared__ double t2[kMaxNm * kMaxNm * kMaxNq];
+  __shared__ double acc[kMaxNm * kMaxNm * kMaxNm];
+  const int64_t cell = blockIdx.x;
+  if (cell >= n_cells) return;  // whole workgroup
+  const int lane = threadIdx.x;
+  const int nm2 = nm * nm, nm3 = nm2 * nm, nq2 = nq * nq, nq3 = nq2 * nq;
+  for (int o = lane; o < nm * nq; o += 64) {
+    const double sv = sh.S[o], dv = sh.D[o];
+    f[0][o] = sv * sv;
+    f[1][o] = dv * dv;
+    f[2][o] = dv * sv;
+  }
+  for (int o = lane; o < nm3; o += 64) acc[o] = 0.0;
+  // component c = xx, xy, xz, yy, yz, zz: factors along x, y, z and weight
+  constexpr int fx[6] = {1, 2, 2, 0, 0, 0}, fy[6] = {0, 2, 0, 1, 2, 0}, fz[6] = {0, 0, 2, 0, 2, 1};
+  constexpr double wc[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};
+  for (int c = 0; c < 6; ++c) {
+    __syncthreads();
+    for (int o = lane; o < nq3; o += 64) g[o] = static_cast<double>(G[g_index(nq, cell, c, o)]);
+    __syncthreads();
+    const double *X = f[fx[c]], *Y = f[fy[c]], *Z = f[fz[c]];
+    for (int o = lane; o < nm * nq2; o += 64) {  // t1[(q2 nq + q1) nm + a] = sum_q0 X(a,q0) g
+      const int a = o % nm, r = o / nm;
+      double v = 0.0;
+      for (int q0 = 0; q0 < nq; ++q0) v += X[a * nq + q0] * g[r * nq + q0];
+      t1[o] = v;
+    }
+    __syncthreads();
+    for (int o = lane; o < nm2 * nq; o += 64) {  // t2[(q2 nm + b) nm + a] = sum_q1 Y(b,q1) t1
+      const int a = o % nm, b = (o / nm) % nm, q2 = o / nm2;
+      double v = 0.0;
+      for (int q1 = 0; q1 < nq; ++q1) v += Y[b * nq + q1] * t1[(q2 * nq + q1) * nm + a];
+      t2[o] = v;
+    }
+    __syncthreads();
+    for (int o = lane; o < nm3; o += 64) {  // acc[(k nm + b) nm + a] += w sum_q2 Z(k,q2) t2
+      const int a = o % nm, b = (o / nm) % nm, k = o / nm2;
+      double v = 0.0;
+      for (int q2 = 0; q2 < nq; ++q2) v += Z[k * nq + q2] * t2[(q2 * nm + b) * nm + a];
+      acc[o] += wc[c] * v;
+    }
+  }
+  for (int o = lane; o < nm3; o += 64) {  // the same lane wrote acc[o]
+    const uint32_t dof = map[cell * nm3 + o];
+    if (dof != kInvalid) atomic_add(diag + dof, static_cast<T>(acc[o]));
+  }
 }
 
 template <typename T>
 void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
                          const Shape1D<T> &S_and_Sd, T *diag, hipStream_t s) {
   const int nm = degree + 1;
-  const int64_t n = n_cells * nm * nm * nm;
-  if (n == 0) return;
-  hipLaunchKernelGGL(bk3_diagonal_kernel<T>, dim3(static_cast<unsigned>((n + 255) / 256)),
-                     dim3(256), 0, s, nm, nq, n_cells, G, map, S_and_Sd, diag);
+  if (n_cells <= 0) return;
+  PMG_REQUIRE(nm <= kMaxNm && nq <= kMaxNq, PMG_ERR_UNSUPPORTED, "diagonal: degree or nq too large");
+  PMG_REQUIRE(n_cells < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
+  hipLaunchKernelGGL(bk3_diagonal_kernel<T>, dim3(static_cast<unsigned>(n_cells)), dim3(64), 0, s,
+                     nm, nq, n_cells, G, map, S_and_Sd, diag);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_bk3_diagonal<double>(int, int, int64_t, const double *, const uint32_t *,

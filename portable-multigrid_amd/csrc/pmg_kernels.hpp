@@ -184,6 +184,8 @@ template <typename T>
 void launch_set_indexed(int64_t n, const uint32_t *idx, T *dst, T v, hipStream_t s);
 // x[i] = 1/x[i]
 template <typename T> void launch_invert(int64_t n, T *x, hipStream_t s);
+// smallest i with !(d[i] > 0), or -1 (synchronous)
+template <typename T> int64_t first_nonpositive(int64_t n, const T *d, hipStream_t s);
 template <typename TD, typename TS>
 void launch_convert(int64_t n, TD *dst, const TS *src, hipStream_t s);
 // the same, unless *skip != 0 on the device (skip may be null)

@@ -404,11 +404,9 @@ void LaplaceOperatorBK3::compute_diagonal_t() {
                          d, nullptr);
   launch_set_indexed<T>(n_constrained_, constrained_.ptr, d, T(1), nullptr);
   // DoFs touched by no cell (possible only with exotic maps) would divide by 0
-  std::vector<T> h(static_cast<size_t>(n_dofs_));
-  PMG_HIP_CHECK(hipMemcpy(h.data(), d, sizeof(T) * n_dofs_, hipMemcpyDeviceToHost));
-  for (int64_t i = 0; i < n_dofs_; ++i)
-    PMG_REQUIRE(h[i] > T(0), PMG_ERR_INVALID_ARG,
-                "non-positive diagonal entry at DoF " + std::to_string(i));
+  const int64_t bad = first_nonpositive<T>(n_dofs_, d, nullptr);
+  PMG_REQUIRE(bad < 0, PMG_ERR_INVALID_ARG,
+              "non-positive diagonal entry at DoF " + std::to_string(bad));
   launch_invert<T>(n_dofs_, d, nullptr);
   PMG_HIP_CHECK(hipDeviceSynchronize());
 }

@@ -268,11 +268,9 @@ void LaplaceOperatorDG::compute_diagonal() {
     launch_fill<T>(n, d, T(0), nullptr);
     launch_bk3_diagonal<T>(degree_, nq_, n_cells_, static_cast<const T *>(G_.data()), map, shd, d,
                            nullptr);
-    std::vector<T> h(static_cast<size_t>(n));
-    PMG_HIP_CHECK(hipMemcpy(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i)
-      PMG_REQUIRE(h[i] > T(0), PMG_ERR_INVALID_ARG,
-                  "non-positive diagonal entry at DoF " + std::to_string(i));
+    const int64_t bad = first_nonpositive<T>(n, d, nullptr);
+    PMG_REQUIRE(bad < 0, PMG_ERR_INVALID_ARG,
+                "non-positive diagonal entry at DoF " + std::to_string(bad));
     launch_invert<T>(n, d, nullptr);
   };
   if (precision_ == PMG_FP64) run(0.0);

@@ -44,6 +44,12 @@ __global__ void set_idx_k(int64_t n, const uint32_t *idx, T *d, T v) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     d[idx[i]] = v;
 }
+// first[0] = min { i : !(d[i] > 0) } (first[0] = n beforehand)
+template <typename T>
+__global__ void first_nonpositive_k(int64_t n, const T *d, unsigned long long *first) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    if (!(d[i] > T(0))) atomicMin(first, static_cast<unsigned long long>(i));
+}
 template <typename T> __global__ void invert_k(int64_t n, T *x) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     x[i] = T(1) / x[i];
@@ -268,6 +274,22 @@ template <typename T> void launch_invert(int64_t n, T *x, hipStream_t s) {
   hipLaunchKernelGGL(invert_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x);
   PMG_HIP_CHECK(hipGetLastError());
 }
+template <typename T>
+int64_t first_nonpositive(int64_t n, const T *d, hipStream_t s) {
+  if (n <= 0) return -1;
+  DeviceArray<unsigned long long> first(1);
+  const unsigned long long init = static_cast<unsigned long long>(n);
+  PMG_HIP_CHECK(hipMemcpyAsync(first.ptr, &init, sizeof(init), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(first_nonpositive_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, d, first.ptr);
+  PMG_HIP_CHECK(hipGetLastError());
+  unsigned long long r = 0;
+  PMG_HIP_CHECK(hipMemcpyAsync(&r, first.ptr, sizeof(r), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  return r < static_cast<unsigned long long>(n) ? static_cast<int64_t>(r) : -1;
+}
+template int64_t first_nonpositive<double>(int64_t, const double *, hipStream_t);
+template int64_t first_nonpositive<float>(int64_t, const float *, hipStream_t);
+
 template <typename TD, typename TS>
 void launch_convert(int64_t n, TD *d, const TS *src, hipStream_t s) {
   if (n <= 0) return;
