@@ -54,6 +54,118 @@ __device__ __forceinline__ void bk3_load_g(const T *Gc, T (&g)[6][NQ]) {
 // (ta,tb)), in two halves that all lanes of the workgroup must call (they
 // hold barriers).  P0..P2: three NQ^3 LDS buffers of this cell.
 //
+// ---- even-odd 1D contractions (tables: Shape1D, fill_even_odd)
+// out[r] = sum_k S[k][r] x[k]   (values at the Gauss points)
+template <int NM, int NQ, typename T>
+__device__ __forceinline__ void eo_interp(const Shape1D<T> &sh, const T (&x)[NM], T (&out)[NQ]) {
+  constexpr int HM = NM / 2, HQ = NQ / 2, HM1 = HM > 0 ? HM : 1;
+  T e[HM1], o[HM1];
+#pragma unroll
+  for (int k = 0; k < HM; ++k) {
+    e[k] = x[k] + x[NM - 1 - k];
+    o[k] = x[k] - x[NM - 1 - k];
+  }
+#pragma unroll
+  for (int r = 0; r < (NQ + 1) / 2; ++r) {
+    T a = (NM % 2) ? sh.SM[r] * x[HM] : T(0);
+#pragma unroll
+    for (int k = 0; k < HM; ++k) a += sh.SE[k * kEO + r] * e[k];
+    if (r < HQ) {
+      T b = 0;
+#pragma unroll
+      for (int k = 0; k < HM; ++k) b += sh.SO[k * kEO + r] * o[k];
+      out[r] = a + b;
+      out[NQ - 1 - r] = a - b;
+    } else {
+      out[r] = a;  // middle point (odd nq)
+    }
+  }
+}
+// out[k] = sum_r S[k][r] x[r]   (test with the basis)
+template <int NM, int NQ, typename T>
+__device__ __forceinline__ void eo_interp_t(const Shape1D<T> &sh, const T (&x)[NQ], T (&out)[NM]) {
+  constexpr int HM = NM / 2, HQ = NQ / 2, HQ1 = HQ > 0 ? HQ : 1;
+  T e[HQ1], o[HQ1];
+#pragma unroll
+  for (int r = 0; r < HQ; ++r) {
+    e[r] = x[r] + x[NQ - 1 - r];
+    o[r] = x[r] - x[NQ - 1 - r];
+  }
+#pragma unroll
+  for (int k = 0; k < HM; ++k) {
+    T a = (NQ % 2) ? sh.SE[k * kEO + HQ] * x[HQ] : T(0), b = 0;
+#pragma unroll
+    for (int r = 0; r < HQ; ++r) {
+      a += sh.SE[k * kEO + r] * e[r];
+      b += sh.SO[k * kEO + r] * o[r];
+    }
+    out[k] = a + b;
+    out[NM - 1 - k] = a - b;
+  }
+  if (NM % 2) {
+    T a = (NQ % 2) ? sh.SM[HQ] * x[HQ] : T(0);
+#pragma unroll
+    for (int r = 0; r < HQ; ++r) a += sh.SM[r] * e[r];
+    out[HM] = a;
+  }
+}
+// out[p] = sum_n D[n][p] x[n]   (derivative at the Gauss points)
+template <int NQ, typename T>
+__device__ __forceinline__ void eo_deriv(const Shape1D<T> &sh, const T (&x)[NQ], T (&out)[NQ]) {
+  constexpr int HQ = NQ / 2, HQ1 = HQ > 0 ? HQ : 1;
+  T e[HQ1], o[HQ1];
+#pragma unroll
+  for (int n = 0; n < HQ; ++n) {
+    e[n] = x[n] + x[NQ - 1 - n];
+    o[n] = x[n] - x[NQ - 1 - n];
+  }
+#pragma unroll
+  for (int p = 0; p < HQ; ++p) {
+    T a = (NQ % 2) ? sh.FM[p] * x[HQ] : T(0), b = 0;
+#pragma unroll
+    for (int n = 0; n < HQ; ++n) {
+      a += sh.FE[n * kEO + p] * e[n];
+      b += sh.FO[n * kEO + p] * o[n];
+    }
+    out[p] = a + b;
+    out[NQ - 1 - p] = b - a;
+  }
+  if (NQ % 2) {
+    T b = 0;
+#pragma unroll
+    for (int n = 0; n < HQ; ++n) b += sh.FO[n * kEO + HQ] * o[n];
+    out[HQ] = b;
+  }
+}
+// out[r] = sum_n D[r][n] x[n]   (test with the derivative)
+template <int NQ, typename T>
+__device__ __forceinline__ void eo_deriv_t(const Shape1D<T> &sh, const T (&x)[NQ], T (&out)[NQ]) {
+  constexpr int HQ = NQ / 2, HQ1 = HQ > 0 ? HQ : 1;
+  T e[HQ1], o[HQ1];
+#pragma unroll
+  for (int n = 0; n < HQ; ++n) {
+    e[n] = x[n] + x[NQ - 1 - n];
+    o[n] = x[n] - x[NQ - 1 - n];
+  }
+#pragma unroll
+  for (int r = 0; r < HQ; ++r) {
+    T a = (NQ % 2) ? sh.TC[r] * x[HQ] : T(0), b = 0;
+#pragma unroll
+    for (int n = 0; n < HQ; ++n) {
+      a += sh.TE[r * kEO + n] * e[n];
+      b += sh.TO[r * kEO + n] * o[n];
+    }
+    out[r] = a + b;
+    out[NQ - 1 - r] = b - a;
+  }
+  if (NQ % 2) {
+    T b = 0;
+#pragma unroll
+    for (int n = 0; n < HQ; ++n) b += sh.TM[n] * o[n];
+    out[HQ] = b;
+  }
+}
+
 // bk3_forward: u[k] is the source column (i,j) = (ta,tb) over k (read only
 // where ta,tb < NM).  Leaves the values at the Gauss points in
 // P0[r][q][p] (p along x) and d/dxi_0 of them in P2.  zbuf[0..zn) (LDS,
@@ -66,13 +178,10 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
   // ---- phase 1: interpolate the column in z
   if (active && ta < NM && tb < NM) {
+    T w[NQ];
+    eo_interp<NM, NQ, T>(sh, u, w);
 #pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      T v = 0;
-#pragma unroll
-      for (int k = 0; k < NM; ++k) v += sh.S[k * NQ + r] * u[k];
-      P0[r * NM2 + tb * NM + ta] = v;  // A[r][j][i]
-    }
+    for (int r = 0; r < NQ; ++r) P0[r * NM2 + tb * NM + ta] = w[r];  // A[r][j][i]
   }
   __syncthreads();
 
@@ -84,35 +193,24 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
     T a[NM];
 #pragma unroll
     for (int j = 0; j < NM; ++j) a[j] = P0[tb * NM2 + j * NM + ta];
+    T w[NQ];
+    eo_interp<NM, NQ, T>(sh, a, w);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      T v = 0;
-#pragma unroll
-      for (int j = 0; j < NM; ++j) v += sh.S[j * NQ + q] * a[j];
-      P1[(tb * NQ + q) * NM + ta] = v;
-    }
+    for (int q = 0; q < NQ; ++q) P1[(tb * NQ + q) * NM + ta] = w[q];
   }
   __syncthreads();
 
   // ---- phase 3: x-interpolation + x-gradient, lane (q,r) = (ta,tb)
   if (active) {
-    T b[NM], U[NQ];
+    T b[NM], U[NQ], dU[NQ];
 #pragma unroll
     for (int i = 0; i < NM; ++i) b[i] = P1[(tb * NQ + ta) * NM + i];
+    eo_interp<NM, NQ, T>(sh, b, U);
+    eo_deriv<NQ, T>(sh, U, dU);
 #pragma unroll
     for (int p = 0; p < NQ; ++p) {
-      T v = 0;
-#pragma unroll
-      for (int i = 0; i < NM; ++i) v += sh.S[i * NQ + p] * b[i];
-      U[p] = v;
-      P0[tb * NQ2 + ta * NQ + p] = v;  // U[r][q][p]
-    }
-#pragma unroll
-    for (int p = 0; p < NQ; ++p) {
-      T v = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + p] * U[n];
-      P2[tb * NQ2 + ta * NQ + p] = v;  // du/dxi_0
+      P0[tb * NQ2 + ta * NQ + p] = U[p];   // U[r][q][p]
+      P2[tb * NQ2 + ta * NQ + p] = dU[p];  // du/dxi_0
     }
   }
   __syncthreads();
@@ -131,96 +229,74 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
   // ---- phase 4: y-gradient, lane (p,r) = (ta,tb)
   if (active) {
-    T c[NQ];
+    T c[NQ], w[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; ++n) c[n] = P0[tb * NQ2 + n * NQ + ta];
+    eo_deriv<NQ, T>(sh, c, w);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      T v = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + q] * c[n];
-      P1[tb * NQ2 + q * NQ + ta] = v;  // du/dxi_1
-    }
+    for (int q = 0; q < NQ; ++q) P1[tb * NQ2 + q * NQ + ta] = w[q];  // du/dxi_1
   }
   __syncthreads();
 
   // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
   if (active) {
     if (GMODE == kGLateNT) bk3_load_g<NQ, T, GMODE>(Gc, g);
-    T col[NQ], fz[NQ];
+    T col[NQ], uzs[NQ], fz[NQ], w[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; ++n) col[n] = P0[n * NQ2 + tb * NQ + ta];
+    eo_deriv<NQ, T>(sh, col, uzs);
 #pragma unroll
     for (int r = 0; r < NQ; ++r) {
-      T uz = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) uz += sh.D[n * NQ + r] * col[n];
+      const T uz = uzs[r];
       const int id = r * NQ2 + tb * NQ + ta;
       const T ux = P2[id], uy = P1[id];
       P2[id] = g[0][r] * ux + g[1][r] * uy + g[2][r] * uz;
       P1[id] = g[1][r] * ux + g[3][r] * uy + g[4][r] * uz;
       fz[r] = g[2][r] * ux + g[4][r] * uy + g[5][r] * uz;
     }
+    eo_deriv_t<NQ, T>(sh, fz, w);
 #pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      T v = 0;
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[r * NQ + n] * fz[n];
-      P0[r * NQ2 + tb * NQ + ta] = v;
-    }
+    for (int r = 0; r < NQ; ++r) P0[r * NQ2 + tb * NQ + ta] = w[r];
   }
   __syncthreads();
 
   // ---- phase 6: x-integration accumulated onto the z part, lane (q,r)
   if (active) {
-    T f[NQ];
+    T f[NQ], w[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; ++n) f[n] = P2[tb * NQ2 + ta * NQ + n];
+    eo_deriv_t<NQ, T>(sh, f, w);
 #pragma unroll
-    for (int p = 0; p < NQ; ++p) {
-      T v = P0[tb * NQ2 + ta * NQ + p];
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[p * NQ + n] * f[n];
-      P0[tb * NQ2 + ta * NQ + p] = v;
-    }
+    for (int p = 0; p < NQ; ++p) P0[tb * NQ2 + ta * NQ + p] += w[p];
   }
   __syncthreads();
 
   // ---- phase 7: y-integration + y back-interpolation, lane (p,r)
   if (active) {
-    T f[NQ], acc[NQ];
+    T f[NQ], acc[NQ], w[NM];
 #pragma unroll
     for (int n = 0; n < NQ; ++n) f[n] = P1[tb * NQ2 + n * NQ + ta];
+    eo_deriv_t<NQ, T>(sh, f, acc);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       T v = P0[tb * NQ2 + q * NQ + ta];
       if (HASV) v += V[tb * NQ2 + q * NQ + ta];
-#pragma unroll
-      for (int n = 0; n < NQ; ++n) v += sh.D[q * NQ + n] * f[n];
-      acc[q] = v;
+      acc[q] += v;
     }
+    eo_interp_t<NM, NQ, T>(sh, acc, w);
 #pragma unroll
-    for (int j = 0; j < NM; ++j) {
-      T v = 0;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) v += sh.S[j * NQ + q] * acc[q];
-      P1[tb * NQ2 + j * NQ + ta] = v;  // W[r][j][p]
-    }
+    for (int j = 0; j < NM; ++j) P1[tb * NQ2 + j * NQ + ta] = w[j];  // W[r][j][p]
   }
   __syncthreads();
 
   // ---- phase 8: x back-interpolation, lane (j,r) = (ta,tb)
   if (active && ta < NM) {
-    T l[NQ];
+    T l[NQ], w[NM];
 #pragma unroll
     for (int p = 0; p < NQ; ++p) l[p] = P1[tb * NQ2 + ta * NQ + p];
+    eo_interp_t<NM, NQ, T>(sh, l, w);
 #pragma unroll
-    for (int i = 0; i < NM; ++i) {
-      T v = 0;
-#pragma unroll
-      for (int p = 0; p < NQ; ++p) v += sh.S[i * NQ + p] * l[p];
-      P2[tb * NM2 + ta * NM + i] = v;  // Z[r][j][i]
-    }
+    for (int i = 0; i < NM; ++i) P2[tb * NM2 + ta * NM + i] = w[i];  // Z[r][j][i]
   }
   __syncthreads();
 
@@ -229,13 +305,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
     T z[NQ];
 #pragma unroll
     for (int r = 0; r < NQ; ++r) z[r] = P2[r * NM2 + tb * NM + ta];
-#pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      T v = 0;
-#pragma unroll
-      for (int r = 0; r < NQ; ++r) v += sh.S[k * NQ + r] * z[r];
-      out[k] = v;
-    }
+    eo_interp_t<NM, NQ, T>(sh, z, out);
   }
 }
 

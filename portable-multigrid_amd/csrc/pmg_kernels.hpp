@@ -15,11 +15,52 @@ namespace pmg {
 constexpr int kMaxNm = 8;
 constexpr int kMaxNq = 8;
 
+// Even-odd tables (stride kEO): the Gauss-Lobatto basis at the Gauss points is
+// centro-symmetric, S[k][r] = S[m-k][n-r], and the collocation derivative
+// centro-antisymmetric, D[n][p] = -D[N-n][N-p], so each 1D contraction splits
+// into half-size even and odd parts (half the products, half the
+// coefficients held in scalar registers).  m = nm-1, N = nq-1, hm = nm/2,
+// hq = nq/2; see fill_even_odd.
+constexpr int kEO = (kMaxNq + 1) / 2;
 template <typename T>
 struct Shape1D {
   T S[kMaxNm * kMaxNq];  // S[i*nq+q]
   T D[kMaxNq * kMaxNq];  // D[n*nq+q]
+  T SE[kEO * kEO], SO[kEO * kEO], SM[kEO];  // (S[k][r] +- S[m-k][r]) / 2, S[hm][r]
+  T FE[kEO * kEO], FO[kEO * kEO], FM[kEO];  // (D[n][p] +- D[N-n][p]) / 2, D[hq][p]
+  T TE[kEO * kEO], TO[kEO * kEO], TC[kEO], TM[kEO];  // (D[r][n] -+ D[N-r][n]) / 2, D[r][hq], D[hq][n]
 };
+
+// Fills the even-odd tables of sh from S [nm][nq] and D [nq][nq] (row-major).
+template <typename T>
+inline void fill_even_odd(Shape1D<T> &sh, const double *S, const double *D, int nm, int nq) {
+  const int m = nm - 1, N = nq - 1, hm = nm / 2, hq = nq / 2, cq = (nq + 1) / 2;
+  auto s = [&](int k, int r) { return S[k * nq + r]; };
+  auto d = [&](int n, int p) { return D[n * nq + p]; };
+  for (int k = 0; k < hm; ++k)
+    for (int r = 0; r < cq; ++r) {
+      sh.SE[k * kEO + r] = static_cast<T>(0.5 * (s(k, r) + s(m - k, r)));
+      if (r < hq) sh.SO[k * kEO + r] = static_cast<T>(0.5 * (s(k, r) - s(m - k, r)));
+    }
+  if (nm % 2)
+    for (int r = 0; r < cq; ++r) sh.SM[r] = static_cast<T>(s(hm, r));
+  for (int n = 0; n < hq; ++n)
+    for (int p = 0; p < cq; ++p) {
+      if (p < hq) sh.FE[n * kEO + p] = static_cast<T>(0.5 * (d(n, p) + d(N - n, p)));
+      sh.FO[n * kEO + p] = static_cast<T>(0.5 * (d(n, p) - d(N - n, p)));
+    }
+  for (int r = 0; r < hq; ++r) {
+    for (int n = 0; n < hq; ++n) {
+      sh.TE[r * kEO + n] = static_cast<T>(0.5 * (d(r, n) - d(N - r, n)));
+      sh.TO[r * kEO + n] = static_cast<T>(0.5 * (d(r, n) + d(N - r, n)));
+    }
+    if (nq % 2) {
+      sh.FM[r] = static_cast<T>(d(hq, r));
+      sh.TC[r] = static_cast<T>(d(r, hq));
+      sh.TM[r] = static_cast<T>(d(hq, r));
+    }
+  }
+}
 
 // Cells per workgroup ("batch") of the cell kernels: each cell owns an
 // nq x nq lane tile of a 256-lane workgroup, E = floor(256 / nq^2).  (Cubes of

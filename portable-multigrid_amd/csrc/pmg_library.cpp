@@ -101,6 +101,8 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   shape_ = make_shape_data(degree, nq);
   sh64_ = to_shape<double>(shape_.S, shape_.D);
   sh32_ = to_shape<float>(shape_.S, shape_.D);
+  fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
+  fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
   shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
   shd32_ = to_shape<float>(shape_.S, shape_.Sd);
   // Internal cell order.  Default: the caller's order (on lexicographic

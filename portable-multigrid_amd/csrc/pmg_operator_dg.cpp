@@ -163,6 +163,8 @@ LaplaceOperatorDG::LaplaceOperatorDG(int degree, int nq, int precision, int64_t 
   shape_ = make_shape_data(degree, nq);
   sh64_ = make_shape<double>(shape_.S, shape_.D);
   sh32_ = make_shape<float>(shape_.S, shape_.D);
+  fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
+  fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
   shd64_ = make_shape<double>(shape_.S, shape_.Sd);
   shd32_ = make_shape<float>(shape_.S, shape_.Sd);
   const int nm = degree + 1;
