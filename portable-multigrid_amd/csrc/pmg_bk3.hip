@@ -727,52 +727,45 @@ __device__ void cell_gradient(const T *__restrict__ vec, const uint32_t *__restr
 {
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   if (ta < NM && tb < NM) {
-    T u[NM];
+    T u[NM], w[NQ];
     for (int k = 0; k < NM; ++k) {
       const uint32_t d = map[cell * NM3 + k * NM2 + tb * NM + ta];
       u[k] = d == kInvalid ? T(0) : vec[d];
     }
-    for (int r = 0; r < NQ; ++r) {
-      T v = 0;
-      for (int k = 0; k < NM; ++k) v += sh.S[k * NQ + r] * u[k];
-      P0[r * NM2 + tb * NM + ta] = v;
-    }
+    eo_interp<NM, NQ, T>(sh, u, w);
+    for (int r = 0; r < NQ; ++r) P0[r * NM2 + tb * NM + ta] = w[r];
   }
   __syncthreads();
   if (ta < NM) {
-    for (int q = 0; q < NQ; ++q) {
-      T v = 0;
-      for (int j = 0; j < NM; ++j) v += sh.S[j * NQ + q] * P0[tb * NM2 + j * NM + ta];
-      P1[(tb * NQ + q) * NM + ta] = v;
+    T a[NM], w[NQ];
+    for (int j = 0; j < NM; ++j) a[j] = P0[tb * NM2 + j * NM + ta];
+    eo_interp<NM, NQ, T>(sh, a, w);
+    for (int q = 0; q < NQ; ++q) P1[(tb * NQ + q) * NM + ta] = w[q];
+  }
+  __syncthreads();
+  {
+    T b[NM], U[NQ], dU[NQ];
+    for (int i = 0; i < NM; ++i) b[i] = P1[(tb * NQ + ta) * NM + i];
+    eo_interp<NM, NQ, T>(sh, b, U);
+    eo_deriv<NQ, T>(sh, U, dU);
+    __syncthreads();
+    for (int p = 0; p < NQ; ++p) {
+      P0[tb * NQ2 + ta * NQ + p] = U[p];
+      P2[tb * NQ2 + ta * NQ + p] = dU[p];
     }
   }
   __syncthreads();
   {
-    T U[NQ];
-    for (int p = 0; p < NQ; ++p) {
-      T v = 0;
-      for (int i = 0; i < NM; ++i) v += sh.S[i * NQ + p] * P1[(tb * NQ + ta) * NM + i];
-      U[p] = v;
-    }
-    __syncthreads();
-    for (int p = 0; p < NQ; ++p) {
-      P0[tb * NQ2 + ta * NQ + p] = U[p];
-      T v = 0;
-      for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + p] * U[n];
-      P2[tb * NQ2 + ta * NQ + p] = v;
-    }
+    T c[NQ], w[NQ];
+    for (int n = 0; n < NQ; ++n) c[n] = P0[tb * NQ2 + n * NQ + ta];
+    eo_deriv<NQ, T>(sh, c, w);
+    for (int q = 0; q < NQ; ++q) P1[tb * NQ2 + q * NQ + ta] = w[q];
   }
   __syncthreads();
-  for (int q = 0; q < NQ; ++q) {
-    T v = 0;
-    for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + q] * P0[tb * NQ2 + n * NQ + ta];
-    P1[tb * NQ2 + q * NQ + ta] = v;
-  }
-  __syncthreads();
-  for (int r = 0; r < NQ; ++r) {
-    T v = 0;
-    for (int n = 0; n < NQ; ++n) v += sh.D[n * NQ + r] * P0[n * NQ2 + tb * NQ + ta];
-    gz[r] = v;
+  {
+    T col[NQ];
+    for (int n = 0; n < NQ; ++n) col[n] = P0[n * NQ2 + tb * NQ + ta];
+    eo_deriv<NQ, T>(sh, col, gz);
   }
 }
 
