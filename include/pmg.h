@@ -186,9 +186,10 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_po
  *  2 patch: one launch; each workgroup's batch of cells accumulates the union
  *    of its DoFs in LDS and writes it once, plain stores for DoFs no other
  *    batch touches, atomics for the rest (fewer, longer-run atomics).
- * Default: patch for n_q_points_1d <= 4 (FP64) or <= 6 (FP32), atomic
- * otherwise (the faster of the two on MI355X for each level type of the MG
- * hierarchies); PMG_BK3_SCHEDULE=atomic|colored|patch overrides it.
+ * Default: patch for n_q_points_1d <= 4 (FP64) or <= 6 (FP32) on operators
+ * of more than 4096 cells, atomic otherwise (the faster of the two on MI355X
+ * for each level type and size of the MG hierarchies);
+ * PMG_BK3_SCHEDULE=atomic|colored|patch overrides it.
  * g_load: 0 non-temporal G loads at kernel entry, 1 ordinary loads at entry,
  * 2 non-temporal loads at first use, -1 the schedule's default (0 for patch,
  * 2 otherwise; PMG_BK3_GLOAD overrides the constructor's choice).  Variant 1,

@@ -200,11 +200,13 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     // otherwise.  From vmult timings on MI355X over the level types of the MG
     // hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3 cells), patch / atomic
     // time: FP64 nq 2..4 0.66-0.83, nq 5 0.87-1.06, nq 6 0.95-1.02, nq >= 7
-    // 1.22-1.26; FP32 nq 2..6 0.57-0.96, nq >= 7 1.07-1.13.
+    // 1.22-1.26; FP32 nq 2..6 0.57-0.96, nq >= 7 1.07-1.13.  Levels of at
+    // most 4096 cells are launch-bound: there the atomic kernel is as fast or
+    // faster (Q4 FP32 2^3..16^3: 8.3-11.7 vs 8.8-13.9 us per launch).
     // PMG_BK3_SCHEDULE=atomic|colored|patch and PMG_BK3_GLOAD=0|1|2 override.
     const char *env = std::getenv("PMG_BK3_SCHEDULE");
     const char *genv = std::getenv("PMG_BK3_GLOAD");
-    int kind = nq <= (precision == PMG_FP32 ? 6 : 4) ? kPatch : kAtomic;
+    int kind = nq <= (precision == PMG_FP32 ? 6 : 4) && n_cells > 4096 ? kPatch : kAtomic;
     if (env)
       kind = std::string(env) == "colored" ? kColored
              : std::string(env) == "atomic" ? kAtomic : kPatch;

@@ -123,9 +123,10 @@ def test_chebyshev_matches_oracle(pmg, orc, p, deform):
     assert rel_err(x.cpu().numpy(), xr) < 1e-11
 
 
+@pytest.mark.parametrize("sched", ["patch", "atomic"])
 @pytest.mark.parametrize("constrained", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_chebyshev_fused_output_prep(pmg, constrained, prec):
+def test_chebyshev_fused_output_prep(pmg, constrained, prec, sched):
     """The smoother's updates write the next matvec's output (src on
     constrained rows, 0 elsewhere) and then run only the cell kernel.  Compare
     with the first-kind recurrence driven from the host through the operator's
@@ -143,6 +144,7 @@ def test_chebyshev_fused_output_prep(pmg, constrained, prec):
                                                 np.zeros(0, dtype=np.uint32),
                                                 cell_vertices=pmg.box_cell_vertices(b),
                                                 precision=precision)
+    op.set_schedule(pmg.SCHEDULE_PATCH if sched == "patch" else pmg.SCHEDULE_ATOMIC)
     op.compute_diagonal()
     sm = pmg.PreconditionChebyshev(op, 3, 15.0, 10)
     mx, _, deg = sm.info()
