@@ -54,6 +54,23 @@ __device__ __forceinline__ void bk3_load_g(const T *Gc, T (&g)[6][NQ]) {
 // (ta,tb)), in two halves that all lanes of the workgroup must call (they
 // hold barriers).  P0..P2: three NQ^3 LDS buffers of this cell.
 //
+// Barrier between the phases of the cell sweeps.  A cell's nq x nq lane tile
+// lies within one 64-lane wave when 64 % nq^2 == 0 (nq = 2, 4, 8), and the
+// phases exchange data only through the cell's own LDS region, so only lanes
+// of that wave take part: LDS instructions of a wave are processed in order,
+// and a wavefront-scope fence keeps the compiler from reordering them.  Other
+// tiles straddle waves and take a workgroup barrier.
+template <int NQ>
+__device__ __forceinline__ void cell_sync() {
+  if constexpr (64 % (NQ * NQ) == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // ---- even-odd 1D contractions (tables: Shape1D, fill_even_odd)
 // out[r] = sum_k S[k][r] x[k]   (values at the Gauss points)
 template <int NM, int NQ, typename T>
@@ -183,10 +200,15 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
 #pragma unroll
     for (int r = 0; r < NQ; ++r) P0[r * NM2 + tb * NM + ta] = w[r];  // A[r][j][i]
   }
-  __syncthreads();
-
-  if (zbuf)
+  if (zbuf) {
+    // zbuf is shared by the workgroup: every lane has consumed u before it is
+    // cleared, and the clear is complete before any lane adds into it
+    __syncthreads();
     for (int i = threadIdx.x; i < zn; i += blockDim.x) zbuf[i] = T(0);
+    __syncthreads();
+  } else {
+    cell_sync<NQ>();
+  }
 
   // ---- phase 2: y-interpolation, lane (i,r) = (ta,tb): B[r][q][i]
   if (active && ta < NM) {
@@ -198,7 +220,7 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
 #pragma unroll
     for (int q = 0; q < NQ; ++q) P1[(tb * NQ + q) * NM + ta] = w[q];
   }
-  __syncthreads();
+  cell_sync<NQ>();
 
   // ---- phase 3: x-interpolation + x-gradient, lane (q,r) = (ta,tb)
   if (active) {
@@ -213,7 +235,7 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
       P2[tb * NQ2 + ta * NQ + p] = dU[p];  // du/dxi_0
     }
   }
-  __syncthreads();
+  cell_sync<NQ>();
 }
 
 // bk3_backward: from the state bk3_forward leaves, the remaining gradient,
@@ -236,7 +258,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int q = 0; q < NQ; ++q) P1[tb * NQ2 + q * NQ + ta] = w[q];  // du/dxi_1
   }
-  __syncthreads();
+  cell_sync<NQ>();
 
   // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
   if (active) {
@@ -258,7 +280,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int r = 0; r < NQ; ++r) P0[r * NQ2 + tb * NQ + ta] = w[r];
   }
-  __syncthreads();
+  cell_sync<NQ>();
 
   // ---- phase 6: x-integration accumulated onto the z part, lane (q,r)
   if (active) {
@@ -269,7 +291,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int p = 0; p < NQ; ++p) P0[tb * NQ2 + ta * NQ + p] += w[p];
   }
-  __syncthreads();
+  cell_sync<NQ>();
 
   // ---- phase 7: y-integration + y back-interpolation, lane (p,r)
   if (active) {
@@ -287,7 +309,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int j = 0; j < NM; ++j) P1[tb * NQ2 + j * NQ + ta] = w[j];  // W[r][j][p]
   }
-  __syncthreads();
+  cell_sync<NQ>();
 
   // ---- phase 8: x back-interpolation, lane (j,r) = (ta,tb)
   if (active && ta < NM) {
@@ -298,7 +320,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int i = 0; i < NM; ++i) P2[tb * NM2 + ta * NM + i] = w[i];  // Z[r][j][i]
   }
-  __syncthreads();
+  cell_sync<NQ>();
 
   // ---- phase 9: z back-interpolation, lane (i,j) = (ta,tb)
   if (active && ta < NM && tb < NM) {

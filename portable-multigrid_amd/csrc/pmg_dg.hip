@@ -151,7 +151,7 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
       }
       B[t] = um;
     }
-    __syncthreads();
+    cell_sync<NQ>();  // face scratch and V are per cell
 
     // B: neighbour value and normal derivative at face point (a,b)
     T vpq = 0, dpq = 0;
@@ -171,7 +171,7 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
       }
       B[NQ2 + t] = vpq;
     }
-    __syncthreads();
+    cell_sync<NQ>();  // face scratch and V are per cell
 
     // C: tangential derivatives by collocation on the face Gauss grid, flux
     T cval = 0, cn = 0;
@@ -215,7 +215,7 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
       A[t] = td * mo1;        // tangential test coefficients
       A[NQ2 + t] = td * mo2;
     }
-    __syncthreads();
+    cell_sync<NQ>();  // face scratch and V are per cell
 
     // D: tangential test terms become value terms by the transposed
     // collocation derivative; inject value and normal-derivative tests into V
@@ -228,7 +228,7 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
 #pragma unroll
       for (int r = 0; r < NQ; ++r) Vl[r * snq] += c * fs.L[s * NQ + r] + cn * fs.Ld[s * NQ + r];
     }
-    __syncthreads();
+    cell_sync<NQ>();  // face scratch and V are per cell
 #pragma unroll
     for (int k = 0; k < NM; ++k) wcur[k] = wnext[k];
   }
