@@ -31,7 +31,10 @@ namespace pmg {
 // every x-line (j,k) of its DoF map is a run of consecutive DoFs starting at
 // lines[cell][k][j]; other cells read the full map.  The line starts and the
 // flag are loaded together, so compressed cells need no dependent step.
-template <int NM, int NQ, typename T, bool ATOMIC = true, int GMODE = kGEarlyNT>
+// WT: wave-aligned lane layout (Bk3Tile::per_wave whole cells per wave, the
+// workgroup has Bk3Tile::wave_lanes lanes): every phase of the cell sweeps is
+// wave-local, no workgroup barrier.
+template <int NM, int NQ, typename T, bool ATOMIC = true, int GMODE = kGEarlyNT, bool WT = false>
 __device__ __forceinline__ void
 bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
           const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
@@ -42,12 +45,15 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   constexpr int E = Bk3Tile<NQ>::cells;
 
   const int tid = threadIdx.x;
-  const int e = tid / NQ2, t = tid - e * NQ2, ta = t % NQ, tb = t / NQ;
+  constexpr int CW = Bk3Tile<NQ>::per_wave;
+  const int wl = tid & 63, wc = wl / NQ2;
+  const int e = WT ? (tid >> 6) * CW + wc : tid / NQ2;
+  const int t = WT ? wl - wc * NQ2 : tid - e * NQ2, ta = t % NQ, tb = t / NQ;
   const int64_t pos = cell_base + e;
-  const bool active = (e < E) && (pos < n_range);
+  const bool active = (!WT || wc < CW) && (e < E) && (pos < n_range);
   const int64_t cell = active ? (cell_ids ? static_cast<int64_t>(cell_ids[pos]) : pos) : 0;
   T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
-  const T *Gc = G + g_index(NQ, cell, 0, t);
+  const T *Gc = G + g_index(NQ, cell, 0, active ? t : 0);
 
   T g[6][NQ];
   if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
@@ -71,7 +77,8 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
 #pragma unroll
     for (int k = 0; k < NM; ++k) u[k] = (idx[k] == kInvalid) ? T(0) : x[idx[k]];
   }
-  bk3_cell_apply<NM, NQ, T, GMODE>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3, active, ta, tb);
+  bk3_cell_apply<NM, NQ, T, GMODE, WT || (64 % NQ2 == 0)>(u, v, g, Gc, sh, P0, P0 + NQ3,
+                                                         P0 + 2 * NQ3, active, ta, tb);
   // scatter-add
   if (active && ta < NM && tb < NM) {
 #pragma unroll
@@ -83,8 +90,19 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   }
 }
 
+// Wave-aligned layout for the cells kernel where it measured faster: FP32
+// at nq = 7 (Q6 48^3: 424 -> 398 us).  FP64 loses with it (nq = 5: 497 ->
+// 530 us, nq = 7: 572 -> 866 us): the larger workgroups meet the FP64 kernels'
+// register budget and lose occupancy, and a fifth of the lanes idle.
+template <int NQ, typename T>
+constexpr bool bk3_wave_tile() { return std::is_same<T, float>::value && NQ == 7; }
+template <int NQ, typename T>
+constexpr int bk3_cells_lanes() {
+  return bk3_wave_tile<NQ, T>() ? Bk3Tile<NQ>::wave_lanes : Bk3Tile<NQ>::lanes;
+}
+
 template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((bk3_cells_lanes<NQ, T>()))
 bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
                  const uint8_t *__restrict__ lflag, const uint32_t *__restrict__ cell_ids,
@@ -93,8 +111,8 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   constexpr int E = Bk3Tile<NQ>::cells;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  bk3_batch<NM, NQ, T, ATOMIC, GMODE>(x, y, G, map, lines, lflag, cell_ids, n_range, sh,
-                                      static_cast<int64_t>(lb) * E, lds);
+  bk3_batch<NM, NQ, T, ATOMIC, GMODE, bk3_wave_tile<NQ, T>()>(
+      x, y, G, map, lines, lflag, cell_ids, n_range, sh, static_cast<int64_t>(lb) * E, lds);
 }
 
 // ---------------------------------------------------------------- patch schedule
@@ -496,8 +514,9 @@ static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
   const int64_t nb = (n_range + Tile::cells - 1) / Tile::cells;
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
-  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, ATOMIC, GMODE>), dim3(grid), dim3(Tile::lanes), 0,
-                     s, x, y, G, map, lines, lflag, cell_ids, n_range, sh);
+  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, ATOMIC, GMODE>), dim3(grid),
+                     dim3(bk3_cells_lanes<NQ, T>()), 0, s, x, y, G, map, lines, lflag, cell_ids,
+                     n_range, sh);
   PMG_HIP_CHECK(hipGetLastError());
 }
 

@@ -12,6 +12,10 @@ template <int NQ>
 struct Bk3Tile {
   static constexpr int NQ2 = NQ * NQ;
   static constexpr int cells = g_batch_cells(NQ);
+  // wave-aligned layout: per_wave whole cells in each 64-lane wave (lanes
+  // beyond per_wave nq^2 idle), the batch's cells over wave_lanes lanes
+  static constexpr int per_wave = 64 / NQ2 > 0 ? 64 / NQ2 : 1;
+  static constexpr int wave_lanes = ((cells + per_wave - 1) / per_wave) * 64;
   static constexpr int lanes = ((cells * NQ2 + 63) / 64) * 64;
   static_assert(cells * NQ2 <= 256, "a batch must fit one 256-lane workgroup");
 };
@@ -58,11 +62,12 @@ __device__ __forceinline__ void bk3_load_g(const T *Gc, T (&g)[6][NQ]) {
 // lies within one 64-lane wave when 64 % nq^2 == 0 (nq = 2, 4, 8), and the
 // phases exchange data only through the cell's own LDS region, so only lanes
 // of that wave take part: LDS instructions of a wave are processed in order,
-// and a wavefront-scope fence keeps the compiler from reordering them.  Other
-// tiles straddle waves and take a workgroup barrier.
-template <int NQ>
+// and a wavefront-scope fence keeps the compiler from reordering them.  So
+// are the tiles of the wave-aligned layout (WAVE = true, Bk3Tile::per_wave).
+// Other tiles straddle waves and take a workgroup barrier.
+template <int NQ, bool WAVE = (64 % (NQ * NQ) == 0)>
 __device__ __forceinline__ void cell_sync() {
-  if constexpr (64 % (NQ * NQ) == 0) {
+  if constexpr (WAVE) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -187,7 +192,7 @@ __device__ __forceinline__ void eo_deriv_t(const Shape1D<T> &sh, const T (&x)[NQ
 // where ta,tb < NM).  Leaves the values at the Gauss points in
 // P0[r][q][p] (p along x) and d/dxi_0 of them in P2.  zbuf[0..zn) (LDS,
 // optional) is cleared by the whole workgroup after every lane has consumed u.
-template <int NM, int NQ, typename T>
+template <int NM, int NQ, typename T, bool WAVE = (64 % (NQ * NQ) == 0)>
 __device__ __forceinline__ void
 bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const bool active,
             const int ta, const int tb, T *zbuf = nullptr, const int zn = 0)
@@ -207,7 +212,7 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
     for (int i = threadIdx.x; i < zn; i += blockDim.x) zbuf[i] = T(0);
     __syncthreads();
   } else {
-    cell_sync<NQ>();
+    cell_sync<NQ, WAVE>();
   }
 
   // ---- phase 2: y-interpolation, lane (i,r) = (ta,tb): B[r][q][i]
@@ -220,7 +225,7 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
 #pragma unroll
     for (int q = 0; q < NQ; ++q) P1[(tb * NQ + q) * NM + ta] = w[q];
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 
   // ---- phase 3: x-interpolation + x-gradient, lane (q,r) = (ta,tb)
   if (active) {
@@ -235,7 +240,7 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
       P2[tb * NQ2 + ta * NQ + p] = dU[p];  // du/dxi_0
     }
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 }
 
 // bk3_backward: from the state bk3_forward leaves, the remaining gradient,
@@ -243,7 +248,7 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
 // and the test with grad phi; with HASV also sum_q phi(q) V[q] for a value
 // coefficient V[r][q][p] at the Gauss points (LDS).  out[k] receives the
 // result for column (i,j) = (ta,tb) over k.
-template <int NM, int NQ, typename T, int GMODE, bool HASV>
+template <int NM, int NQ, typename T, int GMODE, bool HASV, bool WAVE = (64 % (NQ * NQ) == 0)>
 __device__ __forceinline__ void
 bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *P0, T *P1,
              T *P2, const T *V, const bool active, const int ta, const int tb)
@@ -258,7 +263,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int q = 0; q < NQ; ++q) P1[tb * NQ2 + q * NQ + ta] = w[q];  // du/dxi_1
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 
   // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
   if (active) {
@@ -280,7 +285,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int r = 0; r < NQ; ++r) P0[r * NQ2 + tb * NQ + ta] = w[r];
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 
   // ---- phase 6: x-integration accumulated onto the z part, lane (q,r)
   if (active) {
@@ -291,7 +296,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int p = 0; p < NQ; ++p) P0[tb * NQ2 + ta * NQ + p] += w[p];
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 
   // ---- phase 7: y-integration + y back-interpolation, lane (p,r)
   if (active) {
@@ -309,7 +314,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int j = 0; j < NM; ++j) P1[tb * NQ2 + j * NQ + ta] = w[j];  // W[r][j][p]
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 
   // ---- phase 8: x back-interpolation, lane (j,r) = (ta,tb)
   if (active && ta < NM) {
@@ -320,7 +325,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 #pragma unroll
     for (int i = 0; i < NM; ++i) P2[tb * NM2 + ta * NM + i] = w[i];  // Z[r][j][i]
   }
-  cell_sync<NQ>();
+  cell_sync<NQ, WAVE>();
 
   // ---- phase 9: z back-interpolation, lane (i,j) = (ta,tb)
   if (active && ta < NM && tb < NM) {
@@ -332,14 +337,14 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 }
 
 // Both halves: out = A_e u (see bk3_forward / bk3_backward).
-template <int NM, int NQ, typename T, int GMODE>
+template <int NM, int NQ, typename T, int GMODE, bool WAVE = (64 % (NQ * NQ) == 0)>
 __device__ __forceinline__ void
 bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh,
                T *P0, T *P1, T *P2, const bool active, const int ta, const int tb,
                T *zbuf = nullptr, const int zn = 0)
 {
-  bk3_forward<NM, NQ, T>(u, sh, P0, P1, P2, active, ta, tb, zbuf, zn);
-  bk3_backward<NM, NQ, T, GMODE, false>(out, g, Gc, sh, P0, P1, P2, nullptr, active, ta, tb);
+  bk3_forward<NM, NQ, T, WAVE>(u, sh, P0, P1, P2, active, ta, tb, zbuf, zn);
+  bk3_backward<NM, NQ, T, GMODE, false, WAVE>(out, g, Gc, sh, P0, P1, P2, nullptr, active, ta, tb);
 }
 
 }  // namespace pmg
