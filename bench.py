@@ -337,6 +337,18 @@ def _roof(t, nbytes):
             "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS}
 
 
+def _box_subdomain(n, p):
+    """An n^3 box as one subdomain whose x = 1 face is the interface: the
+    physical boundary DoFs (every other face) and the interface cells."""
+    import numpy as np
+    n1 = p * n + 1
+    d = np.arange(n1 ** 3, dtype=np.int64)
+    ix, iy, iz = d % n1, (d // n1) % n1, d // (n1 * n1)
+    phys = (ix == 0) | (iy == 0) | (iy == n1 - 1) | (iz == 0) | (iz == n1 - 1)
+    cells = np.arange(n ** 3, dtype=np.int64)
+    return d[phys].astype(np.uint32), cells[cells % n == n - 1].astype(np.uint32)
+
+
 def bench_rows(pmg):
     """One line per kernel of the hot path, timed alone with HIP events (20
     launches after a warm-up) on its own configuration's sizes, against its
@@ -344,6 +356,7 @@ def bench_rows(pmg):
     not in the byte model): operator setup (geometry, diagonal), the BK3 and
     SIP-DG operators, h-, p- and DG<->CG transfers, the Chebyshev step, the
     V-cycle and one outer CG iteration, in FP64 and FP32."""
+    import numpy as np
     import torch
     rows = {}
     # practical HBM ceiling on this box: a plain 1 GiB -> 1 GiB device copy
@@ -376,6 +389,19 @@ def bench_rows(pmg):
         rows[f"a3_a4_operator_setup_q4_64cube_{tag}"] = {
             "time_ms": t_setup * 1e3, "note": "host DoF map upload + G kernel (setup, not a hot op)"}
         rows[f"a5_diagonal_q4_64cube_{tag}"] = {"time_ms": t_diag * 1e3, "note": "setup"}
+        # subdomain operator (x = 1 face the interface): vmult_neumann moves the
+        # vmult's bytes; the interface-cell range s(N + 2 N_i + 6 C_i nq^3),
+        # N_i the DoFs of its C_i cells (the zero fill of dst is the N)
+        pmap = pmg.box_dofmap(fine, 4, False)
+        phys, icells = _box_subdomain(64, 4)
+        op.set_subdomain(pmap, phys, icells)
+        n_i = len(np.unique(pmap.reshape(-1, 125)[icells]))
+        rows[f"f3_subdomain_vmult_neumann_q4_64cube_{tag}"] = _roof(
+            time_events(lambda: op.vmult_neumann(y, x), 20), by)
+        rows[f"f3_subdomain_vmult_interface_cells_q4_64cube_{tag}"] = dict(
+            _roof(time_events(lambda: op.vmult_interface_cell_range(y, x), 20),
+                  sz[prec] * (N + 2 * n_i + 6 * len(icells) * 125)), interface_cells=len(icells))
+        del pmap, phys, icells
         sm = pmg.PreconditionChebyshev(op, 3, 15.0, 10)
         b = torch.rand(N, dtype=dt[prec], device="cuda")
         t_step = time_events(lambda: sm.step(y, b), 20)

@@ -149,6 +149,27 @@ int pmg_operator_Tvmult(pmg_operator op, void *dst, const void *src, pmg_stream 
 /* dst += sum_cells A_e src, no framing (the raw cell loop a subdomain
  * operator such as SubdomainLaplaceOperator::vmult_bk3 calls) */
 int pmg_operator_apply_add(pmg_operator op, void *dst, const void *src, pmg_stream stream);
+/* Domain-decomposition subdomain operator: replaces the index sets and cell
+ * loops of Portable::SubdomainLaplaceOperator<dim,p,number>
+ * (include/operators/portable_subdomain_laplace_operator.h:23-175).  The
+ * subdomain's operator is created with every subdomain constraint (physical
+ * boundary and interface DoFs): its vmult is vmult_bk3 (:311-382) and its
+ * apply_add the bare cell loop.  set_subdomain adds the plain index set of the
+ * set-up at :1056-1061 -- dofmap (host, caller cell order, as given to create)
+ * with only the physical_dofs eliminated -- and the interface cells (host,
+ * caller indices of the cells with a face on the interface, :1181-1203).
+ * Copies what it needs; may be called again to replace the sets. */
+int pmg_operator_set_subdomain(pmg_operator op, const uint32_t *dofmap,
+                               const uint32_t *physical_dofs, int64_t n_physical,
+                               const uint32_t *interface_cells, int64_t n_interface_cells);
+/* vmult_neumann (:384-454): dst = sum over every cell of A_e src with the
+ * plain indices, then dst_b = src_b on the physical boundary DoFs.  Interface
+ * DoFs are free rows.  src and dst must not alias. */
+int pmg_operator_vmult_neumann(pmg_operator op, void *dst, const void *src, pmg_stream stream);
+/* vmult_interface_cell_range (:456-533): the same over the interface cells
+ * only (rows no interface cell touches are 0, physical rows = src). */
+int pmg_operator_vmult_interface_cells(pmg_operator op, void *dst, const void *src,
+                                       pmg_stream stream);
 /* compute_diagonal(): diag(A), 1 on constrained rows, then inverted
  * (portable_laplace_operator_bk3.h:533-561). Synchronous. */
 int pmg_operator_compute_diagonal(pmg_operator op);

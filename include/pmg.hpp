@@ -148,6 +148,21 @@ class LaplaceOperatorBK3 : public LaplaceOperatorBase {
   void set_schedule(int schedule, int g_load = -1) {
     check(pmg_operator_set_schedule(h_.get(), schedule, g_load));
   }
+  // SubdomainLaplaceOperator: this operator (every subdomain constraint) is
+  // vmult_bk3; the plain index set and the interface cells (host arrays)
+  // enable vmult_neumann and vmult_interface_cell_range
+  void set_subdomain(const uint32_t *dofmap, const uint32_t *physical_dofs, int64_t n_physical,
+                     const uint32_t *interface_cells, int64_t n_interface_cells) {
+    check(pmg_operator_set_subdomain(h_.get(), dofmap, physical_dofs, n_physical,
+                                     interface_cells, n_interface_cells));
+  }
+  void vmult_bk3(void *dst, const void *src, pmg_stream s = nullptr) const { vmult(dst, src, s); }
+  void vmult_neumann(void *dst, const void *src, pmg_stream s = nullptr) const {
+    check(pmg_operator_vmult_neumann(h_.get(), dst, src, s));
+  }
+  void vmult_interface_cell_range(void *dst, const void *src, pmg_stream s = nullptr) const {
+    check(pmg_operator_vmult_interface_cells(h_.get(), dst, src, s));
+  }
 };
 
 class LaplaceOperatorDG : public LaplaceOperatorBase {

@@ -82,6 +82,8 @@ def lib():
             "pmo_geometry": (None, [_MP, _i, _vp, _vp, _vp]),
             "pmo_bk3_cells": (None, [_i, _i, _L, _D, _D, _D, _U, _D, _D, _i, _vp]),
             "pmo_bk3_vmult": (None, [_i, _i, _L, _D, _D, _D, _U, _L, _U, _L, _D, _D, _i, _vp]),
+            "pmo_bk3_vmult_subdomain": (None, [_i, _i, _L, _D, _D, _D, _U, _L, _U, _L, _vp, _L,
+                                                _D, _D]),
             "pmo_bk3_element_matrices": (None, [_i, _i, _L, _D, _D]),
             "pmo_bk3_diagonal": (None, [_i, _i, _L, _D, _U, _L, _U, _L, _D]),
             "pmo_cell_colors": (None, [_MP, _I]),
@@ -222,6 +224,18 @@ class LaplaceBK3:
         lib().pmo_bk3_vmult(self.p, self.nq, self.C, self.S, self.D, self.G, self.map, self.N,
                             self.cons, len(self.cons), np.ascontiguousarray(src, np.float64),
                             dst, nt, _ptr(self.colors))
+        return dst
+
+    def vmult_subdomain(self, src, physical, cells=None):
+        """vmult_neumann (cells None) or vmult_interface_cell_range (the listed
+        cells) of the subdomain operator: plain map, physical DoFs eliminated."""
+        dst = np.zeros(self.N)
+        phys = np.ascontiguousarray(physical, dtype=np.uint32)
+        sel = None if cells is None else np.ascontiguousarray(cells, dtype=np.uint32)
+        lib().pmo_bk3_vmult_subdomain(self.p, self.nq, self.C, self.S, self.D, self.G,
+                                      self.plain_map, self.N, phys, len(phys), _ptr(sel),
+                                      0 if sel is None else len(sel),
+                                      np.ascontiguousarray(src, np.float64), dst)
         return dst
 
     def cells_add(self, src, dst, n_threads=1):

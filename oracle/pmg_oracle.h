@@ -17,6 +17,10 @@
  *                     derivatives, Q1 27-point stencil, x^T A x = int |grad u|^2
  *                                                             test_oracle_bk3.py
  *   diagonal          dense element matrix diagonal            test_oracle_bk3.py
+ *   subdomain vmult   dense assembly over the listed cells with the physical
+ *                     rows/columns replaced by identity; A 1 = 0 without
+ *                     physical DoFs; all-boundary case = the BK3 vmult
+ *                                                             test_oracle_bk3.py
  *   h/p transfers     exact prolongation of polynomials, <Px,y> = <x,Ry>
  *                                                             test_oracle_transfer.py
  *   Chebyshev / CG    textbook Chebyshev error bound on a 1D Laplacian,
@@ -83,6 +87,13 @@ void pmo_bk3_vmult(int p, int nq, long n_cells, const double *S, const double *D
                    const double *G, const uint32_t *map, long n_dofs,
                    const uint32_t *constrained, long n_constrained,
                    const double *src, double *dst, int n_threads, const int *colors);
+/* subdomain vmult (Neumann / interface cell range): plain map with the
+ * physical boundary DoFs eliminated, over every cell (cells NULL) or the
+ * n_list listed cells, dst_b = src_b on the physical boundary DoFs */
+void pmo_bk3_vmult_subdomain(int p, int nq, long n_cells, const double *S, const double *D,
+                             const double *G, const uint32_t *plain_map, long n_dofs,
+                             const uint32_t *physical, long n_physical, const uint32_t *cells,
+                             long n_list, const double *src, double *dst);
 /* dense element matrices K[c][nm^3][nm^3] with exact derivatives */
 void pmo_bk3_element_matrices(int p, int nq, long n_cells, const double *G, double *K);
 /* diag(A) (1 on constrained rows) with exact derivatives */

@@ -159,6 +159,39 @@ void pmo_bk3_vmult(int p, int nq, long n_cells, const double *S, const double *D
   for (long i = 0; i < n_constrained; ++i) dst[constrained[i]] = src[constrained[i]];
 }
 
+/* SubdomainLaplaceOperator::vmult_neumann and ::vmult_interface_cell_range
+ * (include/operators/portable_subdomain_laplace_operator.h:384-454, :456-533):
+ * dst = 0; the cell loop with the plain indices, in which only the physical
+ * boundary DoFs are eliminated (:1056-1061), over every cell (cells == NULL)
+ * or over the listed cells (the interface cells, :1181-1203); then
+ * dst_b = src_b on the physical boundary DoFs. */
+void pmo_bk3_vmult_subdomain(int p, int nq, long n_cells, const double *S, const double *D,
+                             const double *G, const uint32_t *plain_map, long n_dofs,
+                             const uint32_t *physical, long n_physical, const uint32_t *cells,
+                             long n_list, const double *src, double *dst)
+{
+  const int nm = p + 1, nm3 = nm * nm * nm, nq3 = nq * nq * nq;
+  const long n_sel = cells ? n_list : n_cells;
+  unsigned char *phys = calloc((size_t)n_dofs + 1, 1);
+  uint32_t *map = calloc((size_t)(n_sel * nm3 + 1), sizeof(uint32_t));
+  double *g = calloc((size_t)(n_sel * 6 * nq3 + 1), sizeof(double));
+  for (long i = 0; i < n_physical; ++i) phys[physical[i]] = 1;
+  for (long s = 0; s < n_sel; ++s) {
+    const long c = cells ? (long)cells[s] : s;
+    for (int i = 0; i < nm3; ++i) {
+      const uint32_t d = plain_map[c * nm3 + i];
+      map[s * nm3 + i] = (d == PMO_INVALID || phys[d]) ? PMO_INVALID : d;
+    }
+    memcpy(g + s * 6 * nq3, G + c * 6 * nq3, sizeof(double) * 6 * nq3);
+  }
+  memset(dst, 0, sizeof(double) * n_dofs);
+  cells_range(p, nq, S, D, g, map, src, dst, n_sel, NULL, 0);
+  for (long i = 0; i < n_physical; ++i) dst[physical[i]] = src[physical[i]];
+  free(g);
+  free(map);
+  free(phys);
+}
+
 void pmo_cell_colors(const pmo_mesh *m, int *colors)
 {
   long c = 0;

@@ -120,6 +120,9 @@ def lib():
         "pmg_operator_vmult": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_Tvmult": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_apply_add": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_operator_set_subdomain": (_i, [_vp, _vp, _vp, _i64, _vp, _i64]),
+        "pmg_operator_vmult_neumann": (_i, [_vp, _vp, _vp, _vp]),
+        "pmg_operator_vmult_interface_cells": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_compute_diagonal": (_i, [_vp]),
         "pmg_operator_diagonal_inverse": (_i, [_vp, _vp, _vp]),
         "pmg_operator_set_coefficient": (_i, [_vp, _vp, _vp]),
@@ -313,6 +316,25 @@ class LaplaceOperatorBK3:
     def Tvmult(self, dst, src):
         _check(lib().pmg_operator_Tvmult(self.h, _dev_ptr(dst, self.precision, self._m),
                                          _dev_ptr(src, self.precision, self._m), _stream()))
+
+    def set_subdomain(self, dofmap, physical_dofs, interface_cells):
+        """SubdomainLaplaceOperator index sets: the plain map (caller cell
+        order) with only the physical boundary DoFs eliminated, and the
+        interface cells (caller indices)."""
+        dofmap = np.ascontiguousarray(dofmap, dtype=np.uint32)
+        phys = np.ascontiguousarray(physical_dofs, dtype=np.uint32)
+        cells = np.ascontiguousarray(interface_cells, dtype=np.uint32)
+        _check(lib().pmg_operator_set_subdomain(self.h, _np_ptr(dofmap), _np_ptr(phys), len(phys),
+                                                _np_ptr(cells), len(cells)))
+
+    def vmult_neumann(self, dst, src):
+        _check(lib().pmg_operator_vmult_neumann(self.h, _dev_ptr(dst, self.precision, self._m),
+                                                _dev_ptr(src, self.precision, self._m), _stream()))
+
+    def vmult_interface_cell_range(self, dst, src):
+        _check(lib().pmg_operator_vmult_interface_cells(
+            self.h, _dev_ptr(dst, self.precision, self._m), _dev_ptr(src, self.precision, self._m),
+            _stream()))
 
     def apply_add(self, dst, src):
         _check(lib().pmg_operator_apply_add(self.h, _dev_ptr(dst, self.precision, self._m),

@@ -575,6 +575,32 @@ void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const ui
                                        std::to_string(degree) + ", " + std::to_string(nq) + ")");
 }
 
+template <typename T>
+void launch_bk3_cell_list(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
+                          const uint32_t *lines, const uint8_t *lflag, const uint32_t *cell_ids,
+                          int64_t n_list, const Shape1D<T> &sh, hipStream_t s) {
+#define PMG_BK3_CASE(P, Q)                                                                  \
+  if (degree == P && nq == Q) {                                                             \
+    bk3_launch_range<P + 1, Q, T, true, kGLateNT>(x, y, G, map, lines, lflag, cell_ids,     \
+                                                  n_list, sh, s);                           \
+    return;                                                                                 \
+  }
+  PMG_BK3_CASE(1, 2) PMG_BK3_CASE(1, 3) PMG_BK3_CASE(2, 3) PMG_BK3_CASE(2, 4)
+  PMG_BK3_CASE(3, 4) PMG_BK3_CASE(3, 5) PMG_BK3_CASE(4, 5) PMG_BK3_CASE(4, 6)
+  PMG_BK3_CASE(5, 6) PMG_BK3_CASE(5, 7) PMG_BK3_CASE(6, 7) PMG_BK3_CASE(6, 8)
+#undef PMG_BK3_CASE
+  throw Error(PMG_ERR_UNSUPPORTED, "BK3 kernel: unsupported (degree, n_q_points_1d) = (" +
+                                       std::to_string(degree) + ", " + std::to_string(nq) + ")");
+}
+template void launch_bk3_cell_list<double>(int, int, const double *, double *, const double *,
+                                           const uint32_t *, const uint32_t *, const uint8_t *,
+                                           const uint32_t *, int64_t, const Shape1D<double> &,
+                                           hipStream_t);
+template void launch_bk3_cell_list<float>(int, int, const float *, float *, const float *,
+                                          const uint32_t *, const uint32_t *, const uint8_t *,
+                                          const uint32_t *, int64_t, const Shape1D<float> &,
+                                          hipStream_t);
+
 template void launch_bk3_cells<double>(int, int, const double *, double *, const double *,
                                        const uint32_t *, int64_t, const Shape1D<double> &,
                                        hipStream_t, const Bk3Schedule *, bool);

@@ -167,6 +167,30 @@ int pmg_operator_apply_add(pmg_operator op, void *dst, const void *src, pmg_stre
   });
 }
 
+int pmg_operator_set_subdomain(pmg_operator op, const uint32_t *dofmap,
+                               const uint32_t *physical_dofs, int64_t n_physical,
+                               const uint32_t *interface_cells, int64_t n_interface_cells) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    bk3_of(op).set_subdomain(dofmap, physical_dofs, n_physical, interface_cells, n_interface_cells);
+  });
+}
+
+int pmg_operator_vmult_neumann(pmg_operator op, void *dst, const void *src, pmg_stream s) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    bk3_of(op).vmult_subdomain(dst, src, false, as_stream(s));
+  });
+}
+
+int pmg_operator_vmult_interface_cells(pmg_operator op, void *dst, const void *src,
+                                       pmg_stream s) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    bk3_of(op).vmult_subdomain(dst, src, true, as_stream(s));
+  });
+}
+
 int pmg_operator_compute_diagonal(pmg_operator op) {
   return guard([&] {
     PMG_NOT_NULL(op);

@@ -72,6 +72,35 @@ std::vector<double> tridiag_eigenvalues(const std::vector<double> &d, const std:
 }  // namespace
 
 // ============================================================ operator
+// Compressed indices: a cell whose every x-line (j,k) is a run of consecutive
+// unconstrained DoFs needs only the line starts (nm^2 instead of nm^3 index
+// loads).  Uploaded, and true, when at least half of the cells qualify (on
+// lexicographic numberings all but the boundary cells do).
+static bool compress_lines(const std::vector<uint32_t> &imap, int64_t n_cells, int nm,
+                           int64_t n_dofs, DeviceArray<uint32_t> &lines_out,
+                           DeviceArray<uint8_t> &flag_out) {
+  if (n_cells <= 0) return false;
+  const int64_t nm3 = int64_t(nm) * nm * nm;
+  std::vector<uint32_t> lines(static_cast<size_t>(n_cells * nm * nm), 0u);
+  std::vector<uint8_t> flag(static_cast<size_t>(n_cells), 0);
+  int64_t n_comp = 0;
+  for (int64_t c = 0; c < n_cells; ++c) {
+    bool ok = true;
+    for (int kj = 0; kj < nm * nm && ok; ++kj) {
+      const uint32_t *ln = imap.data() + c * nm3 + int64_t(kj) * nm;
+      ok = ln[0] != kInvalid && int64_t(ln[0]) + nm <= n_dofs;
+      for (int i = 1; i < nm && ok; ++i) ok = ln[i] == ln[0] + uint32_t(i);
+      lines[c * nm * nm + kj] = ok ? ln[0] : 0u;
+    }
+    flag[c] = ok ? 1 : 0;
+    n_comp += ok;
+  }
+  if (2 * n_comp < n_cells) return false;
+  lines_out.upload(lines);
+  flag_out.upload(flag);
+  return true;
+}
+
 LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_t n_cells,
                                        const uint32_t *dofmap, int64_t n_dofs,
                                        const uint32_t *constrained, int64_t n_constrained,
@@ -163,32 +192,9 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     }
   map_.upload(imap);
   perm_.upload(perm);
-  // Compressed indices: a cell whose every x-line (j,k) is a run of
-  // consecutive unconstrained DoFs needs only the line starts (nm^2 instead
-  // of nm^3 index loads).  Used when at least half of the cells qualify (on
-  // lexicographic numberings all but the boundary cells do).
-  if (n_cells > 0) {
-    const int nm = degree + 1;
-    std::vector<uint32_t> lines(static_cast<size_t>(n_cells * nm * nm), 0u);
-    std::vector<uint8_t> flag(static_cast<size_t>(n_cells), 0);
-    int64_t n_comp = 0;
-    for (int64_t c = 0; c < n_cells; ++c) {
-      bool ok = true;
-      for (int kj = 0; kj < nm * nm && ok; ++kj) {
-        const uint32_t *ln = imap.data() + c * nm3 + int64_t(kj) * nm;
-        ok = ln[0] != kInvalid && int64_t(ln[0]) + nm <= n_dofs;
-        for (int i = 1; i < nm && ok; ++i) ok = ln[i] == ln[0] + uint32_t(i);
-        lines[c * nm * nm + kj] = ok ? ln[0] : 0u;
-      }
-      flag[c] = ok ? 1 : 0;
-      n_comp += ok;
-    }
-    if (2 * n_comp >= n_cells) {
-      lines_.upload(lines);
-      lflag_.upload(flag);
-      sched_.lines = lines_.ptr;
-      sched_.lflag = lflag_.ptr;
-    }
+  if (compress_lines(imap, n_cells, degree + 1, n_dofs, lines_, lflag_)) {
+    sched_.lines = lines_.ptr;
+    sched_.lflag = lflag_.ptr;
   }
   constrained_.upload(constrained, static_cast<size_t>(n_constrained));
   if (n_constrained > 0) cmask_.upload(cmask);
@@ -368,6 +374,84 @@ void LaplaceOperatorBK3::vmult(void *dst, const void *src, hipStream_t s) const 
   PMG_REQUIRE(dst != src, PMG_ERR_INVALID_ARG, "vmult: dst and src must not alias");
   if (precision_ == PMG_FP64) vmult_t(static_cast<double *>(dst), static_cast<const double *>(src), s);
   else vmult_t(static_cast<float *>(dst), static_cast<const float *>(src), s);
+}
+
+void LaplaceOperatorBK3::set_subdomain(const uint32_t *dofmap, const uint32_t *physical,
+                                       int64_t n_physical, const uint32_t *interface_cells,
+                                       int64_t n_interface) {
+  PMG_REQUIRE(dofmap, PMG_ERR_INVALID_ARG, "subdomain dofmap is NULL");
+  PMG_REQUIRE(n_physical >= 0 && (physical || n_physical == 0), PMG_ERR_INVALID_ARG,
+              "bad physical boundary DoF list");
+  PMG_REQUIRE(n_interface >= 0 && (interface_cells || n_interface == 0), PMG_ERR_INVALID_ARG,
+              "bad interface cell list");
+  const int nm = degree_ + 1;
+  const int64_t nm3 = int64_t(nm) * nm * nm;
+  std::vector<uint32_t> perm(static_cast<size_t>(n_cells_));
+  if (n_cells_ > 0)
+    PMG_HIP_CHECK(hipMemcpy(perm.data(), perm_.ptr, perm.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost));
+  std::vector<uint8_t> pmask(static_cast<size_t>(n_dofs_), 0);
+  for (int64_t i = 0; i < n_physical; ++i) {
+    PMG_REQUIRE(int64_t(physical[i]) < n_dofs_, PMG_ERR_INVALID_ARG,
+                "physical boundary DoF out of range");
+    pmask[physical[i]] = 1;
+  }
+  // the plain indices (setup of plain_dof_indices_per_color,
+  // portable_subdomain_laplace_operator.h:1056-1061), in device cell order
+  std::vector<uint32_t> smap(static_cast<size_t>(n_cells_ * nm3));
+  for (int64_t c = 0; c < n_cells_; ++c)
+    for (int64_t i = 0; i < nm3; ++i) {
+      const uint32_t d = dofmap[int64_t(perm[c]) * nm3 + i];
+      PMG_REQUIRE(d == kInvalid || int64_t(d) < n_dofs_, PMG_ERR_INVALID_ARG,
+                  "subdomain dofmap entry out of range");
+      smap[c * nm3 + i] = (d != kInvalid && pmask[d]) ? kInvalid : d;
+    }
+  // interface cells: caller index -> device position, ascending
+  std::vector<uint32_t> where(static_cast<size_t>(n_cells_));
+  for (int64_t c = 0; c < n_cells_; ++c) where[perm[c]] = static_cast<uint32_t>(c);
+  std::vector<uint32_t> ids(static_cast<size_t>(n_interface));
+  for (int64_t i = 0; i < n_interface; ++i) {
+    PMG_REQUIRE(int64_t(interface_cells[i]) < n_cells_, PMG_ERR_INVALID_ARG,
+                "interface cell out of range");
+    ids[i] = where[interface_cells[i]];
+  }
+  std::sort(ids.begin(), ids.end());
+  PMG_REQUIRE(std::adjacent_find(ids.begin(), ids.end()) == ids.end(), PMG_ERR_INVALID_ARG,
+              "interface cell listed twice");
+  smap_.upload(smap);
+  slines_ok_ = compress_lines(smap, n_cells_, nm, n_dofs_, slines_, sflag_);
+  phys_.upload(physical, static_cast<size_t>(n_physical));
+  icells_.upload(ids);
+  n_phys_ = n_physical;
+  n_icells_ = n_interface;
+  has_subdomain_ = true;
+}
+
+template <typename T>
+void LaplaceOperatorBK3::vmult_subdomain_t(T *dst, const T *src, bool interface_only,
+                                           hipStream_t s) const {
+  const Shape1D<T> &sh = *reinterpret_cast<const Shape1D<T> *>(
+      std::is_same<T, double>::value ? static_cast<const void *>(&sh64_) : static_cast<const void *>(&sh32_));
+  launch_fill<T>(n_dofs_, dst, T(0), s);
+  launch_bk3_cell_list<T>(degree_, nq_, src, dst, static_cast<const T *>(G_.data()), smap_.ptr,
+                          slines_ok_ ? slines_.ptr : nullptr, slines_ok_ ? sflag_.ptr : nullptr,
+                          interface_only ? icells_.ptr : nullptr,
+                          interface_only ? n_icells_ : n_cells_, sh, s);
+  launch_copy_indexed<T>(n_phys_, phys_.ptr, dst, src, s);
+}
+
+// vmult_neumann / vmult_interface_cell_range
+// (portable_subdomain_laplace_operator.h:384-454, :456-533)
+void LaplaceOperatorBK3::vmult_subdomain(void *dst, const void *src, bool interface_only,
+                                         hipStream_t s) const {
+  PMG_REQUIRE(has_subdomain_, PMG_ERR_INVALID_ARG,
+              "no subdomain index set: call pmg_operator_set_subdomain first");
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  PMG_REQUIRE(dst != src, PMG_ERR_INVALID_ARG, "vmult: dst and src must not alias");
+  if (precision_ == PMG_FP64)
+    vmult_subdomain_t(static_cast<double *>(dst), static_cast<const double *>(src), interface_only, s);
+  else
+    vmult_subdomain_t(static_cast<float *>(dst), static_cast<const float *>(src), interface_only, s);
 }
 
 bool LaplaceOperatorBK3::supports_prepared_vmult() const {

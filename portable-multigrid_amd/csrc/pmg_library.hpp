@@ -110,9 +110,23 @@ class LaplaceOperatorBK3 : public Operator {
   // initial guess; r,z,p,q work vectors of m() entries; info = 3 device doubles
   void coarse_pcg(const void *b, void *x, void *r, void *z, void *p, void *q, double rel_tol,
                   int max_it, double *info, hipStream_t s) const;
+  // Domain-decomposition subdomain operator (SubdomainLaplaceOperator,
+  // include/operators/portable_subdomain_laplace_operator.h): this operator,
+  // built with every subdomain constraint, is vmult_bk3; set_subdomain adds
+  // the plain index set (dofmap in caller cell order, only the physical
+  // boundary DoFs eliminated) and the interface cells (caller indices).
+  void set_subdomain(const uint32_t *dofmap, const uint32_t *physical, int64_t n_physical,
+                     const uint32_t *interface_cells, int64_t n_interface);
+  bool has_subdomain() const { return has_subdomain_; }
+  int64_t n_interface_cells() const { return n_icells_; }
+  // dst = A_plain src over every cell (vmult_neumann) or over the interface
+  // cells (vmult_interface_cell_range); dst = src on the physical DoFs
+  void vmult_subdomain(void *dst, const void *src, bool interface_only, hipStream_t s) const;
 
  private:
   template <typename T> void vmult_t(T *dst, const T *src, hipStream_t s) const;
+  template <typename T>
+  void vmult_subdomain_t(T *dst, const T *src, bool interface_only, hipStream_t s) const;
   template <typename T> void compute_diagonal_t();
   template <typename T> void fill_geometry_t(const double *kappa, hipStream_t s);
   void build_coloring();
@@ -137,6 +151,12 @@ class LaplaceOperatorBK3 : public Operator {
   DeviceArray<uint16_t> pline_;  // compressed patch indices (x-line slot starts)
   DeviceArray<uint8_t> pflag_;
   Bk3Schedule sched_;
+  // subdomain index set: plain map (physical DoFs eliminated), its
+  // compressed lines, the physical DoFs and the interface cells (positions)
+  DeviceArray<uint32_t> smap_, slines_, phys_, icells_;
+  DeviceArray<uint8_t> sflag_;
+  int64_t n_phys_ = 0, n_icells_ = 0;
+  bool has_subdomain_ = false, slines_ok_ = false;
 
  public:
   // kAtomic (one launch, device FP atomics per cell), kColored (one launch per

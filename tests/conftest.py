@@ -52,3 +52,20 @@ def rel_err(a, b):
 def seeded_uniform(n, seed, lo=-1.0, hi=1.0):
     """U[lo,hi] from numpy's PCG64 with a fixed seed."""
     return np.random.Generator(np.random.PCG64(seed)).uniform(lo, hi, n)
+
+
+def subdomain_sets(nx, ny, nz, p):
+    """A box as one subdomain whose x = hi face is the interface to its
+    neighbour (lexicographic Q_p numbering, x fastest).  Returns the physical
+    boundary DoFs (on any other face, edges shared with the interface
+    included), the interface-only DoFs, and the interface cells (a face on
+    x = hi), as SubdomainDoFHandler classifies them."""
+    n1 = np.array([p * nx + 1, p * ny + 1, p * nz + 1])
+    d = np.arange(int(n1.prod()), dtype=np.int64)
+    ix, iy, iz = d % n1[0], (d // n1[0]) % n1[1], d // (n1[0] * n1[1])
+    physical = (ix == 0) | (iy == 0) | (iy == n1[1] - 1) | (iz == 0) | (iz == n1[2] - 1)
+    interface = (ix == n1[0] - 1) & ~physical
+    c = np.arange(nx * ny * nz, dtype=np.int64)
+    cells = c[c % nx == nx - 1]
+    return (d[physical].astype(np.uint32), d[interface].astype(np.uint32),
+            cells.astype(np.uint32))

@@ -156,3 +156,43 @@ def test_gradient_matches_finite_difference_of_energy(orc):
     op.G = orc.scale_G(G0, kap + dk, 3)
     a1 = u @ op.vmult(lam)
     assert abs((g * dk / kap).sum() + (a1 - a0)) < 1e-12 * abs(a1 - a0) + 1e-15
+
+
+@pytest.mark.parametrize("p,nq,deform", [(2, 3, 1), (3, 5, 0)])
+def test_subdomain_vmults_equal_dense_assembly(orc, p, nq, deform):
+    """vmult_neumann and vmult_interface_cell_range restated against a dense
+    assembly of exact element matrices over the same cells, with the physical
+    boundary rows and columns replaced by the identity."""
+    from conftest import subdomain_sets
+    m = orc.mesh(3, deform=deform, amplitude=0.06)
+    op = orc.LaplaceBK3(m, p, nq)
+    K = op.element_matrices()
+    N, nm3 = op.N, (p + 1) ** 3
+    phys, iface, icells = subdomain_sets(3, 3, 3, p)
+    assert len(icells) == 9 and len(iface) == (3 * p - 1) ** 2
+
+    def dense(cells):
+        A = np.zeros((N, N))
+        for c in cells:
+            idx = op.plain_map[c * nm3:(c + 1) * nm3]
+            A[np.ix_(idx, idx)] += K[c]
+        A[phys, :] = 0.0
+        A[:, phys] = 0.0
+        A[phys, phys] = 1.0
+        return A
+
+    x = seeded_uniform(N, 11)
+    y_neu = op.vmult_subdomain(x, phys)
+    assert rel_err(y_neu, dense(range(op.C)) @ x) < 1e-13
+    y_int = op.vmult_subdomain(x, phys, icells)
+    assert rel_err(y_int, dense(icells) @ x) < 1e-13
+    # interface cells + the other cells = every cell, on the free rows
+    others = np.setdiff1d(np.arange(op.C), icells).astype(np.uint32)
+    y_oth = op.vmult_subdomain(x, phys, others)
+    free = np.setdiff1d(np.arange(N), phys)
+    assert rel_err((y_int + y_oth)[free], y_neu[free]) < 1e-13
+    # with every boundary DoF physical it is the operator's vmult
+    assert rel_err(op.vmult_subdomain(x, op.cons), op.vmult(x)) < 1e-14
+    # without physical DoFs constants are in its kernel
+    zero = op.vmult_subdomain(np.ones(N), np.zeros(0, np.uint32))
+    assert np.abs(zero).max() < 1e-12 * np.abs(y_neu).max()
