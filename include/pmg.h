@@ -296,9 +296,12 @@ int pmg_chebyshev_info(pmg_smoother sm, double *max_eigenvalue, double *min_eige
  * MultigridSolver::vmult (tests/common/portable_multigrid_solver.h:171-271).
  * Level l = 0 is the coarsest.  transfers[l] maps level l-1 -> l
  * (transfers[0] ignored), smoothers[l] for l >= 1.  All levels share one
- * precision (the V-cycle precision); outer_precision is the precision of the
- * vectors passed to pmg_multigrid_vmult (FP64 outer with FP32 levels is the
- * mixed-precision preconditioner; conversion happens at the V-cycle boundary).
+ * precision (the V-cycle precision), except that the coarsest operator may be
+ * FP64 under FP32 levels: the coarse-grid solve then runs in FP64 on its own
+ * vectors, converted from and to FP32 around it (transfers[1] stays FP32).
+ * outer_precision is the precision of the vectors passed to
+ * pmg_multigrid_vmult (FP64 outer with FP32 levels is the mixed-precision
+ * preconditioner; conversion happens at the V-cycle boundary).
  * coarse: PMG_COARSE_PCG (Jacobi-PCG to coarse_tol relative) or
  * PMG_COARSE_CHEBYSHEV (Chebyshev auto-degree to 1e-3, the reference). */
 int pmg_multigrid_create(int n_levels, const pmg_operator *level_operators,

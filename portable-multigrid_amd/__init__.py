@@ -594,7 +594,7 @@ def vector_dot(a, b, precision=FP64):
 def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=3,
                     smoothing_range=15.0, eig_cg_n_iterations=10, coarse=COARSE_PCG,
                     coarse_tol=1e-12, prolongation_scale=1.0, outer_precision=None,
-                    dg_penalty_kind=0, dg_metric=0, coefficient=None):
+                    dg_penalty_kind=0, dg_metric=0, coefficient=None, coarse_precision=None):
     """V-cycle over `levels` = [(box, degree) or (box, degree, "dg"), ...]
     from coarsest to finest.
 
@@ -606,7 +606,13 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
     n_q_points_1d: None -> degree+1 per level, or a callable degree -> nq.
     coefficient: None, or a callable (box, n_q_points_1d) -> CUDA float64
     tensor kappa [n_cells * nq^3] applied to each continuous level before its
-    diagonal and smoother are set up."""
+    diagonal and smoother are set up.
+    coarse_precision: the coarsest operator's precision.  None: FP64 when FP32
+    levels precondition an FP64 outer solve (the coarse-grid solve stays in
+    FP64, like the diagonal inversion and the outer CG), else `precision`."""
+    if coarse_precision is None:
+        mixed = precision == FP32 and outer_precision == FP64 and len(levels) > 1
+        coarse_precision = FP64 if mixed else precision
     ops, trs, sms = [], [None], [None]
     for l, lev in enumerate(levels):
         b, p = lev[0], lev[1]
@@ -616,7 +622,7 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
             op = LaplaceOperatorDG(b, p, nq, precision, penalty_kind=dg_penalty_kind,
                                    metric=dg_metric)
         else:
-            op = LaplaceOperatorBK3(b, p, nq, precision)
+            op = LaplaceOperatorBK3(b, p, nq, coarse_precision if l == 0 else precision)
             if coefficient is not None:
                 op.set_coefficient(coefficient(b, op.nq))
         op.compute_diagonal()
@@ -766,7 +772,7 @@ def coarsen_boxes(fine_box, n_levels):
 def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precision=FP64,
                         smoother_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
                         coarse=COARSE_PCG, coarse_tol=1e-12, prolongation_scale=1.0,
-                        outer_precision=None, coefficient=None):
+                        outer_precision=None, coefficient=None, coarse_precision=None):
     """h-only hierarchy at fixed degree (source/geometric_multigrid_bk3/program.cc):
     level 0 has fine_box coarsened n_levels-1 times."""
     boxes = coarsen_boxes(fine_box, n_levels)
@@ -774,5 +780,5 @@ def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precisio
     mg, ops, trs, sms = build_hierarchy([(b, degree) for b in boxes], nqf, precision,
                                         smoother_degree, smoothing_range, eig_cg_n_iterations,
                                         coarse, coarse_tol, prolongation_scale, outer_precision,
-                                        coefficient=coefficient)
+                                        coefficient=coefficient, coarse_precision=coarse_precision)
     return mg, ops, trs, sms, boxes

@@ -689,10 +689,11 @@ template void launch_geometry<float>(int, int64_t, const double *, const double 
 // products) instead of a quadrature loop per DoF.  One wave per cell, double
 // accumulation.  Setup only (compute_diagonal,
 // portable_laplace_operator_bk3.h:533-561).
-template <typename T>
+template <typename TG>
 __global__ void __launch_bounds__(64)
-bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const T *__restrict__ G,
-                    const uint32_t *__restrict__ map, const Shape1D<T> sh, T *__restrict__ diag)
+bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const TG *__restrict__ G,
+                    const uint32_t *__restrict__ map, const Shape1D<double> sh,
+                    double *__restrict__ diag)
 {
   __shared__ double f[3][kMaxNm * kMaxNq];  // S^2, D^2, DS
   __shared__ double g[kMaxNq * kMaxNq * kMaxNq];
@@ -741,13 +742,13 @@ bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const T *
   }
   for (int o = lane; o < nm3; o += 64) {  // the same lane wrote acc[o]
     const uint32_t dof = map[cell * nm3 + o];
-    if (dof != kInvalid) atomic_add(diag + dof, static_cast<T>(acc[o]));
+    if (dof != kInvalid) atomic_add(diag + dof, acc[o]);
   }
 }
 
 template <typename T>
 void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
-                         const Shape1D<T> &S_and_Sd, T *diag, hipStream_t s) {
+                         const Shape1D<double> &S_and_Sd, double *diag, hipStream_t s) {
   const int nm = degree + 1;
   if (n_cells <= 0) return;
   PMG_REQUIRE(nm <= kMaxNm && nq <= kMaxNq, PMG_ERR_UNSUPPORTED, "diagonal: degree or nq too large");
@@ -759,7 +760,7 @@ void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const 
 template void launch_bk3_diagonal<double>(int, int, int64_t, const double *, const uint32_t *,
                                           const Shape1D<double> &, double *, hipStream_t);
 template void launch_bk3_diagonal<float>(int, int, int64_t, const float *, const uint32_t *,
-                                         const Shape1D<float> &, float *, hipStream_t);
+                                         const Shape1D<double> &, double *, hipStream_t);
 
 // ---------------------------------------------------------------- adjoint gradient
 // One workgroup per cell, one lane per quadrature point column (p,q); reference

@@ -134,7 +134,6 @@ template <typename T>
 void launch_geometry(int nq, int64_t n_cells, const double *cell_vertices, const double *qpts,
                      const double *qwts, const double *kappa, const uint32_t *perm, T *G,
                      hipStream_t s);
-// diag[map[c][i]] += (A_e)_ii
 // y += sum A_e x over the cells cell_ids[0 .. n_list) (device positions), or
 // over cells 0 .. n_list when cell_ids is null: the atomic scatter with late
 // G loads; lines/lflag optional compressed indices of map (Bk3Schedule)
@@ -142,9 +141,11 @@ template <typename T>
 void launch_bk3_cell_list(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
                           const uint32_t *lines, const uint8_t *lflag, const uint32_t *cell_ids,
                           int64_t n_list, const Shape1D<T> &sh, hipStream_t s);
+// diag[map[c][i]] += (A_e)_ii, accumulated in FP64 from G of either precision
+// (S_and_Sd: shape values and exact derivatives)
 template <typename T>
 void launch_bk3_diagonal(int degree, int nq, int64_t n_cells, const T *G, const uint32_t *map,
-                         const Shape1D<T> &S_and_Sd, T *diag, hipStream_t s);
+                         const Shape1D<double> &S_and_Sd, double *diag, hipStream_t s);
 // out[perm[c]][q] = -(grad u)^T G_q (grad lambda)
 template <typename T>
 void launch_bk3_grad_coefficient(int degree, int nq, int64_t n_cells, const T *G,

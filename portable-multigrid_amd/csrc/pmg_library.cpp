@@ -133,7 +133,6 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
   fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
   shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
-  shd32_ = to_shape<float>(shape_.S, shape_.Sd);
   // Internal cell order.  Default: the caller's order (on lexicographic
   // structured meshes a batch is a run of cells along x, whose DoF rows are
   // long and contiguous -- measured fastest for the atomic scatter).
@@ -482,20 +481,29 @@ void LaplaceOperatorBK3::apply_add(void *dst, const void *src, hipStream_t s) co
                             static_cast<const float *>(G_.data()), map_.ptr, n_cells_, sh32_, s, &sched_);
 }
 
+// The diagonal is assembled and inverted in FP64 at every level precision
+// (an FP32 level's G, FP64 arithmetic), then stored in the level precision.
 template <typename T>
 void LaplaceOperatorBK3::compute_diagonal_t() {
-  const Shape1D<T> &shd = *reinterpret_cast<const Shape1D<T> *>(
-      std::is_same<T, double>::value ? static_cast<const void *>(&shd64_) : static_cast<const void *>(&shd32_));
-  T *d = static_cast<T *>(inv_diag_.data());
-  launch_fill<T>(n_dofs_, d, T(0), nullptr);
-  launch_bk3_diagonal<T>(degree_, nq_, n_cells_, static_cast<const T *>(G_.data()), map_.ptr, shd,
-                         d, nullptr);
-  launch_set_indexed<T>(n_constrained_, constrained_.ptr, d, T(1), nullptr);
+  DeviceArray<double> tmp;
+  double *d = nullptr;
+  if (std::is_same<T, double>::value) {
+    d = static_cast<double *>(inv_diag_.data());
+  } else {
+    tmp.resize(static_cast<size_t>(n_dofs_));
+    d = tmp.ptr;
+  }
+  launch_fill<double>(n_dofs_, d, 0.0, nullptr);
+  launch_bk3_diagonal<T>(degree_, nq_, n_cells_, static_cast<const T *>(G_.data()), map_.ptr,
+                         shd64_, d, nullptr);
+  launch_set_indexed<double>(n_constrained_, constrained_.ptr, d, 1.0, nullptr);
   // DoFs touched by no cell (possible only with exotic maps) would divide by 0
-  const int64_t bad = first_nonpositive<T>(n_dofs_, d, nullptr);
+  const int64_t bad = first_nonpositive<double>(n_dofs_, d, nullptr);
   PMG_REQUIRE(bad < 0, PMG_ERR_INVALID_ARG,
               "non-positive diagonal entry at DoF " + std::to_string(bad));
-  launch_invert<T>(n_dofs_, d, nullptr);
+  launch_invert<double>(n_dofs_, d, nullptr);
+  if (!std::is_same<T, double>::value)
+    launch_convert<float, double>(n_dofs_, static_cast<float *>(inv_diag_.data()), d, nullptr);
   PMG_HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -1141,10 +1149,15 @@ VCycleMultigrid::VCycleMultigrid(std::vector<const Operator *> ops,
   check_precision(outer_precision);
   PMG_REQUIRE(coarse_kind == PMG_COARSE_PCG || coarse_kind == PMG_COARSE_CHEBYSHEV,
               PMG_ERR_INVALID_ARG, "unknown coarse solver");
-  precision_ = ops_[0]->precision();
+  for (int l = 0; l < L; ++l)
+    PMG_REQUIRE(ops_[l], PMG_ERR_INVALID_ARG, "missing level operator");
+  // one level precision; the coarsest operator may be FP64 under FP32 levels
+  // (the coarse-grid solve kept in FP64)
+  precision_ = ops_[L - 1]->precision();
+  coarse64_ = L >= 2 && precision_ == PMG_FP32 && ops_[0]->precision() == PMG_FP64;
   for (int l = 0; l < L; ++l) {
-    PMG_REQUIRE(ops_[l] && ops_[l]->precision() == precision_, PMG_ERR_INVALID_ARG,
-                "all level operators must exist and share one precision");
+    PMG_REQUIRE(ops_[l]->precision() == precision_ || (l == 0 && coarse64_), PMG_ERR_INVALID_ARG,
+                "level operators must share one precision (the coarsest may be FP64)");
     PMG_REQUIRE(ops_[l]->has_diagonal(), PMG_ERR_INVALID_ARG,
                 "level operators need compute_diagonal()");
     if (l > 0) {
@@ -1168,7 +1181,11 @@ VCycleMultigrid::VCycleMultigrid(std::vector<const Operator *> ops,
     def_[l].reinit(precision_, ops_[l]->m());
     t_[l].reinit(precision_, ops_[l]->m());
   }
-  coarse_work_.reinit(precision_, ops_[0]->m());
+  coarse_work_.reinit(ops_[0]->precision(), ops_[0]->m());
+  if (coarse64_) {
+    c64_def_.reinit(PMG_FP64, ops_[0]->m());
+    c64_sol_.reinit(PMG_FP64, ops_[0]->m());
+  }
   // The V-cycle is a fixed launch sequence on device buffers of its own; it is
   // captured once into a HIP graph and replayed, unless the coarse solve needs
   // host round trips (CG on a large coarse level).  PMG_VCYCLE_GRAPH=0 turns
@@ -1230,24 +1247,33 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
   const bool f64 = precision_ == PMG_FP64;
   const int64_t n = ops_[l]->m();
   if (l == 0) {
+    const bool c64 = ops_[0]->precision() == PMG_FP64;
+    void *cdef = coarse64_ ? c64_def_.data() : def_[0].data();
+    void *csol = coarse64_ ? c64_sol_.data() : sol_[0].data();
+    if (coarse64_)
+      launch_convert<double, float>(n, static_cast<double *>(cdef),
+                                    static_cast<const float *>(def_[0].data()), s);
     if (coarse_kind_ == PMG_COARSE_PCG) {
-      if (f64) launch_fill<double>(n, static_cast<double *>(sol_[0].data()), 0.0, s);
-      else launch_fill<float>(n, static_cast<float *>(sol_[0].data()), 0.0f, s);
+      if (c64) launch_fill<double>(n, static_cast<double *>(csol), 0.0, s);
+      else launch_fill<float>(n, static_cast<float *>(csol), 0.0f, s);
       // FP32 arithmetic cannot reduce the residual below ~1e-7 relative
-      const double tol = f64 ? coarse_tol_ : std::max(coarse_tol_, 1e-6);
+      const double tol = c64 ? coarse_tol_ : std::max(coarse_tol_, 1e-6);
       const int max_it = static_cast<int>(std::min<int64_t>(100000, 4 * n + 100));
       const auto *bk3 = dynamic_cast<const LaplaceOperatorBK3 *>(ops_[0]);
       if (bk3 && bk3->n_cells() <= kCoarseSingleWorkgroupCells) {
         CGWork &w = coarse_work_;
-        bk3->coarse_pcg(def_[0].data(), sol_[0].data(), w.r.data(), w.z.data(), w.p.data(),
-                            w.q.data(), tol, max_it, w.sc.ptr + 4, s);
+        bk3->coarse_pcg(cdef, csol, w.r.data(), w.z.data(), w.p.data(), w.q.data(), tol, max_it,
+                        w.sc.ptr + 4, s);
       } else {
-        solve_cg(*ops_[0], PMG_PRECOND_JACOBI, nullptr, def_[0].data(), sol_[0].data(), tol,
-                 max_it, coarse_work_, s, 4);
+        solve_cg(*ops_[0], PMG_PRECOND_JACOBI, nullptr, cdef, csol, tol, max_it, coarse_work_, s,
+                 4);
       }
     } else {
-      coarse_cheby_->vmult(sol_[0].data(), def_[0].data(), s);
+      coarse_cheby_->vmult(csol, cdef, s);
     }
+    if (coarse64_)
+      launch_convert<float, double>(n, static_cast<float *>(sol_[0].data()),
+                                    static_cast<const double *>(csol), s);
     return;
   }
   // pre-smoothing; its last update prepares t for the residual's matvec

@@ -136,7 +136,7 @@ class LaplaceOperatorBK3 : public Operator {
   int64_t n_cells_, n_dofs_, n_constrained_;
   ShapeData shape_;
   Shape1D<double> sh64_{}, shd64_{};
-  Shape1D<float> sh32_{}, shd32_{};
+  Shape1D<float> sh32_{};
   DeviceArray<uint32_t> map_, constrained_, perm_;
   DeviceArray<double> vertices_, qpts_, qwts_;
   DeviceVector G_, inv_diag_;
@@ -209,7 +209,7 @@ class LaplaceOperatorDG : public Operator {
   int64_t n_cells_, nm3_, n_faces_ = 0;
   ShapeData shape_;
   Shape1D<double> sh64_{}, shd64_{};
-  Shape1D<float> sh32_{}, shd32_{};
+  Shape1D<float> sh32_{};
   DgShape<double> fs64_{};
   DgShape<float> fs32_{};
   std::vector<double> sigma_;
@@ -337,6 +337,10 @@ class VCycleMultigrid {
   std::unique_ptr<PreconditionChebyshev> coarse_cheby_;
   mutable std::vector<DeviceVector> sol_, def_, t_;
   mutable CGWork coarse_work_;
+  // FP64 coarse level under FP32 levels: the coarse solve's own FP64 defect
+  // and solution, converted from and to the level-0 vectors around it
+  bool coarse64_ = false;
+  mutable DeviceVector c64_def_, c64_sol_;
   mutable bool use_graph_ = false;  // cleared for good if capturing fails
   mutable hipStream_t cap_stream_ = nullptr;
   mutable hipGraph_t graph_ = nullptr;

@@ -166,7 +166,6 @@ LaplaceOperatorDG::LaplaceOperatorDG(int degree, int nq, int precision, int64_t 
   fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
   fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
   shd64_ = make_shape<double>(shape_.S, shape_.Sd);
-  shd32_ = make_shape<float>(shape_.S, shape_.Sd);
   const int nm = degree + 1;
   const std::vector<double> gll = gauss_lobatto_points(nm);
   std::vector<double> val(nm), d0(nm), d1(nm);
@@ -261,19 +260,23 @@ void LaplaceOperatorDG::compute_diagonal() {
   const int64_t n = m();
   inv_diag_.reinit(precision_, n);
   const uint32_t *map = identity_map();
+  // assembled and inverted in FP64 at both level precisions, stored in the
+  // level precision
   auto run = [&](auto zero) {
     using T = decltype(zero);
     const bool f64 = std::is_same<T, double>::value;
-    const Shape1D<T> &shd = *reinterpret_cast<const Shape1D<T> *>(
-        f64 ? static_cast<const void *>(&shd64_) : static_cast<const void *>(&shd32_));
-    T *d = static_cast<T *>(inv_diag_.data());
-    launch_fill<T>(n, d, T(0), nullptr);
-    launch_bk3_diagonal<T>(degree_, nq_, n_cells_, static_cast<const T *>(G_.data()), map, shd, d,
-                           nullptr);
-    const int64_t bad = first_nonpositive<T>(n, d, nullptr);
+    DeviceArray<double> tmp;
+    if (!f64) tmp.resize(static_cast<size_t>(n));
+    double *d = f64 ? static_cast<double *>(inv_diag_.data()) : tmp.ptr;
+    launch_fill<double>(n, d, 0.0, nullptr);
+    launch_bk3_diagonal<T>(degree_, nq_, n_cells_, static_cast<const T *>(G_.data()), map, shd64_,
+                           d, nullptr);
+    const int64_t bad = first_nonpositive<double>(n, d, nullptr);
     PMG_REQUIRE(bad < 0, PMG_ERR_INVALID_ARG,
                 "non-positive diagonal entry at DoF " + std::to_string(bad));
-    launch_invert<T>(n, d, nullptr);
+    launch_invert<double>(n, d, nullptr);
+    if (!f64) launch_convert<float, double>(n, static_cast<float *>(inv_diag_.data()), d, nullptr);
+    PMG_HIP_CHECK(hipDeviceSynchronize());  // before tmp is released
   };
   if (precision_ == PMG_FP64) run(0.0);
   else run(0.0f);

@@ -78,7 +78,8 @@ int main(int argc, char **argv) {
       const Level &L = levels[l];
       const pmg_box b = make_box(L.cells);
       if (L.dg) ops.emplace_back(new LaplaceOperatorDG(b, L.degree, L.degree + 1, lp));
-      else ops.emplace_back(new LaplaceOperatorBK3(b, L.degree, L.degree + 1, lp));
+      // the coarse-grid solve stays FP64 under FP32 levels
+      else ops.emplace_back(new LaplaceOperatorBK3(b, L.degree, L.degree + 1, l == 0 ? PMG_FP64 : lp));
       ops.back()->compute_diagonal();
       if (l == 0) continue;
       const Level &C = levels[l - 1];

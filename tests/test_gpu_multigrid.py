@@ -228,6 +228,40 @@ def test_vcycle_and_mgcg_match_oracle(pmg, orc, levels_spec):
     assert rel_err(x32.cpu().numpy(), x.cpu().numpy()) < 1e-8
 
 
+def test_mixed_vcycle_keeps_coarse_solve_and_diagonal_in_fp64(pmg, orc):
+    """FP32 levels under an FP64 outer solve: the coarsest operator is FP64
+    (the coarse-grid solve runs in FP64), the FP32 levels' inverse diagonals
+    are assembled and inverted in FP64 and rounded once, and the V-cycle
+    agrees with the FP64 oracle to FP32 accuracy; an FP32 coarse level asked
+    for explicitly still works."""
+    import torch
+    spec = [(2, 2), (4, 2), (8, 2)]
+    print(f"[mg mixed coarse] {spec}", flush=True)
+    boxes = [pmg.box(n, deform=1, amplitude=0.05) for n, _ in spec]
+    levels = list(zip(boxes, [p for _, p in spec]))
+    mg, ops, _, _ = pmg.build_hierarchy(levels, precision=pmg.FP32, outer_precision=pmg.FP64)
+    assert ops[0].precision == pmg.FP64 and all(o.precision == pmg.FP32 for o in ops[1:])
+    op64 = pmg.LaplaceOperatorBK3(boxes[-1], 2, None, pmg.FP64)
+    op64.compute_diagonal()
+    inv64 = op64.get_matrix_diagonal_inverse().cpu().numpy()
+    inv32 = ops[-1].get_matrix_diagonal_inverse().cpu().numpy().astype(np.float64)
+    assert rel_err(inv32, inv64) < 3e-7
+    ref = orc.Multigrid([orc.mesh(n, deform=1, amplitude=0.05) for n, _ in spec],
+                        [p for _, p in spec], coarse_kind=1, coarse_tol=1e-12)
+    r = seeded_uniform(ops[-1].m(), 73)
+    r[orc.boundary_dofs(orc.mesh(spec[-1][0]), spec[-1][1])] = 0
+    zr = ref.vmult(r)
+    z = torch.zeros(len(r), dtype=torch.float64, device="cuda")
+    mg.vmult(z, _t(r))
+    assert rel_err(z.cpu().numpy(), zr) < 1e-4
+    mg_c, ops_c, _, _ = pmg.build_hierarchy(levels, precision=pmg.FP32, outer_precision=pmg.FP64,
+                                            coarse_precision=pmg.FP32)
+    assert ops_c[0].precision == pmg.FP32
+    z.zero_()
+    mg_c.vmult(z, _t(r))
+    assert rel_err(z.cpu().numpy(), zr) < 1e-4
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_vcycle_with_gathered_restriction_matches_oracle(pmg, orc, monkeypatch, prec):
     """The V-cycle with every restriction through the cell buffer and the
