@@ -27,7 +27,9 @@ Also reported on the same problem:
                 vmult + Jacobi-PCG to 1e-8; SIP-DG Q3 on 32^3 deformed cells
                 (vmult roofline, DG MG-CG FP64 and mixed); ph-MG Q6 on 48^3
                 deformed cells (FP64 vs FP32 V-cycle); adjoint gradient 32^3 Q4
-                with a 20-direction finite-difference check.
+                with a 20-direction finite-difference check; the primary
+                MG-CG at 128^3 Q4 (135M DoF), FP64 vs FP32 V-cycle; the
+                subdomain operator's vmults (in rows).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
                        [--no-configs]
@@ -473,6 +475,24 @@ def bench_rows(pmg):
     return rows
 
 
+def bench_mgcg_128(pmg):
+    """The 64^3 Q4 MG-CG of the primary line at 8x the size: 128^3 Q4,
+    135 005 697 DoF, 7 h-levels, FP64 and FP32-V-cycle / FP64-CG to 1e-10
+    (the reference logs its mixed speed-up at 17M and 67M DoF)."""
+    import torch
+    b = pmg.box(128)
+    op = pmg.LaplaceOperatorBK3(b, 4, 5, pmg.FP64)
+    rhs = op.assemble_rhs(1)
+    r64, x64 = mg_solve(pmg, b, 4, 7, pmg.FP64, op, rhs, 1e-10)
+    log(f"[bench] 128^3 FP64 MG-CG: {r64}")
+    r32, x32 = mg_solve(pmg, b, 4, 7, pmg.FP32, op, rhs, 1e-10)
+    r32["rel_diff_vs_fp64_solution"] = ((x32 - x64).abs().max() / x64.abs().max()).item()
+    del x64, x32, rhs, op
+    torch.cuda.empty_cache()
+    return {"dofs": 135005697, "fp64": r64, "fp32_vcycle_fp64_cg": r32,
+            "speedup_mixed": r64["time_s"] / r32["time_s"], "levels": 7}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -489,10 +509,12 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the other configurations (Jacobi-PCG 16^3, SIP-DG, ph-MG Q6, adjoint)")
+                    help="skip the other configurations (Jacobi-PCG 16^3, SIP-DG, ph-MG Q6, adjoint, "
+                         "MG-CG 128^3)")
     ap.add_argument("--config", action="append", default=None,
                     help="run only these configuration legs (repeatable): rows, bk3_q4_16cube_jacobi_pcg, "
-                         "sipdg_q3_32cube_deformed, ph_mg_q6_48cube_deformed, adjoint_q4_32cube")
+                         "sipdg_q3_32cube_deformed, ph_mg_q6_48cube_deformed, adjoint_q4_32cube, "
+                         "mgcg_q4_128cube")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -589,7 +611,8 @@ def main():
                          ("bk3_q4_16cube_jacobi_pcg", bench_config1),
                          ("sipdg_q3_32cube_deformed", bench_config3),
                          ("ph_mg_q6_48cube_deformed", bench_config4),
-                         ("adjoint_q4_32cube", bench_config5)]:
+                         ("adjoint_q4_32cube", bench_config5),
+                         ("mgcg_q4_128cube", bench_mgcg_128)]:
             if args.config and name not in args.config:
                 continue
             log(f"[bench] configuration {name}")
