@@ -292,30 +292,29 @@ void LaplaceOperatorBK3::build_coloring() {
   }
 }
 
-// DoF patches of the batches (patch schedule), built on first use from the
-// device map.
-void LaplaceOperatorBK3::build_patches_once() {
-  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
-  const int E = g_batch_cells(nq_);
-  if (n_cells_ == 0 || n_dofs_ >= (int64_t(1) << 31) || E * nm3 >= 0xFFFF) return;
-  std::vector<uint32_t> imap(static_cast<size_t>(n_cells_ * nm3));
-  PMG_HIP_CHECK(hipMemcpy(imap.data(), map_.ptr, imap.size() * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost));
-  const CellPatches P = build_patches(n_cells_, nm3, imap.data(), n_dofs_, E);
-  pdof_.upload(P.pdof);
-  pcount_.upload(P.pcount);
-  pidx_.upload(P.pidx);
-  sched_.pdof = pdof_.ptr;
-  sched_.pcount = pcount_.ptr;
-  sched_.pidx = pidx_.ptr;
+// DoF patches of the batches (patch schedule) of a map in device cell order:
+// uploaded into pa, and sc points at them; sc is left without patches when the
+// patch slots would not fit 16 bits.
+static void build_patch_schedule(const std::vector<uint32_t> &imap, int64_t n_cells, int degree,
+                                 int nq, int64_t n_dofs, PatchArrays &pa, Bk3Schedule &sc) {
+  const int nm = degree + 1;
+  const int64_t nm3 = int64_t(nm) * nm * nm;
+  const int E = g_batch_cells(nq);
+  if (n_cells == 0 || n_dofs >= (int64_t(1) << 31) || E * nm3 >= 0xFFFF) return;
+  const CellPatches P = build_patches(n_cells, nm3, imap.data(), n_dofs, E);
+  pa.pdof.upload(P.pdof);
+  pa.pcount.upload(P.pcount);
+  pa.pidx.upload(P.pidx);
+  sc.pdof = pa.pdof.ptr;
+  sc.pcount = pa.pcount.ptr;
+  sc.pidx = pa.pidx.ptr;
   // x-lines whose patch slots are consecutive need only their first slot
   // (nm^2 instead of nm^3 16-bit loads per cell); used when at least half of
   // the cells qualify
-  const int nm = degree_ + 1;
-  std::vector<uint16_t> pline(static_cast<size_t>(n_cells_ * nm * nm), 0);
-  std::vector<uint8_t> pflag(static_cast<size_t>(n_cells_), 0);
+  std::vector<uint16_t> pline(static_cast<size_t>(n_cells * nm * nm), 0);
+  std::vector<uint8_t> pflag(static_cast<size_t>(n_cells), 0);
   int64_t n_comp = 0;
-  for (int64_t c = 0; c < n_cells_; ++c) {
+  for (int64_t c = 0; c < n_cells; ++c) {
     bool ok = true;
     for (int kj = 0; kj < nm * nm && ok; ++kj) {
       const uint16_t *ln = P.pidx.data() + c * nm3 + int64_t(kj) * nm;
@@ -326,12 +325,22 @@ void LaplaceOperatorBK3::build_patches_once() {
     pflag[c] = ok ? 1 : 0;
     n_comp += ok;
   }
-  if (2 * n_comp >= n_cells_) {
-    pline_.upload(pline);
-    pflag_.upload(pflag);
-    sched_.pline = pline_.ptr;
-    sched_.pflag = pflag_.ptr;
+  if (2 * n_comp >= n_cells) {
+    pa.pline.upload(pline);
+    pa.pflag.upload(pflag);
+    sc.pline = pa.pline.ptr;
+    sc.pflag = pa.pflag.ptr;
   }
+}
+
+// The main map's patches, built on first use from the device map.
+void LaplaceOperatorBK3::build_patches_once() {
+  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
+  std::vector<uint32_t> imap(static_cast<size_t>(n_cells_ * nm3));
+  if (!imap.empty())
+    PMG_HIP_CHECK(hipMemcpy(imap.data(), map_.ptr, imap.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost));
+  build_patch_schedule(imap, n_cells_, degree_, nq_, n_dofs_, patches_, sched_);
 }
 
 void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {
@@ -418,7 +427,20 @@ void LaplaceOperatorBK3::set_subdomain(const uint32_t *dofmap, const uint32_t *p
   PMG_REQUIRE(std::adjacent_find(ids.begin(), ids.end()) == ids.end(), PMG_ERR_INVALID_ARG,
               "interface cell listed twice");
   smap_.upload(smap);
-  slines_ok_ = compress_lines(smap, n_cells_, nm, n_dofs_, slines_, sflag_);
+  // its scatter schedule: the main map's kind (patch or atomic), same G loads
+  ssched_ = Bk3Schedule{};
+  if (compress_lines(smap, n_cells_, nm, n_dofs_, slines_, sflag_)) {
+    ssched_.lines = slines_.ptr;
+    ssched_.lflag = sflag_.ptr;
+  }
+  spatches_ = PatchArrays{};
+  if (sched_.kind == kPatch) {
+    build_patch_schedule(smap, n_cells_, degree_, nq_, n_dofs_, spatches_, ssched_);
+    if (ssched_.pdof) {
+      ssched_.kind = kPatch;
+      ssched_.gmode = sched_.gmode;
+    }
+  }
   phys_.upload(physical, static_cast<size_t>(n_physical));
   icells_.upload(ids);
   n_phys_ = n_physical;
@@ -431,11 +453,14 @@ void LaplaceOperatorBK3::vmult_subdomain_t(T *dst, const T *src, bool interface_
                                            hipStream_t s) const {
   const Shape1D<T> &sh = *reinterpret_cast<const Shape1D<T> *>(
       std::is_same<T, double>::value ? static_cast<const void *>(&sh64_) : static_cast<const void *>(&sh32_));
+  const T *G = static_cast<const T *>(G_.data());
   launch_fill<T>(n_dofs_, dst, T(0), s);
-  launch_bk3_cell_list<T>(degree_, nq_, src, dst, static_cast<const T *>(G_.data()), smap_.ptr,
-                          slines_ok_ ? slines_.ptr : nullptr, slines_ok_ ? sflag_.ptr : nullptr,
-                          interface_only ? icells_.ptr : nullptr,
-                          interface_only ? n_icells_ : n_cells_, sh, s);
+  if (interface_only)
+    launch_bk3_cell_list<T>(degree_, nq_, src, dst, G, smap_.ptr, ssched_.lines, ssched_.lflag,
+                            icells_.ptr, n_icells_, sh, s);
+  else
+    launch_bk3_cells<T>(degree_, nq_, src, dst, G, smap_.ptr, n_cells_, sh, s, &ssched_,
+                        /*assign=*/true);
   launch_copy_indexed<T>(n_phys_, phys_.ptr, dst, src, s);
 }
 

@@ -81,6 +81,13 @@ class Operator {
 };
 
 // Continuous Q_p Laplace operator (LaplaceOperatorBK3).
+// Device arrays of a patch schedule (Bk3Schedule::pdof .. pflag)
+struct PatchArrays {
+  DeviceArray<uint32_t> pdof, pcount;
+  DeviceArray<uint16_t> pidx, pline;
+  DeviceArray<uint8_t> pflag;
+};
+
 class LaplaceOperatorBK3 : public Operator {
  public:
   LaplaceOperatorBK3(int degree, int nq, int precision, int64_t n_cells, const uint32_t *dofmap,
@@ -142,21 +149,23 @@ class LaplaceOperatorBK3 : public Operator {
   DeviceVector G_, inv_diag_;
   bool has_diag_ = false;
   // cell colouring for the atomics-free schedule, DoF patches for the patch schedule
-  DeviceArray<uint32_t> cell_ids_, pdof_, pcount_;
-  DeviceArray<uint16_t> pidx_;
+  DeviceArray<uint32_t> cell_ids_;
+  PatchArrays patches_;
   // compressed DoF indices (x-line starts) and their per-cell flag
   DeviceArray<uint32_t> lines_;
   DeviceArray<uint8_t> lflag_;
   DeviceArray<uint8_t> cmask_;  // 1 on constrained DoFs
-  DeviceArray<uint16_t> pline_;  // compressed patch indices (x-line slot starts)
-  DeviceArray<uint8_t> pflag_;
   Bk3Schedule sched_;
-  // subdomain index set: plain map (physical DoFs eliminated), its
-  // compressed lines, the physical DoFs and the interface cells (positions)
+  // subdomain index set: plain map (physical DoFs eliminated) with its own
+  // scatter schedule (compressed lines; patches where the main map's default
+  // is the patch schedule), the physical DoFs and the interface cells
+  // (device positions)
   DeviceArray<uint32_t> smap_, slines_, phys_, icells_;
   DeviceArray<uint8_t> sflag_;
+  PatchArrays spatches_;
+  Bk3Schedule ssched_;
   int64_t n_phys_ = 0, n_icells_ = 0;
-  bool has_subdomain_ = false, slines_ok_ = false;
+  bool has_subdomain_ = false;
 
  public:
   // kAtomic (one launch, device FP atomics per cell), kColored (one launch per

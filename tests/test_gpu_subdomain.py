@@ -120,3 +120,28 @@ def test_subdomain_errors(pmg):
     with pytest.raises(pmg.PMGError, match="alias"):
         op.vmult_interface_cell_range(x, x)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_subdomain_neumann_on_patch_schedule(pmg, orc, prec):
+    """The plain index set gets the main map's scatter schedule: the patch
+    schedule when the operator uses it (forced here, and the FP32 default on
+    more than 4096 cells)."""
+    import torch
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    dt, tol = (torch.float64, 1e-12) if prec == "fp64" else (torch.float32, 2e-6)
+    for (n, p, nq, force) in [(4, 3, 4, True), (17, 2, 3, prec == "fp64")]:
+        print(f"[subdomain patch {prec}] {n}^3 p={p} nq={nq} forced={force}", flush=True)
+        b = pmg.box(n, deform=1, amplitude=0.05)
+        ref = orc.LaplaceBK3(orc.mesh(n, deform=1, amplitude=0.05), p, nq)
+        op = pmg.LaplaceOperatorBK3(b, p, nq, precision)
+        if force:
+            op.set_schedule(pmg.SCHEDULE_PATCH, -1)
+        phys, _, icells = subdomain_sets(n, n, n, p)
+        op.set_subdomain(pmg.box_dofmap(b, p, False), phys, icells)
+        x = seeded_uniform(op.m(), 500 + n)
+        if prec == "fp32":
+            x = x.astype(np.float32).astype(np.float64)
+        assert rel_err(_apply(op, op.vmult_neumann, x, dt), ref.vmult_subdomain(x, phys)) < tol
+        assert rel_err(_apply(op, op.vmult_interface_cell_range, x, dt),
+                       ref.vmult_subdomain(x, phys, icells)) < tol
