@@ -158,7 +158,9 @@ int pmg_operator_apply_add(pmg_operator op, void *dst, const void *src, pmg_stre
  * set-up at :1056-1061 -- dofmap (host, caller cell order, as given to create)
  * with only the physical_dofs eliminated -- and the interface cells (host,
  * caller indices of the cells with a face on the interface, :1181-1203).
- * Copies what it needs; may be called again to replace the sets. */
+ * Copies what it needs; may be called again to replace the sets.  The Neumann
+ * vmult uses the operator's scatter schedule as it is at this call (patch or
+ * atomic; colored falls back to atomic). */
 int pmg_operator_set_subdomain(pmg_operator op, const uint32_t *dofmap,
                                const uint32_t *physical_dofs, int64_t n_physical,
                                const uint32_t *interface_cells, int64_t n_interface_cells);
