@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <thread>
+#include <utility>
 
 namespace pmg {
 
@@ -84,39 +86,65 @@ CellPatches build_patches(int64_t n_cells, int64_t nm3, const uint32_t *map, int
   if (int64_t(E) * nm3 >= 0xFFFF) throw std::length_error("patch index exceeds 16 bits");
   CellPatches P;
   const int64_t nb = (n_cells + E - 1) / E, pmax = int64_t(E) * nm3;
-  std::vector<int32_t> mult(static_cast<size_t>(n_dofs), 0);   // cells per DoF
-  std::vector<int32_t> local(static_cast<size_t>(n_dofs), 0);  // cells of this batch per DoF
-  std::vector<int32_t> slot(static_cast<size_t>(n_dofs), -1);  // patch index in this batch
-  for (int64_t e = 0; e < n_cells * nm3; ++e)
-    if (map[e] != kNoDof) ++mult[map[e]];
-  P.pdof.assign(static_cast<size_t>(nb * pmax), 0u);
-  P.pcount.assign(static_cast<size_t>(nb), 0u);
-  P.pidx.assign(static_cast<size_t>(n_cells * nm3), 0xFFFFu);
-  std::vector<uint32_t> dofs;
-  for (int64_t b = 0; b < nb; ++b) {
-    const int64_t c0 = b * E, c1 = std::min(n_cells, c0 + E);
-    dofs.clear();
-    for (int64_t e = c0 * nm3; e < c1 * nm3; ++e) {
-      const uint32_t d = map[e];
-      if (d == kNoDof) continue;
-      if (local[d]++ == 0) dofs.push_back(d);
+  RawVector<int32_t> mult(static_cast<size_t>(n_dofs));  // cells per DoF
+  P.pdof.resize(static_cast<size_t>(nb * pmax));
+  P.pcount.resize(static_cast<size_t>(nb));
+  P.pidx.resize(static_cast<size_t>(n_cells * nm3));
+  // Worker threads over contiguous cell (batch) ranges: first the DoF
+  // multiplicities (relaxed atomic increments), then the batches, which are
+  // independent given those; each writes only its own slices.  Per batch, the
+  // sorted (DoF, entry) keys give the patch (ascending DoFs, so the
+  // write-back coalesces), each DoF's count in the batch and every entry's slot.
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int64_t n_threads = std::min<int64_t>(std::min<unsigned>(hw, 16u), std::max<int64_t>(nb / 64, 1));
+  auto run = [n_threads](auto &&fn) {
+    std::vector<std::thread> pool;
+    for (int64_t t = 1; t < n_threads; ++t) pool.emplace_back(fn, t);
+    fn(int64_t(0));
+    for (auto &th : pool) th.join();
+  };
+  run([&](int64_t t) {
+    const int64_t d0 = n_dofs * t / n_threads, d1 = n_dofs * (t + 1) / n_threads;
+    std::fill(mult.begin() + d0, mult.begin() + d1, 0);
+  });
+  run([&](int64_t t) {
+    const int64_t e0 = n_cells * nm3 * t / n_threads, e1 = n_cells * nm3 * (t + 1) / n_threads;
+    for (int64_t e = e0; e < e1; ++e)
+      if (map[e] != kNoDof) __atomic_fetch_add(&mult[map[e]], 1, __ATOMIC_RELAXED);
+  });
+  std::vector<int64_t> shared(static_cast<size_t>(n_threads), 0), owned(shared);
+  run([&](int64_t t) {
+    const int64_t b0 = nb * t / n_threads, b1 = nb * (t + 1) / n_threads;
+    std::vector<uint64_t> ent;  // DoF << 16 | entry offset in the batch (< 2^16)
+    ent.reserve(static_cast<size_t>(pmax));
+    for (int64_t b = b0; b < b1; ++b) {
+      const int64_t c0 = b * E, c1 = std::min(n_cells, c0 + E);
+      std::fill(P.pidx.begin() + c0 * nm3, P.pidx.begin() + c1 * nm3, uint16_t(0xFFFF));
+      ent.clear();
+      for (int64_t e = c0 * nm3; e < c1 * nm3; ++e)
+        if (map[e] != kNoDof) ent.push_back(uint64_t(map[e]) << 16 | uint64_t(e - c0 * nm3));
+      std::sort(ent.begin(), ent.end());
+      uint32_t n = 0;
+      for (size_t k = 0; k < ent.size();) {
+        const uint32_t d = static_cast<uint32_t>(ent[k] >> 16);
+        size_t r = k;
+        while (r < ent.size() && (ent[r] >> 16) == d) {
+          P.pidx[c0 * nm3 + static_cast<int64_t>(ent[r] & 0xFFFFu)] = static_cast<uint16_t>(n);
+          ++r;
+        }
+        const bool sh = static_cast<int64_t>(r - k) < mult[d];
+        P.pdof[b * pmax + n] = d | (sh ? 0x80000000u : 0u);
+        ++(sh ? shared[t] : owned[t]);
+        ++n;
+        k = r;
+      }
+      P.pcount[b] = n;
+      std::fill(P.pdof.begin() + b * pmax + n, P.pdof.begin() + (b + 1) * pmax, 0u);
     }
-    std::sort(dofs.begin(), dofs.end());  // ascending: the patch write-back coalesces
-    P.pcount[b] = static_cast<uint32_t>(dofs.size());
-    for (size_t i = 0; i < dofs.size(); ++i) {
-      const uint32_t d = dofs[i];
-      slot[d] = static_cast<int32_t>(i);
-      const bool shared = local[d] < mult[d];
-      P.pdof[b * pmax + i] = d | (shared ? 0x80000000u : 0u);
-      if (shared) ++P.n_shared;
-      else ++P.n_owned;
-    }
-    for (int64_t e = c0 * nm3; e < c1 * nm3; ++e)
-      if (map[e] != kNoDof) P.pidx[e] = static_cast<uint16_t>(slot[map[e]]);
-    for (uint32_t d : dofs) {
-      slot[d] = -1;
-      local[d] = 0;
-    }
+  });
+  for (int64_t t = 0; t < n_threads; ++t) {
+    P.n_shared += shared[t];
+    P.n_owned += owned[t];
   }
   return P;
 }

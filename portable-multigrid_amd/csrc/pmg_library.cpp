@@ -302,9 +302,9 @@ static void build_patch_schedule(const std::vector<uint32_t> &imap, int64_t n_ce
   const int E = g_batch_cells(nq);
   if (n_cells == 0 || n_dofs >= (int64_t(1) << 31) || E * nm3 >= 0xFFFF) return;
   const CellPatches P = build_patches(n_cells, nm3, imap.data(), n_dofs, E);
-  pa.pdof.upload(P.pdof);
-  pa.pcount.upload(P.pcount);
-  pa.pidx.upload(P.pidx);
+  pa.pdof.upload(P.pdof.data(), P.pdof.size());
+  pa.pcount.upload(P.pcount.data(), P.pcount.size());
+  pa.pidx.upload(P.pidx.data(), P.pidx.size());
   sc.pdof = pa.pdof.ptr;
   sc.pcount = pa.pcount.ptr;
   sc.pidx = pa.pidx.ptr;
