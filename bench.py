@@ -416,6 +416,16 @@ def bench_rows(pmg):
         torch.cuda.empty_cache()
 
         coarse = pmg.box(32)
+        if prec == pmg.FP64:  # set-up of the fine h-transfer: host weights vs setup_weights on the device
+            for mode in (False, True):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                tr = pmg.GeometricTransfer(coarse, 4, prec, device_weights=mode)
+                torch.cuda.synchronize()
+                rows[f"f2_h_transfer_setup_q4_32to64_{'device' if mode else 'host'}_weights"] = {
+                    "time_ms": (time.perf_counter() - t0) * 1e3,
+                    "note": "box maps on the host included (setup, not a hot op)"}
+                del tr
         tr = pmg.GeometricTransfer(coarse, 4, prec)
         xc = torch.rand(tr.n_c, dtype=dt[prec], device="cuda")
         yf = torch.zeros(tr.n_f, dtype=dt[prec], device="cuda")

@@ -267,6 +267,19 @@ int pmg_transfer_create(int kind, int precision, int degree_coarse, int degree_f
                         int64_t n_cells, const uint32_t *map_coarse, const uint32_t *map_fine,
                         const double *weights, int64_t n_dofs_coarse, int64_t n_dofs_fine,
                         double prolongation_scale, pmg_transfer *tr);
+/* Kinds H and P with the weights built on the device from the fine level's
+ * constraints, as GeometricTransfer::setup_weights and its PolynomialTransfer
+ * counterpart compute them (include/multigrid/portable_geometric_transfer.h:1344-1380,
+ * portable_polynomial_transfer.h:1155-1360): 1/multiplicity of each fine DoF
+ * over map_fine, 0 on the constrained_fine DoFs (host list).  The library
+ * keeps them as owner marks, so the caller builds and passes no weights. */
+int pmg_transfer_create_from_constraints(int kind, int precision, int degree_coarse,
+                                         int degree_fine, int64_t n_cells,
+                                         const uint32_t *map_coarse, const uint32_t *map_fine,
+                                         const uint32_t *constrained_fine,
+                                         int64_t n_constrained_fine, int64_t n_dofs_coarse,
+                                         int64_t n_dofs_fine, double prolongation_scale,
+                                         pmg_transfer *tr);
 int pmg_transfer_destroy(pmg_transfer tr);
 /* dst_fine += P src_coarse (prolongate_and_add) */
 int pmg_transfer_prolongate_and_add(pmg_transfer tr, void *dst_fine, const void *src_coarse,

@@ -121,6 +121,8 @@ def lib():
         "pmg_operator_Tvmult": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_apply_add": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_set_subdomain": (_i, [_vp, _vp, _vp, _i64, _vp, _i64]),
+        "pmg_transfer_create_from_constraints": (_i, [_i, _i, _i, _i, _i64, _vp, _vp, _vp, _i64,
+                                                      _i64, _i64, _d, P(_vp)]),
         "pmg_operator_vmult_neumann": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_vmult_interface_cells": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_compute_diagonal": (_i, [_vp]),
@@ -442,7 +444,8 @@ class LaplaceOperatorDG(LaplaceOperatorBK3):
 
 # ------------------------------------------------------------------ transfers
 class _Transfer:
-    def __init__(self, kind, precision, pc, pf, mc, mf, w, n_c, n_f, scale):
+    def __init__(self, kind, precision, pc, pf, mc, mf, w, n_c, n_f, scale, cons=None):
+        """cons (with w None): the fine constrained DoFs, weights built on the device."""
         self.precision = precision
         self.n_c, self.n_f = n_c, n_f
         self.h = ctypes.c_void_p()
@@ -450,6 +453,13 @@ class _Transfer:
         mf = None if mf is None else np.ascontiguousarray(mf, dtype=np.uint32)
         w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
         n_cells = len(mc) // (pc + 1) ** 3
+        if w is None and cons is not None:
+            cons = np.ascontiguousarray(cons, dtype=np.uint32)
+            _check(lib().pmg_transfer_create_from_constraints(
+                kind, precision, pc, pf, n_cells, _np_ptr(mc), None if mf is None else _np_ptr(mf),
+                _np_ptr(cons), len(cons),
+                n_c, n_f, scale, ctypes.byref(self.h)))
+            return
         _check(lib().pmg_transfer_create(kind, precision, pc, pf, n_cells, _np_ptr(mc),
                                          None if mf is None else _np_ptr(mf),
                                          None if w is None else _np_ptr(w), n_c, n_f, scale,
@@ -477,20 +487,24 @@ class GeometricTransfer(_Transfer):
     prolongation_scale = 2.0 reproduces the reference HEAD's doubled
     h-prolongation (portable_geometric_transfer.h:805-855)."""
 
-    def __init__(self, coarse_box, degree, precision=FP64, prolongation_scale=1.0):
+    def __init__(self, coarse_box, degree, precision=FP64, prolongation_scale=1.0,
+                 device_weights=False):
         mc, mf, w = box_h_transfer_maps(coarse_box, degree)
-        super().__init__(TRANSFER_H, precision, degree, degree, mc, mf, w,
-                         box_n_dofs(coarse_box, degree), box_n_dofs(refine(coarse_box), degree),
-                         prolongation_scale)
+        fine = refine(coarse_box)
+        cons = box_boundary_dofs(fine, degree) if device_weights else None
+        super().__init__(TRANSFER_H, precision, degree, degree, mc, mf,
+                         None if device_weights else w, box_n_dofs(coarse_box, degree),
+                         box_n_dofs(fine, degree), prolongation_scale, cons)
 
 
 class PolynomialTransfer(_Transfer):
     """p-transfer on one box, degree pc -> pf (PolynomialTransfer<3,pc,pf>)."""
 
-    def __init__(self, b, pc, pf, precision=FP64):
+    def __init__(self, b, pc, pf, precision=FP64, device_weights=False):
         mc, mf, w = box_p_transfer_maps(b, pc, pf)
-        super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, w, box_n_dofs(b, pc),
-                         box_n_dofs(b, pf), 1.0)
+        cons = box_boundary_dofs(b, pf) if device_weights else None
+        super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, None if device_weights else w,
+                         box_n_dofs(b, pc), box_n_dofs(b, pf), 1.0, cons)
 
 
 class ContinuousTransfer(_Transfer):

@@ -258,6 +258,26 @@ int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops
 }
 
 // ---------------------------------------------------------------- transfers
+int pmg_transfer_create_from_constraints(int kind, int precision, int pc, int pf,
+                                         int64_t n_cells, const uint32_t *map_coarse,
+                                         const uint32_t *map_fine,
+                                         const uint32_t *constrained_fine,
+                                         int64_t n_constrained_fine, int64_t n_dofs_coarse,
+                                         int64_t n_dofs_fine, double scale, pmg_transfer *tr) {
+  return guard([&] {
+    PMG_NOT_NULL(tr);
+    *tr = nullptr;
+    PMG_REQUIRE(kind == PMG_TRANSFER_H || kind == PMG_TRANSFER_P, PMG_ERR_UNSUPPORTED,
+                "weights from constraints exist for h- and p-transfers");
+    PMG_REQUIRE(n_constrained_fine >= 0, PMG_ERR_INVALID_ARG, "negative constrained count");
+    auto h = std::make_unique<pmg_transfer_s>();
+    h->impl = std::make_unique<pmg::MGTransfer>(kind, precision, pc, pf, n_cells, map_coarse, map_fine,
+                                                nullptr, n_dofs_coarse, n_dofs_fine, scale,
+                                                constrained_fine, n_constrained_fine);
+    *tr = h.release();
+  });
+}
+
 int pmg_transfer_create(int kind, int precision, int pc, int pf, int64_t n_cells,
                         const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
                         int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale, pmg_transfer *tr) {

@@ -200,6 +200,17 @@ struct Transfer1D {
   T M[kMaxNm * (2 * kMaxNm)];  // M[i*nmf + j]
 };
 
+// Owner form of a plain fine transfer map, built on the device: the weights
+// of setup_weights (1/multiplicity, 0 on constrained fine DoFs) become bit 31
+// on the first entry naming each unconstrained DoF.  cons: byte mask over the
+// n_dofs fine DoFs; first: n_dofs words of scratch.  Synchronous on s.
+void launch_owner_marks(int64_t nw, int64_t n_dofs, uint32_t *mf, const uint8_t *cons,
+                        uint32_t *first, hipStream_t s);
+// x-line words {first DoF, owner bits} of an owner-form map with lines of nmf
+// entries; *bad becomes 1 when a line is not a run of consecutive DoFs
+void launch_fine_lines(int64_t n_lines, int nmf, const uint32_t *mf, uint2 *fl, uint32_t *bad,
+                       hipStream_t s);
+
 // w == nullptr: owner mode (map_f bit 31 marks the owning entry of each fine
 // DoF, see pmg_transfer.hip); otherwise weighted, w pre-scaled for the
 // prolongation (scale is only used in owner mode).  fine_lines (owner mode,

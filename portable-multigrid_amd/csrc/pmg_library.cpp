@@ -667,7 +667,8 @@ double LaplaceOperatorBK3::algorithmic_flops() const {
 // ============================================================ transfers
 MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
                        const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
-                       int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale)
+                       int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale,
+                       const uint32_t *constrained_fine, int64_t n_constrained_fine)
     : kind_(kind), precision_(precision), n_cells_(n_cells), n_dofs_c_(n_dofs_coarse),
       n_dofs_f_(n_dofs_fine) {
   check_precision(precision);
@@ -707,7 +708,11 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     }
     return;
   }
-  PMG_REQUIRE(map_coarse && map_fine && weights, PMG_ERR_INVALID_ARG, "NULL setup array");
+  const bool device_weights = weights == nullptr && n_constrained_fine >= 0;
+  PMG_REQUIRE(map_coarse && map_fine && (weights || device_weights), PMG_ERR_INVALID_ARG,
+              "NULL setup array");
+  PMG_REQUIRE(!device_weights || constrained_fine || n_constrained_fine == 0, PMG_ERR_INVALID_ARG,
+              "NULL constrained fine DoF list");
   std::vector<double> M;
   if (kind == PMG_TRANSFER_H) {
     PMG_REQUIRE(pc == pf && pc >= 1 && pc <= 6, PMG_ERR_UNSUPPORTED,
@@ -758,6 +763,40 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     cell_buf_.reinit(precision, n_cells * nc3);
   }
   const size_t nw = static_cast<size_t>(n_cells * nf3);
+  if (device_weights) {
+    // setup_weights (portable_geometric_transfer.h:1344-1380) on the device:
+    // the weights are 1/multiplicity and 0 on constrained fine DoFs, which is
+    // the owner form (each unconstrained DoF's first entry marked)
+    PMG_REQUIRE(nw < (size_t(1) << 32) && n_dofs_fine < (int64_t(1) << 31), PMG_ERR_SIZE,
+                "fine map too large for owner marks");
+    std::vector<uint8_t> cmask(static_cast<size_t>(n_dofs_fine), 0);
+    for (int64_t i = 0; i < n_constrained_fine; ++i) {
+      PMG_REQUIRE(int64_t(constrained_fine[i]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
+                  "constrained fine DoF out of range");
+      cmask[constrained_fine[i]] = 1;
+    }
+    DeviceArray<uint8_t> dmask;
+    dmask.upload(cmask);
+    DeviceArray<uint32_t> first(static_cast<size_t>(std::max<int64_t>(n_dofs_fine, 1)));
+    map_f_.upload(map_fine, nw);
+    launch_owner_marks(static_cast<int64_t>(nw), n_dofs_fine, map_f_.ptr, dmask.ptr, first.ptr,
+                       nullptr);
+    owner_ = true;
+    scale_ = scale;
+    if (nmf_ <= 32 && n_cells > 0) {
+      const int64_t nl = n_cells * nmf_ * nmf_;
+      fine_lines_.resize(static_cast<size_t>(2 * nl));
+      DeviceArray<uint32_t> bad(1);
+      PMG_HIP_CHECK(hipMemset(bad.ptr, 0, sizeof(uint32_t)));
+      launch_fine_lines(nl, nmf_, map_f_.ptr, reinterpret_cast<uint2 *>(fine_lines_.ptr), bad.ptr,
+                        nullptr);
+      uint32_t hbad = 1;
+      PMG_HIP_CHECK(hipMemcpy(&hbad, bad.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+      if (hbad) fine_lines_.release();
+    }
+    PMG_HIP_CHECK(hipDeviceSynchronize());
+    return;
+  }
   // Owner mode when the weights are 1/multiplicity (0 on constrained DoFs,
   // consistently for every entry of a DoF), as the reference's setup_weights
   // builds them: mark the first entry of each weighted fine DoF, drop weights.

@@ -231,9 +231,12 @@ class LaplaceOperatorDG : public Operator {
 
 class MGTransfer {
  public:
+  // weights == nullptr: the weights of setup_weights (1/multiplicity, 0 on the
+  // n_constrained_fine listed fine DoFs) are built on the device, in owner form
   MGTransfer(int kind, int precision, int degree_coarse, int degree_fine, int64_t n_cells,
              const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
-             int64_t n_dofs_coarse, int64_t n_dofs_fine, double prolongation_scale);
+             int64_t n_dofs_coarse, int64_t n_dofs_fine, double prolongation_scale,
+             const uint32_t *constrained_fine = nullptr, int64_t n_constrained_fine = -1);
   void prolongate_and_add(void *dst_fine, const void *src_coarse, hipStream_t s) const;
   void restrict_and_add(void *dst_coarse, const void *src_fine, hipStream_t s) const;
   // dst_coarse += R (b - t) (the V-cycle's defect restriction, one pass)

@@ -21,6 +21,8 @@
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
+#include <algorithm>
+
 namespace pmg {
 
 template <int NF>
@@ -316,5 +318,70 @@ template void launch_restrict<float>(int, int, int64_t, const Transfer1D<float> 
                                      const uint32_t *, const uint32_t *, const float *,
                                      const float *, float *, hipStream_t, const float *, float *,
                                      const uint2 *);
+
+// ---------------------------------------------------------------- set-up
+namespace {
+unsigned setup_grid(int64_t n) {
+  return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)));
+}
+
+__global__ void owner_first_kernel(const int64_t nw, const uint32_t *__restrict__ mf,
+                                   const uint8_t *__restrict__ cons, uint32_t *__restrict__ first)
+{
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < nw;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t d = mf[e];
+    if (!cons[d]) atomicMin(first + d, static_cast<uint32_t>(e));
+  }
+}
+
+__global__ void owner_mark_kernel(const int64_t nw, uint32_t *__restrict__ mf,
+                                  const uint8_t *__restrict__ cons,
+                                  const uint32_t *__restrict__ first)
+{
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < nw;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t d = mf[e];
+    if (!cons[d] && first[d] == static_cast<uint32_t>(e)) mf[e] = d | 0x80000000u;
+  }
+}
+
+__global__ void fine_lines_kernel(const int64_t nl, const int nmf, const uint32_t *__restrict__ mf,
+                                  uint2 *__restrict__ fl, uint32_t *__restrict__ bad)
+{
+  for (int64_t l = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; l < nl;
+       l += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t *e = mf + l * nmf;
+    const uint32_t d0 = e[0] & 0x7FFFFFFFu;
+    uint32_t bits = 0;
+    bool ok = true;
+    for (int a = 0; a < nmf; ++a) {
+      ok = ok && (e[a] & 0x7FFFFFFFu) == d0 + static_cast<uint32_t>(a);
+      bits |= (e[a] >> 31) << a;
+    }
+    fl[l] = make_uint2(d0, bits);
+    if (!ok) atomicOr(bad, 1u);
+  }
+}
+}  // namespace
+
+void launch_owner_marks(int64_t nw, int64_t n_dofs, uint32_t *mf, const uint8_t *cons,
+                        uint32_t *first, hipStream_t s) {
+  if (nw <= 0) return;
+  PMG_HIP_CHECK(hipMemsetAsync(first, 0xFF, static_cast<size_t>(n_dofs) * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(owner_first_kernel, dim3(setup_grid(nw)), dim3(256), 0, s, nw, mf, cons, first);
+  PMG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(owner_mark_kernel, dim3(setup_grid(nw)), dim3(256), 0, s, nw, mf, cons, first);
+  PMG_HIP_CHECK(hipGetLastError());
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void launch_fine_lines(int64_t n_lines, int nmf, const uint32_t *mf, uint2 *fl, uint32_t *bad,
+                       hipStream_t s) {
+  if (n_lines <= 0) return;
+  hipLaunchKernelGGL(fine_lines_kernel, dim3(setup_grid(n_lines)), dim3(256), 0, s, n_lines, nmf, mf,
+                     fl, bad);
+  PMG_HIP_CHECK(hipGetLastError());
+}
 
 }  // namespace pmg

@@ -414,3 +414,45 @@ def test_vcycle_graph_replays_match_eager(pmg, monkeypatch, prec):
         torch.cuda.synchronize()
         assert rel_err(zg.cpu().numpy(), ze.cpu().numpy()) < tol, rep
         del junk
+
+
+@pytest.mark.parametrize("kind,p,pf", [("h", 1, 1), ("h", 2, 2), ("h", 4, 4), ("p", 1, 2),
+                                       ("p", 2, 5), ("p", 3, 6)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_transfer_weights_built_on_device_match_host_weights(pmg, kind, p, pf, prec):
+    """setup_weights on the device (from the fine constraints) gives the
+    transfer that the host weights give, whose parity with the oracle the
+    other transfer tests pin."""
+    import torch
+    print(f"[device weights] {kind} p={p} pf={pf} {prec}", flush=True)
+    precision, dt = (pmg.FP64, torch.float64) if prec == "fp64" else (pmg.FP32, torch.float32)
+    b = pmg.box(3, deform=1, amplitude=0.05)
+    if kind == "h":
+        a = pmg.GeometricTransfer(b, p, precision)
+        d = pmg.GeometricTransfer(b, p, precision, device_weights=True)
+    else:
+        a = pmg.PolynomialTransfer(b, p, pf, precision)
+        d = pmg.PolynomialTransfer(b, p, pf, precision, device_weights=True)
+    xc = torch.tensor(seeded_uniform(a.n_c, 81), dtype=dt, device="cuda")
+    xf = torch.tensor(seeded_uniform(a.n_f, 82), dtype=dt, device="cuda")
+    ya, yd = (torch.zeros(a.n_f, dtype=dt, device="cuda") for _ in range(2))
+    a.prolongate_and_add(ya, xc)
+    d.prolongate_and_add(yd, xc)
+    tol = 1e-14 if prec == "fp64" else 1e-6
+    assert rel_err(yd.cpu().numpy(), ya.cpu().numpy()) < tol
+    ra, rd = (torch.zeros(a.n_c, dtype=dt, device="cuda") for _ in range(2))
+    a.restrict_and_add(ra, xf)
+    d.restrict_and_add(rd, xf)
+    assert rel_err(rd.cpu().numpy(), ra.cpu().numpy()) < tol
+
+
+def test_transfer_from_constraints_errors(pmg):
+    b = pmg.box(2)
+    mc, mf, _ = pmg.box_h_transfer_maps(b, 2)
+    n_c, n_f = pmg.box_n_dofs(b, 2), pmg.box_n_dofs(pmg.refine(b), 2)
+    with pytest.raises(pmg.PMGError, match="out of range"):
+        pmg._Transfer(pmg.TRANSFER_H, pmg.FP64, 2, 2, mc, mf, None, n_c, n_f, 1.0,
+                      cons=np.array([n_f], np.uint32))
+    with pytest.raises(pmg.PMGError, match="h- and p-transfers"):
+        pmg._Transfer(pmg.TRANSFER_DG_CG, pmg.FP64, 2, 2, pmg.box_dofmap(b, 2), None, None, n_c,
+                      pmg.box_n_cells(b) * 27, 1.0, cons=np.zeros(0, np.uint32))
