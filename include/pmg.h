@@ -98,11 +98,13 @@ int pmg_box_boundary_dofs(const pmg_box *box, int degree, uint32_t *dofs, int64_
  * setup_weights (include/multigrid/portable_geometric_transfer.h:1344-1486):
  * coarse map [C_c][(p+1)^3] (constrained -> invalid), fine patch map
  * [C_c][(2p+1)^3] (plain), weights [C_c][(2p+1)^3] (1/multiplicity, 0 on
- * constrained fine DoFs).  The fine box is the coarse box refined once. */
+ * constrained fine DoFs; weights = NULL skips them, for
+ * pmg_transfer_create_from_constraints).  The fine box is the coarse box
+ * refined once. */
 int pmg_box_h_transfer_maps(const pmg_box *coarse, int degree, uint32_t *map_coarse,
                             uint32_t *map_fine, double *weights);
 /* p-transfer maps of PolynomialTransfer<dim,pc,pf>::setup_weights_and_dof_indices
- * (include/multigrid/portable_polynomial_transfer.h:1155-1360) */
+ * (include/multigrid/portable_polynomial_transfer.h:1155-1360); weights may be NULL */
 int pmg_box_p_transfer_maps(const pmg_box *box, int degree_coarse, int degree_fine,
                             uint32_t *map_coarse, uint32_t *map_fine, double *weights);
 /* DG face lists of the box (the face_info of LaplaceOperatorDG::compute_face_info,
@@ -177,6 +179,17 @@ int pmg_operator_vmult_interface_cells(pmg_operator op, void *dst, const void *s
 int pmg_operator_compute_diagonal(pmg_operator op);
 /* get_matrix_diagonal_inverse(): copies the inverse diagonal into out (device, m()) */
 int pmg_operator_diagonal_inverse(pmg_operator op, void *out, pmg_stream stream);
+/* el(i, i) (portable_laplace_operator_bk3.h:586-595): 1 / (inverse diagonal)_ii
+ * of the last compute_diagonal() (constrained rows: 1).  Synchronous. */
+int pmg_operator_diagonal_entry(pmg_operator op, int64_t row, double *value);
+/* vmult_dummy(dst, src, ghost_exchange_on, computation_on)
+ * (include/base/portable_laplace_operator_base.h:27-31, implementation
+ * portable_laplace_operator_bk3.h:233-357), the reference's instrument for
+ * timing the halo exchange and the cell loop apart.  On one GPU there is no
+ * ghost exchange, so it is: dst = 0; computation_on: dst += sum_cells A_e src;
+ * ghost_exchange_on: dst_c = src_c on the constrained rows (both on: vmult). */
+int pmg_operator_vmult_dummy(pmg_operator op, void *dst, const void *src, int ghost_exchange_on,
+                             int computation_on, pmg_stream stream);
 /* Variable coefficient a(u,v) = sum_q kappa_q grad u . grad v JxW: kappa is a
  * DEVICE array [C][nq^3] of doubles; G <- kappa * G_geometry.  Requires an
  * operator created from cell vertices.  kappa = NULL restores kappa = 1. */
@@ -281,6 +294,23 @@ int pmg_transfer_create_from_constraints(int kind, int precision, int degree_coa
                                          int64_t n_dofs_fine, double prolongation_scale,
                                          pmg_transfer *tr);
 int pmg_transfer_destroy(pmg_transfer tr);
+/* kind, precision, degrees, and how the fine side is stored (layout 0:
+ * explicit weights, 1: owner marks in the full fine map, 2: owner marks as
+ * one word per fine x-line, used when every fine x-line is a run of
+ * consecutive DoFs).  Any output may be NULL. */
+int pmg_transfer_info(pmg_transfer tr, int *kind, int *precision, int *degree_coarse,
+                      int *degree_fine, int *layout);
+/* MGTransferBase::reinit(mf_coarse, mf_fine, constraints_coarse,
+ * constraints_fine) (include/base/portable_mg_transfer_base.h:32-35): rebuild
+ * the transfer in place for new maps (coarse constraints as invalid entries
+ * of map_coarse) and new fine constraints (host list; weights built on the
+ * device as pmg_transfer_create_from_constraints does; ignored for DG_CG),
+ * keeping kind, precision, degrees and prolongation scale.  A V-cycle built
+ * over the handle notices the rebuild at its next application and captures
+ * its graph again; its level sizes must still match. */
+int pmg_transfer_reinit(pmg_transfer tr, int64_t n_cells, const uint32_t *map_coarse,
+                        const uint32_t *map_fine, const uint32_t *constrained_fine,
+                        int64_t n_constrained_fine, int64_t n_dofs_coarse, int64_t n_dofs_fine);
 /* dst_fine += P src_coarse (prolongate_and_add) */
 int pmg_transfer_prolongate_and_add(pmg_transfer tr, void *dst_fine, const void *src_coarse,
                                     pmg_stream stream);

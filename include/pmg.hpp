@@ -78,7 +78,21 @@ class LaplaceOperatorBase {
   void apply_add(void *dst, const void *src, pmg_stream s = nullptr) const {
     check(pmg_operator_apply_add(h_.get(), dst, src, s));
   }
+  // vmult_dummy (portable_laplace_operator_base.h:27-31): one GPU has no
+  // ghost exchange, so ghost_exchange_on only adds the constrained-row copy
+  void vmult_dummy(void *dst, const void *src, bool ghost_exchange_on, bool computation_on,
+                   pmg_stream s = nullptr) const {
+    check(pmg_operator_vmult_dummy(h_.get(), dst, src, ghost_exchange_on ? 1 : 0,
+                                   computation_on ? 1 : 0, s));
+  }
   void compute_diagonal() { check(pmg_operator_compute_diagonal(h_.get())); }
+  // el(row, col) (portable_laplace_operator_bk3.h:586-595): diagonal entries only
+  double el(int64_t row, int64_t col) const {
+    if (row != col) throw Error(PMG_ERR_UNSUPPORTED, "el: only diagonal entries");
+    double v = 0;
+    check(pmg_operator_diagonal_entry(h_.get(), row, &v));
+    return v;
+  }
   // copies the inverse diagonal into out (device, m() entries)
   void get_matrix_diagonal_inverse(void *out, pmg_stream s = nullptr) const {
     check(pmg_operator_diagonal_inverse(h_.get(), out, s));
@@ -202,6 +216,14 @@ class MGTransferBase {
   }
   void restrict_and_add(void *dst_coarse, const void *src_fine, pmg_stream s = nullptr) const {
     check(pmg_transfer_restrict_and_add(h_.get(), dst_coarse, src_fine, s));
+  }
+  // MGTransferBase::reinit: new maps (coarse constraints as invalid entries)
+  // and fine constraints; weights built on the device
+  void reinit(int64_t n_cells, const uint32_t *map_coarse, const uint32_t *map_fine,
+              const uint32_t *constrained_fine, int64_t n_constrained_fine, int64_t n_dofs_coarse,
+              int64_t n_dofs_fine) {
+    check(pmg_transfer_reinit(h_.get(), n_cells, map_coarse, map_fine, constrained_fine,
+                              n_constrained_fine, n_dofs_coarse, n_dofs_fine));
   }
   pmg_transfer handle() const { return h_.get(); }
 

@@ -32,6 +32,7 @@ TRANSFER_H, TRANSFER_P, TRANSFER_DG_CG = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
 COARSE_CHEBYSHEV, COARSE_PCG = 0, 1
 SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH = 0, 1, 2
+TRANSFER_LAYOUT_WEIGHTED, TRANSFER_LAYOUT_OWNER_MAP, TRANSFER_LAYOUT_OWNER_LINES = 0, 1, 2
 INVALID_DOF = 0xFFFFFFFF
 
 OK, ERR_INVALID_ARG, ERR_SIZE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED, ERR_ALLOC = range(7)
@@ -127,6 +128,9 @@ def lib():
         "pmg_operator_vmult_interface_cells": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_operator_compute_diagonal": (_i, [_vp]),
         "pmg_operator_diagonal_inverse": (_i, [_vp, _vp, _vp]),
+        "pmg_operator_diagonal_entry": (_i, [_vp, _i64, P(_d)]),
+        "pmg_operator_vmult_dummy": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+        "pmg_transfer_reinit": (_i, [_vp, _i64, _vp, _vp, _vp, _i64, _i64, _i64]),
         "pmg_operator_set_coefficient": (_i, [_vp, _vp, _vp]),
         "pmg_operator_gradient_coefficient": (_i, [_vp, _vp, _vp, _vp, _vp]),
         "pmg_operator_algorithmic_bytes": (_i, [_vp, P(_d), P(_d)]),
@@ -136,6 +140,7 @@ def lib():
         "pmg_transfer_create": (_i, [_i, _i, _i, _i, _i64, _vp, _vp, _vp, _i64, _i64, _d,
                                      P(_vp)]),
         "pmg_transfer_destroy": (_i, [_vp]),
+        "pmg_transfer_info": (_i, [_vp, P(_i), P(_i), P(_i), P(_i), P(_i)]),
         "pmg_transfer_prolongate_and_add": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_transfer_restrict_and_add": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_chebyshev_create": (_i, [_vp, _i, _d, _i, P(_vp)]),
@@ -213,23 +218,25 @@ def box_boundary_dofs(b, p):
     return out
 
 
-def box_h_transfer_maps(coarse, p):
+def box_h_transfer_maps(coarse, p, weights=True):
+    """Coarse map, fine patch map and (weights=True, else None) the host
+    weights 1/multiplicity of the h-transfer coarse -> refine(coarse)."""
     C = box_n_cells(coarse)
     mc = np.zeros(C * (p + 1) ** 3, dtype=np.uint32)
     mf = np.zeros(C * (2 * p + 1) ** 3, dtype=np.uint32)
-    w = np.zeros(C * (2 * p + 1) ** 3)
+    w = np.zeros(C * (2 * p + 1) ** 3) if weights else None
     _check(lib().pmg_box_h_transfer_maps(ctypes.byref(coarse), p, _np_ptr(mc), _np_ptr(mf),
-                                         _np_ptr(w)))
+                                         None if w is None else _np_ptr(w)))
     return mc, mf, w
 
 
-def box_p_transfer_maps(b, pc, pf):
+def box_p_transfer_maps(b, pc, pf, weights=True):
     C = box_n_cells(b)
     mc = np.zeros(C * (pc + 1) ** 3, dtype=np.uint32)
     mf = np.zeros(C * (pf + 1) ** 3, dtype=np.uint32)
-    w = np.zeros(C * (pf + 1) ** 3)
+    w = np.zeros(C * (pf + 1) ** 3) if weights else None
     _check(lib().pmg_box_p_transfer_maps(ctypes.byref(b), pc, pf, _np_ptr(mc), _np_ptr(mf),
-                                         _np_ptr(w)))
+                                         None if w is None else _np_ptr(w)))
     return mc, mf, w
 
 
@@ -344,6 +351,20 @@ class LaplaceOperatorBK3:
 
     def compute_diagonal(self):
         _check(lib().pmg_operator_compute_diagonal(self.h))
+
+    def el(self, row, col):
+        """Diagonal entry A_ii = 1 / (inverse diagonal)_ii (row == col only)."""
+        if row != col:
+            raise PMGError(ERR_UNSUPPORTED, "el: only diagonal entries")
+        v = ctypes.c_double()
+        _check(lib().pmg_operator_diagonal_entry(self.h, int(row), ctypes.byref(v)))
+        return v.value
+
+    def vmult_dummy(self, dst, src, ghost_exchange_on, computation_on):
+        _check(lib().pmg_operator_vmult_dummy(self.h, _dev_ptr(dst, self.precision, self._m),
+                                              _dev_ptr(src, self.precision, self._m),
+                                              int(bool(ghost_exchange_on)),
+                                              int(bool(computation_on)), _stream()))
 
     def get_matrix_diagonal_inverse(self):
         out = self.initialize_dof_vector()
@@ -480,6 +501,28 @@ class _Transfer:
             self.h, _dev_ptr(dst, self.precision, self.n_c), _dev_ptr(src, self.precision, self.n_f),
             _stream()))
 
+    def reinit(self, mc, mf, cons, n_c, n_f):
+        """MGTransferBase::reinit: new maps, fine constraints (device weights) and sizes."""
+        mc = np.ascontiguousarray(mc, dtype=np.uint32)
+        mf = None if mf is None else np.ascontiguousarray(mf, dtype=np.uint32)
+        cons = np.ascontiguousarray(cons if cons is not None else [], dtype=np.uint32)
+        pc = self.info()[2]
+        n_cells = len(mc) // (pc + 1) ** 3
+        _check(lib().pmg_transfer_reinit(self.h, n_cells, _np_ptr(mc),
+                                         None if mf is None else _np_ptr(mf), _np_ptr(cons),
+                                         len(cons), n_c, n_f))
+        self.n_c, self.n_f = n_c, n_f
+
+    def info(self):
+        """(kind, precision, degree_coarse, degree_fine, layout)"""
+        v = [ctypes.c_int() for _ in range(5)]
+        _check(lib().pmg_transfer_info(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def layout(self):
+        """TRANSFER_LAYOUT_WEIGHTED, _OWNER_MAP or _OWNER_LINES (fine map storage)."""
+        return self.info()[4]
+
 
 class GeometricTransfer(_Transfer):
     """h-transfer coarse box -> once-refined box at degree p (GeometricTransfer<3,p>).
@@ -489,7 +532,7 @@ class GeometricTransfer(_Transfer):
 
     def __init__(self, coarse_box, degree, precision=FP64, prolongation_scale=1.0,
                  device_weights=False):
-        mc, mf, w = box_h_transfer_maps(coarse_box, degree)
+        mc, mf, w = box_h_transfer_maps(coarse_box, degree, weights=not device_weights)
         fine = refine(coarse_box)
         cons = box_boundary_dofs(fine, degree) if device_weights else None
         super().__init__(TRANSFER_H, precision, degree, degree, mc, mf,
@@ -501,7 +544,7 @@ class PolynomialTransfer(_Transfer):
     """p-transfer on one box, degree pc -> pf (PolynomialTransfer<3,pc,pf>)."""
 
     def __init__(self, b, pc, pf, precision=FP64, device_weights=False):
-        mc, mf, w = box_p_transfer_maps(b, pc, pf)
+        mc, mf, w = box_p_transfer_maps(b, pc, pf, weights=not device_weights)
         cons = box_boundary_dofs(b, pf) if device_weights else None
         super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, None if device_weights else w,
                          box_n_dofs(b, pc), box_n_dofs(b, pf), 1.0, cons)
@@ -652,11 +695,12 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
             assert same and pc == p, "a DG level sits on the CG level of the same box and degree"
             trs.append(ContinuousTransfer(b, p, precision))
         elif same:
-            trs.append(PolynomialTransfer(b, pc, p, precision))
+            trs.append(PolynomialTransfer(b, pc, p, precision, device_weights=True))
         else:
             assert (2 * bc.nx, 2 * bc.ny, 2 * bc.nz) == (b.nx, b.ny, b.nz) and pc == p, \
                 "consecutive levels must differ by one refinement or by the degree"
-            trs.append(GeometricTransfer(bc, p, precision, prolongation_scale))
+            trs.append(GeometricTransfer(bc, p, precision, prolongation_scale,
+                                         device_weights=True))
         sms.append(PreconditionChebyshev(op, smoother_degree, smoothing_range,
                                          eig_cg_n_iterations))
     mg = VCycleMultigrid(ops, trs, sms, coarse, coarse_tol,

@@ -208,6 +208,46 @@ int pmg_operator_diagonal_inverse(pmg_operator op, void *out, pmg_stream s) {
   });
 }
 
+int pmg_operator_diagonal_entry(pmg_operator op, int64_t row, double *value) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    PMG_NOT_NULL(value);
+    PMG_REQUIRE(row >= 0 && row < op->impl->m(), PMG_ERR_INVALID_ARG, "row out of range");
+    const auto *inv = static_cast<const char *>(op->impl->inverse_diagonal());
+    double d = 0.0;
+    if (op->impl->precision() == PMG_FP64) {
+      PMG_HIP_CHECK(hipMemcpy(&d, inv + row * sizeof(double), sizeof(double),
+                              hipMemcpyDeviceToHost));
+    } else {
+      float f = 0.0f;
+      PMG_HIP_CHECK(hipMemcpy(&f, inv + row * sizeof(float), sizeof(float), hipMemcpyDeviceToHost));
+      d = f;
+    }
+    *value = 1.0 / d;
+  });
+}
+
+int pmg_operator_vmult_dummy(pmg_operator op, void *dst, const void *src, int ghost_exchange_on,
+                             int computation_on, pmg_stream s) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    PMG_NOT_NULL(dst);
+    PMG_NOT_NULL(src);
+    PMG_REQUIRE(dst != src, PMG_ERR_INVALID_ARG, "vmult_dummy: dst and src must not alias");
+    const pmg::Operator &A = *op->impl;
+    if (ghost_exchange_on && computation_on) {
+      A.vmult(dst, src, as_stream(s));
+      return;
+    }
+    if (A.precision() == PMG_FP64)
+      pmg::launch_fill<double>(A.m(), static_cast<double *>(dst), 0.0, as_stream(s));
+    else
+      pmg::launch_fill<float>(A.m(), static_cast<float *>(dst), 0.0f, as_stream(s));
+    if (computation_on) A.apply_add(dst, src, as_stream(s));
+    if (ghost_exchange_on) A.copy_constrained(dst, src, as_stream(s));
+  });
+}
+
 int pmg_operator_set_coefficient(pmg_operator op, const double *kappa, pmg_stream s) {
   return guard([&] {
     PMG_NOT_NULL(op);
@@ -293,6 +333,30 @@ int pmg_transfer_create(int kind, int precision, int pc, int pf, int64_t n_cells
 
 int pmg_transfer_destroy(pmg_transfer tr) {
   return guard([&] { delete tr; });
+}
+
+int pmg_transfer_info(pmg_transfer tr, int *kind, int *precision, int *degree_coarse,
+                      int *degree_fine, int *layout) {
+  return guard([&] {
+    PMG_NOT_NULL(tr);
+    const pmg::MGTransfer &t = *tr->impl;
+    if (kind) *kind = t.kind();
+    if (precision) *precision = t.precision();
+    if (degree_coarse) *degree_coarse = t.degree_coarse();
+    if (degree_fine) *degree_fine = t.degree_fine();
+    if (layout) *layout = t.layout();
+  });
+}
+
+int pmg_transfer_reinit(pmg_transfer tr, int64_t n_cells, const uint32_t *map_coarse,
+                        const uint32_t *map_fine, const uint32_t *constrained_fine,
+                        int64_t n_constrained_fine, int64_t n_dofs_coarse, int64_t n_dofs_fine) {
+  return guard([&] {
+    PMG_NOT_NULL(tr);
+    PMG_REQUIRE(n_cells >= 0, PMG_ERR_INVALID_ARG, "negative cell count");
+    tr->impl->reinit(n_cells, map_coarse, map_fine, constrained_fine, n_constrained_fine,
+                     n_dofs_coarse, n_dofs_fine);
+  });
 }
 
 int pmg_transfer_prolongate_and_add(pmg_transfer tr, void *dst, const void *src, pmg_stream s) {

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <exception>
 #include <thread>
 #include <utility>
 
@@ -97,11 +98,33 @@ CellPatches build_patches(int64_t n_cells, int64_t nm3, const uint32_t *map, int
   // write-back coalesces), each DoF's count in the batch and every entry's slot.
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const int64_t n_threads = std::min<int64_t>(std::min<unsigned>(hw, 16u), std::max<int64_t>(nb / 64, 1));
+  // Exception-safe: a worker's exception is caught in the worker and
+  // rethrown here after every started thread has been joined; if a thread
+  // cannot be started, its share runs on the calling thread.
   auto run = [n_threads](auto &&fn) {
+    std::vector<std::exception_ptr> err(static_cast<size_t>(n_threads));
+    auto guarded = [&fn, &err](int64_t t) {
+      try {
+        fn(t);
+      } catch (...) {
+        err[static_cast<size_t>(t)] = std::current_exception();
+      }
+    };
     std::vector<std::thread> pool;
-    for (int64_t t = 1; t < n_threads; ++t) pool.emplace_back(fn, t);
-    fn(int64_t(0));
+    std::vector<int64_t> inline_shares;
+    pool.reserve(static_cast<size_t>(n_threads));
+    for (int64_t t = 1; t < n_threads; ++t) {
+      try {
+        pool.emplace_back(guarded, t);
+      } catch (...) {
+        inline_shares.push_back(t);
+      }
+    }
+    guarded(int64_t(0));
+    for (int64_t t : inline_shares) guarded(t);
     for (auto &th : pool) th.join();
+    for (auto &e : err)
+      if (e) std::rethrow_exception(e);
   };
   run([&](int64_t t) {
     const int64_t d0 = n_dofs * t / n_threads, d1 = n_dofs * (t + 1) / n_threads;

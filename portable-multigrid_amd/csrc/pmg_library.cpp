@@ -496,6 +496,16 @@ void LaplaceOperatorBK3::vmult_prepared(void *dst, const void *src, hipStream_t 
                             &sched_, /*assign=*/true);
 }
 
+void LaplaceOperatorBK3::copy_constrained(void *dst, const void *src, hipStream_t s) const {
+  PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
+  if (precision_ == PMG_FP64)
+    launch_copy_indexed<double>(n_constrained_, constrained_.ptr, static_cast<double *>(dst),
+                                static_cast<const double *>(src), s);
+  else
+    launch_copy_indexed<float>(n_constrained_, constrained_.ptr, static_cast<float *>(dst),
+                               static_cast<const float *>(src), s);
+}
+
 void LaplaceOperatorBK3::apply_add(void *dst, const void *src, hipStream_t s) const {
   PMG_REQUIRE(dst && src && dst != src, PMG_ERR_INVALID_ARG, "bad vectors");
   if (precision_ == PMG_FP64)
@@ -669,8 +679,8 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
                        const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
                        int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale,
                        const uint32_t *constrained_fine, int64_t n_constrained_fine)
-    : kind_(kind), precision_(precision), n_cells_(n_cells), n_dofs_c_(n_dofs_coarse),
-      n_dofs_f_(n_dofs_fine) {
+    : kind_(kind), precision_(precision), pc_(pc), pf_(pf), pscale_(scale), n_cells_(n_cells),
+      n_dofs_c_(n_dofs_coarse), n_dofs_f_(n_dofs_fine) {
   check_precision(precision);
   PMG_REQUIRE(kind == PMG_TRANSFER_H || kind == PMG_TRANSFER_P || kind == PMG_TRANSFER_DG_CG,
               PMG_ERR_UNSUPPORTED, "transfer kind not supported");
@@ -866,6 +876,19 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
   }
 }
 
+void MGTransfer::reinit(int64_t n_cells, const uint32_t *map_coarse, const uint32_t *map_fine,
+                        const uint32_t *constrained_fine, int64_t n_constrained_fine,
+                        int64_t n_dofs_coarse, int64_t n_dofs_fine) {
+  PMG_REQUIRE(kind_ == PMG_TRANSFER_DG_CG || n_constrained_fine >= 0, PMG_ERR_INVALID_ARG,
+              "negative constrained DoF count");
+  MGTransfer fresh(kind_, precision_, pc_, pf_, n_cells, map_coarse, map_fine, nullptr,
+                   n_dofs_coarse, n_dofs_fine, pscale_, constrained_fine,
+                   kind_ == PMG_TRANSFER_DG_CG ? -1 : n_constrained_fine);
+  const uint64_t gen = generation_ + 1;
+  *this = std::move(fresh);  // the old device buffers are released here
+  generation_ = gen;
+}
+
 void MGTransfer::prolongate_and_add(void *dst, const void *src, hipStream_t s) const {
   PMG_REQUIRE(dst && src, PMG_ERR_INVALID_ARG, "NULL vector");
   if (kind_ == PMG_TRANSFER_DG_CG) {
@@ -958,22 +981,35 @@ void CGWork::reinit(int precision, int64_t n) {
   }
 }
 
+CGSpace CGWork::space() {
+  host.allocate(1);
+  CGSpace w;
+  w.sc = sc.ptr;
+  w.partials = partials.ptr;
+  w.r = r.data();
+  w.z = z.data();
+  w.p = p.data();
+  w.q = q.data();
+  w.host_rr = host.ptr;
+  return w;
+}
+
 namespace {
 template <typename T>
 CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
-                    const T *b, T *x, double rel_tol, int max_it, CGWork &w, hipStream_t s,
-                    int check_every) {
+                    const T *b, T *x, double rel_tol, int max_it, const CGSpace &w,
+                    hipStream_t s, int check_every) {
   const int64_t n = op.m();
-  w.reinit(op.precision(), n);
-  T *r = static_cast<T *>(w.r.data()), *z = static_cast<T *>(w.z.data());
-  T *p = static_cast<T *>(w.p.data()), *q = static_cast<T *>(w.q.data());
+  T *r = static_cast<T *>(w.r), *z = static_cast<T *>(w.z);
+  T *p = static_cast<T *>(w.p), *q = static_cast<T *>(w.q);
   const T *pinv = precond == PMG_PRECOND_JACOBI ? static_cast<const T *>(op.inverse_diagonal()) : nullptr;
-  double *sc = w.sc.ptr;
+  double *sc = w.sc;
+  double *partials = w.partials;
   // r = b - A x
   op.vmult(q, x, s);
   launch_residual<T>(n, q, b, s);  // q = b - A x
   launch_copy<T>(n, r, q, s);
-  launch_dot<T>(n, r, r, w.partials.ptr, sc + 2, s);
+  launch_dot<T>(n, r, r, partials, sc + 2, s);
   double rr = 0;
   PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
   PMG_HIP_CHECK(hipStreamSynchronize(s));
@@ -986,7 +1022,6 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
   }
   int it = 0;
   static const bool verbose = std::getenv("PMG_CG_VERBOSE") != nullptr;
-  w.host.allocate(1);
   const double target2 = target * target;
   if (precond == PMG_PRECOND_MG) mg->stage(r, s, nullptr);
   while (it < max_it) {
@@ -995,30 +1030,35 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
       // (the same values as after a conversion, one pass less)
       const void *zl = mg->run(s);
       if (mg->level_precision() == PMG_FP64) {
-        launch_dot<T, double>(n, r, static_cast<const double *>(zl), w.partials.ptr, sc + 0, s, true);
+        launch_dot<T, double>(n, r, static_cast<const double *>(zl), partials, sc + 0, s, true);
         launch_cg_update_xp<T, double>(n, x, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
       } else {
-        launch_dot<T, float>(n, r, static_cast<const float *>(zl), w.partials.ptr, sc + 0, s, true);
+        launch_dot<T, float>(n, r, static_cast<const float *>(zl), partials, sc + 0, s, true);
         launch_cg_update_xp<T, float>(n, x, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s);
       }
     } else {
-      launch_cg_precond_dot<T>(n, r, pinv, z, w.partials.ptr, s);
-      launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 0, 1, s);
+      launch_cg_precond_dot<T>(n, r, pinv, z, partials, s);
+      launch_reduce_partials(partials, kDotBlocks, sc, 0, 1, s);
       launch_cg_update_xp<T>(n, x, p, z, sc, it == 0 ? 1 : 0, s);
     }
     op.vmult(q, p, s);
-    launch_dot<T>(n, p, q, w.partials.ptr, sc + 1, s);
-    launch_cg_update_r<T>(n, r, q, sc, w.partials.ptr, s);
-    launch_reduce_partials(w.partials.ptr, kDotBlocks, sc, 2, 1, s);
-    launch_cg_scalars(sc, 0, s, w.host.ptr, target2);
+    launch_dot<T>(n, p, q, partials, sc + 1, s);
+    launch_cg_update_r<T>(n, r, q, sc, partials, s);
+    launch_reduce_partials(partials, kDotBlocks, sc, 2, 1, s);
+    launch_cg_scalars(sc, 0, s, w.host_rr, target2);
     ++it;
     const bool check = it % check_every == 0 || it == max_it;
     // the next iteration's V-cycle input is queued before the host waits for
     // r.r, so the GPU stays busy; it skips itself once r.r has converged
     if (precond == PMG_PRECOND_MG && it < max_it) mg->stage(r, s, check ? sc + 7 : nullptr);
     if (check) {
-      PMG_HIP_CHECK(hipStreamSynchronize(s));
-      rr = w.host.ptr[0];
+      if (w.host_rr) {
+        PMG_HIP_CHECK(hipStreamSynchronize(s));
+        rr = w.host_rr[0];
+      } else {
+        PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        PMG_HIP_CHECK(hipStreamSynchronize(s));
+      }
       res.res = std::sqrt(rr);
       if (verbose) std::fprintf(stderr, "[pmg cg] it %d  |r|/|r0| = %.3e\n", it, res.res / res.res0);
       if (!(res.res > target)) {  // also stops on NaN
@@ -1036,6 +1076,15 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
 CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, CGWork &work,
                   hipStream_t s, int check_every) {
+  work.reinit(op.precision(), op.m());
+  return solve_cg(op, precond, mg, b, x, rel_tol, max_it, work.space(), s, check_every);
+}
+
+CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
+                  const void *b, void *x, double rel_tol, int max_it, const CGSpace &work,
+                  hipStream_t s, int check_every) {
+  PMG_REQUIRE(work.sc && work.partials && work.r && work.z && work.p && work.q,
+              PMG_ERR_INVALID_ARG, "CG work space missing");
   PMG_REQUIRE(b && x && b != x, PMG_ERR_INVALID_ARG, "bad vectors");
   PMG_REQUIRE(precond >= 0 && precond <= 2, PMG_ERR_INVALID_ARG, "unknown preconditioner");
   PMG_REQUIRE(precond != PMG_PRECOND_MG || (mg && mg->outer_precision() == op.precision()),
@@ -1367,7 +1416,29 @@ void VCycleMultigrid::stage(const void *src, hipStream_t s, const double *skip) 
   else launch_convert_unless<float, float>(n, static_cast<float *>(d), static_cast<const float *>(src), skip, s);
 }
 
+// A transfer rebuilt by reinit() has new buffers and maybe new sizes: the
+// sizes must still fit the level vectors, and a graph captured with the old
+// buffers is dropped (the next run captures again).
+void VCycleMultigrid::check_transfers() const {
+  bool changed = xfer_gen_.size() != xfer_.size();
+  for (size_t l = 1; l < xfer_.size() && !changed; ++l)
+    changed = xfer_[l] && xfer_[l]->generation() != xfer_gen_[l];
+  if (!changed) return;
+  for (size_t l = 1; l < xfer_.size(); ++l)
+    PMG_REQUIRE(xfer_[l]->n_dofs_coarse() == ops_[l - 1]->m() &&
+                    xfer_[l]->n_dofs_fine() == ops_[l]->m(),
+                PMG_ERR_SIZE, "transfer sizes no longer match the V-cycle levels");
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  xfer_gen_.assign(xfer_.size(), 0);
+  for (size_t l = 1; l < xfer_.size(); ++l)
+    if (xfer_[l]) xfer_gen_[l] = xfer_[l]->generation();
+}
+
 const void *VCycleMultigrid::run(hipStream_t s) const {
+  check_transfers();
   if (use_graph_ && !graph_exec_) capture();  // may fall back to the eager path
   if (use_graph_) PMG_HIP_CHECK(hipGraphLaunch(graph_exec_, s));
   else v_cycle(levels() - 1, s);

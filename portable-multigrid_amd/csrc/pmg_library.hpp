@@ -23,12 +23,16 @@ constexpr int64_t kFusedPrepMaxDofs = int64_t(1) << 20;
 constexpr int64_t kPreparedOutputMaxBytes = int64_t(32) << 20;
 
 
-// Untyped owning device buffer of either precision.
+// Untyped owning device buffer of either precision.  reinit keeps the
+// buffer (and its address) when precision and length are unchanged: device
+// pointers to it may already be baked into a captured V-cycle graph (the
+// inverse diagonal a later compute_diagonal() refills, for one).
 struct DeviceVector {
   DeviceArray<char> raw;
   int precision = PMG_FP64;
   int64_t n = 0;
   void reinit(int prec, int64_t count) {
+    if (raw.ptr && prec == precision && count == n) return;
     precision = prec;
     n = count;
     raw.resize(static_cast<size_t>(count) * element_size(prec));
@@ -78,6 +82,10 @@ class Operator {
   virtual void vmult_prepared(void *dst, const void *src, hipStream_t s) const {
     vmult(dst, src, s);
   }
+  // dst_c = src_c on the constrained rows (copy_constrained_values); none by default
+  virtual void copy_constrained(void *dst, const void *src, hipStream_t s) const {
+    (void)dst; (void)src; (void)s;
+  }
 };
 
 // Continuous Q_p Laplace operator (LaplaceOperatorBK3).
@@ -98,6 +106,7 @@ class LaplaceOperatorBK3 : public Operator {
   bool supports_prepared_vmult() const override;
   const uint8_t *constrained_mask() const override { return cmask_.ptr; }
   void vmult_prepared(void *dst, const void *src, hipStream_t s) const override;
+  void copy_constrained(void *dst, const void *src, hipStream_t s) const override;
   void compute_diagonal() override;
   const void *inverse_diagonal() const override;
   void set_coefficient(const double *kappa, hipStream_t s);
@@ -243,12 +252,29 @@ class MGTransfer {
   void restrict_residual_and_add(void *dst_coarse, const void *b, const void *t,
                                  hipStream_t s) const;
   int precision() const { return precision_; }
+  int kind() const { return kind_; }
+  int degree_coarse() const { return pc_; }
+  int degree_fine() const { return pf_; }
   int64_t n_dofs_coarse() const { return n_dofs_c_; }
   int64_t n_dofs_fine() const { return n_dofs_f_; }
+  // fine-map storage: 0 weighted (explicit weights), 1 owner marks in the
+  // full fine map, 2 owner marks as x-line words {first DoF, owner bits}
+  int layout() const { return !owner_ ? 0 : (fine_lines_.ptr ? 2 : 1); }
+  // MGTransferBase::reinit (include/base/portable_mg_transfer_base.h:32-35):
+  // rebuild in place from new maps and fine constraints (weights built on
+  // the device; DG-CG: constraints unused), keeping kind, precision, degrees
+  // and prolongation scale.  generation() changes, so a V-cycle that
+  // captured this transfer's buffers in its graph captures it again.
+  void reinit(int64_t n_cells, const uint32_t *map_coarse, const uint32_t *map_fine,
+              const uint32_t *constrained_fine, int64_t n_constrained_fine,
+              int64_t n_dofs_coarse, int64_t n_dofs_fine);
+  uint64_t generation() const { return generation_; }
 
  private:
   void restrict_sub(void *dst, const void *src, const void *sub, hipStream_t s) const;
-  int kind_, precision_, nmc_, nmf_;
+  int kind_, precision_, pc_ = 0, pf_ = 0, nmc_, nmf_;
+  double pscale_ = 1.0;
+  uint64_t generation_ = 0;
   int64_t n_cells_, n_dofs_c_, n_dofs_f_;
   bool owner_ = false;  // owner-computes mode (weights 1/multiplicity), see pmg_transfer.hip
   double scale_ = 1.0;
@@ -289,12 +315,23 @@ class PreconditionChebyshev {
   mutable DeviceVector tmp_, old_;
 };
 
-// Scalars and work space of a CG run on the device.
+// Work space of one CG run as raw device pointers: 8 scalars, kDotBlocks
+// partial sums, and r, z, p, q (m() entries of the operator precision).
+// host_rr: a pinned host double the device stores r.r into at every
+// iteration, or null (r.r is then copied back at each convergence check).
+struct CGSpace {
+  double *sc = nullptr, *partials = nullptr;
+  void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *host_rr = nullptr;
+};
+
+// Scalars and work space of a CG run on the device (owning).
 struct CGWork {
   DeviceArray<double> sc, partials;
   DeviceVector r, z, p, q;
   PinnedDoubles host;  // r.r of the last iteration, stored by the device
   void reinit(int precision, int64_t n);
+  CGSpace space();  // allocates host on first use
 };
 
 // Preconditioned CG: precond 0 none, 1 Jacobi, 2 callback.
@@ -308,6 +345,10 @@ class VCycleMultigrid;
 
 CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, CGWork &work,
+                  hipStream_t s, int check_every = 1);
+// The same on caller-provided work space (allocates nothing itself).
+CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
+                  const void *b, void *x, double rel_tol, int max_it, const CGSpace &space,
                   hipStream_t s, int check_every = 1);
 
 // Coarse levels with at most this many cells are solved by the one-workgroup PCG kernel.
@@ -357,6 +398,9 @@ class VCycleMultigrid {
   mutable hipStream_t cap_stream_ = nullptr;
   mutable hipGraph_t graph_ = nullptr;
   mutable hipGraphExec_t graph_exec_ = nullptr;
+  // transfers' generation() at capture: a reinit'ed transfer re-captures
+  mutable std::vector<uint64_t> xfer_gen_;
+  void check_transfers() const;
 };
 
 }  // namespace pmg

@@ -156,7 +156,7 @@ int pmg_box_h_transfer_maps(const pmg_box *coarse, int p, uint32_t *map_c, uint3
                             double *w) {
   return guard([&] {
     check_box(coarse);
-    PMG_REQUIRE(map_c && map_f && w, PMG_ERR_INVALID_ARG, "NULL output");
+    PMG_REQUIRE(map_c && map_f, PMG_ERR_INVALID_ARG, "NULL output");
     PMG_REQUIRE(pmg_box_dofmap(coarse, p, 1, map_c) == PMG_OK, PMG_ERR_INVALID_ARG,
                 "coarse map failed");
     pmg_box fine = *coarse;
@@ -177,7 +177,7 @@ int pmg_box_h_transfer_maps(const pmg_box *coarse, int p, uint32_t *map_c, uint3
                 *m++ = static_cast<uint32_t>(
                     g.index(2 * int64_t(p) * cx + i, 2 * int64_t(p) * cy + j, 2 * int64_t(p) * cz + k));
         }
-    patch_weights(pmg_box_n_cells(coarse), nf3, map_f, g, w);
+    if (w) patch_weights(pmg_box_n_cells(coarse), nf3, map_f, g, w);
   });
 }
 
@@ -186,11 +186,11 @@ int pmg_box_p_transfer_maps(const pmg_box *b, int pc, int pf, uint32_t *map_c, u
   return guard([&] {
     check_box(b);
     PMG_REQUIRE(pc >= 1 && pf > pc, PMG_ERR_INVALID_ARG, "need 1 <= degree_coarse < degree_fine");
-    PMG_REQUIRE(map_c && map_f && w, PMG_ERR_INVALID_ARG, "NULL output");
+    PMG_REQUIRE(map_c && map_f, PMG_ERR_INVALID_ARG, "NULL output");
     PMG_REQUIRE(pmg_box_dofmap(b, pc, 1, map_c) == PMG_OK, PMG_ERR_INVALID_ARG, "coarse map failed");
     PMG_REQUIRE(pmg_box_dofmap(b, pf, 0, map_f) == PMG_OK, PMG_ERR_INVALID_ARG, "fine map failed");
     const int64_t nf3 = int64_t(pf + 1) * (pf + 1) * (pf + 1);
-    patch_weights(pmg_box_n_cells(b), nf3, map_f, Grid(b, pf), w);
+    if (w) patch_weights(pmg_box_n_cells(b), nf3, map_f, Grid(b, pf), w);
   });
 }
 

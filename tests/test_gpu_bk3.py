@@ -204,3 +204,35 @@ def test_internal_cell_orders(pmg, orc, order, monkeypatch):
     dst = opg.initialize_dof_vector()
     opg.vmult(dst, torch.tensor(x, device="cuda"))
     assert rel_err(dst.cpu().numpy(), y) < 1e-12
+
+
+@pytest.mark.parametrize("n,p,nq,deform", [(3, 2, 3, 1), (2, 4, 5, 0), (4, 4, 5, 1), (2, 6, 7, 1),
+                                           (3, 3, 5, 2), (2, 1, 2, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_bk3_rhs_and_l2_error_match_oracle(pmg, orc, n, p, nq, deform, prec):
+    """assemble_rhs (f = 1 and f = 3 pi^2 sin sin sin, Gauss p+1 and nq) and
+    l2_error (u = 0 and sin sin sin, Gauss p+2) against the oracle's
+    quadrature sums (whose closed forms test_oracle_pinning pins): 1e-12 in
+    FP64; FP32 vectors within FP32 rounding."""
+    import torch
+    print(f"[bk3 rhs/l2] n={n} p={p} nq={nq} deform={deform} {prec}", flush=True)
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    b = pmg.box(n, deform=deform, amplitude=0.05)
+    m = orc.mesh(n, deform=deform, amplitude=0.05)
+    op = pmg.LaplaceOperatorBK3(b, p, nq, precision)
+    ref = orc.LaplaceBK3(m, p, nq)
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    for f_kind in (0, 1):
+        for q in (0, nq):
+            got = op.assemble_rhs(f_kind, q).double().cpu().numpy()
+            exp = ref.rhs(f_kind, None if q == 0 else q)
+            assert rel_err(got, exp) < tol, (f_kind, q)
+    u = seeded_uniform(op.m(), 95)
+    u[orc.boundary_dofs(m, p)] = 0
+    ut = torch.tensor(u, dtype=op.dtype, device="cuda")
+    uq = ut.double().cpu().numpy()
+    for u_kind in (0, 1):
+        for q in (0, p + 3):
+            e = op.l2_error(ut, u_kind, q)
+            er = ref.l2_error(uq, u_kind, None if q == 0 else q)
+            assert abs(e - er) <= (1e-12 if prec == "fp64" else 1e-6) * er, (u_kind, q)

@@ -416,6 +416,42 @@ def test_vcycle_graph_replays_match_eager(pmg, monkeypatch, prec):
         del junk
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_vcycle_graph_sees_level_data_changed_after_capture(pmg, monkeypatch, prec):
+    """Level data changed behind the same handles after the graph was captured
+    (a new coefficient and a recomputed inverse diagonal) is what the replay
+    uses: the graph keeps pointing at the buffers the operator refills."""
+    import torch
+    print(f"[graph after set_coefficient] {prec}", flush=True)
+    p = pmg.FP64 if prec == "fp64" else pmg.FP32
+    levels = pmg.ph_levels(pmg.box(8, deform=1, amplitude=0.05), 4, 3)
+    mg_g, ops_g, _, _ = pmg.build_hierarchy(levels, precision=p, outer_precision=pmg.FP64)
+    monkeypatch.setenv("PMG_VCYCLE_GRAPH", "0")
+    mg_e, ops_e, _, _ = pmg.build_hierarchy(levels, precision=p, outer_precision=pmg.FP64)
+    assert mg_g.uses_graph() and not mg_e.uses_graph()
+    n = ops_g[-1].m()
+    r = _t(seeded_uniform(n, 410))
+    z0 = torch.zeros_like(r)
+    mg_g.vmult(z0, r)  # captures the graph
+    torch.cuda.synchronize()
+    for ops in (ops_g, ops_e):
+        for lev in (ops[-1], ops[-2]):
+            kappa = torch.full((lev.n_cells * lev.nq ** 3,), 2.5, dtype=torch.float64,
+                               device="cuda")
+            lev.set_coefficient(kappa)
+            lev.compute_diagonal()
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for rep in range(2):
+        zg, ze = torch.zeros_like(r), torch.zeros_like(r)
+        mg_g.vmult(zg, r)
+        torch.cuda.synchronize()
+        mg_e.vmult(ze, r)
+        torch.cuda.synchronize()
+        assert rel_err(zg.cpu().numpy(), ze.cpu().numpy()) < tol, rep
+    # the changed levels do change the V-cycle (kappa 2.5 on the two finest)
+    assert rel_err(zg.cpu().numpy(), z0.cpu().numpy()) > 1e-3
+
+
 @pytest.mark.parametrize("kind,p,pf", [("h", 1, 1), ("h", 2, 2), ("h", 4, 4), ("p", 1, 2),
                                        ("p", 2, 5), ("p", 3, 6)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -456,3 +492,87 @@ def test_transfer_from_constraints_errors(pmg):
     with pytest.raises(pmg.PMGError, match="h- and p-transfers"):
         pmg._Transfer(pmg.TRANSFER_DG_CG, pmg.FP64, 2, 2, pmg.box_dofmap(b, 2), None, None, n_c,
                       pmg.box_n_cells(b) * 27, 1.0, cons=np.zeros(0, np.uint32))
+
+
+def test_config2_q4_h_hierarchy_matches_oracle(pmg, orc):
+    """Geometric MG-CG configured as the 64^3 Q4 workload, at 8^3: Q4 on every
+    h-level (2^3, 4^3, 8^3), Gauss nq = 5, Chebyshev degree 3 over the Jacobi
+    diagonal (range 15, 10 eigenvalue steps), coarse Jacobi-PCG to 1e-12,
+    f = 3 pi^2 sin sin sin, CG to 1e-10: the V-cycle, the iteration count
+    (+-1), the solution and the L2 error agree with the oracle; the FP32
+    V-cycle inside FP64 CG reaches the same residual and error."""
+    import torch
+    print("[config2 8^3 Q4]", flush=True)
+    mg, ops, _, _, boxes = pmg.geometric_hierarchy(pmg.box(8), 4, 3, 5)
+    ref = orc.Multigrid([orc.mesh(2), orc.mesh(4), orc.mesh(8)], [4, 4, 4], nqs=[5, 5, 5],
+                        cheby_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
+                        coarse_kind=1, coarse_tol=1e-12)
+    fine = ops[-1]
+    for l in range(1, 3):  # the smoothers' eigenvalue estimates
+        mx, mn, deg = pmg.PreconditionChebyshev(ops[l], 3, 15.0, 10).info()
+        rmx, rmn, rdeg = ref.smoother_info(l)
+        assert deg == rdeg == 3 and abs(mx - rmx) <= 1e-10 * rmx and abs(mn - rmn) <= 1e-9 * rmx
+    r = seeded_uniform(fine.m(), 72)
+    r[orc.boundary_dofs(orc.mesh(8), 4)] = 0
+    z = fine.initialize_dof_vector()
+    mg.vmult(z, _t(r))
+    assert rel_err(z.cpu().numpy(), ref.vmult(r)) < 1e-9
+    b = fine.assemble_rhs(1)
+    bo = orc.LaplaceBK3(orc.mesh(8), 4, 5).rhs(1)
+    assert rel_err(b.cpu().numpy(), bo) < 1e-12
+    x = fine.initialize_dof_vector()
+    its, r0, res = pmg.solve_cg(fine, b, x, pmg.PRECOND_MG, mg, 1e-10, 200)
+    xr, rits, rr0, rres = ref.pcg(bo, precond=2, rel_tol=1e-10)
+    assert abs(its - rits) <= 1, (its, rits)
+    assert abs(r0 - rr0) <= 1e-12 * rr0 and res <= 1e-10 * r0
+    assert rel_err(x.cpu().numpy(), xr) < 1e-9
+    e_gpu = fine.l2_error(x, 1)
+    e_ref = orc.LaplaceBK3(orc.mesh(8), 4, 5).l2_error(xr, 1)
+    assert abs(e_gpu - e_ref) <= 1e-6 * e_ref  # both are discretisation errors of ~1e-7
+    mg32, _, _, _, _ = pmg.geometric_hierarchy(pmg.box(8), 4, 3, 5, precision=pmg.FP32,
+                                               outer_precision=pmg.FP64)
+    x32 = fine.initialize_dof_vector()
+    its32, _, res32 = pmg.solve_cg(fine, b, x32, pmg.PRECOND_MG, mg32, 1e-10, 200)
+    assert res32 <= 1e-10 * r0 and abs(its32 - its) <= 1
+    assert rel_err(x32.cpu().numpy(), xr) < 1e-8
+    assert abs(fine.l2_error(x32, 1) - e_ref) <= 1e-6 * e_ref
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_config4_ph_hierarchy_6_3_1_matches_oracle(pmg, orc, prec):
+    """ph-multigrid configured as the 48^3 Q6 workload, at 6^3 deformed
+    cells: p 6 -> 3 -> 1 on the fine mesh, then Q1 h-levels 6^3 -> 3^3,
+    nq = p + 1 per level; FP64 throughout, or FP32 V-cycle (smoother,
+    transfers, levels) inside FP64 CG with the coarse solve in FP64.  The
+    V-cycle (FP64), the iteration count (+-1) and the solution agree with the
+    FP64 oracle; the mixed solve's distance to the oracle solution is the
+    FP64-oracle error this configuration reports."""
+    import torch
+    print(f"[config4 6^3 Q6 6-3-1 {prec}]", flush=True)
+    fine_box = pmg.box(6, deform=1, amplitude=0.05)
+    levels = pmg.ph_levels(fine_box, 6, 2, p_sequence=[6, 3, 1])
+    assert [(lv[0].nx, lv[1]) for lv in levels] == [(3, 1), (6, 1), (6, 3), (6, 6)]
+    p = pmg.FP64 if prec == "fp64" else pmg.FP32
+    mg, ops, _, _ = pmg.build_hierarchy(levels, precision=p, outer_precision=pmg.FP64)
+    m3, m6 = orc.mesh(3, deform=1, amplitude=0.05), orc.mesh(6, deform=1, amplitude=0.05)
+    ref = orc.Multigrid([m3, m6, m6, m6], [1, 1, 3, 6], coarse_kind=1, coarse_tol=1e-12)
+    if prec == "fp64":
+        fine = ops[-1]
+        r = seeded_uniform(fine.m(), 73)
+        r[orc.boundary_dofs(m6, 6)] = 0
+        z = fine.initialize_dof_vector()
+        mg.vmult(z, _t(r))
+        assert rel_err(z.cpu().numpy(), ref.vmult(r)) < 1e-9
+    fine = pmg.LaplaceOperatorBK3(fine_box, 6)
+    b = fine.assemble_rhs(0)
+    bo = orc.LaplaceBK3(m6, 6).rhs(0)
+    assert rel_err(b.cpu().numpy(), bo) < 1e-12
+    x = fine.initialize_dof_vector()
+    its, r0, res = pmg.solve_cg(fine, b, x, pmg.PRECOND_MG, mg, 1e-10, 200)
+    xr, rits, rr0, _ = ref.pcg(bo, precond=2, rel_tol=1e-10)
+    assert res <= 1e-10 * r0
+    assert abs(its - rits) <= 1, (its, rits)
+    err = rel_err(x.cpu().numpy(), xr)
+    print(f"[config4 6^3 {prec}] its {its} (oracle {rits}), |u - u_oracle|/|u_oracle| = {err:.2e}",
+          flush=True)
+    assert err < (1e-9 if prec == "fp64" else 1e-8)

@@ -140,18 +140,29 @@ def test_transfers_equivariant_under_renumbering(pmg, kind):
     mf2 = _permute_map(mf, nmf3, pif, rho)
     w2 = w.reshape(C, nmf3)[rho].ravel()
     tr2 = pmg._Transfer(k, pmg.FP64, pc, pf, mc2, mf2, w2, nc, nf, 1.0)
+    # the same transfer with its weights built on the device from the fine
+    # constraints: on a shuffled fine numbering the x-line words do not apply
+    # and the owner marks stay a full map
+    fine_box = pmg.refine(bc) if kind == "h" else bc
+    cons2 = pif[pmg.box_boundary_dofs(fine_box, pf).astype(np.int64)].astype(np.uint32)
+    tr3 = pmg._Transfer(k, pmg.FP64, pc, pf, mc2, mf2, None, nc, nf, 1.0, cons=cons2)
+    assert tr1.layout() == pmg.TRANSFER_LAYOUT_OWNER_LINES
+    assert tr2.layout() == pmg.TRANSFER_LAYOUT_OWNER_MAP
+    assert tr3.layout() == pmg.TRANSFER_LAYOUT_OWNER_MAP
     xc, yf0 = seeded_uniform(nc, 523), seeded_uniform(nf, 524)
     yf1 = torch.tensor(yf0, device="cuda")
     tr1.prolongate_and_add(yf1, torch.tensor(xc, device="cuda"))
-    yf2 = torch.tensor(_to(pif, yf0), device="cuda")
-    tr2.prolongate_and_add(yf2, torch.tensor(_to(pic, xc), device="cuda"))
-    assert rel_err(yf2.cpu().numpy()[pif], yf1.cpu().numpy()) < 1e-13
+    for tr in (tr2, tr3):
+        yf2 = torch.tensor(_to(pif, yf0), device="cuda")
+        tr.prolongate_and_add(yf2, torch.tensor(_to(pic, xc), device="cuda"))
+        assert rel_err(yf2.cpu().numpy()[pif], yf1.cpu().numpy()) < 1e-13
     xf, yc0 = seeded_uniform(nf, 525), seeded_uniform(nc, 526)
     yc1 = torch.tensor(yc0, device="cuda")
     tr1.restrict_and_add(yc1, torch.tensor(xf, device="cuda"))
-    yc2 = torch.tensor(_to(pic, yc0), device="cuda")
-    tr2.restrict_and_add(yc2, torch.tensor(_to(pif, xf), device="cuda"))
-    assert rel_err(yc2.cpu().numpy()[pic], yc1.cpu().numpy()) < 1e-13
+    for tr in (tr2, tr3):
+        yc2 = torch.tensor(_to(pic, yc0), device="cuda")
+        tr.restrict_and_add(yc2, torch.tensor(_to(pif, xf), device="cuda"))
+        assert rel_err(yc2.cpu().numpy()[pic], yc1.cpu().numpy()) < 1e-13
 
 
 def _dg_pair(pmg, b, p, nq, seed):
