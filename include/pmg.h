@@ -374,6 +374,23 @@ int pmg_cg_solve(pmg_operator op, int precond, pmg_multigrid mg, const void *b, 
                  double rel_tol, int max_it, int *iterations, double *res0, double *res,
                  pmg_stream stream);
 
+/* The same CG without any allocation (no hipMalloc, hipFree, host pinning
+ * or allocating library call inside the call).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  r, z, p, q, the scalars
+ * and the dot-product partial sums live in a fixed file-scope device array
+ * of 64 MiB that every GPU the library runs on holds until the process
+ * ends: m() may be at most 2 096 096 (FP64 operator) or 4 192 192 (FP32),
+ * larger sizes return PMG_ERR_SIZE.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two host threads, or unordered
+ * streams) race.  Preconditioners allocate nothing either, once set up: a
+ * V-cycle captures its HIP graph at its first application, so apply it
+ * once (pmg_multigrid_vmult or pmg_cg_solve) before calling this from code
+ * that must not allocate.  Results equal pmg_cg_solve's. */
+int pmg_cg_solve_noalloc(pmg_operator op, int precond, pmg_multigrid mg, const void *b, void *x,
+                         double rel_tol, int max_it, int *iterations, double *res0, double *res,
+                         pmg_stream stream);
+
 /* ------------------------------------------------------------ vectors (device)
  * Small helpers for callers without a device-vector library. */
 int pmg_vector_dot(int precision, int64_t n, const void *a, const void *b, double *result,

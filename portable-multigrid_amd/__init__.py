@@ -131,6 +131,8 @@ def lib():
         "pmg_operator_diagonal_entry": (_i, [_vp, _i64, P(_d)]),
         "pmg_operator_vmult_dummy": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
         "pmg_transfer_reinit": (_i, [_vp, _i64, _vp, _vp, _vp, _i64, _i64, _i64]),
+        "pmg_cg_solve_noalloc": (_i, [_vp, _i, _vp, _vp, _vp, _d, _i, P(_i), P(_d), P(_d),
+                                      _vp]),
         "pmg_operator_set_coefficient": (_i, [_vp, _vp, _vp]),
         "pmg_operator_gradient_coefficient": (_i, [_vp, _vp, _vp, _vp, _vp]),
         "pmg_operator_algorithmic_bytes": (_i, [_vp, P(_d), P(_d)]),

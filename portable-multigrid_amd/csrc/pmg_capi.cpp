@@ -477,6 +477,25 @@ int pmg_cg_solve(pmg_operator op, int precond, pmg_multigrid mg, const void *b, 
   return g != PMG_OK ? g : status;
 }
 
+int pmg_cg_solve_noalloc(pmg_operator op, int precond, pmg_multigrid mg, const void *b, void *x,
+                         double rel_tol, int max_it, int *iterations, double *res0, double *res,
+                         pmg_stream s) {
+  int status = PMG_OK;
+  const int g = guard([&] {
+    PMG_NOT_NULL(op);
+    const auto r = pmg::solve_cg_noalloc(*op->impl, precond, mg ? mg->impl.get() : nullptr, b, x,
+                                         rel_tol, max_it, as_stream(s));
+    if (iterations) *iterations = r.iterations;
+    if (res0) *res0 = r.res0;
+    if (res) *res = r.res;
+    if (!r.converged) {
+      status = PMG_ERR_NOT_CONVERGED;
+      pmg::set_last_error("CG did not reach the tolerance in max_it iterations");
+    }
+  });
+  return g != PMG_OK ? g : status;
+}
+
 // ---------------------------------------------------------------- vectors
 int pmg_vector_dot(int precision, int64_t n, const void *a, const void *b, double *result,
                    pmg_stream s) {

@@ -351,6 +351,12 @@ CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, const CGSpace &space,
                   hipStream_t s, int check_every = 1);
 
+// solve_cg on the library's fixed device work array (pmg_noalloc.hip):
+// allocates nothing; throws PMG_ERR_SIZE above cg_noalloc_max_dofs(precision).
+CGResult solve_cg_noalloc(const Operator &op, int precond, const VCycleMultigrid *mg,
+                          const void *b, void *x, double rel_tol, int max_it, hipStream_t s);
+int64_t cg_noalloc_max_dofs(int precision);
+
 // Coarse levels with at most this many cells are solved by the one-workgroup PCG kernel.
 constexpr int64_t kCoarseSingleWorkgroupCells = 4096;
 
