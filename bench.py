@@ -107,6 +107,72 @@ def time_cell_kernel(op, x, y, steps):
     return e0.elapsed_time(e1) / 1e3 / steps
 
 
+STAGES = ("pre_smooth", "residual_matvec", "restrict", "coarse_solve", "prolongate",
+          "post_smooth")
+
+
+def vcycle_bytes(pmg, ops, smoother_degree=3):
+    """Algorithmic HBM bytes of one V-cycle application, level by level: each
+    level l >= 1 (N DoFs, s bytes per entry, A = its operator's algorithmic
+    bytes) runs a Chebyshev vmult of degree k (k-1 matvecs; a start update
+    reading b and D^-1 and writing x, k-1 updates of 7 s N, the prepared
+    output s N), the residual matvec, the restriction (fill of the coarse
+    defect, reads of d and t, read-modify-write of the coarse vector), the
+    prolongation (coarse read, fine read-modify-write) and a Chebyshev step
+    (k matvecs, a start update of 6 s N, k-1 updates of 7 s N).  The coarse
+    solve's bytes are not counted (one workgroup on a few thousand DoFs)."""
+    k = smoother_degree
+    total = 0.0
+    for l in range(1, len(ops)):
+        op, oc = ops[l], ops[l - 1]
+        s = 8 if op.precision == pmg.FP64 else 4
+        n, nc = op.m(), oc.m()
+        a, _ = op.algorithmic_bytes()
+        pre = (k - 1) * a + 3 * s * n + (k - 1) * 7 * s * n + s * n
+        post = k * a + 6 * s * n + (k - 1) * 7 * s * n
+        restrict = s * nc + 2 * s * n + 2 * s * nc
+        prolong = s * nc + 2 * s * n
+        total += pre + a + restrict + prolong + post
+    return total
+
+
+def cg_iteration_bytes(outer, level_bytes_per_entry):
+    """Algorithmic bytes of one outer FP64 CG iteration around the V-cycle:
+    staging r into the V-cycle (8N + sN), r.z (8N + sN), x and p updates
+    (32N + sN), the vmult (zero fill 8N + the operator's bytes), p.q (16N),
+    the r update (24N)."""
+    n, s = outer.m(), level_bytes_per_entry
+    a, _ = outer.algorithmic_bytes()
+    return a + 96 * n + 3 * s * n
+
+
+def solve_roofline(pmg, ops, outer, its, t):
+    s = 8 if ops[-1].precision == pmg.FP64 else 4
+    vb, cb = vcycle_bytes(pmg, ops), cg_iteration_bytes(outer, s)
+    nbytes = its * (vb + cb)
+    return {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": nbytes / t / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": nbytes,
+            "vcycle_bytes": vb, "cg_iteration_bytes": cb,
+            "model": "iterations x (V-cycle + outer CG iteration) algorithmic bytes / solve time; "
+                     "coarse solve not counted"}
+
+
+def level_table(pmg, mg, ops, src, n_apps=5):
+    """Per-level stage times of the V-cycle (eager, device events), the
+    reference's print_wall_times table, in microseconds per application."""
+    t = mg.level_times(src, n_apps)
+    rows = []
+    for l, op in enumerate(ops):
+        row = {"level": l, "dofs": op.m(), "degree": op.degree,
+               "precision": "fp64" if op.precision == pmg.FP64 else "fp32"}
+        for j, name in enumerate(STAGES):
+            if (l == 0) == (name == "coarse_solve"):
+                row[name + "_us"] = float(t[l, j] * 1e6)
+        rows.append(row)
+    return {"stages_us_per_application": rows, "eager_vcycle_us": float(t.sum() * 1e6),
+            "applications": n_apps}
+
+
 def mg_solve(pmg, fine_box, degree, levels, level_precision, fine_op, b, tol):
     import torch
     t0 = time.perf_counter()
@@ -125,9 +191,82 @@ def mg_solve(pmg, fine_box, degree, levels, level_precision, fine_op, b, tol):
         results.append((time.perf_counter() - t0, its, r0, r, x))
     t, its, r0, r, x = results[-1]
     err = fine_op.l2_error(x, 1)
+    out = {"time_s": t, "iterations": its, "rel_residual": r / r0, "l2_error": err,
+           "setup_s": setup, "roofline": solve_roofline(pmg, ops, fine_op, its, t),
+           "levels_table": level_table(pmg, mg, ops, b)}
     del mg, ops, trs, sms
-    return {"time_s": t, "iterations": its, "rel_residual": r / r0, "l2_error": err,
-            "setup_s": setup}, x
+    return out, x
+
+
+def host_threads():
+    """Every host core this job may use: the OpenMP thread budget the job was
+    given (OMP_NUM_THREADS), else the CPUs of this process's affinity mask."""
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    aff = len(os.sched_getaffinity(0))
+    return max(1, int(omp)) if omp.isdigit() else max(1, aff)
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model,
+            "threads_used": host_threads()}
+
+
+def oracle_check_config4(pmg, n=24):
+    """The ph-multigrid configuration (deformed cube, Q6, p 6 -> 3 -> 1, then Q1
+    h-levels down to 3^3) at n^3 cells, where the FP64 oracle's MG-CG solves
+    it on the host cores in seconds: the GPU's FP64 and FP32-V-cycle/FP64-CG
+    solutions against the oracle's FP64 solution (max-norm and Euclidean
+    relative differences), and the iteration counts."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    import oracle
+    oracle.build()
+    fine = pmg.box(n, deform=1, amplitude=0.05)
+    h = 1
+    while (n >> h) >= 3 and (n >> (h - 1)) % 2 == 0:
+        h += 1
+    levels = pmg.ph_levels(fine, 6, h, p_sequence=[6, 3, 1])
+    meshes = [oracle.mesh(lv[0].nx, deform=1, amplitude=0.05) for lv in levels]
+    degs = [lv[1] for lv in levels]
+    t0 = time.perf_counter()
+    ref = oracle.Multigrid(meshes, degs, coarse_kind=1, coarse_tol=1e-12,
+                           n_threads=host_threads())
+    t_setup = time.perf_counter() - t0
+    bo = oracle.LaplaceBK3(meshes[-1], 6).rhs(0)
+    t0 = time.perf_counter()
+    xo, its_o, _, _ = ref.pcg(bo, precond=2, rel_tol=1e-10, max_it=200)
+    t_solve = time.perf_counter() - t0
+    del ref
+    out = {"workload": f"ph-MG Q6, {n}^3 deformed cells, p 6->3->1, Q1 h-levels "
+                       f"{n}->{levels[0][0].nx}, f = 1, CG to 1e-10",
+           "dofs": int(len(bo)), "levels": len(levels),
+           "oracle": {"iterations": its_o, "setup_s": t_setup, "solve_s": t_solve,
+                      "threads": host_threads()}}
+    outer = pmg.LaplaceOperatorBK3(fine, 6)
+    b = outer.assemble_rhs(0)
+    out["rhs_rel_diff"] = float(np.abs(b.cpu().numpy() - bo).max() / np.abs(bo).max())
+    for key, prec in (("fp64", pmg.FP64), ("fp32_vcycle_fp64_cg", pmg.FP32)):
+        mg, ops, _, _ = pmg.build_hierarchy(levels, precision=prec, outer_precision=pmg.FP64)
+        x = outer.initialize_dof_vector()
+        its, r0, r = pmg.solve_cg(outer, b, x, pmg.PRECOND_MG, mg, 1e-10, 200)
+        xg = x.cpu().numpy()
+        out[key] = {"iterations": its, "rel_residual": r / r0,
+                    "max_rel_diff_vs_oracle": float(np.abs(xg - xo).max() / np.abs(xo).max()),
+                    "l2_rel_diff_vs_oracle": float(np.linalg.norm(xg - xo) / np.linalg.norm(xo))}
+        del mg, ops
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(n_cells_dir, degree, nq, seconds):
@@ -140,7 +279,7 @@ def cpu_baseline(n_cells_dir, degree, nq, seconds):
     op = oracle.LaplaceBK3(m, degree, nq)
     log(f"[cpu] oracle setup {time.perf_counter() - t0:.1f}s, N={op.N}")
     x = np.random.Generator(np.random.PCG64(1)).uniform(-1, 1, op.N)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
 
     def rate(nt, budget):
         n, t = 0, 0.0
@@ -170,6 +309,7 @@ def cpu_baseline(n_cells_dir, degree, nq, seconds):
     t_solve = time.perf_counter() - t0
     log(f"[cpu] oracle MG-CG {n_s}^3 Q{degree}: setup {t_setup:.1f}s, solve {t_solve:.2f}s, {its} its")
     return {"value": multi, "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "host": host_info(),
             "sample": f"{n2} FP64 vmults of the same 64^3 Q4 operator with {threads} OpenMP "
                       f"threads ({t2:.1f}s); oracle = plain-C restatement of the reference "
                       f"BK3 kernel",
@@ -277,21 +417,22 @@ def bench_config4(pmg):
     rhs = outer.assemble_rhs(1)
     r64, x64 = solve_timed(pmg, outer, rhs, mg, pmg.PRECOND_MG, 1e-10)
     r64["setup_s"] = setup
-    r64["l2_error_vs_sin"] = outer.l2_error(x64, 1)
+    r64["roofline"] = solve_roofline(pmg, ops, outer, r64["iterations"], r64["time_s"])
+    r64["levels_table"] = level_table(pmg, mg, ops, rhs)
     del mg, ops[:-1]
     t0 = time.perf_counter()
     mg32, ops32, _, _ = pmg.build_hierarchy(levels, precision=pmg.FP32, outer_precision=pmg.FP64)
     setup = time.perf_counter() - t0
     r32, x32 = solve_timed(pmg, outer, rhs, mg32, pmg.PRECOND_MG, 1e-10)
     r32["setup_s"] = setup
-    r32["l2_error_vs_sin"] = outer.l2_error(x32, 1)
+    r32["roofline"] = solve_roofline(pmg, ops32, outer, r32["iterations"], r32["time_s"])
+    r32["levels_table"] = level_table(pmg, mg32, ops32, rhs)
     r32["rel_diff_vs_fp64_solution"] = ((x32 - x64).abs().max() / x64.abs().max()).item()
     return {"workload": "ph-MG Q6, 48^3 deformed cells, p 6->3->1, h 48->3 at Q1",
             "dofs": outer.m(), "levels": len(levels), "fp64": r64, "fp32_vcycle_fp64_cg": r32,
             "speedup_mixed": r64["time_s"] / r32["time_s"],
-            "note": "the deformation moves the boundary, so sin sin sin is not the exact "
-                    "solution there: l2_error_vs_sin is a fixed offset, the FP64/mixed "
-                    "difference is the precision figure"}
+            "fp64_oracle_error": "see cpu_baseline.ph_mg_q6_oracle (the same hierarchy at 24^3, "
+                                 "solved by the FP64 oracle)"}
 
 
 def bench_config5(pmg, n_dir=20):
@@ -554,6 +695,8 @@ def main():
     log(f"[bench] cell kernel {t_kernel * 1e6:.1f} us -> {achieved:.0f} GB/s "
         f"({achieved / HBM_PEAK_GBS:.3f} of peak), {alg_flops / t_kernel / 1e12:.2f} TFLOP/s")
 
+    fma64, fma32 = pmg.measure_fma_rate(pmg.FP64), pmg.measure_fma_rate(pmg.FP32)
+    log(f"[bench] FMA rate FP64 {fma64 / 1e12:.1f} TFLOP/s, FP32 {fma32 / 1e12:.1f} TFLOP/s")
     out = {
         "metric": METRIC, "value": gdofs, "unit": "GDoF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_step * 1e3, "higher_is_better": True,
@@ -572,6 +715,12 @@ def main():
                      "fp64_tflops": alg_flops / t_kernel / 1e12,
                      "fp64_frac": alg_flops / t_kernel / 1e12 / FP64_PEAK_TFLOPS},
     }
+    out["peaks_measured"] = {
+        "fp64_fma_tflops": fma64 / 1e12, "fp32_fma_tflops": fma32 / 1e12,
+        "fp64_vendor_tflops": FP64_PEAK_TFLOPS, "fp32_vendor_tflops": 157.3,
+        "note": "independent FMA chains in every lane (pmg_measure_fma_rate); FP32 is "
+                "issued unpacked here (packed FP32 disabled), hence at the FP64 rate",
+        "hbm_copy_note": "configs.rows.hbm_copy_1gib is the measured copy rate"}
     # The reference's own logs hold 14.16 GDoF/s for this vmult
     # (same DoF count, nq = p+1) on a GH200; they are run logs, not a
     # publication, so vs_baseline stays null and the ratio is reported here.
@@ -645,6 +794,9 @@ def main():
             out["cpu_baseline"]["mgcg_fp64"]["gpu_same_problem"] = r_gpu
             out["cpu_baseline"]["mgcg_fp64"]["gpu_speedup"] = (
                 out["cpu_baseline"]["mgcg_fp64"]["solve_s"] / r_gpu["time_s"])
+            log("[bench] ph-MG Q6 against the FP64 oracle")
+            out["cpu_baseline"]["ph_mg_q6_oracle"] = oracle_check_config4(pmg)
+            log(f"[bench] {json.dumps(out['cpu_baseline']['ph_mg_q6_oracle'])}")
     print(json.dumps(out), flush=True)
 
 

@@ -220,12 +220,17 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_po
  *  1 colored: one launch per colour of a greedy cell colouring with a plain
  *    read-modify-write scatter (atomics-free);
  *  2 patch: one launch; each workgroup's batch of cells accumulates the union
- *    of its DoFs in LDS and writes it once, plain stores for DoFs no other
- *    batch touches, atomics for the rest (fewer, longer-run atomics).
+ *    of its DoFs in LDS and adds it once per DoF, with device atomics in
+ *    sorted DoF order (fewer, longer-run atomics);
+ *  3 gather: two launches and no atomics on the output: the cells store
+ *    their values to a per-cell buffer, then every DoF sums its entries
+ *    through the inverse map (built on the device at the first selection;
+ *    extra device memory n_cells (p+1)^3 (4 + s) + 4 n_dofs bytes); the
+ *    summation order is fixed, so results repeat bit for bit.
  * Default: patch for n_q_points_1d <= 4 (FP64) or <= 6 (FP32) on operators
  * of more than 4096 cells, atomic otherwise (the faster of the two on MI355X
  * for each level type and size of the MG hierarchies);
- * PMG_BK3_SCHEDULE=atomic|colored|patch overrides it.
+ * PMG_BK3_SCHEDULE=atomic|colored|patch|gather overrides it.
  * g_load: 0 non-temporal G loads at kernel entry, 1 ordinary loads at entry,
  * 2 non-temporal loads at first use, -1 the schedule's default (0 for patch,
  * 2 otherwise; PMG_BK3_GLOAD overrides the constructor's choice).  Variant 1,
@@ -361,6 +366,16 @@ int pmg_multigrid_destroy(pmg_multigrid mg);
 int pmg_multigrid_vmult(pmg_multigrid mg, void *dst, const void *src, pmg_stream stream);
 /* *uses_graph = 1 if vmult replays a captured HIP graph (see above) */
 int pmg_multigrid_uses_graph(pmg_multigrid mg, int *uses_graph);
+/* Per-level wall times of the V-cycle, as MultigridSolver::print_wall_times
+ * tabulates them (tests/common/portable_multigrid_solver.h:108-132,215-268):
+ * n_applications eager V-cycles on src (outer precision; the defect is
+ * staged once) with device events around every stage.
+ * times[l * 6 + stage] = seconds per application at level l (0 = coarsest),
+ * stage 0 pre-smoothing, 1 residual matvec, 2 restriction (the defect is
+ * formed inside it), 3 coarse solve (level 0 only), 4 prolongation,
+ * 5 post-smoothing.  A measurement call: it creates events.  Synchronous. */
+int pmg_multigrid_level_times(pmg_multigrid mg, const void *src, int n_applications,
+                              double *times, pmg_stream stream);
 
 /* ------------------------------------------------------------ CG
  * Preconditioned CG of deal.II SolverCG as driven by MultigridSolver::solve_cg
@@ -397,6 +412,11 @@ int pmg_vector_dot(int precision, int64_t n, const void *a, const void *b, doubl
                    pmg_stream stream);
 int pmg_vector_convert(int64_t n, int dst_precision, void *dst, int src_precision,
                        const void *src, pmg_stream stream);
+/* Measurement only (no reference counterpart): the device's vector FMA rate
+ * in flop/s for the given precision, from a kernel of independent FMA chains
+ * in every lane (the FLOP peak the roofline fractions are quoted against is
+ * checked with it).  Allocates a scratch array.  Synchronous. */
+int pmg_measure_fma_rate(int precision, double *flops_per_second, pmg_stream stream);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ FP64, FP32 = 0, 1
 TRANSFER_H, TRANSFER_P, TRANSFER_DG_CG = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
 COARSE_CHEBYSHEV, COARSE_PCG = 0, 1
-SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH = 0, 1, 2
+SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH, SCHEDULE_GATHER = 0, 1, 2, 3
 TRANSFER_LAYOUT_WEIGHTED, TRANSFER_LAYOUT_OWNER_MAP, TRANSFER_LAYOUT_OWNER_LINES = 0, 1, 2
 INVALID_DOF = 0xFFFFFFFF
 
@@ -154,9 +154,11 @@ def lib():
         "pmg_multigrid_destroy": (_i, [_vp]),
         "pmg_multigrid_vmult": (_i, [_vp, _vp, _vp, _vp]),
         "pmg_multigrid_uses_graph": (_i, [_vp, P(_i)]),
+        "pmg_multigrid_level_times": (_i, [_vp, _vp, _i, _vp, _vp]),
         "pmg_cg_solve": (_i, [_vp, _i, _vp, _vp, _vp, _d, _i, P(_i), P(_d), P(_d), _vp]),
         "pmg_vector_dot": (_i, [_i, _i64, _vp, _vp, P(_d), _vp]),
         "pmg_vector_convert": (_i, [_i64, _i, _vp, _i, _vp, _vp]),
+        "pmg_measure_fma_rate": (_i, [_i, P(_d), _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -602,6 +604,7 @@ class VCycleMultigrid:
     def __init__(self, operators, transfers, smoothers, coarse=COARSE_PCG, coarse_tol=1e-12,
                  outer_precision=FP64):
         L = len(operators)
+        self.L = L
         self._keep = (operators, transfers, smoothers)
         self.outer_precision = outer_precision
         self.n_fine = operators[-1].m()
@@ -622,6 +625,18 @@ class VCycleMultigrid:
         _check(lib().pmg_multigrid_vmult(self.h, _dev_ptr(dst, p, self.n_fine),
                                          _dev_ptr(src, p, self.n_fine), _stream()))
 
+    LEVEL_STAGES = ("pre_smooth", "residual_matvec", "restrict", "coarse_solve", "prolongate",
+                    "post_smooth")
+
+    def level_times(self, src, n_applications=5):
+        """Seconds per application of each stage at each level (eager V-cycles,
+        device events): array [levels][6] in the order of LEVEL_STAGES."""
+        out = np.zeros(self.L * 6)
+        _check(lib().pmg_multigrid_level_times(self.h, _dev_ptr(src, self.outer_precision,
+                                                                 self.n_fine),
+                                               int(n_applications), _np_ptr(out), _stream()))
+        return out.reshape(self.L, 6)
+
     def uses_graph(self):
         """True if the V-cycle body replays a captured HIP graph."""
         v = ctypes.c_int()
@@ -640,6 +655,13 @@ def solve_cg(op, b, x, precond=PRECOND_JACOBI, mg=None, rel_tol=1e-10, max_it=10
     if st != OK and (raise_on_failure or st != ERR_NOT_CONVERGED):
         _check(st)
     return its.value, r0.value, r.value
+
+
+def measure_fma_rate(precision=FP64):
+    """Device vector FMA rate in flop/s (measurement only)."""
+    v = ctypes.c_double()
+    _check(lib().pmg_measure_fma_rate(precision, ctypes.byref(v), _stream()))
+    return v.value
 
 
 def vector_dot(a, b, precision=FP64):

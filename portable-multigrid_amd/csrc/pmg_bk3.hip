@@ -20,13 +20,20 @@
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
 namespace pmg {
 
 // One batch of E = Bk3Tile<NQ>::cells cells at positions cell_base .. cell_base+E-1
 // of a range of n_range cells (cell index = cell_ids[pos] if cell_ids, else pos),
 // processed by the first E*NQ^2 lanes of the workgroup (all lanes must call it:
-// it holds workgroup barriers).  lds: E * 3 * NQ^3 elements.  ATOMIC = false is
-// only valid when no two cells of the range share a DoF (one colour).
+// it holds workgroup barriers).  lds: E * 3 * NQ^3 elements.  SCATTER:
+// kScatterAtomic adds with device atomics; kScatterPlain with a plain
+// read-modify-write, valid only when no two cells of the range share a DoF
+// (one colour); kScatterStore stores the cell's values to y[cell][local] (a
+// per-cell buffer laid out like the DoF map), every entry, for a gather pass.
 // Compressed indices (optional, lines != nullptr): for a cell with lflag set,
 // every x-line (j,k) of its DoF map is a run of consecutive DoFs starting at
 // lines[cell][k][j]; other cells read the full map.  The line starts and the
@@ -34,7 +41,10 @@ namespace pmg {
 // WT: wave-aligned lane layout (Bk3Tile::per_wave whole cells per wave, the
 // workgroup has Bk3Tile::wave_lanes lanes): every phase of the cell sweeps is
 // wave-local, no workgroup barrier.
-template <int NM, int NQ, typename T, bool ATOMIC = true, int GMODE = kGEarlyNT, bool WT = false>
+enum : int { kScatterAtomic = 0, kScatterPlain = 1, kScatterStore = 2 };
+
+template <int NM, int NQ, typename T, int SCATTER = kScatterAtomic, int GMODE = kGEarlyNT,
+          bool WT = false>
 __device__ __forceinline__ void
 bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
           const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
@@ -81,12 +91,18 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                                                          P0 + 2 * NQ3, active, ta, tb);
   // scatter-add
   if (active && ta < NM && tb < NM) {
+    if (SCATTER == kScatterStore) {
+      T *yc = y + cell * NM3 + tb * NM + ta;
 #pragma unroll
-    for (int k = 0; k < NM; ++k)
-      if (idx[k] != kInvalid) {
-        if (ATOMIC) atomic_add(y + idx[k], v[k]);
-        else y[idx[k]] += v[k];  // one colour: this cell is the only writer
-      }
+      for (int k = 0; k < NM; ++k) yc[k * NM2] = v[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NM; ++k)
+        if (idx[k] != kInvalid) {
+          if (SCATTER == kScatterAtomic) atomic_add(y + idx[k], v[k]);
+          else y[idx[k]] += v[k];  // one colour: this cell is the only writer
+        }
+    }
   }
 }
 
@@ -101,7 +117,7 @@ constexpr int bk3_cells_lanes() {
   return bk3_wave_tile<NQ, T>() ? Bk3Tile<NQ>::wave_lanes : Bk3Tile<NQ>::lanes;
 }
 
-template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
+template <int NM, int NQ, typename T, int SCATTER, int GMODE>
 __global__ void __launch_bounds__((bk3_cells_lanes<NQ, T>()))
 bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
@@ -111,7 +127,7 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   constexpr int E = Bk3Tile<NQ>::cells;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  bk3_batch<NM, NQ, T, ATOMIC, GMODE, bk3_wave_tile<NQ, T>()>(
+  bk3_batch<NM, NQ, T, SCATTER, GMODE, bk3_wave_tile<NQ, T>()>(
       x, y, G, map, lines, lflag, cell_ids, n_range, sh, static_cast<int64_t>(lb) * E, lds);
 }
 
@@ -119,12 +135,14 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 // One batch of E cells per workgroup.  The batch's patch (the union of its
 // cells' unconstrained DoFs, sorted) is read once into LDS, each cell reads
 // its source column from the patch, and the cells' results are summed into
-// the patch with LDS atomics; the patch is then written once: DoFs that no
-// other batch touches with plain (coalescing) stores, DoFs shared with other
-// batches with device atomics, in sorted DoF order, so a wave's atomics cover
-// long runs of consecutive DoFs.  ASSIGN: y[d] = value for owned DoFs
-// (vmult; y must be zero on shared DoFs), else y[d] += value.
-template <int NM, int NQ, typename T, int GMODE, bool ASSIGN>
+// the patch with LDS atomics; the patch is then added to y once per DoF, in
+// sorted DoF order, so a wave's device atomics cover long runs of consecutive
+// DoFs.  Every DoF is added atomically, also those no other batch touches:
+// plain stores into cache lines that other batches' atomics also update were
+// measured slower on MI355X (Q4 64^3 FP64 549 -> 500 us, FP32 330 -> 321 us,
+// Q6 48^3 FP32 480 -> 406 us with all-atomic write-out), so pdof's bit 31
+// (shared with another batch) is not used here.
+template <int NM, int NQ, typename T, int GMODE>
 __global__ void __launch_bounds__(256)
 bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ pcount,
@@ -197,11 +215,7 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   for (int j = 0; j < PL; ++j) {
     const int i = tid + j * LANES;
     if (i < np) {
-      const uint32_t w = pw[j];
-      const uint32_t d = w & 0x7FFFFFFFu;
-      if (w & 0x80000000u) atomic_add(y + d, patch[i]);
-      else if (ASSIGN) y[d] = patch[i];
-      else y[d] += patch[i];
+      atomic_add(y + (pw[j] & 0x7FFFFFFFu), patch[i]);
     }
   }
 }
@@ -214,12 +228,9 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
   const int64_t nb = (n_cells + Tile::cells - 1) / Tile::cells;
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
-  if (assign)
-    hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true>), dim3(grid), dim3(Tile::lanes),
-                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, n_cells, sh);
-  else
-    hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, false>), dim3(grid), dim3(Tile::lanes),
-                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, n_cells, sh);
+  (void)assign;  // y += patch either way: callers of vmult zero the free rows first
+  hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE>), dim3(grid), dim3(Tile::lanes), 0, s,
+                     x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, n_cells, sh);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -504,7 +515,107 @@ template void launch_bk3_coarse_pcg<float>(int, int, int64_t, const float *, con
                                            float *, float *, float *, double, int, double *,
                                            hipStream_t);
 
-template <int NM, int NQ, typename T, bool ATOMIC, int GMODE>
+// ---------------------------------------------------------------- gather schedule
+// Second pass of the gather schedule: y[i] (=, or += if !assign) sum of
+// buf[list[k]] over k in [off[i], off[i+1]); DoFs without entries (the
+// constrained ones) are left untouched.  off and list are read once.
+template <typename T>
+__global__ void __launch_bounds__(256)
+bk3_gather_kernel(const int64_t n, const uint32_t *__restrict__ off,
+                  const uint32_t *__restrict__ list, const T *__restrict__ buf, T *__restrict__ y,
+                  const int assign)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k0 = __builtin_nontemporal_load(off + i);
+  const uint32_t k1 = __builtin_nontemporal_load(off + i + 1);
+  if (k0 == k1) return;
+  T acc = T(0);
+  for (uint32_t k = k0; k < k1; ++k) acc += buf[__builtin_nontemporal_load(list + k)];
+  y[i] = assign ? acc : y[i] + acc;
+}
+
+template <typename T>
+static void launch_bk3_gather(int64_t n, const uint32_t *off, const uint32_t *list, const T *buf,
+                              T *y, bool assign, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t nb = (n + 255) / 256;
+  PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many DoFs for one launch");
+  hipLaunchKernelGGL(bk3_gather_kernel<T>, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, n,
+                     off, list, buf, y, assign ? 1 : 0);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256)
+imap_count_kernel(const int64_t n_pos, const uint32_t *__restrict__ map, uint32_t *cnt) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pos) return;
+  const uint32_t d = map[e];
+  if (d != kInvalid) atomicAdd(cnt + d, 1u);
+}
+__global__ void __launch_bounds__(256)
+imap_fill_kernel(const int64_t n_pos, const uint32_t *__restrict__ map, uint32_t *cursor,
+                 uint32_t *__restrict__ list) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pos) return;
+  const uint32_t d = map[e];
+  if (d != kInvalid) list[atomicAdd(cursor + d, 1u)] = static_cast<uint32_t>(e);
+}
+// each DoF's entries in ascending order (insertion sort; hex meshes have <= 8)
+__global__ void __launch_bounds__(256)
+imap_sort_kernel(const int64_t n, const uint32_t *__restrict__ off, uint32_t *__restrict__ list) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k0 = off[i], k1 = off[i + 1];
+  for (uint32_t k = k0 + 1; k < k1; ++k) {
+    const uint32_t v = list[k];
+    uint32_t j = k;
+    while (j > k0 && list[j - 1] > v) {
+      list[j] = list[j - 1];
+      --j;
+    }
+    list[j] = v;
+  }
+}
+
+void build_inverse_map(int64_t n_pos, const uint32_t *map, int64_t n_dofs, uint32_t *off,
+                       uint32_t **list_out, int64_t *n_entries, hipStream_t s) {
+  PMG_REQUIRE(n_pos < (int64_t(1) << 32) && n_dofs < (int64_t(1) << 31), PMG_ERR_SIZE,
+              "inverse map: too many entries for 32-bit positions");
+  *list_out = nullptr;
+  *n_entries = 0;
+  DeviceArray<uint32_t> cnt(static_cast<size_t>(n_dofs + 1));
+  PMG_HIP_CHECK(hipMemsetAsync(cnt.ptr, 0, cnt.bytes(), s));
+  const unsigned gp = static_cast<unsigned>((n_pos + 255) / 256);
+  const unsigned gd = static_cast<unsigned>((n_dofs + 255) / 256);
+  if (n_pos > 0) hipLaunchKernelGGL(imap_count_kernel, dim3(gp), dim3(256), 0, s, n_pos, map, cnt.ptr);
+  PMG_HIP_CHECK(hipGetLastError());
+  size_t tmp_bytes = 0;
+  PMG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt.ptr, off,
+                                                 static_cast<int>(n_dofs + 1), s));
+  DeviceArray<char> tmp(std::max<size_t>(tmp_bytes, 1));
+  PMG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tmp_bytes, cnt.ptr, off,
+                                                 static_cast<int>(n_dofs + 1), s));
+  uint32_t total = 0;
+  PMG_HIP_CHECK(hipMemcpyAsync(&total, off + n_dofs, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  uint32_t *list = nullptr;
+  if (total > 0) {
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(&list), size_t(total) * sizeof(uint32_t));
+    if (e != hipSuccess) throw Error(PMG_ERR_ALLOC, "hipMalloc of the inverse map failed");
+    PMG_HIP_CHECK(hipMemcpyAsync(cnt.ptr, off, size_t(n_dofs) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, s));  // cursors
+    hipLaunchKernelGGL(imap_fill_kernel, dim3(gp), dim3(256), 0, s, n_pos, map, cnt.ptr, list);
+    PMG_HIP_CHECK(hipGetLastError());
+    if (n_dofs > 0) hipLaunchKernelGGL(imap_sort_kernel, dim3(gd), dim3(256), 0, s, n_dofs, off, list);
+    PMG_HIP_CHECK(hipGetLastError());
+    PMG_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  *list_out = list;
+  *n_entries = total;
+}
+
+template <int NM, int NQ, typename T, int SCATTER, int GMODE>
 static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
                              const uint32_t *lines, const uint8_t *lflag,
                              const uint32_t *cell_ids, int64_t n_range, const Shape1D<T> &sh,
@@ -514,7 +625,7 @@ static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
   const int64_t nb = (n_range + Tile::cells - 1) / Tile::cells;
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
-  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, ATOMIC, GMODE>), dim3(grid),
+  hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, SCATTER, GMODE>), dim3(grid),
                      dim3(bk3_cells_lanes<NQ, T>()), 0, s, x, y, G, map, lines, lflag, cell_ids,
                      n_range, sh);
   PMG_HIP_CHECK(hipGetLastError());
@@ -533,12 +644,25 @@ static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_
       bk3_launch_patch<NM, NQ, T, kGLateNT>(assign, x, y, G, *sc, n_cells, sh, s);
     return;
   }
+  if (sc && sc->kind == kGather && sc->gbuf) {
+    // cells -> per-cell buffer (plain coalesced stores), then every DoF sums
+    // its entries: no atomics on the output, and no zero fill before a vmult
+    T *buf = static_cast<T *>(sc->gbuf);
+    if (FULL && sc->gmode == kGEarlyNT)
+      bk3_launch_range<NM, NQ, T, kScatterStore, kGEarlyNT>(x, buf, G, map, sc->lines, sc->lflag,
+                                                           nullptr, n_cells, sh, s);
+    else
+      bk3_launch_range<NM, NQ, T, kScatterStore, kGLateNT>(x, buf, G, map, sc->lines, sc->lflag,
+                                                          nullptr, n_cells, sh, s);
+    launch_bk3_gather<T>(sc->gn, sc->goff, sc->glist, buf, y, assign, s);
+    return;
+  }
   if (sc && sc->kind == kColored) {
     // one launch per colour, no atomics; launches are stream-ordered, so a
     // colour sees the updates of the previous ones
     for (size_t c = 0; c + 1 < sc->color_offsets.size(); ++c) {
       const int64_t b = sc->color_offsets[c], n = sc->color_offsets[c + 1] - b;
-      bk3_launch_range<NM, NQ, T, false, kGEarlyNT>(x, y, G, map, sc->lines, sc->lflag,
+      bk3_launch_range<NM, NQ, T, kScatterPlain, kGEarlyNT>(x, y, G, map, sc->lines, sc->lflag,
                                                     sc->cell_ids + b, n, sh, s);
     }
     return;
@@ -547,12 +671,12 @@ static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_
   const uint32_t *lines = sc ? sc->lines : nullptr;
   const uint8_t *lflag = sc ? sc->lflag : nullptr;
   if (FULL && gmode == kGEarly)
-    bk3_launch_range<NM, NQ, T, true, (FULL ? kGEarly : kGLateNT)>(x, y, G, map, lines, lflag,
+    bk3_launch_range<NM, NQ, T, kScatterAtomic, (FULL ? kGEarly : kGLateNT)>(x, y, G, map, lines, lflag,
                                                                    nullptr, n_cells, sh, s);
   else if (FULL && gmode == kGEarlyNT)
-    bk3_launch_range<NM, NQ, T, true, kGEarlyNT>(x, y, G, map, lines, lflag, nullptr, n_cells, sh, s);
+    bk3_launch_range<NM, NQ, T, kScatterAtomic, kGEarlyNT>(x, y, G, map, lines, lflag, nullptr, n_cells, sh, s);
   else
-    bk3_launch_range<NM, NQ, T, true, kGLateNT>(x, y, G, map, lines, lflag, nullptr, n_cells, sh, s);
+    bk3_launch_range<NM, NQ, T, kScatterAtomic, kGLateNT>(x, y, G, map, lines, lflag, nullptr, n_cells, sh, s);
 }
 
 template <typename T>
@@ -581,7 +705,7 @@ void launch_bk3_cell_list(int degree, int nq, const T *x, T *y, const T *G, cons
                           int64_t n_list, const Shape1D<T> &sh, hipStream_t s) {
 #define PMG_BK3_CASE(P, Q)                                                                  \
   if (degree == P && nq == Q) {                                                             \
-    bk3_launch_range<P + 1, Q, T, true, kGLateNT>(x, y, G, map, lines, lflag, cell_ids,     \
+    bk3_launch_range<P + 1, Q, T, kScatterAtomic, kGLateNT>(x, y, G, map, lines, lflag, cell_ids,     \
                                                   n_list, sh, s);                           \
     return;                                                                                 \
   }

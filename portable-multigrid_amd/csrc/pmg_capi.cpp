@@ -283,8 +283,8 @@ int pmg_operator_set_schedule(pmg_operator op, int schedule, int g_load) {
   return guard([&] {
     PMG_NOT_NULL(op);
     PMG_REQUIRE(g_load >= -1 && g_load <= 2, PMG_ERR_INVALID_ARG, "g_load must be -1, 0, 1 or 2");
-    PMG_REQUIRE(schedule >= 0 && schedule <= 2, PMG_ERR_INVALID_ARG,
-                "schedule must be 0 (atomic), 1 (colored) or 2 (patch)");
+    PMG_REQUIRE(schedule >= 0 && schedule <= 3, PMG_ERR_INVALID_ARG,
+                "schedule must be 0 (atomic), 1 (colored), 2 (patch) or 3 (gather)");
     bk3_of(op).set_schedule(schedule, g_load);
   });
 }
@@ -450,6 +450,14 @@ int pmg_multigrid_uses_graph(pmg_multigrid mg, int *uses_graph) {
   });
 }
 
+int pmg_multigrid_level_times(pmg_multigrid mg, const void *src, int n_applications,
+                              double *times, pmg_stream s) {
+  return guard([&] {
+    PMG_NOT_NULL(mg);
+    mg->impl->level_times(src, n_applications, times, as_stream(s));
+  });
+}
+
 int pmg_multigrid_vmult(pmg_multigrid mg, void *dst, const void *src, pmg_stream s) {
   return guard([&] {
     PMG_NOT_NULL(mg);
@@ -530,6 +538,14 @@ int pmg_vector_convert(int64_t n, int dst_precision, void *dst, int src_precisio
       pmg::launch_convert<float, double>(n, static_cast<float *>(dst), static_cast<const double *>(src), st);
     else
       pmg::launch_convert<float, float>(n, static_cast<float *>(dst), static_cast<const float *>(src), st);
+  });
+}
+
+int pmg_measure_fma_rate(int precision, double *flops_per_second, pmg_stream s) {
+  return guard([&] {
+    pmg::check_precision(precision);
+    PMG_NOT_NULL(flops_per_second);
+    *flops_per_second = pmg::measure_fma_rate(precision, as_stream(s));
   });
 }
 

@@ -6,13 +6,12 @@
 #include <cstdint>
 
 namespace pmg {
-// F[c][q] = f(X_q) JxW_q on the trilinear cells (cv = [C][8][3] vertices)
-void launch_f_at_q(int kind, int nq, int64_t n_cells, const double *cv, const double *qp,
-                   const double *qw, double *F, hipStream_t s);
-// b[map[c][i]] += sum_q S_i(q) F[c][q]; S = [(p+1)*nq] (double)
+// b[map[c][i]] += sum_q S_i(q) f(X_q) JxW_q on the trilinear cells
+// (cv = [C][8][3] vertices, S = [(p+1)*nq] shape values, double)
 template <typename T>
-void launch_integrate(int nm, int nq, int64_t n_cells, const double *S, const uint32_t *map,
-                      const double *F, T *b, hipStream_t s);
+void launch_integrate(int kind, int nm, int nq, int64_t n_cells, const double *cv,
+                      const double *qp, const double *qw, const double *S, const uint32_t *map,
+                      T *b, hipStream_t s);
 // per-block partial sums of (u_h - u)^2 JxW with Gauss(nq)
 template <typename T>
 void launch_l2_error(int kind, int nm, int nq, int64_t n_cells, const double *cv, const double *qp,

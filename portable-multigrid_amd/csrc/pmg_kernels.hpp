@@ -96,9 +96,13 @@ __host__ __device__ inline int64_t g_size(int nq, int64_t n_cells) {
 //           (no two cells of a colour share a DoF);
 //  kPatch   all cells in one launch; a batch gathers the union of its cells'
 //           DoFs ("patch") into LDS, its cells accumulate into the patch with
-//           LDS atomics, and the patch is written once: DoFs no other batch
-//           touches with plain stores, shared ones with device atomics.
-enum Bk3ScheduleKind : int { kAtomic = 0, kColored = 1, kPatch = 2 };
+//           LDS atomics, and the patch is added to y once per DoF with device
+//           atomics in sorted DoF order (fewer, longer-run atomics);
+//  kGather  two launches, no atomics on the output: the cells store their
+//           nm^3 values to a per-cell buffer laid out like the DoF map, then
+//           every DoF sums its buffer entries through the inverse map (CSR,
+//           entries in ascending order: the result is reproducible bit for bit).
+enum Bk3ScheduleKind : int { kAtomic = 0, kColored = 1, kPatch = 2, kGather = 3 };
 struct Bk3Schedule {
   int kind = kAtomic;
   int gmode = 2;                        // G loads: 0 early non-temporal, 1 early, 2 late non-temporal
@@ -120,10 +124,23 @@ struct Bk3Schedule {
   // x-line are consecutive; null when too few cells qualify
   const uint16_t *pline = nullptr;
   const uint8_t *pflag = nullptr;
+  // gather data (kGather): inverse map of the gn DoFs, goff[gn + 1] into
+  // glist (entry positions cell * nm^3 + local, ascending per DoF), and the
+  // per-cell buffer gbuf [n_cells][nm^3] of the operator precision
+  const uint32_t *goff = nullptr, *glist = nullptr;
+  void *gbuf = nullptr;
+  int64_t gn = 0;
 };
 
-// y += sum_cells A_e x.  assign (patch schedule only): DoFs owned by one
-// batch are overwritten instead of added to (y must be zero on the others)
+// Inverse map of a DoF map for the gather schedule, built on the device:
+// off[n_dofs + 1] and list[off[n_dofs]] with the positions e of map[e] == d
+// in ascending order for each DoF d (invalid entries are in no list).
+// list_out is allocated here (it needs the entry count).  Synchronous.
+void build_inverse_map(int64_t n_positions, const uint32_t *map, int64_t n_dofs, uint32_t *off,
+                       uint32_t **list_out, int64_t *n_entries, hipStream_t s);
+
+// y += sum_cells A_e x.  assign is accepted for the callers' intent (vmult,
+// which zeroes the free rows first) and does not change the result.
 template <typename T>
 void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const uint32_t *map,
                       int64_t n_cells, const Shape1D<T> &sh, hipStream_t s,
@@ -300,5 +317,10 @@ void launch_reduce_partials(const double *partials, int n_partials, double *sc, 
 // also: sc[7] = 1 once sc[2] = r.r <= target2 (or NaN); host[0] = r.r (pinned, optional)
 void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host = nullptr,
                        double target2 = -1.0);
+
+// Measured FMA issue rate of the device (flop/s) in the given precision:
+// independent FMA chains in every lane of 16 workgroups per CU (measurement
+// only, allocates a scratch array).  Synchronous.
+double measure_fma_rate(int precision, hipStream_t s);
 
 }  // namespace pmg

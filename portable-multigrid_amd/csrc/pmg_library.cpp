@@ -213,8 +213,10 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     const char *genv = std::getenv("PMG_BK3_GLOAD");
     int kind = nq <= (precision == PMG_FP32 ? 6 : 4) && n_cells > 4096 ? kPatch : kAtomic;
     if (env)
-      kind = std::string(env) == "colored" ? kColored
-             : std::string(env) == "atomic" ? kAtomic : kPatch;
+      kind = std::string(env) == "colored"  ? kColored
+             : std::string(env) == "atomic" ? kAtomic
+             : std::string(env) == "gather" ? kGather
+                                            : kPatch;
     set_schedule(kind, genv ? std::atoi(genv) : -1);
   }
   G_.reinit(precision, g_size(nq, n_cells));
@@ -343,11 +345,33 @@ void LaplaceOperatorBK3::build_patches_once() {
   build_patch_schedule(imap, n_cells_, degree_, nq_, n_dofs_, patches_, sched_);
 }
 
+// The gather schedule's inverse map (built on the device) and cell buffer.
+void LaplaceOperatorBK3::build_gather_once() {
+  if (sched_.gbuf || n_cells_ == 0) return;
+  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
+  if (n_cells_ * nm3 >= (int64_t(1) << 32) || n_dofs_ >= (int64_t(1) << 31)) return;
+  goff_.resize(static_cast<size_t>(n_dofs_ + 1));
+  uint32_t *list = nullptr;
+  int64_t ne = 0;
+  build_inverse_map(n_cells_ * nm3, map_.ptr, n_dofs_, goff_.ptr, &list, &ne, nullptr);
+  glist_.release();
+  glist_.ptr = list;
+  glist_.n = static_cast<size_t>(ne);
+  gbuf_.reinit(precision_, n_cells_ * nm3);
+  sched_.goff = goff_.ptr;
+  sched_.glist = glist_.ptr;
+  sched_.gbuf = gbuf_.data();
+  sched_.gn = n_dofs_;
+}
+
 void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {
+  PMG_REQUIRE(kind >= kAtomic && kind <= kGather, PMG_ERR_INVALID_ARG, "unknown schedule");
   if (kind == kPatch && sched_.pdof == nullptr) build_patches_once();
   if (kind == kColored && sched_.color_offsets.empty()) build_coloring();
+  if (kind == kGather && sched_.gbuf == nullptr) build_gather_once();
   if (kind == kColored && sched_.color_offsets.empty()) kind = kAtomic;
   if (kind == kPatch && sched_.pdof == nullptr) kind = kAtomic;
+  if (kind == kGather && sched_.gbuf == nullptr) kind = kAtomic;
   sched_.kind = kind;
   // -1: the schedule's measured best G-load policy (the patch kernel's first
   // phase only gathers indices and x, which the G stream then overlaps)
@@ -369,9 +393,11 @@ void LaplaceOperatorBK3::vmult_t(T *dst, const T *src, hipStream_t s) const {
   // cells (saves a launch, which dominates there); large ones: a zero fill
   // and an indexed copy of the constrained rows (fewer bytes than the fused
   // pass's byte mask and conditional loads, measured faster at 17M DoFs).
-  const bool fused = n_constrained_ > 0 && n_dofs_ <= kFusedPrepMaxDofs;
+  // The gather schedule overwrites every free row itself: no fill.
+  const bool gather = sched_.kind == kGather;
+  const bool fused = !gather && n_constrained_ > 0 && n_dofs_ <= kFusedPrepMaxDofs;
   if (fused) launch_prep_constrained<T>(n_dofs_, cmask_.ptr, dst, src, s);
-  else launch_fill<T>(n_dofs_, dst, T(0), s);
+  else if (!gather) launch_fill<T>(n_dofs_, dst, T(0), s);
   launch_bk3_cells<T>(degree_, nq_, src, dst, static_cast<const T *>(G_.data()), map_.ptr,
                       n_cells_, sh, s, &sched_, /*assign=*/true);
   if (!fused) launch_copy_indexed<T>(n_constrained_, constrained_.ptr, dst, src, s);
@@ -586,17 +612,18 @@ void assemble_rhs_cells(int precision, int degree, int64_t n_cells, int64_t n_do
   PMG_REQUIRE(nq >= 1 && nq <= 12, PMG_ERR_INVALID_ARG, "n_q_points_1d out of range");
   const ShapeData sd = make_shape_data(degree, std::max(nq, degree + 1));
   PMG_REQUIRE(sd.n_q_points_1d == nq, PMG_ERR_INVALID_ARG, "need n_q_points_1d >= degree+1");
-  DeviceArray<double> S, qp, qw, F(static_cast<size_t>(n_cells * nq * nq * nq));
+  DeviceArray<double> S, qp, qw;
   S.upload(sd.S);
   qp.upload(sd.quad.points);
   qw.upload(sd.quad.weights);
-  launch_f_at_q(f_kind, nq, n_cells, vertices, qp.ptr, qw.ptr, F.ptr, s);
   if (precision == PMG_FP64) {
     launch_fill<double>(n_dofs, static_cast<double *>(b), 0.0, s);
-    launch_integrate<double>(degree + 1, nq, n_cells, S.ptr, map, F.ptr, static_cast<double *>(b), s);
+    launch_integrate<double>(f_kind, degree + 1, nq, n_cells, vertices, qp.ptr, qw.ptr, S.ptr, map,
+                             static_cast<double *>(b), s);
   } else {
     launch_fill<float>(n_dofs, static_cast<float *>(b), 0.0f, s);
-    launch_integrate<float>(degree + 1, nq, n_cells, S.ptr, map, F.ptr, static_cast<float *>(b), s);
+    launch_integrate<float>(f_kind, degree + 1, nq, n_cells, vertices, qp.ptr, qw.ptr, S.ptr, map,
+                            static_cast<float *>(b), s);
   }
   PMG_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
 }
@@ -614,7 +641,7 @@ double l2_error_cells(int precision, int degree, int64_t n_cells, const double *
   S.upload(sd.S);
   qp.upload(sd.quad.points);
   qw.upload(sd.quad.weights);
-  const int nb = 2048;
+  const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(n_cells, 16384)));
   DeviceArray<double> partials(nb);
   if (precision == PMG_FP64)
     launch_l2_error<double>(u_kind, degree + 1, nq, n_cells, vertices, qp.ptr, qw.ptr, S.ptr, map,
@@ -1356,10 +1383,53 @@ void VCycleMultigrid::capture() const {
   if (hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0) != hipSuccess) give_up();
 }
 
+void VCycleMultigrid::mark(int level, int stage, bool begin, hipStream_t s) const {
+  if (!prof_) return;
+  if (begin) {
+    StageEvent ev{level, stage, nullptr, nullptr};
+    PMG_HIP_CHECK(hipEventCreate(&ev.begin));
+    PMG_HIP_CHECK(hipEventCreate(&ev.end));
+    prof_->push_back(ev);
+    PMG_HIP_CHECK(hipEventRecord(ev.begin, s));
+  } else {
+    PMG_HIP_CHECK(hipEventRecord(prof_->back().end, s));
+  }
+}
+
+void VCycleMultigrid::level_times(const void *src, int n_applications, double *times,
+                                  hipStream_t s) const {
+  PMG_REQUIRE(src && times && n_applications > 0, PMG_ERR_INVALID_ARG, "bad arguments");
+  std::vector<StageEvent> events;
+  events.reserve(static_cast<size_t>(n_applications) * levels() * kStages);
+  struct Reset {  // the events are destroyed and profiling is off on every exit
+    const VCycleMultigrid *mg;
+    std::vector<StageEvent> *ev;
+    ~Reset() {
+      mg->prof_ = nullptr;
+      for (auto &e : *ev) {
+        if (e.begin) (void)hipEventDestroy(e.begin);
+        if (e.end) (void)hipEventDestroy(e.end);
+      }
+    }
+  } reset{this, &events};
+  stage(src, s, nullptr);
+  prof_ = &events;
+  for (int a = 0; a < n_applications; ++a) v_cycle(levels() - 1, s);
+  prof_ = nullptr;
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < levels() * kStages; ++i) times[i] = 0.0;
+  for (const auto &e : events) {
+    float ms = 0.f;
+    PMG_HIP_CHECK(hipEventElapsedTime(&ms, e.begin, e.end));
+    times[e.level * kStages + e.stage] += 1e-3 * ms / n_applications;
+  }
+}
+
 void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
   const bool f64 = precision_ == PMG_FP64;
   const int64_t n = ops_[l]->m();
   if (l == 0) {
+    mark(0, 3, true, s);
     const bool c64 = ops_[0]->precision() == PMG_FP64;
     void *cdef = coarse64_ ? c64_def_.data() : def_[0].data();
     void *csol = coarse64_ ? c64_sol_.data() : sol_[0].data();
@@ -1387,21 +1457,31 @@ void VCycleMultigrid::v_cycle(int l, hipStream_t s) const {
     if (coarse64_)
       launch_convert<float, double>(n, static_cast<float *>(sol_[0].data()),
                                     static_cast<const double *>(csol), s);
+    mark(0, 3, false, s);
     return;
   }
   // pre-smoothing; its last update prepares t for the residual's matvec
-  if (sm_[l]->vmult_and_prepare(sol_[l].data(), def_[l].data(), t_[l].data(), s))
-    ops_[l]->vmult_prepared(t_[l].data(), sol_[l].data(), s);
-  else
-    ops_[l]->vmult(t_[l].data(), sol_[l].data(), s);
+  mark(l, 0, true, s);
+  const bool prepared = sm_[l]->vmult_and_prepare(sol_[l].data(), def_[l].data(), t_[l].data(), s);
+  mark(l, 0, false, s);
+  mark(l, 1, true, s);
+  if (prepared) ops_[l]->vmult_prepared(t_[l].data(), sol_[l].data(), s);
+  else ops_[l]->vmult(t_[l].data(), sol_[l].data(), s);
+  mark(l, 1, false, s);
+  mark(l, 2, true, s);
   const int64_t nc = ops_[l - 1]->m();
   if (f64) launch_fill<double>(nc, static_cast<double *>(def_[l - 1].data()), 0.0, s);
   else launch_fill<float>(nc, static_cast<float *>(def_[l - 1].data()), 0.0f, s);
   // coarse defect = R (d - A s): the residual is formed inside the restriction
   xfer_[l]->restrict_residual_and_add(def_[l - 1].data(), def_[l].data(), t_[l].data(), s);
+  mark(l, 2, false, s);
   v_cycle(l - 1, s);
+  mark(l, 4, true, s);
   xfer_[l]->prolongate_and_add(sol_[l].data(), sol_[l - 1].data(), s);
+  mark(l, 4, false, s);
+  mark(l, 5, true, s);
   sm_[l]->step(sol_[l].data(), def_[l].data(), s);
+  mark(l, 5, false, s);
 }
 
 void VCycleMultigrid::stage(const void *src, hipStream_t s, const double *skip) const {

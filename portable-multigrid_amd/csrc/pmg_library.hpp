@@ -147,6 +147,7 @@ class LaplaceOperatorBK3 : public Operator {
   template <typename T> void fill_geometry_t(const double *kappa, hipStream_t s);
   void build_coloring();
   void build_patches_once();
+  void build_gather_once();
 
   int degree_, nq_, precision_;
   int64_t n_cells_, n_dofs_, n_constrained_;
@@ -160,6 +161,9 @@ class LaplaceOperatorBK3 : public Operator {
   // cell colouring for the atomics-free schedule, DoF patches for the patch schedule
   DeviceArray<uint32_t> cell_ids_;
   PatchArrays patches_;
+  // gather schedule: inverse map (CSR) and the per-cell buffer
+  DeviceArray<uint32_t> goff_, glist_;
+  DeviceVector gbuf_;
   // compressed DoF indices (x-line starts) and their per-cell flag
   DeviceArray<uint32_t> lines_;
   DeviceArray<uint8_t> lflag_;
@@ -178,8 +182,9 @@ class LaplaceOperatorBK3 : public Operator {
 
  public:
   // kAtomic (one launch, device FP atomics per cell), kColored (one launch per
-  // colour, plain scatter) or kPatch (batch patches in LDS, see pmg_bk3.hip);
-  // a schedule whose set-up data is missing falls back to kAtomic.
+  // colour, plain scatter), kPatch (batch patches in LDS) or kGather (cell
+  // buffer + inverse-map gather), see pmg_bk3.hip; a schedule whose set-up
+  // data is missing falls back to kAtomic.
   // gmode: G load policy (see pmg_bk3.hip)
   void set_schedule(int kind, int gmode);
   int n_colors() const { return static_cast<int>(sched_.color_offsets.size()) - 1; }
@@ -384,10 +389,24 @@ class VCycleMultigrid {
   int levels() const { return static_cast<int>(ops_.size()); }
   // the V-cycle body runs as a captured HIP graph (no host sync inside it)
   bool uses_graph() const { return use_graph_; }
+  // Per-level stage times of n_applications eager V-cycles on src (the
+  // reference's timing table, tests/common/portable_multigrid_solver.h:108-132):
+  // times[l * kStages + stage] = average seconds per application, stages
+  // pre-smoothing, residual matvec, restriction (with the defect), coarse
+  // solve (level 0 only), prolongation, post-smoothing.  Synchronous.
+  static constexpr int kStages = 6;
+  void level_times(const void *src, int n_applications, double *times, hipStream_t s) const;
 
  private:
   void v_cycle(int level, hipStream_t s) const;
   void capture() const;
+  // stage events of level_times (null outside it)
+  struct StageEvent {
+    int level, stage;
+    hipEvent_t begin, end;
+  };
+  mutable std::vector<StageEvent> *prof_ = nullptr;
+  void mark(int level, int stage, bool begin, hipStream_t s) const;
   std::vector<const Operator *> ops_;
   std::vector<const MGTransfer *> xfer_;
   std::vector<const PreconditionChebyshev *> sm_;

@@ -8,6 +8,9 @@
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
+#include <algorithm>
+#include <type_traits>
+
 namespace pmg {
 
 namespace {
@@ -429,5 +432,59 @@ PMG_CU(float, double)
 PMG_CU(double, double)
 PMG_CU(float, float)
 #undef PMG_CU
+
+// ---------------------------------------------------------------- FMA rate probe
+// Measurement only: each lane runs kFmaChains independent dependency chains
+// of kFmaIters fused multiply-adds (2 flops each), so the issue rate, not the
+// latency, bounds it; the sum is stored so nothing is eliminated.
+constexpr int kFmaChains = 8, kFmaIters = 2048;
+template <typename T>
+__global__ void __launch_bounds__(256) fma_probe_k(T seed, T *out) {
+  T a[kFmaChains];
+#pragma unroll
+  for (int c = 0; c < kFmaChains; ++c) a[c] = seed + T(c) + T(threadIdx.x) * T(1e-6);
+  const T m = T(0.999999), add = T(1e-7);
+  for (int i = 0; i < kFmaIters; ++i) {
+#pragma unroll
+    for (int c = 0; c < kFmaChains; ++c) {
+      if constexpr (std::is_same<T, double>::value) a[c] = __builtin_fma(a[c], m, add);
+      else a[c] = __builtin_fmaf(a[c], m, add);
+    }
+  }
+  T sum = 0;
+#pragma unroll
+  for (int c = 0; c < kFmaChains; ++c) sum += a[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
+double measure_fma_rate(int precision, hipStream_t s) {
+  int dev = 0, n_cu = 0;
+  PMG_HIP_CHECK(hipGetDevice(&dev));
+  PMG_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  const unsigned blocks = static_cast<unsigned>(std::max(n_cu, 1)) * 16;
+  const size_t lanes = size_t(blocks) * 256;
+  DeviceArray<double> out(lanes);
+  hipEvent_t e0, e1;
+  PMG_HIP_CHECK(hipEventCreate(&e0));
+  PMG_HIP_CHECK(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (int rep = 0; rep < 4; ++rep) {
+    PMG_HIP_CHECK(hipEventRecord(e0, s));
+    if (precision == PMG_FP64)
+      hipLaunchKernelGGL(fma_probe_k<double>, dim3(blocks), dim3(256), 0, s, 1.0, out.ptr);
+    else
+      hipLaunchKernelGGL(fma_probe_k<float>, dim3(blocks), dim3(256), 0, s, 1.0f,
+                         reinterpret_cast<float *>(out.ptr));
+    PMG_HIP_CHECK(hipGetLastError());
+    PMG_HIP_CHECK(hipEventRecord(e1, s));
+    PMG_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (rep > 0) best = std::min(best, ms);  // the first launch loads the code object
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 2.0 * kFmaChains * kFmaIters * double(lanes) / (1e-3 * best);
+}
 
 }  // namespace pmg

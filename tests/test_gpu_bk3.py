@@ -151,10 +151,12 @@ def test_bad_arguments_raise(pmg):
     (4, 5, 1, 0), (4, 5, 0, 1), (4, 5, 0, 0), (4, 5, 2, 2), (4, 6, 1, 0), (4, 6, 2, 2),
     (2, 3, 1, 0), (6, 7, 1, 0), (1, 2, 2, 2), (3, 4, 2, 2), (5, 7, 2, 2), (6, 8, 2, 2),
     (1, 2, 0, 2), (3, 4, 0, 2), (4, 5, 2, 0), (4, 5, 2, 1), (3, 4, 2, 0), (6, 7, 2, 0),
-    (1, 2, 2, -1), (5, 6, 0, -1)])
+    (1, 2, 2, -1), (5, 6, 0, -1), (1, 2, 3, -1), (2, 3, 3, -1), (3, 4, 3, -1), (4, 5, 3, -1),
+    (4, 5, 3, 0), (4, 6, 3, -1), (5, 6, 3, -1), (6, 7, 3, -1), (6, 8, 3, -1)])
 def test_schedules_agree(pmg, orc, p, nq, sched, gload):
     """Atomics-free colour-by-colour scatter, the batch-patch scatter, the
-    per-cell atomic scatter and the G-load variants all give the oracle's result."""
+    per-cell atomic scatter, the cell-buffer gather and the G-load variants
+    all give the oracle's result."""
     import torch
     op, ref = _pair(pmg, orc, 3, p, nq, 1, pmg.FP64)
     op.set_schedule(sched, gload)
@@ -175,7 +177,7 @@ def test_internal_cell_orders(pmg, orc, order, monkeypatch):
     op, ref = _pair(pmg, orc, n, p, nq, 1, pmg.FP64)
     x = seeded_uniform(op.m(), 61)
     y = ref.vmult(x)
-    for sched in (0, 2):
+    for sched in (0, 2, 3):
         op.set_schedule(sched, 2)
         dst = op.initialize_dof_vector()
         op.vmult(dst, torch.tensor(x, device="cuda"))
@@ -236,3 +238,40 @@ def test_bk3_rhs_and_l2_error_match_oracle(pmg, orc, n, p, nq, deform, prec):
             e = op.l2_error(ut, u_kind, q)
             er = ref.l2_error(uq, u_kind, None if q == 0 else q)
             assert abs(e - er) <= (1e-12 if prec == "fp64" else 1e-6) * er, (u_kind, q)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_colored_schedule_is_bitwise_reproducible(pmg, prec):
+    """The atomics-free colored and gather schedules add every cell's values
+    in one fixed order, so repeated vmults (and a solve driven by them) are
+    bitwise equal;
+    the default atomic scatter may differ in the last bits between runs (its
+    summation order is the hardware's), within 1e-15 relative in FP64."""
+    import torch
+    print(f"[determinism] {prec}", flush=True)
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    b = pmg.box(6, deform=1, amplitude=0.05)
+    op = pmg.LaplaceOperatorBK3(b, 4, None, precision)
+    x = torch.tensor(seeded_uniform(op.m(), 121), dtype=op.dtype, device="cuda")
+    for sched in (pmg.SCHEDULE_GATHER, pmg.SCHEDULE_COLORED):
+        op.set_schedule(sched)
+        ys = []
+        for _ in range(4):
+            y = op.initialize_dof_vector()
+            op.vmult(y, x)
+            ys.append(y)
+        assert all(torch.equal(ys[0], y) for y in ys[1:]), sched
+    op.compute_diagonal()
+    sol = []
+    for _ in range(2):
+        u = op.initialize_dof_vector()
+        pmg.solve_cg(op, x, u, pmg.PRECOND_JACOBI, None, 1e-6 if prec == "fp32" else 1e-10, 2000)
+        sol.append(u)
+    assert torch.equal(sol[0], sol[1])
+    op.set_schedule(pmg.SCHEDULE_ATOMIC)
+    ya, yb = op.initialize_dof_vector(), op.initialize_dof_vector()
+    op.vmult(ya, x)
+    op.vmult(yb, x)
+    tol = 1e-15 if prec == "fp64" else 1e-6
+    assert rel_err(ya.double().cpu().numpy(), ys[0].double().cpu().numpy()) < tol * 10
+    assert rel_err(ya.double().cpu().numpy(), yb.double().cpu().numpy()) < tol * 10

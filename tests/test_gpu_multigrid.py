@@ -123,7 +123,7 @@ def test_chebyshev_matches_oracle(pmg, orc, p, deform):
     assert rel_err(x.cpu().numpy(), xr) < 1e-11
 
 
-@pytest.mark.parametrize("sched", ["patch", "atomic"])
+@pytest.mark.parametrize("sched", ["patch", "atomic", "gather"])
 @pytest.mark.parametrize("constrained", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_chebyshev_fused_output_prep(pmg, constrained, prec, sched):
@@ -144,7 +144,8 @@ def test_chebyshev_fused_output_prep(pmg, constrained, prec, sched):
                                                 np.zeros(0, dtype=np.uint32),
                                                 cell_vertices=pmg.box_cell_vertices(b),
                                                 precision=precision)
-    op.set_schedule(pmg.SCHEDULE_PATCH if sched == "patch" else pmg.SCHEDULE_ATOMIC)
+    op.set_schedule({"patch": pmg.SCHEDULE_PATCH, "atomic": pmg.SCHEDULE_ATOMIC,
+                     "gather": pmg.SCHEDULE_GATHER}[sched])
     op.compute_diagonal()
     sm = pmg.PreconditionChebyshev(op, 3, 15.0, 10)
     mx, _, deg = sm.info()

@@ -62,7 +62,8 @@ def test_bk3_equivariant_under_renumbering(pmg, p, nq, prec):
     op1.vmult(y1, torch.tensor(x, dtype=dt, device="cuda"))
     y1 = y1.cpu().numpy().astype(np.float64)
     x2 = torch.tensor(_to(pi, x), dtype=dt, device="cuda")
-    for sched in (None, pmg.SCHEDULE_ATOMIC, pmg.SCHEDULE_COLORED, pmg.SCHEDULE_PATCH):
+    for sched in (None, pmg.SCHEDULE_ATOMIC, pmg.SCHEDULE_COLORED, pmg.SCHEDULE_PATCH,
+                  pmg.SCHEDULE_GATHER):
         if sched is not None:
             op2.set_schedule(sched)
         y2 = op2.initialize_dof_vector()
