@@ -226,17 +226,28 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_po
  *    their values to a per-cell buffer, then every DoF sums its entries
  *    through the inverse map (built on the device at the first selection;
  *    extra device memory n_cells (p+1)^3 (4 + s) + 4 n_dofs bytes); the
- *    summation order is fixed, so results repeat bit for bit.
+ *    summation order is fixed, so results repeat bit for bit;
+ *  4 grid: the same two passes without the inverse map, for an operator
+ *    declared a lexicographic box (pmg_operator_set_box_structure): the cells
+ *    store to the box's exploded grid and every DoF sums its <= 8 adjacent
+ *    copies (coalesced, no atomics, bit-reproducible; n_cells (p+1)^3 s bytes).
  * Default: patch for n_q_points_1d <= 4 (FP64) or <= 6 (FP32) on operators
  * of more than 4096 cells, atomic otherwise (the faster of the two on MI355X
  * for each level type and size of the MG hierarchies);
- * PMG_BK3_SCHEDULE=atomic|colored|patch|gather overrides it.
+ * PMG_BK3_SCHEDULE=atomic|colored|patch|gather|grid overrides it.
  * g_load: 0 non-temporal G loads at kernel entry, 1 ordinary loads at entry,
  * 2 non-temporal loads at first use, -1 the schedule's default (0 for patch,
  * 2 otherwise; PMG_BK3_GLOAD overrides the constructor's choice).  Variant 1,
  * and variant 0 of the atomic schedule, exist for degree 4 only; elsewhere
  * they fall back to 2. */
 int pmg_operator_set_schedule(pmg_operator op, int schedule, int g_load);
+/* Declares the operator's cells the lexicographic box of nx x ny x nz cells
+ * (x fastest) with the lexicographic DoF numbering on the (p nx+1)(p ny+1)
+ * (p nz+1) grid -- what pmg_box_dofmap produces; constrained DoFs may appear
+ * as 0xFFFFFFFF.  Verified on the device against the operator's map
+ * (PMG_ERR_INVALID_ARG if it differs); enables schedule 4 (grid).  The
+ * reference's MatrixFree knows no such structure: no counterpart. */
+int pmg_operator_set_box_structure(pmg_operator op, int nx, int ny, int nz);
 /* Bytes one vmult must move (BK3: s(2N + 6 C nq^3); DG: adds the face data
  * and connectivity) and its flop count. */
 int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops);

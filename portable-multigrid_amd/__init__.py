@@ -31,7 +31,7 @@ FP64, FP32 = 0, 1
 TRANSFER_H, TRANSFER_P, TRANSFER_DG_CG = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
 COARSE_CHEBYSHEV, COARSE_PCG = 0, 1
-SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH, SCHEDULE_GATHER = 0, 1, 2, 3
+SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH, SCHEDULE_GATHER, SCHEDULE_GRID = 0, 1, 2, 3, 4
 TRANSFER_LAYOUT_WEIGHTED, TRANSFER_LAYOUT_OWNER_MAP, TRANSFER_LAYOUT_OWNER_LINES = 0, 1, 2
 INVALID_DOF = 0xFFFFFFFF
 
@@ -138,6 +138,7 @@ def lib():
         "pmg_operator_algorithmic_bytes": (_i, [_vp, P(_d), P(_d)]),
         "pmg_operator_assemble_rhs": (_i, [_vp, _i, _i, _vp, _vp]),
         "pmg_operator_set_schedule": (_i, [_vp, _i, _i]),
+        "pmg_operator_set_box_structure": (_i, [_vp, _i, _i, _i]),
         "pmg_operator_l2_error": (_i, [_vp, _vp, _i, _i, P(_d), _vp]),
         "pmg_transfer_create": (_i, [_i, _i, _i, _i, _i64, _vp, _vp, _vp, _i64, _i64, _d,
                                      P(_vp)]),
@@ -388,6 +389,11 @@ class LaplaceOperatorBK3:
         _check(lib().pmg_operator_gradient_coefficient(
             self.h, _dev_ptr(u, self.precision, self._m), _dev_ptr(lam, self.precision, self._m),
             _dev_ptr(out, self.precision, self.n_cells * self.nq ** 3), _stream()))
+
+    def set_box_structure(self, nx, ny, nz):
+        """Declare the cells a lexicographic nx x ny x nz box (verified on the
+        device); enables SCHEDULE_GRID."""
+        _check(lib().pmg_operator_set_box_structure(self.h, int(nx), int(ny), int(nz)))
 
     def set_schedule(self, schedule=SCHEDULE_ATOMIC, g_load=-1):
         """Cell-loop schedule: SCHEDULE_ATOMIC (per-cell device atomics),

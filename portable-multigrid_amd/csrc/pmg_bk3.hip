@@ -33,7 +33,10 @@ namespace pmg {
 // kScatterAtomic adds with device atomics; kScatterPlain with a plain
 // read-modify-write, valid only when no two cells of the range share a DoF
 // (one colour); kScatterStore stores the cell's values to y[cell][local] (a
-// per-cell buffer laid out like the DoF map), every entry, for a gather pass.
+// per-cell buffer laid out like the DoF map), every entry, for a gather pass;
+// kScatterGrid stores them to the exploded grid of a lexicographic box of
+// gnx x gny cells per layer (each cell owns an nm^3 block of it, see
+// bk3_grid_sum_kernel).
 // Compressed indices (optional, lines != nullptr): for a cell with lflag set,
 // every x-line (j,k) of its DoF map is a run of consecutive DoFs starting at
 // lines[cell][k][j]; other cells read the full map.  The line starts and the
@@ -41,7 +44,7 @@ namespace pmg {
 // WT: wave-aligned lane layout (Bk3Tile::per_wave whole cells per wave, the
 // workgroup has Bk3Tile::wave_lanes lanes): every phase of the cell sweeps is
 // wave-local, no workgroup barrier.
-enum : int { kScatterAtomic = 0, kScatterPlain = 1, kScatterStore = 2 };
+enum : int { kScatterAtomic = 0, kScatterPlain = 1, kScatterStore = 2, kScatterGrid = 3 };
 
 template <int NM, int NQ, typename T, int SCATTER = kScatterAtomic, int GMODE = kGEarlyNT,
           bool WT = false>
@@ -49,7 +52,8 @@ __device__ __forceinline__ void
 bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
           const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
           const uint8_t *__restrict__ lflag, const uint32_t *__restrict__ cell_ids,
-          const int64_t n_range, const Shape1D<T> &sh, const int64_t cell_base, T *lds)
+          const int64_t n_range, const Shape1D<T> &sh, const int64_t cell_base, T *lds,
+          const int gnx = 0, const int gny = 0)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells;
@@ -95,6 +99,13 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
       T *yc = y + cell * NM3 + tb * NM + ta;
 #pragma unroll
       for (int k = 0; k < NM; ++k) yc[k * NM2] = v[k];
+    } else if (SCATTER == kScatterGrid) {
+      const int c32 = static_cast<int>(cell), cyz = c32 / gnx, cx = c32 - cyz * gnx;
+      const int cz = cyz / gny, cy = cyz - cz * gny;
+      const int64_t row = int64_t(gnx) * NM, plane = row * gny * NM;
+      T *yc = y + cz * NM * plane + (cy * NM + tb) * row + cx * NM + ta;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) yc[k * plane] = idx[k] == kInvalid ? T(0) : v[k];
     } else {
 #pragma unroll
       for (int k = 0; k < NM; ++k)
@@ -122,13 +133,14 @@ __global__ void __launch_bounds__((bk3_cells_lanes<NQ, T>()))
 bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ map, const uint32_t *__restrict__ lines,
                  const uint8_t *__restrict__ lflag, const uint32_t *__restrict__ cell_ids,
-                 const int64_t n_range, const Shape1D<T> sh)
+                 const int64_t n_range, const Shape1D<T> sh, const int gnx, const int gny)
 {
   constexpr int E = Bk3Tile<NQ>::cells;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
   bk3_batch<NM, NQ, T, SCATTER, GMODE, bk3_wave_tile<NQ, T>()>(
-      x, y, G, map, lines, lflag, cell_ids, n_range, sh, static_cast<int64_t>(lb) * E, lds);
+      x, y, G, map, lines, lflag, cell_ids, n_range, sh, static_cast<int64_t>(lb) * E, lds, gnx,
+      gny);
 }
 
 // ---------------------------------------------------------------- patch schedule
@@ -546,6 +558,139 @@ static void launch_bk3_gather(int64_t n, const uint32_t *off, const uint32_t *li
   PMG_HIP_CHECK(hipGetLastError());
 }
 
+// Second pass of the grid schedule on a lexicographic box of nx x ny x nz
+// cells (degree p, nm = p+1): the first pass stored every cell's values to an
+// exploded grid of nx nm x ny nm x nz nm entries, cell (cx,cy,cz) at the
+// block starting at (cx nm, cy nm, cz nm).  Along each axis a DoF of grid
+// index I = q p + r has one copy at q nm + r, or two adjacent ones, q nm - 1
+// and q nm, when it lies on a face between two cells (r = 0, 0 < q < n), so
+// y[I,J,K] is the sum of at most 2 x 2 x 2 neighbouring entries: every access
+// coalesces, nothing is atomic, and the summation order is fixed.
+// Constrained rows (cmask != 0) are left untouched.
+template <int P>
+__device__ __forceinline__ int grid_copies(int I, int n, int (&e)[2]) {
+  constexpr int NM = P + 1;
+  const int q = I / P, r = I - q * P;  // division by a constant
+  if (q == n) {
+    e[0] = q * NM - 1;
+    return 1;
+  }
+  if (r == 0 && q > 0) {
+    e[0] = q * NM - 1;
+    e[1] = q * NM;
+    return 2;
+  }
+  e[0] = q * NM + r;
+  return 1;
+}
+
+// Each lane sums DPL DoFs 256 apart, with every load issued before the first
+// sum (one memory latency per lane instead of DPL).  Constrained DoFs are
+// summed too: their buffer entries are zero (the first pass stores zero for
+// invalid map entries), and the caller restores the constrained rows.
+template <int P, typename T, int DPL>
+__global__ void __launch_bounds__(256)
+bk3_grid_sum_kernel(const int nx, const int ny, const int nz, const T *__restrict__ buf,
+                    T *__restrict__ y, const int assign)
+{
+  constexpr int NM = P + 1;
+  const int n1x = P * nx + 1, n1y = P * ny + 1;
+  const int64_t n = int64_t(n1x) * n1y * (P * nz + 1);
+  const int64_t row = int64_t(nx) * NM, plane = row * ny * NM;
+  const int base = static_cast<int>(blockIdx.x) * (256 * DPL) + threadIdx.x;
+  T v[DPL][8];
+  int cnt[DPL];
+#pragma unroll
+  for (int d = 0; d < DPL; ++d) {
+    const int i = base + d * 256;
+    cnt[d] = 0;
+    if (i < n) {
+      const int jk = i / n1x, I = i - jk * n1x;
+      const int K = jk / n1y, J = jk - K * n1y;
+      int ex[2], ey[2], ez[2];
+      const int cx = grid_copies<P>(I, nx, ex), cy = grid_copies<P>(J, ny, ey),
+                cz = grid_copies<P>(K, nz, ez);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            if (a < cz && b < cy && c < cx)
+              v[d][cnt[d]++] = buf[ez[a] * plane + ey[b] * row + ex[c]];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DPL; ++d) {
+    const int i = base + d * 256;
+    if (i < n) {
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < cnt[d]) acc += v[d][k];
+      y[i] = assign ? acc : y[i] + acc;
+    }
+  }
+}
+
+template <typename T>
+static void launch_bk3_grid_sum(int p, const int *dims, const T *buf, T *y, bool assign,
+                                hipStream_t s) {
+  constexpr int DPL = 2;
+  const int64_t n = (int64_t(p) * dims[0] + 1) * (int64_t(p) * dims[1] + 1) *
+                    (int64_t(p) * dims[2] + 1);
+  if (n <= 0) return;
+  const int64_t nb = (n + 256 * DPL - 1) / (256 * DPL);
+  PMG_REQUIRE(n + 256 * DPL < (int64_t(1) << 31), PMG_ERR_SIZE, "box too large for the grid schedule");
+#define PMG_GRID_SUM_CASE(P)                                                                  \
+  if (p == P) {                                                                               \
+    hipLaunchKernelGGL((bk3_grid_sum_kernel<P, T, DPL>), dim3(static_cast<unsigned>(nb)),     \
+                       dim3(256), 0, s, dims[0], dims[1], dims[2], buf, y, assign ? 1 : 0);   \
+    PMG_HIP_CHECK(hipGetLastError());                                                         \
+    return;                                                                                   \
+  }
+  PMG_GRID_SUM_CASE(1) PMG_GRID_SUM_CASE(2) PMG_GRID_SUM_CASE(3) PMG_GRID_SUM_CASE(4)
+  PMG_GRID_SUM_CASE(5) PMG_GRID_SUM_CASE(6)
+#undef PMG_GRID_SUM_CASE
+  throw Error(PMG_ERR_UNSUPPORTED, "grid schedule: unsupported degree");
+}
+
+// 1 where the map is not the lexicographic box map: entry (cell, k, j, i) of
+// cell (cx, cy, cz) must be the grid DoF (p cx + i, p cy + j, p cz + k) or,
+// if that DoF is constrained (cmask, may be null), that DoF or invalid.
+__global__ void __launch_bounds__(256)
+box_map_check_kernel(const int p, const int nx, const int ny, const int64_t n_pos,
+                     const uint32_t *__restrict__ map, const uint8_t *__restrict__ cmask,
+                     uint32_t *bad)
+{
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pos) return;
+  const int nm = p + 1, nm3 = nm * nm * nm;
+  const int64_t c = e / nm3;
+  const int l = static_cast<int>(e - c * nm3), i = l % nm, j = (l / nm) % nm, k = l / (nm * nm);
+  const int64_t cyz = c / nx, cx = c - cyz * nx, cz = cyz / ny, cy = cyz - cz * ny;
+  const int64_t n1x = int64_t(p) * nx + 1, n1y = int64_t(p) * ny + 1;
+  const int64_t d = (p * cx + i) + n1x * ((p * cy + j) + n1y * (p * cz + k));
+  const uint32_t m = map[e];
+  const bool ok = m == kInvalid ? (cmask && cmask[d]) : int64_t(m) == d;
+  if (!ok) atomicOr(bad, 1u);
+}
+
+bool check_box_map(int degree, const int *dims, int64_t n_cells, const uint32_t *map,
+                   const uint8_t *cmask, hipStream_t s) {
+  const int64_t nm = degree + 1, n_pos = n_cells * nm * nm * nm;
+  if (n_pos == 0) return true;
+  DeviceArray<uint32_t> bad(1);
+  PMG_HIP_CHECK(hipMemsetAsync(bad.ptr, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(box_map_check_kernel, dim3(static_cast<unsigned>((n_pos + 255) / 256)),
+                     dim3(256), 0, s, degree, dims[0], dims[1], n_pos, map, cmask, bad.ptr);
+  PMG_HIP_CHECK(hipGetLastError());
+  uint32_t h = 1;
+  PMG_HIP_CHECK(hipMemcpyAsync(&h, bad.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  return h == 0;
+}
+
 __global__ void __launch_bounds__(256)
 imap_count_kernel(const int64_t n_pos, const uint32_t *__restrict__ map, uint32_t *cnt) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -619,7 +764,7 @@ template <int NM, int NQ, typename T, int SCATTER, int GMODE>
 static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
                              const uint32_t *lines, const uint8_t *lflag,
                              const uint32_t *cell_ids, int64_t n_range, const Shape1D<T> &sh,
-                             hipStream_t s) {
+                             hipStream_t s, int gnx = 0, int gny = 0) {
   if (n_range <= 0) return;
   using Tile = Bk3Tile<NQ>;
   const int64_t nb = (n_range + Tile::cells - 1) / Tile::cells;
@@ -627,7 +772,7 @@ static void bk3_launch_range(const T *x, T *y, const T *G, const uint32_t *map,
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
   hipLaunchKernelGGL((bk3_cells_kernel<NM, NQ, T, SCATTER, GMODE>), dim3(grid),
                      dim3(bk3_cells_lanes<NQ, T>()), 0, s, x, y, G, map, lines, lflag, cell_ids,
-                     n_range, sh);
+                     n_range, sh, gnx, gny);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -642,6 +787,21 @@ static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_
       bk3_launch_patch<NM, NQ, T, (FULL ? kGEarly : kGLateNT)>(assign, x, y, G, *sc, n_cells, sh, s);
     else
       bk3_launch_patch<NM, NQ, T, kGLateNT>(assign, x, y, G, *sc, n_cells, sh, s);
+    return;
+  }
+  if (sc && sc->kind == kGrid && sc->gbuf) {
+    // cells -> exploded grid (plain stores), then every DoF sums its at most
+    // 8 adjacent copies: no atomics, no index loads in the second pass
+    T *buf = static_cast<T *>(sc->gbuf);
+    if (FULL && sc->gmode == kGEarlyNT)
+      bk3_launch_range<NM, NQ, T, kScatterGrid, kGEarlyNT>(x, buf, G, map, sc->lines, sc->lflag,
+                                                          nullptr, n_cells, sh, s, sc->grid[0],
+                                                          sc->grid[1]);
+    else
+      bk3_launch_range<NM, NQ, T, kScatterGrid, kGLateNT>(x, buf, G, map, sc->lines, sc->lflag,
+                                                         nullptr, n_cells, sh, s, sc->grid[0],
+                                                         sc->grid[1]);
+    launch_bk3_grid_sum<T>(NM - 1, sc->grid, buf, y, assign, s);
     return;
   }
   if (sc && sc->kind == kGather && sc->gbuf) {

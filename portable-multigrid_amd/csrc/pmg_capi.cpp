@@ -283,9 +283,16 @@ int pmg_operator_set_schedule(pmg_operator op, int schedule, int g_load) {
   return guard([&] {
     PMG_NOT_NULL(op);
     PMG_REQUIRE(g_load >= -1 && g_load <= 2, PMG_ERR_INVALID_ARG, "g_load must be -1, 0, 1 or 2");
-    PMG_REQUIRE(schedule >= 0 && schedule <= 3, PMG_ERR_INVALID_ARG,
-                "schedule must be 0 (atomic), 1 (colored), 2 (patch) or 3 (gather)");
+    PMG_REQUIRE(schedule >= 0 && schedule <= 4, PMG_ERR_INVALID_ARG,
+                "schedule must be 0 (atomic), 1 (colored), 2 (patch), 3 (gather) or 4 (grid)");
     bk3_of(op).set_schedule(schedule, g_load);
+  });
+}
+
+int pmg_operator_set_box_structure(pmg_operator op, int nx, int ny, int nz) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    bk3_of(op).set_box_structure(nx, ny, nz);
   });
 }
 

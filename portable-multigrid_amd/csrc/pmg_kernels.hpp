@@ -101,8 +101,11 @@ __host__ __device__ inline int64_t g_size(int nq, int64_t n_cells) {
 //  kGather  two launches, no atomics on the output: the cells store their
 //           nm^3 values to a per-cell buffer laid out like the DoF map, then
 //           every DoF sums its buffer entries through the inverse map (CSR,
-//           entries in ascending order: the result is reproducible bit for bit).
-enum Bk3ScheduleKind : int { kAtomic = 0, kColored = 1, kPatch = 2, kGather = 3 };
+//           entries in ascending order: the result is reproducible bit for bit);
+//  kGrid    the same two passes for a lexicographic box (map verified): the
+//           buffer is the box's exploded grid, and each DoF sums its <= 8
+//           adjacent copies (coalesced, no index loads, reproducible).
+enum Bk3ScheduleKind : int { kAtomic = 0, kColored = 1, kPatch = 2, kGather = 3, kGrid = 4 };
 struct Bk3Schedule {
   int kind = kAtomic;
   int gmode = 2;                        // G loads: 0 early non-temporal, 1 early, 2 late non-temporal
@@ -130,7 +133,15 @@ struct Bk3Schedule {
   const uint32_t *goff = nullptr, *glist = nullptr;
   void *gbuf = nullptr;
   int64_t gn = 0;
+  // grid data (kGrid, gbuf is the exploded grid): box cells per direction;
+  // the second pass writes the constrained rows too (zero, or y unchanged
+  // when adding): callers restore them
+  int grid[3] = {0, 0, 0};
 };
+// true if map is the lexicographic box map of dims[0] x dims[1] x dims[2]
+// cells at this degree (entries of constrained DoFs may be invalid)
+bool check_box_map(int degree, const int *dims, int64_t n_cells, const uint32_t *map,
+                   const uint8_t *cmask, hipStream_t s);
 
 // Inverse map of a DoF map for the gather schedule, built on the device:
 // off[n_dofs + 1] and list[off[n_dofs]] with the positions e of map[e] == d

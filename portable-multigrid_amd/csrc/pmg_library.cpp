@@ -191,6 +191,8 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     }
   map_.upload(imap);
   perm_.upload(perm);
+  caller_order_ = true;
+  for (int64_t c = 0; c < n_cells && caller_order_; ++c) caller_order_ = perm[c] == uint32_t(c);
   if (compress_lines(imap, n_cells, degree + 1, n_dofs, lines_, lflag_)) {
     sched_.lines = lines_.ptr;
     sched_.lflag = lflag_.ptr;
@@ -216,6 +218,7 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
       kind = std::string(env) == "colored"  ? kColored
              : std::string(env) == "atomic" ? kAtomic
              : std::string(env) == "gather" ? kGather
+             : std::string(env) == "grid"   ? kGrid
                                             : kPatch;
     set_schedule(kind, genv ? std::atoi(genv) : -1);
   }
@@ -345,11 +348,39 @@ void LaplaceOperatorBK3::build_patches_once() {
   build_patch_schedule(imap, n_cells_, degree_, nq_, n_dofs_, patches_, sched_);
 }
 
+void LaplaceOperatorBK3::set_box_structure(int nx, int ny, int nz) {
+  PMG_REQUIRE(nx > 0 && ny > 0 && nz > 0, PMG_ERR_INVALID_ARG, "box dimensions must be positive");
+  PMG_REQUIRE(int64_t(nx) * ny * nz == n_cells_, PMG_ERR_SIZE,
+              "box dimensions do not match the cell count");
+  const int64_t p = degree_;
+  PMG_REQUIRE((p * nx + 1) * (p * ny + 1) * (p * nz + 1) == n_dofs_, PMG_ERR_SIZE,
+              "box dimensions do not match the DoF count");
+  PMG_REQUIRE(caller_order_, PMG_ERR_UNSUPPORTED,
+              "the box structure needs the caller's cell order (PMG_CELL_ORDER unset)");
+  PMG_REQUIRE(p * ny + 1 < 65536 && p * nz + 1 < 65536 && n_cells_ < (int64_t(1) << 31),
+              PMG_ERR_SIZE, "box too large for the grid schedule");
+  const int dims[3] = {nx, ny, nz};
+  PMG_REQUIRE(check_box_map(degree_, dims, n_cells_, map_.ptr, cmask_.ptr, nullptr),
+              PMG_ERR_INVALID_ARG,
+              "the DoF map is not the lexicographic box map of these dimensions");
+  for (int d = 0; d < 3; ++d) sched_.grid[d] = dims[d];
+}
+
+// The cell buffer of the gather and grid schedules (n_cells nm^3 entries).
+bool LaplaceOperatorBK3::ensure_cell_buffer() {
+  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
+  if (n_cells_ == 0 || n_cells_ * nm3 >= (int64_t(1) << 32) || n_dofs_ >= (int64_t(1) << 31))
+    return false;
+  gbuf_.reinit(precision_, n_cells_ * nm3);
+  sched_.gbuf = gbuf_.data();
+  return true;
+}
+
 // The gather schedule's inverse map (built on the device) and cell buffer.
 void LaplaceOperatorBK3::build_gather_once() {
-  if (sched_.gbuf || n_cells_ == 0) return;
+  if (sched_.goff || n_cells_ == 0) return;
   const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
-  if (n_cells_ * nm3 >= (int64_t(1) << 32) || n_dofs_ >= (int64_t(1) << 31)) return;
+  if (!ensure_cell_buffer()) return;
   goff_.resize(static_cast<size_t>(n_dofs_ + 1));
   uint32_t *list = nullptr;
   int64_t ne = 0;
@@ -357,21 +388,21 @@ void LaplaceOperatorBK3::build_gather_once() {
   glist_.release();
   glist_.ptr = list;
   glist_.n = static_cast<size_t>(ne);
-  gbuf_.reinit(precision_, n_cells_ * nm3);
   sched_.goff = goff_.ptr;
   sched_.glist = glist_.ptr;
-  sched_.gbuf = gbuf_.data();
   sched_.gn = n_dofs_;
 }
 
 void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {
-  PMG_REQUIRE(kind >= kAtomic && kind <= kGather, PMG_ERR_INVALID_ARG, "unknown schedule");
+  PMG_REQUIRE(kind >= kAtomic && kind <= kGrid, PMG_ERR_INVALID_ARG, "unknown schedule");
   if (kind == kPatch && sched_.pdof == nullptr) build_patches_once();
   if (kind == kColored && sched_.color_offsets.empty()) build_coloring();
-  if (kind == kGather && sched_.gbuf == nullptr) build_gather_once();
+  if (kind == kGather && sched_.goff == nullptr) build_gather_once();
+  if (kind == kGrid && sched_.grid[0] > 0 && sched_.gbuf == nullptr) ensure_cell_buffer();
   if (kind == kColored && sched_.color_offsets.empty()) kind = kAtomic;
   if (kind == kPatch && sched_.pdof == nullptr) kind = kAtomic;
-  if (kind == kGather && sched_.gbuf == nullptr) kind = kAtomic;
+  if (kind == kGather && sched_.goff == nullptr) kind = kAtomic;
+  if (kind == kGrid && (sched_.grid[0] == 0 || sched_.gbuf == nullptr)) kind = kAtomic;
   sched_.kind = kind;
   // -1: the schedule's measured best G-load policy (the patch kernel's first
   // phase only gathers indices and x, which the G stream then overlaps)
@@ -393,8 +424,8 @@ void LaplaceOperatorBK3::vmult_t(T *dst, const T *src, hipStream_t s) const {
   // cells (saves a launch, which dominates there); large ones: a zero fill
   // and an indexed copy of the constrained rows (fewer bytes than the fused
   // pass's byte mask and conditional loads, measured faster at 17M DoFs).
-  // The gather schedule overwrites every free row itself: no fill.
-  const bool gather = sched_.kind == kGather;
+  // The gather and grid schedules overwrite every free row themselves: no fill.
+  const bool gather = sched_.kind == kGather || sched_.kind == kGrid;
   const bool fused = !gather && n_constrained_ > 0 && n_dofs_ <= kFusedPrepMaxDofs;
   if (fused) launch_prep_constrained<T>(n_dofs_, cmask_.ptr, dst, src, s);
   else if (!gather) launch_fill<T>(n_dofs_, dst, T(0), s);
@@ -520,6 +551,8 @@ void LaplaceOperatorBK3::vmult_prepared(void *dst, const void *src, hipStream_t 
     launch_bk3_cells<float>(degree_, nq_, static_cast<const float *>(src), static_cast<float *>(dst),
                             static_cast<const float *>(G_.data()), map_.ptr, n_cells_, sh32_, s,
                             &sched_, /*assign=*/true);
+  // the grid schedule's second pass overwrites the constrained rows with 0
+  if (sched_.kind == kGrid) copy_constrained(dst, src, s);
 }
 
 void LaplaceOperatorBK3::copy_constrained(void *dst, const void *src, hipStream_t s) const {

@@ -148,6 +148,8 @@ class LaplaceOperatorBK3 : public Operator {
   void build_coloring();
   void build_patches_once();
   void build_gather_once();
+  bool ensure_cell_buffer();
+  bool caller_order_ = true;  // internal cell order = caller order
 
   int degree_, nq_, precision_;
   int64_t n_cells_, n_dofs_, n_constrained_;
@@ -187,6 +189,11 @@ class LaplaceOperatorBK3 : public Operator {
   // data is missing falls back to kAtomic.
   // gmode: G load policy (see pmg_bk3.hip)
   void set_schedule(int kind, int gmode);
+  // Declares the cells a lexicographic nx x ny x nz box with the lexicographic
+  // DoF numbering (verified on the device against the map; throws if not):
+  // enables the grid schedule.
+  void set_box_structure(int nx, int ny, int nz);
+  bool has_box_structure() const { return sched_.grid[0] > 0; }
   int n_colors() const { return static_cast<int>(sched_.color_offsets.size()) - 1; }
   const Bk3Schedule &schedule() const { return sched_; }
 };
