@@ -57,6 +57,32 @@ enum pmg_precond_kind { PMG_PRECOND_NONE = 0, PMG_PRECOND_JACOBI = 1, PMG_PRECON
 
 enum pmg_coarse_kind { PMG_COARSE_CHEBYSHEV = 0, PMG_COARSE_PCG = 1 };
 
+/* Index arrays of an operator (pmg_operator_index_data), element type in
+ * brackets, cells in the operator's internal order (cell order array) */
+enum pmg_index_array {
+  PMG_INDEX_MAP = 0,               /* [uint32] [C][(p+1)^3], constrained entries invalid */
+  PMG_INDEX_CONSTRAINED = 1,       /* [uint32] the constrained DoF list */
+  PMG_INDEX_CELL_ORDER = 2,        /* [uint32] internal -> caller cell index */
+  PMG_INDEX_LINES = 3,             /* [uint32] [C][(p+1)^2] x-line starts (0 where not compressed) */
+  PMG_INDEX_LINE_FLAGS = 4,        /* [uint8]  [C] 1 where every x-line is compressed */
+  PMG_INDEX_PATCH_DOFS = 5,        /* [uint32] [batches][E (p+1)^3] patch DoFs, bit 31 shared */
+  PMG_INDEX_PATCH_COUNTS = 6,      /* [uint32] [batches] */
+  PMG_INDEX_PATCH_SLOTS = 7,       /* [uint16] [C][(p+1)^3] patch slot, 0xFFFF constrained */
+  PMG_INDEX_PATCH_LINES = 8,       /* [uint16] [C][(p+1)^2] first slot of each x-line */
+  PMG_INDEX_PATCH_LINE_FLAGS = 9,  /* [uint8]  [C] */
+  PMG_INDEX_VERTICES = 10          /* [double] [C][8][3] */
+};
+/* Index arrays of a transfer (pmg_transfer_index_data) */
+enum pmg_transfer_index_array {
+  PMG_TRANSFER_INDEX_COARSE_MAP = 0,        /* [uint32] [C][nc^3], constrained invalid */
+  PMG_TRANSFER_INDEX_FINE_MAP = 1,          /* [uint32] [C][nf^3], bit 31 owner mark */
+  PMG_TRANSFER_INDEX_FINE_LINES = 2,        /* [uint32] [C][nf^2][2] {first DoF, owner bits} */
+  PMG_TRANSFER_INDEX_RESTRICT_OFFSETS = 3,  /* [uint32] coarse DoF -> buffer entries (CSR) */
+  PMG_TRANSFER_INDEX_RESTRICT_LIST = 4,     /* [uint32] */
+  PMG_TRANSFER_INDEX_DG_OFFSETS = 5,        /* [uint32] CG DoF -> DG copies (CSR) */
+  PMG_TRANSFER_INDEX_DG_LIST = 6            /* [uint32] */
+};
+
 typedef struct pmg_operator_s *pmg_operator;
 typedef struct pmg_transfer_s *pmg_transfer;
 typedef struct pmg_smoother_s *pmg_smoother;
@@ -133,6 +159,18 @@ int pmg_operator_create(int dim, int degree, int n_q_points_1d, int precision,
                         const uint32_t *constrained_dofs, int64_t n_constrained,
                         const double *cell_vertices, pmg_operator *op);
 /* Same, with G[C][6][nq^3] (xx,xy,xz,yy,yz,zz, already times JxW) given. */
+/* Same for the Q_p operator of a structured box (pmg_box_dofmap's
+ * numbering, every boundary DoF constrained), its index data generated on the
+ * device -- what pmg_box_dofmap, pmg_box_boundary_dofs and
+ * pmg_box_cell_vertices followed by pmg_operator_create give, without a
+ * host pass over the cells: the DoFHandler/MatrixFree set-up of the
+ * reference's drivers (tests/poisson_cube_bk3/program.cc:281-566,
+ * setup_dof_indices_per_color, portable_laplace_operator_bk3.h:360-437).
+ * Declares the box structure (pmg_operator_set_box_structure) where the grid
+ * schedule's index limits allow.  Cells stay in box order (PMG_CELL_ORDER is
+ * not consulted). */
+int pmg_operator_create_box(const pmg_box *box, int degree, int n_q_points_1d, int precision,
+                            pmg_operator *op);
 int pmg_operator_create_from_G(int dim, int degree, int n_q_points_1d, int precision,
                                int64_t n_cells, const uint32_t *dofmap, int64_t n_dofs,
                                const uint32_t *constrained_dofs, int64_t n_constrained,
@@ -251,6 +289,13 @@ int pmg_operator_set_box_structure(pmg_operator op, int nx, int ny, int nz);
 /* Bytes one vmult must move (BK3: s(2N + 6 C nq^3); DG: adds the face data
  * and connectivity) and its flop count. */
 int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops);
+/* Copies the operator's device index array `which` (pmg_index_array) to the
+ * host array `host` of *n elements; host NULL only sets *n (0 when the array
+ * is not built: patch arrays exist once the patch schedule was selected).
+ * The views of the index data get_matrix_free() exposes
+ * (include/base/portable_laplace_operator_base.h:56-59), for inspection and
+ * tests. */
+int pmg_operator_index_data(pmg_operator op, int which, void *host, int64_t *n);
 
 /* ------------------------------------------------------------ SIP-DG operator
  * Replaces Portable::LaplaceOperatorDG<dim,p,n_q_points_1d,number>
@@ -309,6 +354,19 @@ int pmg_transfer_create_from_constraints(int kind, int precision, int degree_coa
                                          int64_t n_constrained_fine, int64_t n_dofs_coarse,
                                          int64_t n_dofs_fine, double prolongation_scale,
                                          pmg_transfer *tr);
+/* The transfer between levels of structured boxes (every boundary DoF
+ * constrained), maps and weights generated on the device: kind H from `box`
+ * to the box refined once (degree_coarse == degree_fine), P on `box` from
+ * degree_coarse to degree_fine, DG_CG on `box` (CG(p) -> cell-major DG(p)).
+ * Equals pmg_transfer_create_from_constraints (H, P) or pmg_transfer_create
+ * (DG_CG) over pmg_box_h/p_transfer_maps / pmg_box_dofmap and the fine
+ * boundary DoFs (GeometricTransfer::setup_dof_indices / setup_weights,
+ * portable_geometric_transfer.h:1344-1486; PolynomialTransfer,
+ * portable_polynomial_transfer.h:1247-1360). */
+int pmg_transfer_create_box(int kind, int precision, const pmg_box *box, int degree_coarse,
+                            int degree_fine, double prolongation_scale, pmg_transfer *tr);
+/* As pmg_operator_index_data, for pmg_transfer_index_array. */
+int pmg_transfer_index_data(pmg_transfer tr, int which, void *host, int64_t *n);
 int pmg_transfer_destroy(pmg_transfer tr);
 /* kind, precision, degrees, and how the fine side is stored (layout 0:
  * explicit weights, 1: owner marks in the full fine map, 2: owner marks as

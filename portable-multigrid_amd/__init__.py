@@ -34,6 +34,17 @@ COARSE_CHEBYSHEV, COARSE_PCG = 0, 1
 SCHEDULE_ATOMIC, SCHEDULE_COLORED, SCHEDULE_PATCH, SCHEDULE_GATHER, SCHEDULE_GRID = 0, 1, 2, 3, 4
 TRANSFER_LAYOUT_WEIGHTED, TRANSFER_LAYOUT_OWNER_MAP, TRANSFER_LAYOUT_OWNER_LINES = 0, 1, 2
 INVALID_DOF = 0xFFFFFFFF
+# index arrays of pmg_operator_index_data / pmg_transfer_index_data: name -> (id, dtype)
+OPERATOR_INDEX = {"map": (0, np.uint32), "constrained": (1, np.uint32),
+                  "cell_order": (2, np.uint32), "lines": (3, np.uint32),
+                  "line_flags": (4, np.uint8), "patch_dofs": (5, np.uint32),
+                  "patch_counts": (6, np.uint32), "patch_slots": (7, np.uint16),
+                  "patch_lines": (8, np.uint16), "patch_line_flags": (9, np.uint8),
+                  "vertices": (10, np.float64)}
+TRANSFER_INDEX = {"coarse_map": (0, np.uint32), "fine_map": (1, np.uint32),
+                  "fine_lines": (2, np.uint32), "restrict_offsets": (3, np.uint32),
+                  "restrict_list": (4, np.uint32), "dg_offsets": (5, np.uint32),
+                  "dg_list": (6, np.uint32)}
 
 OK, ERR_INVALID_ARG, ERR_SIZE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED, ERR_ALLOC = range(7)
 
@@ -115,6 +126,10 @@ def lib():
         "pmg_operator_create": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp, P(_vp)]),
         "pmg_operator_create_from_G": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp,
                                             P(_vp)]),
+        "pmg_operator_create_box": (_i, [_BP, _i, _i, _i, P(_vp)]),
+        "pmg_operator_index_data": (_i, [_vp, _i, _vp, P(_i64)]),
+        "pmg_transfer_create_box": (_i, [_i, _i, _BP, _i, _i, _d, P(_vp)]),
+        "pmg_transfer_index_data": (_i, [_vp, _i, _vp, P(_i64)]),
         "pmg_operator_destroy": (_i, [_vp]),
         "pmg_operator_m": (_i, [_vp, P(_i64)]),
         "pmg_operator_info": (_i, [_vp, P(_i), P(_i), P(_i), P(_i64)]),
@@ -257,6 +272,16 @@ def box_dg_faces(b):
     return inner, bdry
 
 
+def _index_data(fn, h, spec):
+    which, dtype = spec
+    n = ctypes.c_int64(0)
+    _check(fn(h, which, None, ctypes.byref(n)))
+    out = np.empty(n.value, dtype=dtype)
+    if n.value:
+        _check(fn(h, which, _np_ptr(out), ctypes.byref(n)))
+    return out
+
+
 # ------------------------------------------------------------------ operator
 class LaplaceOperatorBK3:
     """Continuous Q_p Laplace operator on the GPU (LaplaceOperatorBK3<3,p,number>).
@@ -270,14 +295,21 @@ class LaplaceOperatorBK3:
         self.precision = precision
         self.h = ctypes.c_void_p()
         if _raw is None:
+            # the box's map, constraints and vertices are generated on the device;
+            # a requested non-default cell order (PMG_CELL_ORDER) goes through
+            # the host arrays
             self.box = b
-            dofmap = box_dofmap(b, degree, True)
-            cons = box_boundary_dofs(b, degree)
-            cv = box_cell_vertices(b)
             n_cells, n_dofs = box_n_cells(b), box_n_dofs(b, degree)
-            _check(lib().pmg_operator_create(3, degree, self.nq, precision, n_cells,
-                                             _np_ptr(dofmap), n_dofs, _np_ptr(cons), len(cons),
-                                             _np_ptr(cv), ctypes.byref(self.h)))
+            if os.environ.get("PMG_CELL_ORDER", "caller") == "caller":
+                _check(lib().pmg_operator_create_box(ctypes.byref(b), degree, self.nq, precision,
+                                                     ctypes.byref(self.h)))
+            else:
+                dofmap = box_dofmap(b, degree, True)
+                cons = box_boundary_dofs(b, degree)
+                cv = box_cell_vertices(b)
+                _check(lib().pmg_operator_create(3, degree, self.nq, precision, n_cells,
+                                                 _np_ptr(dofmap), n_dofs, _np_ptr(cons),
+                                                 len(cons), _np_ptr(cv), ctypes.byref(self.h)))
         else:
             dofmap, n_dofs, cons, cv, G = _raw
             dofmap = np.ascontiguousarray(dofmap, dtype=np.uint32)
@@ -390,6 +422,12 @@ class LaplaceOperatorBK3:
             self.h, _dev_ptr(u, self.precision, self._m), _dev_ptr(lam, self.precision, self._m),
             _dev_ptr(out, self.precision, self.n_cells * self.nq ** 3), _stream()))
 
+    def index_data(self, name):
+        """Copy of the device index array `name` (OPERATOR_INDEX) as numpy; an
+        empty array where it is not built (patch arrays: once SCHEDULE_PATCH
+        was selected)."""
+        return _index_data(lib().pmg_operator_index_data, self.h, OPERATOR_INDEX[name])
+
     def set_box_structure(self, nx, ny, nz):
         """Declare the cells a lexicographic nx x ny x nz box (verified on the
         device); enables SCHEDULE_GRID."""
@@ -475,11 +513,16 @@ class LaplaceOperatorDG(LaplaceOperatorBK3):
 
 # ------------------------------------------------------------------ transfers
 class _Transfer:
-    def __init__(self, kind, precision, pc, pf, mc, mf, w, n_c, n_f, scale, cons=None):
-        """cons (with w None): the fine constrained DoFs, weights built on the device."""
+    def __init__(self, kind, precision, pc, pf, mc, mf, w, n_c, n_f, scale, cons=None, box=None):
+        """cons (with w None): the fine constrained DoFs, weights built on the device.
+        box (maps None): the (coarse) box, maps and weights generated on the device."""
         self.precision = precision
         self.n_c, self.n_f = n_c, n_f
         self.h = ctypes.c_void_p()
+        if box is not None:
+            _check(lib().pmg_transfer_create_box(kind, precision, ctypes.byref(box), pc, pf,
+                                                 scale, ctypes.byref(self.h)))
+            return
         mc = np.ascontiguousarray(mc, dtype=np.uint32)
         mf = None if mf is None else np.ascontiguousarray(mf, dtype=np.uint32)
         w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
@@ -523,6 +566,10 @@ class _Transfer:
                                          len(cons), n_c, n_f))
         self.n_c, self.n_f = n_c, n_f
 
+    def index_data(self, name):
+        """Copy of the device index array `name` (TRANSFER_INDEX) as numpy."""
+        return _index_data(lib().pmg_transfer_index_data, self.h, TRANSFER_INDEX[name])
+
     def info(self):
         """(kind, precision, degree_coarse, degree_fine, layout)"""
         v = [ctypes.c_int() for _ in range(5)]
@@ -542,22 +589,29 @@ class GeometricTransfer(_Transfer):
 
     def __init__(self, coarse_box, degree, precision=FP64, prolongation_scale=1.0,
                  device_weights=False):
-        mc, mf, w = box_h_transfer_maps(coarse_box, degree, weights=not device_weights)
+        """device_weights: maps and weights generated on the device
+        (pmg_transfer_create_box); else host maps with explicit weights."""
         fine = refine(coarse_box)
-        cons = box_boundary_dofs(fine, degree) if device_weights else None
-        super().__init__(TRANSFER_H, precision, degree, degree, mc, mf,
-                         None if device_weights else w, box_n_dofs(coarse_box, degree),
-                         box_n_dofs(fine, degree), prolongation_scale, cons)
+        n_c, n_f = box_n_dofs(coarse_box, degree), box_n_dofs(fine, degree)
+        if device_weights:
+            super().__init__(TRANSFER_H, precision, degree, degree, None, None, None, n_c, n_f,
+                             prolongation_scale, box=coarse_box)
+            return
+        mc, mf, w = box_h_transfer_maps(coarse_box, degree)
+        super().__init__(TRANSFER_H, precision, degree, degree, mc, mf, w, n_c, n_f,
+                         prolongation_scale)
 
 
 class PolynomialTransfer(_Transfer):
     """p-transfer on one box, degree pc -> pf (PolynomialTransfer<3,pc,pf>)."""
 
     def __init__(self, b, pc, pf, precision=FP64, device_weights=False):
-        mc, mf, w = box_p_transfer_maps(b, pc, pf, weights=not device_weights)
-        cons = box_boundary_dofs(b, pf) if device_weights else None
-        super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, None if device_weights else w,
-                         box_n_dofs(b, pc), box_n_dofs(b, pf), 1.0, cons)
+        n_c, n_f = box_n_dofs(b, pc), box_n_dofs(b, pf)
+        if device_weights:
+            super().__init__(TRANSFER_P, precision, pc, pf, None, None, None, n_c, n_f, 1.0, box=b)
+            return
+        mc, mf, w = box_p_transfer_maps(b, pc, pf)
+        super().__init__(TRANSFER_P, precision, pc, pf, mc, mf, w, n_c, n_f, 1.0)
 
 
 class ContinuousTransfer(_Transfer):
@@ -566,9 +620,8 @@ class ContinuousTransfer(_Transfer):
     restriction sums them back (constrained CG DoFs skipped)."""
 
     def __init__(self, b, p, precision=FP64):
-        mc = box_dofmap(b, p, True)
-        super().__init__(TRANSFER_DG_CG, precision, p, p, mc, None, None, box_n_dofs(b, p),
-                         box_n_cells(b) * (p + 1) ** 3, 1.0)
+        super().__init__(TRANSFER_DG_CG, precision, p, p, None, None, None, box_n_dofs(b, p),
+                         box_n_cells(b) * (p + 1) ** 3, 1.0, box=b)
 
 
 # ------------------------------------------------------------------ smoother

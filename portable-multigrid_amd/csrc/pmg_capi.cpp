@@ -85,6 +85,18 @@ int pmg_operator_create(int dim, int degree, int nq, int precision, int64_t n_ce
   });
 }
 
+int pmg_operator_create_box(const pmg_box *box, int degree, int nq, int precision,
+                            pmg_operator *op) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    *op = nullptr;
+    PMG_NOT_NULL(box);
+    auto h = std::make_unique<pmg_operator_s>();
+    h->impl = std::make_unique<pmg::LaplaceOperatorBK3>(*box, degree, nq, precision);
+    *op = h.release();
+  });
+}
+
 int pmg_operator_create_from_G(int dim, int degree, int nq, int precision, int64_t n_cells,
                                const uint32_t *dofmap, int64_t n_dofs, const uint32_t *constrained,
                                int64_t n_constrained, const double *G, pmg_operator *op) {
@@ -296,6 +308,27 @@ int pmg_operator_set_box_structure(pmg_operator op, int nx, int ny, int nz) {
   });
 }
 
+namespace {
+void copy_index_array(const pmg::LaplaceOperatorBK3::IndexArray &a, void *host, int64_t *n) {
+  PMG_NOT_NULL(n);
+  if (host) {
+    PMG_REQUIRE(*n == a.n, PMG_ERR_SIZE,
+                "index array has " + std::to_string(a.n) + " elements, not " + std::to_string(*n));
+    if (a.n > 0)
+      PMG_HIP_CHECK(hipMemcpy(host, a.ptr, static_cast<size_t>(a.n) * a.elem,
+                              hipMemcpyDeviceToHost));
+  }
+  *n = a.n;
+}
+}  // namespace
+
+int pmg_operator_index_data(pmg_operator op, int which, void *host, int64_t *n) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    copy_index_array(bk3_of(op).index_array(which), host, n);
+  });
+}
+
 int pmg_operator_algorithmic_bytes(pmg_operator op, double *bytes, double *flops) {
   return guard([&] {
     PMG_NOT_NULL(op);
@@ -335,6 +368,25 @@ int pmg_transfer_create(int kind, int precision, int pc, int pf, int64_t n_cells
     h->impl = std::make_unique<pmg::MGTransfer>(kind, precision, pc, pf, n_cells, map_coarse, map_fine,
                                                 weights, n_dofs_coarse, n_dofs_fine, scale);
     *tr = h.release();
+  });
+}
+
+int pmg_transfer_create_box(int kind, int precision, const pmg_box *box, int pc, int pf,
+                            double scale, pmg_transfer *tr) {
+  return guard([&] {
+    PMG_NOT_NULL(tr);
+    *tr = nullptr;
+    PMG_NOT_NULL(box);
+    auto h = std::make_unique<pmg_transfer_s>();
+    h->impl = std::make_unique<pmg::MGTransfer>(kind, precision, *box, pc, pf, scale);
+    *tr = h.release();
+  });
+}
+
+int pmg_transfer_index_data(pmg_transfer tr, int which, void *host, int64_t *n) {
+  return guard([&] {
+    PMG_NOT_NULL(tr);
+    copy_index_array(tr->impl->index_array(which), host, n);
   });
 }
 

@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "pmg.h"
+
 namespace pmg {
 
 // ---------------------------------------------------------------- BK3 operator
@@ -149,6 +151,45 @@ bool check_box_map(int degree, const int *dims, int64_t n_cells, const uint32_t 
 // list_out is allocated here (it needs the entry count).  Synchronous.
 void build_inverse_map(int64_t n_positions, const uint32_t *map, int64_t n_dofs, uint32_t *off,
                        uint32_t **list_out, int64_t *n_entries, hipStream_t s);
+
+// ---- index data built on the device (pmg_setup.hip)
+// Box map [C][(p r + 1)^3]: cell (cx, cy, cz) of the box, entry (k, j, i) ->
+// grid point (p r cx + i, ...) of the box refined r times (r = 1: the Q_p
+// DoF map, r = 2: the h-transfer patch map on the fine box), lexicographic on
+// the (p r nx + 1)(p r ny + 1)(p r nz + 1) grid; boundary points invalid when
+// dirichlet.
+void box_map_device(const pmg_box &b, int degree, int refine, bool dirichlet, uint32_t *map,
+                    hipStream_t s);
+int64_t box_boundary_count(const pmg_box &b, int degree);
+// ascending boundary DoF list (list may be null) and the 0/1 mask of the
+// boundary DoFs (mask may be null) of the Q_p grid of the box
+void box_boundary_device(const pmg_box &b, int degree, uint32_t *list, uint8_t *mask,
+                         hipStream_t s);
+// [C][8][3] cell vertices, the host mesh generator's displacements
+void box_vertices_device(const pmg_box &b, double *v, hipStream_t s);
+void iota_device(uint32_t *p, int64_t n, hipStream_t s);
+// mask (n_dofs bytes) = 1 on the listed DoFs, 0 elsewhere; returns the first
+// list position holding a DoF >= n_dofs (-1: none).  Synchronous.
+int64_t mask_from_list_device(int64_t n, const uint32_t *list, int64_t n_dofs, uint8_t *mask,
+                              hipStream_t s);
+// out[c][i] = src[perm[c]][i] (perm null: identity, out == src allowed) with
+// masked DoFs made invalid; returns the first caller position holding an
+// entry >= n_dofs that is not invalid (-1: none).  Synchronous.
+int64_t map_eliminate_device(int64_t n_cells, int nm3, const uint32_t *src, const uint32_t *perm,
+                             const uint8_t *mask, int64_t n_dofs, uint32_t *out, hipStream_t s);
+// x-line compression of a map (lines [C][nm^2], flag [C]; lines of cells
+// that do not qualify are 0) and of patch slots; return the number of
+// qualifying cells.  Synchronous.
+int64_t compress_lines_device(int64_t n_cells, int nm, const uint32_t *map, int64_t n_dofs,
+                              uint32_t *lines, uint8_t *flag, hipStream_t s);
+int64_t compress_patch_lines_device(int64_t n_cells, int nm, const uint16_t *pidx, uint16_t *pline,
+                                    uint8_t *flag, hipStream_t s);
+// list[i] = to[list[i]] (positions -> the numbering a map gives them)
+void remap_list_device(int64_t n, const uint32_t *to, uint32_t *list, hipStream_t s);
+// patches of the batches of E cells (Bk3Schedule::pdof, pcount, pidx)
+bool patch_schedule_fits(int64_t n_cells, int nm3, int E, int64_t n_dofs);
+void build_patches_device(int64_t n_cells, int nm3, int E, const uint32_t *map, int64_t n_dofs,
+                          uint32_t *pdof, uint32_t *pcount, uint16_t *pidx, hipStream_t s);
 
 // y += sum_cells A_e x.  assign is accepted for the callers' intent (vmult,
 // which zeroes the free rows first) and does not change the result.

@@ -74,31 +74,76 @@ std::vector<double> tridiag_eigenvalues(const std::vector<double> &d, const std:
 // ============================================================ operator
 // Compressed indices: a cell whose every x-line (j,k) is a run of consecutive
 // unconstrained DoFs needs only the line starts (nm^2 instead of nm^3 index
-// loads).  Uploaded, and true, when at least half of the cells qualify (on
-// lexicographic numberings all but the boundary cells do).
-static bool compress_lines(const std::vector<uint32_t> &imap, int64_t n_cells, int nm,
-                           int64_t n_dofs, DeviceArray<uint32_t> &lines_out,
-                           DeviceArray<uint8_t> &flag_out) {
+// loads).  Built on the device from the device map; kept, and true, when at
+// least half of the cells qualify (on lexicographic numberings all but the
+// boundary cells do).
+static bool compress_lines(const uint32_t *map, int64_t n_cells, int nm, int64_t n_dofs,
+                           DeviceArray<uint32_t> &lines_out, DeviceArray<uint8_t> &flag_out) {
   if (n_cells <= 0) return false;
-  const int64_t nm3 = int64_t(nm) * nm * nm;
-  std::vector<uint32_t> lines(static_cast<size_t>(n_cells * nm * nm), 0u);
-  std::vector<uint8_t> flag(static_cast<size_t>(n_cells), 0);
-  int64_t n_comp = 0;
-  for (int64_t c = 0; c < n_cells; ++c) {
-    bool ok = true;
-    for (int kj = 0; kj < nm * nm && ok; ++kj) {
-      const uint32_t *ln = imap.data() + c * nm3 + int64_t(kj) * nm;
-      ok = ln[0] != kInvalid && int64_t(ln[0]) + nm <= n_dofs;
-      for (int i = 1; i < nm && ok; ++i) ok = ln[i] == ln[0] + uint32_t(i);
-      lines[c * nm * nm + kj] = ok ? ln[0] : 0u;
-    }
-    flag[c] = ok ? 1 : 0;
-    n_comp += ok;
+  lines_out.resize(static_cast<size_t>(n_cells * nm * nm));
+  flag_out.resize(static_cast<size_t>(n_cells));
+  const int64_t n_comp =
+      compress_lines_device(n_cells, nm, map, n_dofs, lines_out.ptr, flag_out.ptr, nullptr);
+  if (2 * n_comp >= n_cells) return true;
+  lines_out.release();
+  flag_out.release();
+  return false;
+}
+
+void LaplaceOperatorBK3::setup_shape() {
+  check_precision(precision_);
+  PMG_REQUIRE(degree_ >= 1 && degree_ <= 6, PMG_ERR_UNSUPPORTED, "degree must be in [1,6]");
+  PMG_REQUIRE(nq_ == degree_ + 1 || nq_ == degree_ + 2, PMG_ERR_UNSUPPORTED,
+              "n_q_points_1d must be degree+1 or degree+2");
+  PMG_REQUIRE(n_cells_ >= 0 && n_dofs_ >= 0 && n_constrained_ >= 0, PMG_ERR_INVALID_ARG,
+              "negative size");
+  PMG_REQUIRE(n_dofs_ < int64_t(0xFFFFFFFFu), PMG_ERR_SIZE, "n_dofs exceeds 32-bit DoF maps");
+  shape_ = make_shape_data(degree_, nq_);
+  sh64_ = to_shape<double>(shape_.S, shape_.D);
+  sh32_ = to_shape<float>(shape_.S, shape_.D);
+  fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree_ + 1, shape_.n_q_points_1d);
+  fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree_ + 1, shape_.n_q_points_1d);
+  shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
+}
+
+// Compressed lines and the default scatter schedule of the device map.
+void LaplaceOperatorBK3::setup_schedule() {
+  if (compress_lines(map_.ptr, n_cells_, degree_ + 1, n_dofs_, lines_, lflag_)) {
+    sched_.lines = lines_.ptr;
+    sched_.lflag = lflag_.ptr;
   }
-  if (2 * n_comp < n_cells) return false;
-  lines_out.upload(lines);
-  flag_out.upload(flag);
-  return true;
+  // Default scatter schedule: the patch schedule (with early non-temporal G
+  // loads) where a batch holds enough cells for its LDS patch to fold many
+  // shared DoFs before the global update (E = 256/nq^2 cells: FP64 nq <= 4,
+  // i.e. E >= 16; FP32 nq <= 6), the per-cell atomic scatter (late G loads)
+  // otherwise.  From vmult timings on MI355X over the level types of the MG
+  // hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3 cells), patch / atomic
+  // time: FP64 nq 2..4 0.66-0.83, nq 5 0.87-1.06, nq 6 0.95-1.02, nq >= 7
+  // 1.22-1.26; FP32 nq 2..6 0.57-0.96, nq >= 7 1.07-1.13.  Levels of at
+  // most 4096 cells are launch-bound: there the atomic kernel is as fast or
+  // faster (Q4 FP32 2^3..16^3: 8.3-11.7 vs 8.8-13.9 us per launch).
+  // PMG_BK3_SCHEDULE=atomic|colored|patch|gather|grid and PMG_BK3_GLOAD=0|1|2
+  // override.
+  const char *env = std::getenv("PMG_BK3_SCHEDULE");
+  const char *genv = std::getenv("PMG_BK3_GLOAD");
+  int kind = nq_ <= (precision_ == PMG_FP32 ? 6 : 4) && n_cells_ > 4096 ? kPatch : kAtomic;
+  if (env)
+    kind = std::string(env) == "colored"  ? kColored
+           : std::string(env) == "atomic" ? kAtomic
+           : std::string(env) == "gather" ? kGather
+           : std::string(env) == "grid"   ? kGrid
+                                          : kPatch;
+  set_schedule(kind, genv ? std::atoi(genv) : -1);
+}
+
+// G of the trilinear cells from the device vertices (internal cell order).
+void LaplaceOperatorBK3::setup_geometry() {
+  G_.reinit(precision_, g_size(nq_, n_cells_));
+  qpts_.upload(shape_.quad.points);
+  qwts_.upload(shape_.quad.weights);
+  if (precision_ == PMG_FP64) fill_geometry_t<double>(nullptr, nullptr);
+  else fill_geometry_t<float>(nullptr, nullptr);
+  PMG_HIP_CHECK(hipDeviceSynchronize());
 }
 
 LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_t n_cells,
@@ -107,32 +152,13 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
                                        const double *cell_vertices, const double *G)
     : degree_(degree), nq_(nq), precision_(precision), n_cells_(n_cells), n_dofs_(n_dofs),
       n_constrained_(n_constrained) {
-  check_precision(precision);
-  PMG_REQUIRE(degree >= 1 && degree <= 6, PMG_ERR_UNSUPPORTED, "degree must be in [1,6]");
-  PMG_REQUIRE(nq == degree + 1 || nq == degree + 2, PMG_ERR_UNSUPPORTED,
-              "n_q_points_1d must be degree+1 or degree+2");
-  PMG_REQUIRE(n_cells >= 0 && n_dofs >= 0 && n_constrained >= 0, PMG_ERR_INVALID_ARG,
-              "negative size");
-  PMG_REQUIRE(n_dofs < int64_t(0xFFFFFFFFu), PMG_ERR_SIZE, "n_dofs exceeds 32-bit DoF maps");
+  setup_shape();
   PMG_REQUIRE(n_cells == 0 || dofmap, PMG_ERR_INVALID_ARG, "dofmap is NULL");
   PMG_REQUIRE(n_constrained == 0 || constrained, PMG_ERR_INVALID_ARG, "constrained list is NULL");
   PMG_REQUIRE((cell_vertices != nullptr) != (G != nullptr), PMG_ERR_INVALID_ARG,
               "give exactly one of cell_vertices and G");
   const int64_t nm3 = int64_t(degree + 1) * (degree + 1) * (degree + 1);
   const int64_t nq3 = int64_t(nq) * nq * nq;
-  // host-side validation of the index data the kernels will dereference
-  for (int64_t e = 0; e < n_cells * nm3; ++e)
-    PMG_REQUIRE(dofmap[e] == kInvalid || int64_t(dofmap[e]) < n_dofs, PMG_ERR_INVALID_ARG,
-                "dofmap entry out of range at " + std::to_string(e));
-  for (int64_t e = 0; e < n_constrained; ++e)
-    PMG_REQUIRE(int64_t(constrained[e]) < n_dofs, PMG_ERR_INVALID_ARG,
-                "constrained DoF out of range");
-  shape_ = make_shape_data(degree, nq);
-  sh64_ = to_shape<double>(shape_.S, shape_.D);
-  sh32_ = to_shape<float>(shape_.S, shape_.D);
-  fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
-  fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree + 1, shape_.n_q_points_1d);
-  shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
   // Internal cell order.  Default: the caller's order (on lexicographic
   // structured meshes a batch is a run of cells along x, whose DoF rows are
   // long and contiguous -- measured fastest for the atomic scatter).
@@ -140,13 +166,19 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
   // DoFs (group_cells), =morton sorts by the Morton key of the cell centroids.
   // perm_ maps internal -> caller cell index; cell-indexed user data (G,
   // kappa, the coefficient gradient) is translated through it.
-  std::vector<uint32_t> perm(static_cast<size_t>(n_cells));
-  for (int64_t c = 0; c < n_cells; ++c) perm[c] = static_cast<uint32_t>(c);
+  std::vector<uint32_t> perm;
   const char *order_env = std::getenv("PMG_CELL_ORDER");
   const std::string order_kind = order_env ? order_env : "caller";
-  if (order_kind == "grouped" && n_cells > 1)
+  if (order_kind == "grouped" && n_cells > 1) {
+    // group_cells reads the map: check its range on the host first
+    for (int64_t e = 0; e < n_cells * nm3; ++e)
+      PMG_REQUIRE(dofmap[e] == kInvalid || int64_t(dofmap[e]) < n_dofs, PMG_ERR_INVALID_ARG,
+                  "dofmap entry out of range at " + std::to_string(e));
     perm = group_cells(n_cells, nm3, dofmap, n_dofs, g_batch_cells(nq));
+  }
   if (cell_vertices && n_cells > 1 && order_kind == "morton") {
+    perm.resize(static_cast<size_t>(n_cells));
+    for (int64_t c = 0; c < n_cells; ++c) perm[c] = static_cast<uint32_t>(c);
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     std::vector<double> cen(static_cast<size_t>(3 * n_cells));
     for (int64_t c = 0; c < n_cells; ++c)
@@ -178,69 +210,60 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
     }
     std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
   }
+  caller_order_ = true;
+  for (size_t c = 0; c < perm.size() && caller_order_; ++c) caller_order_ = perm[c] == uint32_t(c);
+  if (caller_order_) {
+    perm_.resize(static_cast<size_t>(n_cells));
+    iota_device(perm_.ptr, n_cells, nullptr);
+  } else {
+    perm_.upload(perm);
+  }
   // constrained DoFs are eliminated (rows and columns), as the reference's
   // setup_dof_indices_per_color does: a map entry naming a constrained DoF
-  // is treated as invalid
-  std::vector<uint8_t> cmask(n_constrained > 0 ? static_cast<size_t>(n_dofs) : 0, 0);
-  for (int64_t i = 0; i < n_constrained; ++i) cmask[constrained[i]] = 1;
-  std::vector<uint32_t> imap(static_cast<size_t>(n_cells * nm3));
-  for (int64_t c = 0; c < n_cells; ++c)
-    for (int64_t i = 0; i < nm3; ++i) {
-      const uint32_t d = dofmap[perm[c] * nm3 + i];
-      imap[c * nm3 + i] = (d != kInvalid && !cmask.empty() && cmask[d]) ? kInvalid : d;
-    }
-  map_.upload(imap);
-  perm_.upload(perm);
-  caller_order_ = true;
-  for (int64_t c = 0; c < n_cells && caller_order_; ++c) caller_order_ = perm[c] == uint32_t(c);
-  if (compress_lines(imap, n_cells, degree + 1, n_dofs, lines_, lflag_)) {
-    sched_.lines = lines_.ptr;
-    sched_.lflag = lflag_.ptr;
-  }
+  // is treated as invalid.  Range checks, the mask and the elimination run
+  // on the device.
   constrained_.upload(constrained, static_cast<size_t>(n_constrained));
-  if (n_constrained > 0) cmask_.upload(cmask);
-  {
-    // Default scatter schedule: the patch schedule (with early non-temporal G
-    // loads) where a batch holds enough cells for its LDS patch to fold many
-    // shared DoFs before the global update (E = 256/nq^2 cells: FP64 nq <= 4,
-    // i.e. E >= 16; FP32 nq <= 6), the per-cell atomic scatter (late G loads)
-    // otherwise.  From vmult timings on MI355X over the level types of the MG
-    // hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3 cells), patch / atomic
-    // time: FP64 nq 2..4 0.66-0.83, nq 5 0.87-1.06, nq 6 0.95-1.02, nq >= 7
-    // 1.22-1.26; FP32 nq 2..6 0.57-0.96, nq >= 7 1.07-1.13.  Levels of at
-    // most 4096 cells are launch-bound: there the atomic kernel is as fast or
-    // faster (Q4 FP32 2^3..16^3: 8.3-11.7 vs 8.8-13.9 us per launch).
-    // PMG_BK3_SCHEDULE=atomic|colored|patch and PMG_BK3_GLOAD=0|1|2 override.
-    const char *env = std::getenv("PMG_BK3_SCHEDULE");
-    const char *genv = std::getenv("PMG_BK3_GLOAD");
-    int kind = nq <= (precision == PMG_FP32 ? 6 : 4) && n_cells > 4096 ? kPatch : kAtomic;
-    if (env)
-      kind = std::string(env) == "colored"  ? kColored
-             : std::string(env) == "atomic" ? kAtomic
-             : std::string(env) == "gather" ? kGather
-             : std::string(env) == "grid"   ? kGrid
-                                            : kPatch;
-    set_schedule(kind, genv ? std::atoi(genv) : -1);
+  if (n_constrained > 0) {
+    cmask_.resize(static_cast<size_t>(n_dofs));
+    PMG_REQUIRE(mask_from_list_device(n_constrained, constrained_.ptr, n_dofs, cmask_.ptr,
+                                      nullptr) < 0,
+                PMG_ERR_INVALID_ARG, "constrained DoF out of range");
   }
-  G_.reinit(precision, g_size(nq, n_cells));
+  map_.upload(dofmap, static_cast<size_t>(n_cells * nm3));
+  {
+    DeviceArray<uint32_t> raw;
+    if (!caller_order_) {
+      raw = std::move(map_);
+      map_.resize(static_cast<size_t>(n_cells * nm3));
+    }
+    const int64_t bad = map_eliminate_device(n_cells, static_cast<int>(nm3),
+                                             caller_order_ ? map_.ptr : raw.ptr,
+                                             caller_order_ ? nullptr : perm_.ptr, cmask_.ptr,
+                                             n_dofs, map_.ptr, nullptr);
+    PMG_REQUIRE(bad < 0, PMG_ERR_INVALID_ARG, "dofmap entry out of range at " + std::to_string(bad));
+  }
+  setup_schedule();
   if (cell_vertices) {
-    std::vector<double> iv(static_cast<size_t>(n_cells * 24));
-    for (int64_t c = 0; c < n_cells; ++c)
-      std::copy(cell_vertices + perm[c] * 24, cell_vertices + (perm[c] + 1) * 24, iv.begin() + c * 24);
-    vertices_.upload(iv);
-    qpts_.upload(shape_.quad.points);
-    qwts_.upload(shape_.quad.weights);
-    if (precision == PMG_FP64) fill_geometry_t<double>(nullptr, nullptr);
-    else fill_geometry_t<float>(nullptr, nullptr);
-    PMG_HIP_CHECK(hipDeviceSynchronize());
+    if (caller_order_) {
+      vertices_.upload(cell_vertices, static_cast<size_t>(n_cells * 24));
+    } else {
+      std::vector<double> iv(static_cast<size_t>(n_cells * 24));
+      for (int64_t c = 0; c < n_cells; ++c)
+        std::copy(cell_vertices + perm[c] * 24, cell_vertices + (perm[c] + 1) * 24, iv.begin() + c * 24);
+      vertices_.upload(iv);
+    }
+    setup_geometry();
   } else {
+    G_.reinit(precision, g_size(nq, n_cells));
     // caller layout [cell][comp][q] -> batch-interleaved device layout (padding 0)
     auto relayout = [&](auto *dst) {
-      for (int64_t c = 0; c < n_cells; ++c)
+      for (int64_t c = 0; c < n_cells; ++c) {
+        const int64_t from = caller_order_ ? c : int64_t(perm[c]);
         for (int comp = 0; comp < 6; ++comp)
           for (int q = 0; q < nq3; ++q)
             dst[g_index(nq, c, comp, q)] = static_cast<std::remove_pointer_t<decltype(dst)>>(
-                G[(int64_t(perm[c]) * 6 + comp) * nq3 + q]);
+                G[(from * 6 + comp) * nq3 + q]);
+      }
     };
     if (precision == PMG_FP64) {
       std::vector<double> g(static_cast<size_t>(g_size(nq, n_cells)), 0.0);
@@ -252,6 +275,40 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
       PMG_HIP_CHECK(hipMemcpy(G_.data(), g.data(), G_.raw.bytes(), hipMemcpyHostToDevice));
     }
   }
+}
+
+// The operator of a structured box with every boundary DoF constrained, its
+// index data generated on the device: the lexicographic map with boundary
+// entries invalid, the ascending boundary list, the vertices; cells in the
+// box's lexicographic order.  Declares the box structure (grid schedule)
+// where its index limits allow.
+LaplaceOperatorBK3::LaplaceOperatorBK3(const pmg_box &box, int degree, int nq, int precision)
+    : degree_(degree), nq_(nq), precision_(precision),
+      n_cells_(int64_t(box.nx) * box.ny * box.nz),
+      n_dofs_((int64_t(degree) * box.nx + 1) * (int64_t(degree) * box.ny + 1) *
+              (int64_t(degree) * box.nz + 1)),
+      n_constrained_(0) {
+  PMG_REQUIRE(box.nx > 0 && box.ny > 0 && box.nz > 0, PMG_ERR_INVALID_ARG,
+              "box needs at least one cell per direction");
+  PMG_REQUIRE(box.deform >= 0 && box.deform <= 2, PMG_ERR_INVALID_ARG, "unknown deformation");
+  setup_shape();
+  n_constrained_ = box_boundary_count(box, degree);
+  const int64_t nm3 = int64_t(degree + 1) * (degree + 1) * (degree + 1);
+  map_.resize(static_cast<size_t>(n_cells_ * nm3));
+  box_map_device(box, degree, 1, true, map_.ptr, nullptr);
+  constrained_.resize(static_cast<size_t>(n_constrained_));
+  cmask_.resize(static_cast<size_t>(n_dofs_));
+  box_boundary_device(box, degree, constrained_.ptr, cmask_.ptr, nullptr);
+  perm_.resize(static_cast<size_t>(n_cells_));
+  iota_device(perm_.ptr, n_cells_, nullptr);
+  caller_order_ = true;
+  vertices_.resize(static_cast<size_t>(n_cells_ * 24));
+  box_vertices_device(box, vertices_.ptr, nullptr);
+  const int64_t p = degree;
+  if (p * box.ny + 1 < 65536 && p * box.nz + 1 < 65536 && n_cells_ < (int64_t(1) << 31))
+    for (int d = 0; d < 3; ++d) sched_.grid[d] = d == 0 ? box.nx : d == 1 ? box.ny : box.nz;
+  setup_schedule();
+  setup_geometry();
 }
 
 // Greedy cell colouring (cells sharing a DoF get different colours), cells
@@ -297,55 +354,43 @@ void LaplaceOperatorBK3::build_coloring() {
   }
 }
 
-// DoF patches of the batches (patch schedule) of a map in device cell order:
-// uploaded into pa, and sc points at them; sc is left without patches when the
-// patch slots would not fit 16 bits.
-static void build_patch_schedule(const std::vector<uint32_t> &imap, int64_t n_cells, int degree,
-                                 int nq, int64_t n_dofs, PatchArrays &pa, Bk3Schedule &sc) {
+// DoF patches of the batches (patch schedule) of a device map in device cell
+// order, built on the device into pa, and sc points at them; sc is left
+// without patches when a batch's entries exceed the builder's limit.
+static void build_patch_schedule(const uint32_t *map, int64_t n_cells, int degree, int nq,
+                                 int64_t n_dofs, PatchArrays &pa, Bk3Schedule &sc) {
   const int nm = degree + 1;
-  const int64_t nm3 = int64_t(nm) * nm * nm;
+  const int nm3 = nm * nm * nm;
   const int E = g_batch_cells(nq);
-  if (n_cells == 0 || n_dofs >= (int64_t(1) << 31) || E * nm3 >= 0xFFFF) return;
-  const CellPatches P = build_patches(n_cells, nm3, imap.data(), n_dofs, E);
-  pa.pdof.upload(P.pdof.data(), P.pdof.size());
-  pa.pcount.upload(P.pcount.data(), P.pcount.size());
-  pa.pidx.upload(P.pidx.data(), P.pidx.size());
+  if (!patch_schedule_fits(n_cells, nm3, E, n_dofs)) return;
+  const int64_t nb = (n_cells + E - 1) / E;
+  pa.pdof.resize(static_cast<size_t>(nb * E * nm3));
+  pa.pcount.resize(static_cast<size_t>(nb));
+  pa.pidx.resize(static_cast<size_t>(n_cells * nm3));
+  build_patches_device(n_cells, nm3, E, map, n_dofs, pa.pdof.ptr, pa.pcount.ptr, pa.pidx.ptr,
+                       nullptr);
   sc.pdof = pa.pdof.ptr;
   sc.pcount = pa.pcount.ptr;
   sc.pidx = pa.pidx.ptr;
   // x-lines whose patch slots are consecutive need only their first slot
   // (nm^2 instead of nm^3 16-bit loads per cell); used when at least half of
   // the cells qualify
-  std::vector<uint16_t> pline(static_cast<size_t>(n_cells * nm * nm), 0);
-  std::vector<uint8_t> pflag(static_cast<size_t>(n_cells), 0);
-  int64_t n_comp = 0;
-  for (int64_t c = 0; c < n_cells; ++c) {
-    bool ok = true;
-    for (int kj = 0; kj < nm * nm && ok; ++kj) {
-      const uint16_t *ln = P.pidx.data() + c * nm3 + int64_t(kj) * nm;
-      ok = ln[0] != 0xFFFFu;
-      for (int i = 1; i < nm && ok; ++i) ok = ln[i] == uint16_t(ln[0] + i);
-      pline[c * nm * nm + kj] = ok ? ln[0] : 0;
-    }
-    pflag[c] = ok ? 1 : 0;
-    n_comp += ok;
-  }
+  pa.pline.resize(static_cast<size_t>(n_cells * nm * nm));
+  pa.pflag.resize(static_cast<size_t>(n_cells));
+  const int64_t n_comp =
+      compress_patch_lines_device(n_cells, nm, pa.pidx.ptr, pa.pline.ptr, pa.pflag.ptr, nullptr);
   if (2 * n_comp >= n_cells) {
-    pa.pline.upload(pline);
-    pa.pflag.upload(pflag);
     sc.pline = pa.pline.ptr;
     sc.pflag = pa.pflag.ptr;
+  } else {
+    pa.pline.release();
+    pa.pflag.release();
   }
 }
 
-// The main map's patches, built on first use from the device map.
+// The main map's patches, built on first use.
 void LaplaceOperatorBK3::build_patches_once() {
-  const int64_t nm3 = int64_t(degree_ + 1) * (degree_ + 1) * (degree_ + 1);
-  std::vector<uint32_t> imap(static_cast<size_t>(n_cells_ * nm3));
-  if (!imap.empty())
-    PMG_HIP_CHECK(hipMemcpy(imap.data(), map_.ptr, imap.size() * sizeof(uint32_t),
-                            hipMemcpyDeviceToHost));
-  build_patch_schedule(imap, n_cells_, degree_, nq_, n_dofs_, patches_, sched_);
+  build_patch_schedule(map_.ptr, n_cells_, degree_, nq_, n_dofs_, patches_, sched_);
 }
 
 void LaplaceOperatorBK3::set_box_structure(int nx, int ny, int nz) {
@@ -391,6 +436,27 @@ void LaplaceOperatorBK3::build_gather_once() {
   sched_.goff = goff_.ptr;
   sched_.glist = glist_.ptr;
   sched_.gn = n_dofs_;
+}
+
+LaplaceOperatorBK3::IndexArray LaplaceOperatorBK3::index_array(int which) const {
+  auto arr = [](const auto &a) {
+    return IndexArray{a.ptr, a.ptr ? static_cast<int64_t>(a.n) : 0,
+                      static_cast<int>(sizeof(*a.ptr))};
+  };
+  switch (which) {
+    case PMG_INDEX_MAP: return arr(map_);
+    case PMG_INDEX_CONSTRAINED: return arr(constrained_);
+    case PMG_INDEX_CELL_ORDER: return arr(perm_);
+    case PMG_INDEX_LINES: return arr(lines_);
+    case PMG_INDEX_LINE_FLAGS: return arr(lflag_);
+    case PMG_INDEX_PATCH_DOFS: return arr(patches_.pdof);
+    case PMG_INDEX_PATCH_COUNTS: return arr(patches_.pcount);
+    case PMG_INDEX_PATCH_SLOTS: return arr(patches_.pidx);
+    case PMG_INDEX_PATCH_LINES: return arr(patches_.pline);
+    case PMG_INDEX_PATCH_LINE_FLAGS: return arr(patches_.pflag);
+    case PMG_INDEX_VERTICES: return arr(vertices_);
+    default: throw Error(PMG_ERR_INVALID_ARG, "unknown index array");
+  }
 }
 
 void LaplaceOperatorBK3::set_schedule(int kind, int gmode) {
@@ -451,54 +517,59 @@ void LaplaceOperatorBK3::set_subdomain(const uint32_t *dofmap, const uint32_t *p
               "bad interface cell list");
   const int nm = degree_ + 1;
   const int64_t nm3 = int64_t(nm) * nm * nm;
-  std::vector<uint32_t> perm(static_cast<size_t>(n_cells_));
-  if (n_cells_ > 0)
-    PMG_HIP_CHECK(hipMemcpy(perm.data(), perm_.ptr, perm.size() * sizeof(uint32_t),
-                            hipMemcpyDeviceToHost));
-  std::vector<uint8_t> pmask(static_cast<size_t>(n_dofs_), 0);
-  for (int64_t i = 0; i < n_physical; ++i) {
-    PMG_REQUIRE(int64_t(physical[i]) < n_dofs_, PMG_ERR_INVALID_ARG,
-                "physical boundary DoF out of range");
-    pmask[physical[i]] = 1;
-  }
   // the plain indices (setup of plain_dof_indices_per_color,
-  // portable_subdomain_laplace_operator.h:1056-1061), in device cell order
-  std::vector<uint32_t> smap(static_cast<size_t>(n_cells_ * nm3));
-  for (int64_t c = 0; c < n_cells_; ++c)
-    for (int64_t i = 0; i < nm3; ++i) {
-      const uint32_t d = dofmap[int64_t(perm[c]) * nm3 + i];
-      PMG_REQUIRE(d == kInvalid || int64_t(d) < n_dofs_, PMG_ERR_INVALID_ARG,
-                  "subdomain dofmap entry out of range");
-      smap[c * nm3 + i] = (d != kInvalid && pmask[d]) ? kInvalid : d;
-    }
+  // portable_subdomain_laplace_operator.h:1056-1061), in device cell order,
+  // physical boundary DoFs eliminated: range checks, mask and elimination on
+  // the device
+  DeviceArray<uint32_t> dphys;
+  dphys.upload(physical, static_cast<size_t>(n_physical));
+  DeviceArray<uint8_t> pmask(static_cast<size_t>(std::max<int64_t>(n_dofs_, 1)));
+  PMG_REQUIRE(mask_from_list_device(n_physical, dphys.ptr, n_dofs_, pmask.ptr, nullptr) < 0,
+              PMG_ERR_INVALID_ARG, "physical boundary DoF out of range");
+  DeviceArray<uint32_t> smap, raw;
+  if (caller_order_) {
+    smap.upload(dofmap, static_cast<size_t>(n_cells_ * nm3));
+  } else {
+    raw.upload(dofmap, static_cast<size_t>(n_cells_ * nm3));
+    smap.resize(static_cast<size_t>(n_cells_ * nm3));
+  }
+  PMG_REQUIRE(map_eliminate_device(n_cells_, static_cast<int>(nm3),
+                                   caller_order_ ? smap.ptr : raw.ptr,
+                                   caller_order_ ? nullptr : perm_.ptr, pmask.ptr, n_dofs_,
+                                   smap.ptr, nullptr) < 0,
+              PMG_ERR_INVALID_ARG, "subdomain dofmap entry out of range");
   // interface cells: caller index -> device position, ascending
-  std::vector<uint32_t> where(static_cast<size_t>(n_cells_));
-  for (int64_t c = 0; c < n_cells_; ++c) where[perm[c]] = static_cast<uint32_t>(c);
   std::vector<uint32_t> ids(static_cast<size_t>(n_interface));
-  for (int64_t i = 0; i < n_interface; ++i) {
+  for (int64_t i = 0; i < n_interface; ++i)
     PMG_REQUIRE(int64_t(interface_cells[i]) < n_cells_, PMG_ERR_INVALID_ARG,
                 "interface cell out of range");
-    ids[i] = where[interface_cells[i]];
+  if (caller_order_) {
+    std::copy(interface_cells, interface_cells + n_interface, ids.begin());
+  } else {
+    const std::vector<uint32_t> perm = perm_.download();
+    std::vector<uint32_t> where(static_cast<size_t>(n_cells_));
+    for (int64_t c = 0; c < n_cells_; ++c) where[perm[c]] = static_cast<uint32_t>(c);
+    for (int64_t i = 0; i < n_interface; ++i) ids[i] = where[interface_cells[i]];
   }
   std::sort(ids.begin(), ids.end());
   PMG_REQUIRE(std::adjacent_find(ids.begin(), ids.end()) == ids.end(), PMG_ERR_INVALID_ARG,
               "interface cell listed twice");
-  smap_.upload(smap);
+  smap_ = std::move(smap);
   // its scatter schedule: the main map's kind (patch or atomic), same G loads
   ssched_ = Bk3Schedule{};
-  if (compress_lines(smap, n_cells_, nm, n_dofs_, slines_, sflag_)) {
+  if (compress_lines(smap_.ptr, n_cells_, nm, n_dofs_, slines_, sflag_)) {
     ssched_.lines = slines_.ptr;
     ssched_.lflag = sflag_.ptr;
   }
   spatches_ = PatchArrays{};
   if (sched_.kind == kPatch) {
-    build_patch_schedule(smap, n_cells_, degree_, nq_, n_dofs_, spatches_, ssched_);
+    build_patch_schedule(smap_.ptr, n_cells_, degree_, nq_, n_dofs_, spatches_, ssched_);
     if (ssched_.pdof) {
       ssched_.kind = kPatch;
       ssched_.gmode = sched_.gmode;
     }
   }
-  phys_.upload(physical, static_cast<size_t>(n_physical));
+  phys_ = std::move(dphys);
   icells_.upload(ids);
   n_phys_ = n_physical;
   n_icells_ = n_interface;
@@ -735,24 +806,120 @@ double LaplaceOperatorBK3::algorithmic_flops() const {
 }
 
 // ============================================================ transfers
+// Inverse map (CSR) of a device map, built on the device: off[n_dofs + 1],
+// list = the positions of each DoF's entries in ascending order.
+static void inverse_csr(const uint32_t *map, int64_t n_pos, int64_t n_dofs,
+                        DeviceArray<uint32_t> &off, DeviceArray<uint32_t> &list) {
+  off.resize(static_cast<size_t>(n_dofs + 1));
+  uint32_t *l = nullptr;
+  int64_t ne = 0;
+  build_inverse_map(n_pos, map, n_dofs, off.ptr, &l, &ne, nullptr);
+  list.release();
+  list.ptr = l;
+  list.n = static_cast<size_t>(ne);
+}
+
+// Kind and degree checks; entries per cell and the 1D transfer matrix.
+void MGTransfer::init_kind() {
+  check_precision(precision_);
+  PMG_REQUIRE(kind_ == PMG_TRANSFER_H || kind_ == PMG_TRANSFER_P || kind_ == PMG_TRANSFER_DG_CG,
+              PMG_ERR_UNSUPPORTED, "transfer kind not supported");
+  PMG_REQUIRE(n_cells_ >= 0 && n_dofs_c_ >= 0 && n_dofs_f_ >= 0, PMG_ERR_INVALID_ARG,
+              "negative size");
+  if (kind_ == PMG_TRANSFER_DG_CG) {
+    // ContinuousTransfer (portable_continuous_transfer.h:18-485): CG(p) coarse,
+    // DG(p) fine on the same cells
+    PMG_REQUIRE(pc_ == pf_ && pc_ >= 1 && pc_ <= 6, PMG_ERR_UNSUPPORTED,
+                "DG-CG transfer needs degree_coarse == degree_fine in [1,6]");
+    nmc_ = nmf_ = pc_ + 1;
+    return;
+  }
+  std::vector<double> M;
+  if (kind_ == PMG_TRANSFER_H) {
+    PMG_REQUIRE(pc_ == pf_ && pc_ >= 1 && pc_ <= 6, PMG_ERR_UNSUPPORTED,
+                "h-transfer needs degree_coarse == degree_fine in [1,6]");
+    nmc_ = pc_ + 1;
+    nmf_ = 2 * pc_ + 1;
+    M = h_transfer_matrix(pc_);
+  } else {
+    PMG_REQUIRE(pc_ >= 1 && pf_ > pc_ && pf_ <= 6, PMG_ERR_UNSUPPORTED,
+                "p-transfer needs 1 <= degree_coarse < degree_fine <= 6");
+    nmc_ = pc_ + 1;
+    nmf_ = pf_ + 1;
+    M = p_transfer_matrix(pc_, pf_);
+  }
+  for (size_t i = 0; i < M.size(); ++i) {
+    M64_.M[i] = M[i];
+    M32_.M[i] = static_cast<float>(M[i]);
+  }
+}
+
+// DG-CG: the CG DoFs' DG copies (CSR) for the atomics-free restriction, from
+// the device coarse map; list entries are DG positions cell * n^3 + i, or
+// map_fine of them when the DG numbering is given.
+void MGTransfer::setup_dg_inverse(const uint32_t *map_fine) {
+  const int64_t n3 = int64_t(nmc_) * nmc_ * nmc_;
+  if (n_cells_ * n3 >= (int64_t(1) << 32)) return;
+  inverse_csr(map_c_.ptr, n_cells_ * n3, n_dofs_c_, inv_off_, inv_list_);
+  if (map_fine && inv_list_.n > 0) remap_list_device(static_cast<int64_t>(inv_list_.n), map_fine,
+                                                     inv_list_.ptr, nullptr);
+}
+
+// Restriction without coarse atomics: cells store their coarse values to a
+// buffer [cell][nc^3], and each coarse DoF gathers its entries through the
+// inverse map (CSR over buffer positions; constrained entries in none).  It
+// costs a second launch and the buffer's bytes, so it is used where it
+// measured faster: large levels (>= 2^20 entries) in FP32 or of a p-transfer
+// (Q4 h 64^3 -> 32^3: FP32 76 -> 58 us, FP64 83 -> 87 us; Q6 -> Q3 on 48^3:
+// FP32 134 -> 114 us, FP64 148 -> 139 us).  PMG_RESTRICT_GATHER=0|1 forces
+// either path.
+void MGTransfer::setup_restrict_gather() {
+  const int64_t nc3 = int64_t(nmc_) * nmc_ * nmc_;
+  const bool big = n_cells_ * nc3 >= (int64_t(1) << 20);
+  bool use_gather = big && (precision_ == PMG_FP32 || kind_ == PMG_TRANSFER_P);
+  if (const char *genv = std::getenv("PMG_RESTRICT_GATHER")) use_gather = std::string(genv) == "1";
+  if (use_gather && n_cells_ * nc3 < (int64_t(1) << 32) && n_dofs_c_ > 0) {
+    inverse_csr(map_c_.ptr, n_cells_ * nc3, n_dofs_c_, cinv_off_, cinv_list_);
+    cell_buf_.reinit(precision_, n_cells_ * nc3);
+  }
+}
+
+// setup_weights (portable_geometric_transfer.h:1344-1380) on the device: the
+// weights are 1/multiplicity and 0 on constrained fine DoFs, which is the
+// owner form (each unconstrained DoF's first entry marked in map_f_); fine
+// x-lines that are runs of consecutive DoFs become one word each.
+void MGTransfer::setup_owner(const uint8_t *dmask) {
+  const size_t nw = static_cast<size_t>(n_cells_) * nmf_ * nmf_ * nmf_;
+  PMG_REQUIRE(nw < (size_t(1) << 32) && n_dofs_f_ < (int64_t(1) << 31), PMG_ERR_SIZE,
+              "fine map too large for owner marks");
+  DeviceArray<uint32_t> first(static_cast<size_t>(std::max<int64_t>(n_dofs_f_, 1)));
+  launch_owner_marks(static_cast<int64_t>(nw), n_dofs_f_, map_f_.ptr, dmask, first.ptr, nullptr);
+  owner_ = true;
+  scale_ = pscale_;
+  if (nmf_ <= 32 && n_cells_ > 0) {
+    const int64_t nl = n_cells_ * nmf_ * nmf_;
+    fine_lines_.resize(static_cast<size_t>(2 * nl));
+    DeviceArray<uint32_t> bad(1);
+    PMG_HIP_CHECK(hipMemset(bad.ptr, 0, sizeof(uint32_t)));
+    launch_fine_lines(nl, nmf_, map_f_.ptr, reinterpret_cast<uint2 *>(fine_lines_.ptr), bad.ptr,
+                      nullptr);
+    uint32_t hbad = 1;
+    PMG_HIP_CHECK(hipMemcpy(&hbad, bad.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (hbad) fine_lines_.release();
+  }
+  PMG_HIP_CHECK(hipDeviceSynchronize());
+}
+
 MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
                        const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
                        int64_t n_dofs_coarse, int64_t n_dofs_fine, double scale,
                        const uint32_t *constrained_fine, int64_t n_constrained_fine)
     : kind_(kind), precision_(precision), pc_(pc), pf_(pf), pscale_(scale), n_cells_(n_cells),
       n_dofs_c_(n_dofs_coarse), n_dofs_f_(n_dofs_fine) {
-  check_precision(precision);
-  PMG_REQUIRE(kind == PMG_TRANSFER_H || kind == PMG_TRANSFER_P || kind == PMG_TRANSFER_DG_CG,
-              PMG_ERR_UNSUPPORTED, "transfer kind not supported");
-  PMG_REQUIRE(n_cells >= 0 && n_dofs_coarse >= 0 && n_dofs_fine >= 0, PMG_ERR_INVALID_ARG,
-              "negative size");
+  init_kind();
   if (kind == PMG_TRANSFER_DG_CG) {
-    // ContinuousTransfer (portable_continuous_transfer.h:18-485): CG(p) coarse,
-    // DG(p) fine on the same cells; fine map NULL = cell-major DG numbering
-    PMG_REQUIRE(pc == pf && pc >= 1 && pc <= 6, PMG_ERR_UNSUPPORTED,
-                "DG-CG transfer needs degree_coarse == degree_fine in [1,6]");
+    // fine map NULL = cell-major DG numbering
     PMG_REQUIRE(map_coarse, PMG_ERR_INVALID_ARG, "coarse (CG) map is NULL");
-    nmc_ = nmf_ = pc + 1;
     const int64_t n3 = int64_t(nmc_) * nmc_ * nmc_;
     PMG_REQUIRE(map_fine || n_dofs_fine == n_cells * n3, PMG_ERR_SIZE,
                 "implicit DG numbering needs n_dofs_fine = n_cells (p+1)^3");
@@ -764,18 +931,7 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     }
     map_c_.upload(map_coarse, static_cast<size_t>(n_cells * n3));
     if (map_fine) map_f_.upload(map_fine, static_cast<size_t>(n_cells * n3));
-    if (n_cells * n3 < (int64_t(1) << 32)) {
-      std::vector<uint32_t> off(static_cast<size_t>(n_dofs_coarse + 1), 0u);
-      for (int64_t e = 0; e < n_cells * n3; ++e)
-        if (map_coarse[e] != kInvalid) ++off[map_coarse[e] + 1];
-      for (int64_t i = 0; i < n_dofs_coarse; ++i) off[i + 1] += off[i];
-      std::vector<uint32_t> list(off.back()), fill(off.begin(), off.end() - 1);
-      for (int64_t e = 0; e < n_cells * n3; ++e)
-        if (map_coarse[e] != kInvalid)
-          list[fill[map_coarse[e]]++] = map_fine ? map_fine[e] : static_cast<uint32_t>(e);
-      inv_off_.upload(off);
-      inv_list_.upload(list);
-    }
+    setup_dg_inverse(map_f_.ptr);
     return;
   }
   const bool device_weights = weights == nullptr && n_constrained_fine >= 0;
@@ -783,24 +939,6 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
               "NULL setup array");
   PMG_REQUIRE(!device_weights || constrained_fine || n_constrained_fine == 0, PMG_ERR_INVALID_ARG,
               "NULL constrained fine DoF list");
-  std::vector<double> M;
-  if (kind == PMG_TRANSFER_H) {
-    PMG_REQUIRE(pc == pf && pc >= 1 && pc <= 6, PMG_ERR_UNSUPPORTED,
-                "h-transfer needs degree_coarse == degree_fine in [1,6]");
-    nmc_ = pc + 1;
-    nmf_ = 2 * pc + 1;
-    M = h_transfer_matrix(pc);
-  } else {
-    PMG_REQUIRE(pc >= 1 && pf > pc && pf <= 6, PMG_ERR_UNSUPPORTED,
-                "p-transfer needs 1 <= degree_coarse < degree_fine <= 6");
-    nmc_ = pc + 1;
-    nmf_ = pf + 1;
-    M = p_transfer_matrix(pc, pf);
-  }
-  for (size_t i = 0; i < M.size(); ++i) {
-    M64_.M[i] = M[i];
-    M32_.M[i] = static_cast<float>(M[i]);
-  }
   const int64_t nc3 = int64_t(nmc_) * nmc_ * nmc_, nf3 = int64_t(nmf_) * nmf_ * nmf_;
   for (int64_t e = 0; e < n_cells * nc3; ++e)
     PMG_REQUIRE(map_coarse[e] == kInvalid || int64_t(map_coarse[e]) < n_dofs_coarse,
@@ -809,62 +947,17 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     PMG_REQUIRE(int64_t(map_fine[e]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
                 "fine map entry out of range (fine maps carry no invalid entries)");
   map_c_.upload(map_coarse, static_cast<size_t>(n_cells * nc3));
-  // Restriction without coarse atomics: cells store their coarse values to
-  // a buffer [cell][nc^3], and each coarse DoF gathers its entries through
-  // the inverse map (CSR over buffer positions; constrained entries in none).
-  // It costs a second launch and the buffer's bytes, so it is used where it
-  // measured faster: large levels (>= 2^20 entries) in FP32 or of a
-  // p-transfer (Q4 h 64^3 -> 32^3: FP32 76 -> 58 us, FP64 83 -> 87 us;
-  // Q6 -> Q3 on 48^3: FP32 134 -> 114 us, FP64 148 -> 139 us).
-  // PMG_RESTRICT_GATHER=0|1 forces either path.
-  const bool big = n_cells * nc3 >= (int64_t(1) << 20);
-  bool use_gather = big && (precision == PMG_FP32 || kind == PMG_TRANSFER_P);
-  if (const char *genv = std::getenv("PMG_RESTRICT_GATHER")) use_gather = std::string(genv) == "1";
-  if (use_gather && n_cells * nc3 < (int64_t(1) << 32) && n_dofs_coarse > 0) {
-    std::vector<uint32_t> off(static_cast<size_t>(n_dofs_coarse + 1), 0u);
-    for (int64_t e = 0; e < n_cells * nc3; ++e)
-      if (map_coarse[e] != kInvalid) ++off[map_coarse[e] + 1];
-    for (int64_t i = 0; i < n_dofs_coarse; ++i) off[i + 1] += off[i];
-    std::vector<uint32_t> list(off.back()), fill(off.begin(), off.end() - 1);
-    for (int64_t e = 0; e < n_cells * nc3; ++e)
-      if (map_coarse[e] != kInvalid) list[fill[map_coarse[e]]++] = static_cast<uint32_t>(e);
-    cinv_off_.upload(off);
-    cinv_list_.upload(list);
-    cell_buf_.reinit(precision, n_cells * nc3);
-  }
+  setup_restrict_gather();
   const size_t nw = static_cast<size_t>(n_cells * nf3);
   if (device_weights) {
-    // setup_weights (portable_geometric_transfer.h:1344-1380) on the device:
-    // the weights are 1/multiplicity and 0 on constrained fine DoFs, which is
-    // the owner form (each unconstrained DoF's first entry marked)
-    PMG_REQUIRE(nw < (size_t(1) << 32) && n_dofs_fine < (int64_t(1) << 31), PMG_ERR_SIZE,
-                "fine map too large for owner marks");
-    std::vector<uint8_t> cmask(static_cast<size_t>(n_dofs_fine), 0);
-    for (int64_t i = 0; i < n_constrained_fine; ++i) {
-      PMG_REQUIRE(int64_t(constrained_fine[i]) < n_dofs_fine, PMG_ERR_INVALID_ARG,
-                  "constrained fine DoF out of range");
-      cmask[constrained_fine[i]] = 1;
-    }
-    DeviceArray<uint8_t> dmask;
-    dmask.upload(cmask);
-    DeviceArray<uint32_t> first(static_cast<size_t>(std::max<int64_t>(n_dofs_fine, 1)));
+    DeviceArray<uint32_t> dcons;
+    dcons.upload(constrained_fine, static_cast<size_t>(n_constrained_fine));
+    DeviceArray<uint8_t> dmask(static_cast<size_t>(std::max<int64_t>(n_dofs_fine, 1)));
+    PMG_REQUIRE(mask_from_list_device(n_constrained_fine, dcons.ptr, n_dofs_fine, dmask.ptr,
+                                      nullptr) < 0,
+                PMG_ERR_INVALID_ARG, "constrained fine DoF out of range");
     map_f_.upload(map_fine, nw);
-    launch_owner_marks(static_cast<int64_t>(nw), n_dofs_fine, map_f_.ptr, dmask.ptr, first.ptr,
-                       nullptr);
-    owner_ = true;
-    scale_ = scale;
-    if (nmf_ <= 32 && n_cells > 0) {
-      const int64_t nl = n_cells * nmf_ * nmf_;
-      fine_lines_.resize(static_cast<size_t>(2 * nl));
-      DeviceArray<uint32_t> bad(1);
-      PMG_HIP_CHECK(hipMemset(bad.ptr, 0, sizeof(uint32_t)));
-      launch_fine_lines(nl, nmf_, map_f_.ptr, reinterpret_cast<uint2 *>(fine_lines_.ptr), bad.ptr,
-                        nullptr);
-      uint32_t hbad = 1;
-      PMG_HIP_CHECK(hipMemcpy(&hbad, bad.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
-      if (hbad) fine_lines_.release();
-    }
-    PMG_HIP_CHECK(hipDeviceSynchronize());
+    setup_owner(dmask.ptr);
     return;
   }
   // Owner mode when the weights are 1/multiplicity (0 on constrained DoFs,
@@ -933,6 +1026,66 @@ MGTransfer::MGTransfer(int kind, int precision, int pc, int pf, int64_t n_cells,
     }
     PMG_HIP_CHECK(hipMemcpy(w_prolong_.data(), wp.data(), nw * 4, hipMemcpyHostToDevice));
     PMG_HIP_CHECK(hipMemcpy(w_restrict_.data(), wr.data(), nw * 4, hipMemcpyHostToDevice));
+  }
+}
+
+// The transfer between two levels of structured boxes with every boundary DoF
+// constrained, maps generated on the device: h (coarse box -> the box refined
+// once, degree pc = pf; the fine map is each coarse cell's (2p+1)^3 patch),
+// p (one box, pc -> pf) or DG-CG (one box, degree pc = pf, cell-major DG).
+MGTransfer::MGTransfer(int kind, int precision, const pmg_box &box, int pc, int pf, double scale)
+    : kind_(kind), precision_(precision), pc_(pc), pf_(pf), pscale_(scale),
+      n_cells_(int64_t(box.nx) * box.ny * box.nz), n_dofs_c_(0), n_dofs_f_(0) {
+  PMG_REQUIRE(box.nx > 0 && box.ny > 0 && box.nz > 0, PMG_ERR_INVALID_ARG,
+              "box needs at least one cell per direction");
+  auto grid_size = [&](int p, int r) {
+    return (int64_t(p) * r * box.nx + 1) * (int64_t(p) * r * box.ny + 1) *
+           (int64_t(p) * r * box.nz + 1);
+  };
+  init_kind();
+  n_dofs_c_ = grid_size(pc, 1);
+  const int64_t nc3 = int64_t(nmc_) * nmc_ * nmc_;
+  map_c_.resize(static_cast<size_t>(n_cells_ * nc3));
+  box_map_device(box, pc, 1, true, map_c_.ptr, nullptr);
+  if (kind == PMG_TRANSFER_DG_CG) {
+    n_dofs_f_ = n_cells_ * nc3;
+    setup_dg_inverse(nullptr);
+    PMG_HIP_CHECK(hipDeviceSynchronize());
+    return;
+  }
+  const bool h = kind == PMG_TRANSFER_H;
+  n_dofs_f_ = h ? grid_size(pc, 2) : grid_size(pf, 1);
+  setup_restrict_gather();
+  const int64_t nf3 = int64_t(nmf_) * nmf_ * nmf_;
+  map_f_.resize(static_cast<size_t>(n_cells_ * nf3));
+  box_map_device(box, h ? pc : pf, h ? 2 : 1, false, map_f_.ptr, nullptr);
+  DeviceArray<uint8_t> dmask(static_cast<size_t>(std::max<int64_t>(n_dofs_f_, 1)));
+  if (h) {
+    pmg_box fine = box;
+    fine.nx *= 2;
+    fine.ny *= 2;
+    fine.nz *= 2;
+    box_boundary_device(fine, pc, nullptr, dmask.ptr, nullptr);
+  } else {
+    box_boundary_device(box, pf, nullptr, dmask.ptr, nullptr);
+  }
+  setup_owner(dmask.ptr);
+}
+
+LaplaceOperatorBK3::IndexArray MGTransfer::index_array(int which) const {
+  auto arr = [](const auto &a) {
+    return LaplaceOperatorBK3::IndexArray{a.ptr, a.ptr ? static_cast<int64_t>(a.n) : 0,
+                                          static_cast<int>(sizeof(*a.ptr))};
+  };
+  switch (which) {
+    case PMG_TRANSFER_INDEX_COARSE_MAP: return arr(map_c_);
+    case PMG_TRANSFER_INDEX_FINE_MAP: return arr(map_f_);
+    case PMG_TRANSFER_INDEX_FINE_LINES: return arr(fine_lines_);
+    case PMG_TRANSFER_INDEX_RESTRICT_OFFSETS: return arr(cinv_off_);
+    case PMG_TRANSFER_INDEX_RESTRICT_LIST: return arr(cinv_list_);
+    case PMG_TRANSFER_INDEX_DG_OFFSETS: return arr(inv_off_);
+    case PMG_TRANSFER_INDEX_DG_LIST: return arr(inv_list_);
+    default: throw Error(PMG_ERR_INVALID_ARG, "unknown index array");
   }
 }
 

@@ -101,6 +101,9 @@ class LaplaceOperatorBK3 : public Operator {
   LaplaceOperatorBK3(int degree, int nq, int precision, int64_t n_cells, const uint32_t *dofmap,
                      int64_t n_dofs, const uint32_t *constrained, int64_t n_constrained,
                      const double *cell_vertices, const double *G);
+  // Structured box, every boundary DoF constrained; map, constraints and
+  // vertices generated on the device, box structure declared
+  LaplaceOperatorBK3(const pmg_box &box, int degree, int nq, int precision);
   void vmult(void *dst, const void *src, hipStream_t s) const override;
   void apply_add(void *dst, const void *src, hipStream_t s) const override;
   bool supports_prepared_vmult() const override;
@@ -145,6 +148,9 @@ class LaplaceOperatorBK3 : public Operator {
   void vmult_subdomain_t(T *dst, const T *src, bool interface_only, hipStream_t s) const;
   template <typename T> void compute_diagonal_t();
   template <typename T> void fill_geometry_t(const double *kappa, hipStream_t s);
+  void setup_shape();
+  void setup_schedule();
+  void setup_geometry();
   void build_coloring();
   void build_patches_once();
   void build_gather_once();
@@ -196,6 +202,14 @@ class LaplaceOperatorBK3 : public Operator {
   bool has_box_structure() const { return sched_.grid[0] > 0; }
   int n_colors() const { return static_cast<int>(sched_.color_offsets.size()) - 1; }
   const Bk3Schedule &schedule() const { return sched_; }
+  // Device index data by PMG_INDEX_* (pmg.h): pointer, element count and
+  // element size; {nullptr, 0} where the array is not built
+  struct IndexArray {
+    const void *ptr;
+    int64_t n;
+    int elem;
+  };
+  IndexArray index_array(int which) const;
 };
 
 // Symmetric interior penalty DG Laplacian (LaplaceOperatorDG,
@@ -258,6 +272,10 @@ class MGTransfer {
              const uint32_t *map_coarse, const uint32_t *map_fine, const double *weights,
              int64_t n_dofs_coarse, int64_t n_dofs_fine, double prolongation_scale,
              const uint32_t *constrained_fine = nullptr, int64_t n_constrained_fine = -1);
+  // Between structured boxes, every boundary DoF constrained, maps built on
+  // the device: h (box -> box refined once), p or DG-CG (same box)
+  MGTransfer(int kind, int precision, const pmg_box &box, int degree_coarse, int degree_fine,
+             double prolongation_scale);
   void prolongate_and_add(void *dst_fine, const void *src_coarse, hipStream_t s) const;
   void restrict_and_add(void *dst_coarse, const void *src_fine, hipStream_t s) const;
   // dst_coarse += R (b - t) (the V-cycle's defect restriction, one pass)
@@ -281,9 +299,15 @@ class MGTransfer {
               const uint32_t *constrained_fine, int64_t n_constrained_fine,
               int64_t n_dofs_coarse, int64_t n_dofs_fine);
   uint64_t generation() const { return generation_; }
+  // Device index data by PMG_TRANSFER_INDEX_* (pmg.h)
+  LaplaceOperatorBK3::IndexArray index_array(int which) const;
 
  private:
   void restrict_sub(void *dst, const void *src, const void *sub, hipStream_t s) const;
+  void init_kind();
+  void setup_dg_inverse(const uint32_t *map_fine);
+  void setup_restrict_gather();
+  void setup_owner(const uint8_t *dmask);
   int kind_, precision_, pc_ = 0, pf_ = 0, nmc_, nmf_;
   double pscale_ = 1.0;
   uint64_t generation_ = 0;
