@@ -530,7 +530,9 @@ def bench_rows(pmg):
         y = op.initialize_dof_vector()
         rows[f"a1_a2_bk3_vmult_q4_64cube_{tag}"] = _roof(time_events(lambda: op.vmult(y, x), 20), by)
         rows[f"a3_a4_operator_setup_q4_64cube_{tag}"] = {
-            "time_ms": t_setup * 1e3, "note": "host DoF map upload + G kernel (setup, not a hot op)"}
+            "time_ms": t_setup * 1e3,
+            "note": "box map, constraints, vertices, lines, patches and G built on the device "
+                    "(setup, not a hot op)"}
         rows[f"a5_diagonal_q4_64cube_{tag}"] = {"time_ms": t_diag * 1e3, "note": "setup"}
         # subdomain operator (x = 1 face the interface): vmult_neumann moves the
         # vmult's bytes; the interface-cell range s(N + 2 N_i + 6 C_i nq^3),
@@ -557,7 +559,7 @@ def bench_rows(pmg):
         torch.cuda.empty_cache()
 
         coarse = pmg.box(32)
-        if prec == pmg.FP64:  # set-up of the fine h-transfer: host weights vs setup_weights on the device
+        if prec == pmg.FP64:  # set-up of the fine h-transfer: host maps + weights vs all on the device
             for mode in (False, True):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -565,7 +567,8 @@ def bench_rows(pmg):
                 torch.cuda.synchronize()
                 rows[f"f2_h_transfer_setup_q4_32to64_{'device' if mode else 'host'}_weights"] = {
                     "time_ms": (time.perf_counter() - t0) * 1e3,
-                    "note": "box maps on the host included (setup, not a hot op)"}
+                    "note": ("maps and weights built on the device" if mode else
+                             "box maps and weights built on the host") + " (setup, not a hot op)"}
                 del tr
         tr = pmg.GeometricTransfer(coarse, 4, prec)
         xc = torch.rand(tr.n_c, dtype=dt[prec], device="cuda")

@@ -269,7 +269,7 @@ int pmg_operator_l2_error(pmg_operator op, const void *u, int u_kind, int n_q_po
  *    declared a lexicographic box (pmg_operator_set_box_structure): the cells
  *    store to the box's exploded grid and every DoF sums its <= 8 adjacent
  *    copies (coalesced, no atomics, bit-reproducible; n_cells (p+1)^3 s bytes).
- * Default: patch for n_q_points_1d <= 4 (FP64) or <= 6 (FP32) on operators
+ * Default: patch for n_q_points_1d <= 5 (FP64) or <= 7 (FP32) on operators
  * of more than 4096 cells, atomic otherwise (the faster of the two on MI355X
  * for each level type and size of the MG hierarchies);
  * PMG_BK3_SCHEDULE=atomic|colored|patch|gather|grid overrides it.

@@ -437,7 +437,7 @@ class LaplaceOperatorBK3:
         """Cell-loop schedule: SCHEDULE_ATOMIC (per-cell device atomics),
         SCHEDULE_COLORED (one launch per colour) or SCHEDULE_PATCH (batch
         patches in LDS).  The constructor's default is PATCH for
-        n_q_points_1d <= 4 (FP64) or <= 6 (FP32), ATOMIC otherwise.  g_load:
+        n_q_points_1d <= 5 (FP64) or <= 7 (FP32), ATOMIC otherwise.  g_load:
         G load policy (0 early non-temporal, 1 early, 2 late non-temporal,
         -1 the schedule's default)."""
         _check(lib().pmg_operator_set_schedule(self.h, int(schedule), g_load))

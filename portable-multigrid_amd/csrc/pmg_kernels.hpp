@@ -301,6 +301,8 @@ void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
 
 // ---------------------------------------------------------------- vectors
 template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s);
+// x[i] = (i mod 11) - mean (the eigenvalue estimate's start vector)
+template <typename T> void launch_fill_mod11(int64_t n, T *x, double mean, hipStream_t s);
 template <typename T> void launch_copy(int64_t n, T *dst, const T *src, hipStream_t s);
 // dst[idx[i]] = src[idx[i]]
 template <typename T>

@@ -113,20 +113,22 @@ void LaplaceOperatorBK3::setup_schedule() {
     sched_.lflag = lflag_.ptr;
   }
   // Default scatter schedule: the patch schedule (with early non-temporal G
-  // loads) where a batch holds enough cells for its LDS patch to fold many
-  // shared DoFs before the global update (E = 256/nq^2 cells: FP64 nq <= 4,
-  // i.e. E >= 16; FP32 nq <= 6), the per-cell atomic scatter (late G loads)
-  // otherwise.  From vmult timings on MI355X over the level types of the MG
-  // hierarchies (Q1..Q6, nq = p+1, p+2, 16^3..64^3 cells), patch / atomic
-  // time: FP64 nq 2..4 0.66-0.83, nq 5 0.87-1.06, nq 6 0.95-1.02, nq >= 7
-  // 1.22-1.26; FP32 nq 2..6 0.57-0.96, nq >= 7 1.07-1.13.  Levels of at
-  // most 4096 cells are launch-bound: there the atomic kernel is as fast or
-  // faster (Q4 FP32 2^3..16^3: 8.3-11.7 vs 8.8-13.9 us per launch).
+  // loads) for FP64 nq <= 5 and FP32 nq <= 7 on operators of more than 4096
+  // cells, the per-cell atomic scatter (late G loads) otherwise.  From vmult
+  // timings on MI355X over the level types of the MG hierarchies, patch vs
+  // atomic (us): FP64 Q4 nq 5 32^3 59.7 / 78.6, 64^3 489.5 / 507.3, 128^3
+  // 3567 / 4140; FP64 Q3 nq 4 48^3 89.4 / 127.0; FP64 Q6 nq 7 48^3 728.8 /
+  // 571.2 (atomic); FP32 Q4 nq 5 64^3 295.3 / 439.6; FP32 Q3 nq 4 48^3 70.5 /
+  // 122.9; FP32 Q6 nq 7 24^3 59.1 / 66.4, 48^3 370.8 / 399.0.  The gather
+  // and grid schedules were slower or within 3 % (grid FP32 Q4 64^3 291.6,
+  // Q3 48^3 66.3) and need a cell buffer.  Levels of at most 4096 cells are
+  // launch-bound: there the atomic kernel is as fast or faster (Q4 FP32
+  // 2^3..16^3: 8.3-11.7 vs 8.8-13.9 us per launch).
   // PMG_BK3_SCHEDULE=atomic|colored|patch|gather|grid and PMG_BK3_GLOAD=0|1|2
   // override.
   const char *env = std::getenv("PMG_BK3_SCHEDULE");
   const char *genv = std::getenv("PMG_BK3_GLOAD");
-  int kind = nq_ <= (precision_ == PMG_FP32 ? 6 : 4) && n_cells_ > 4096 ? kPatch : kAtomic;
+  int kind = nq_ <= (precision_ == PMG_FP32 ? 7 : 5) && n_cells_ > 4096 ? kPatch : kAtomic;
   if (env)
     kind = std::string(env) == "colored"  ? kColored
            : std::string(env) == "atomic" ? kAtomic
@@ -1327,12 +1329,11 @@ void estimate_eigenvalues(const Operator &op, int n_it, double &mn, double &mx) 
   T *x = static_cast<T *>(xv.data()), *r = static_cast<T *>(rv.data()), *z = static_cast<T *>(zv.data());
   T *p = static_cast<T *>(pv.data()), *q = static_cast<T *>(qv.data());
   const T *pinv = static_cast<const T *>(op.inverse_diagonal());
-  std::vector<T> h(static_cast<size_t>(n));
-  double mean = 0;
-  for (int64_t i = 0; i < n; ++i) mean += double(i % 11);
-  mean /= std::max<int64_t>(n, 1);
-  for (int64_t i = 0; i < n; ++i) h[i] = static_cast<T>(double(i % 11) - mean);
-  PMG_HIP_CHECK(hipMemcpy(r, h.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+  // start vector r_i = (i mod 11) - mean, filled on the device; the sum of
+  // i mod 11 is an integer below 2^53, so the closed form is exact
+  const int64_t full = n / 11, rest = n % 11;
+  const double mean = double(55 * full + rest * (rest - 1) / 2) / double(std::max<int64_t>(n, 1));
+  launch_fill_mod11<T>(n, r, mean, nullptr);
   launch_fill<T>(n, x, T(0), nullptr);
   double res = std::sqrt(dot_sync<T>(n, r, r, partials, sc, nullptr));
   std::vector<double> al, be;

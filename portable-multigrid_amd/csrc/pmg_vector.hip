@@ -241,8 +241,18 @@ __global__ void convert_unless_k(int64_t n, TD *d, const TS *s, const double *sk
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     d[i] = static_cast<TD>(__builtin_nontemporal_load(s + i));
 }
+template <typename T>
+__global__ void fill_mod11_k(int64_t n, T *x, double mean) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = static_cast<T>(double(i % 11) - mean);
+}
 }  // namespace
 
+template <typename T> void launch_fill_mod11(int64_t n, T *x, double mean, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_mod11_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, mean);
+  PMG_HIP_CHECK(hipGetLastError());
+}
 template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, v);
@@ -381,6 +391,7 @@ void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double t
 
 #define PMG_INST(T)                                                                           \
   template void launch_fill<T>(int64_t, T *, T, hipStream_t);                                 \
+  template void launch_fill_mod11<T>(int64_t, T *, double, hipStream_t);                      \
   template void launch_copy<T>(int64_t, T *, const T *, hipStream_t);                         \
   template void launch_copy_indexed<T>(int64_t, const uint32_t *, T *, const T *, hipStream_t); \
   template void launch_prep_constrained<T>(int64_t, const uint8_t *, T *, const T *, hipStream_t); \
