@@ -134,21 +134,10 @@ class LaplaceOperatorBK3 : public LaplaceOperatorBase {
                               constrained, n_constrained, cell_vertices, &h));
     h_ = Handle<pmg_operator, pmg_operator_destroy>(h);
   }
-  // structured box (dirichlet on the whole boundary)
+  // structured box (dirichlet on the whole boundary), index data built on the device
   LaplaceOperatorBK3(const pmg_box &box, int degree, int n_q_points_1d, int precision) {
-    const int64_t C = pmg_box_n_cells(&box), N = pmg_box_n_dofs(&box, degree);
-    const int64_t nm3 = int64_t(degree + 1) * (degree + 1) * (degree + 1);
-    std::vector<uint32_t> map(static_cast<size_t>(C * nm3));
-    std::vector<double> cv(static_cast<size_t>(C * 24));
-    int64_t nc = 0;
-    check(pmg_box_dofmap(&box, degree, 1, map.data()));
-    check(pmg_box_cell_vertices(&box, cv.data()));
-    check(pmg_box_boundary_dofs(&box, degree, nullptr, &nc));
-    std::vector<uint32_t> cons(static_cast<size_t>(nc));
-    check(pmg_box_boundary_dofs(&box, degree, cons.data(), &nc));
     pmg_operator h = nullptr;
-    check(pmg_operator_create(3, degree, n_q_points_1d, precision, C, map.data(), N, cons.data(),
-                              nc, cv.data(), &h));
+    check(pmg_operator_create_box(&box, degree, n_q_points_1d, precision, &h));
     h_ = Handle<pmg_operator, pmg_operator_destroy>(h);
   }
   void set_coefficient(const double *kappa_device, pmg_stream s = nullptr) {
@@ -234,25 +223,19 @@ class MGTransferBase {
     check(pmg_transfer_create(kind, precision, pc, pf, n_cells, mc, mf, w, nc, nf, scale, &h));
     h_ = Handle<pmg_transfer, pmg_transfer_destroy>(h);
   }
+  void create_box(int kind, int precision, const pmg_box &box, int pc, int pf, double scale) {
+    pmg_transfer h = nullptr;
+    check(pmg_transfer_create_box(kind, precision, &box, pc, pf, scale, &h));
+    h_ = Handle<pmg_transfer, pmg_transfer_destroy>(h);
+  }
   Handle<pmg_transfer, pmg_transfer_destroy> h_;
 };
 
-// coarse box -> once-refined box at one degree
+// coarse box -> once-refined box at one degree (maps and weights built on the device)
 class GeometricTransfer : public MGTransferBase {
  public:
   GeometricTransfer(const pmg_box &coarse, int degree, int precision, double prolongation_scale = 1.0) {
-    pmg_box fine = coarse;
-    fine.nx *= 2;
-    fine.ny *= 2;
-    fine.nz *= 2;
-    const int64_t C = pmg_box_n_cells(&coarse);
-    const int64_t nc3 = int64_t(degree + 1) * (degree + 1) * (degree + 1);
-    const int64_t nf3 = int64_t(2 * degree + 1) * (2 * degree + 1) * (2 * degree + 1);
-    std::vector<uint32_t> mc(static_cast<size_t>(C * nc3)), mf(static_cast<size_t>(C * nf3));
-    std::vector<double> w(static_cast<size_t>(C * nf3));
-    check(pmg_box_h_transfer_maps(&coarse, degree, mc.data(), mf.data(), w.data()));
-    create(PMG_TRANSFER_H, precision, degree, degree, C, mc.data(), mf.data(), w.data(),
-           pmg_box_n_dofs(&coarse, degree), pmg_box_n_dofs(&fine, degree), prolongation_scale);
+    create_box(PMG_TRANSFER_H, precision, coarse, degree, degree, prolongation_scale);
   }
 };
 
@@ -260,13 +243,7 @@ class GeometricTransfer : public MGTransferBase {
 class PolynomialTransfer : public MGTransferBase {
  public:
   PolynomialTransfer(const pmg_box &box, int pc, int pf, int precision) {
-    const int64_t C = pmg_box_n_cells(&box);
-    const int64_t nc3 = int64_t(pc + 1) * (pc + 1) * (pc + 1), nf3 = int64_t(pf + 1) * (pf + 1) * (pf + 1);
-    std::vector<uint32_t> mc(static_cast<size_t>(C * nc3)), mf(static_cast<size_t>(C * nf3));
-    std::vector<double> w(static_cast<size_t>(C * nf3));
-    check(pmg_box_p_transfer_maps(&box, pc, pf, mc.data(), mf.data(), w.data()));
-    create(PMG_TRANSFER_P, precision, pc, pf, C, mc.data(), mf.data(), w.data(),
-           pmg_box_n_dofs(&box, pc), pmg_box_n_dofs(&box, pf), 1.0);
+    create_box(PMG_TRANSFER_P, precision, box, pc, pf, 1.0);
   }
 };
 
@@ -274,12 +251,7 @@ class PolynomialTransfer : public MGTransferBase {
 class ContinuousTransfer : public MGTransferBase {
  public:
   ContinuousTransfer(const pmg_box &box, int degree, int precision) {
-    const int64_t C = pmg_box_n_cells(&box);
-    const int64_t n3 = int64_t(degree + 1) * (degree + 1) * (degree + 1);
-    std::vector<uint32_t> mc(static_cast<size_t>(C * n3));
-    check(pmg_box_dofmap(&box, degree, 1, mc.data()));
-    create(PMG_TRANSFER_DG_CG, precision, degree, degree, C, mc.data(), nullptr, nullptr,
-           pmg_box_n_dofs(&box, degree), C * n3, 1.0);
+    create_box(PMG_TRANSFER_DG_CG, precision, box, degree, degree, 1.0);
   }
 };
 
