@@ -647,6 +647,47 @@ def bench_mgcg_128(pmg):
             "speedup_mixed": r64["time_s"] / r32["time_s"], "levels": 7}
 
 
+def bench_reference_logged(pmg):
+    """The configuration of the reference's own GPU run logs (context, not a
+    target): [-1,0.9]^3 refined to 64^3 cells, continuous Q4 with q = p+1,
+    ph hierarchy p -> max(p-2,1) = 4, 2, 1 over Q1 h-levels 64^3 .. 2^3,
+    Chebyshev degree 5 (range 15, 10 eigenvalue CG iterations), coarse
+    Chebyshev with automatic degree to 1e-3, f = 1, CG to 1e-10
+    (tests/poisson_cube_bk3/program.cc:236-258,300-305,476-505).  The logs
+    predate HEAD's level sequence, so the comparison is indicative."""
+    fine = pmg.box(64, lo=(-1.0, -1.0, -1.0), hi=(0.9, 0.9, 0.9))
+    levels = pmg.ph_levels(fine, 4, 6)
+    out = {"workload": "ph-MG Q4 on [-1,0.9]^3 64^3 cells, p 4->2->1, Q1 h 64->2, Chebyshev 5, "
+                       "coarse Chebyshev auto-degree 1e-3, f = 1, CG to 1e-10",
+           "levels": len(levels),
+           "reference_logged": {"fp64_s": 0.1566, "mixed_s": 0.1108, "iterations": 5,
+                                "hardware": "1x GH200 (inferred)",
+                                "source": "tests/poisson_cube_bk3_fp64/results/single-gpu-batched."
+                                          "out:1243, tests/poisson_cube_bk3/results/"
+                                          "1-node-single-gpu.out:1338"}}
+    outer = None
+    for tag, prec in (("fp64", pmg.FP64), ("fp32_vcycle_fp64_cg", pmg.FP32)):
+        t0 = time.perf_counter()
+        mg, ops, _, _ = pmg.build_hierarchy(levels, precision=prec, smoother_degree=5,
+                                            coarse=pmg.COARSE_CHEBYSHEV,
+                                            outer_precision=pmg.FP64)
+        setup = time.perf_counter() - t0
+        if outer is None:
+            outer = ops[-1]
+            rhs = outer.assemble_rhs(0)
+            out["dofs"] = outer.m()
+        r, _ = solve_timed(pmg, outer, rhs, mg, pmg.PRECOND_MG, 1e-10)
+        r["setup_s"] = setup
+        out[tag] = r
+        del mg
+        if prec == pmg.FP32:
+            del ops
+    out["speedup_mixed"] = out["fp64"]["time_s"] / out["fp32_vcycle_fp64_cg"]["time_s"]
+    out["vs_reference_logged"] = {"fp64": 0.1566 / out["fp64"]["time_s"],
+                                  "mixed": 0.1108 / out["fp32_vcycle_fp64_cg"]["time_s"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -656,7 +697,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cells", type=int, default=64)
-    ap.add_argument("--schedule", choices=["atomic", "colored", "patch"], default=None,
+    ap.add_argument("--schedule", choices=["atomic", "colored", "patch", "gather", "grid"], default=None,
                     help="BK3 cell-loop schedule (default: the library default)")
     ap.add_argument("--gload", type=int, default=None,
                     help="BK3 G-load policy 0/1/2 (default: the schedule's own)")
@@ -668,7 +709,7 @@ def main():
     ap.add_argument("--config", action="append", default=None,
                     help="run only these configuration legs (repeatable): rows, bk3_q4_16cube_jacobi_pcg, "
                          "sipdg_q3_32cube_deformed, ph_mg_q6_48cube_deformed, adjoint_q4_32cube, "
-                         "mgcg_q4_128cube")
+                         "mgcg_q4_128cube, reference_logged_ph_q4_64cube")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -683,8 +724,8 @@ def main():
     log(f"[bench] building operator {n}^3 Q{degree} nq={nq}")
     op = pmg.LaplaceOperatorBK3(fine_box, degree, nq, pmg.FP64)
     if args.schedule is not None or args.gload is not None:
-        kinds = {"atomic": 0, "colored": 1, "patch": 2}
-        op.set_schedule(kinds[args.schedule or "atomic"], -1 if args.gload is None else args.gload)
+        kinds = {"atomic": 0, "colored": 1, "patch": 2, "gather": 3, "grid": 4}
+        op.set_schedule(kinds[args.schedule or "patch"], -1 if args.gload is None else args.gload)
     N = op.m()
     g = torch.Generator(device="cuda").manual_seed(1234)
     x = torch.rand(N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
@@ -774,7 +815,8 @@ def main():
                          ("sipdg_q3_32cube_deformed", bench_config3),
                          ("ph_mg_q6_48cube_deformed", bench_config4),
                          ("adjoint_q4_32cube", bench_config5),
-                         ("mgcg_q4_128cube", bench_mgcg_128)]:
+                         ("mgcg_q4_128cube", bench_mgcg_128),
+                         ("reference_logged_ph_q4_64cube", bench_reference_logged)]:
             if args.config and name not in args.config:
                 continue
             log(f"[bench] configuration {name}")
