@@ -64,8 +64,8 @@ def load_pmg():
     return mod
 
 
-PROFILE = os.path.join(ROOT, "profiles", "bk3_cells_q4_64cube_fp64.md")
-DEFAULT_KERNEL = "bk3_cells_kernel<5, 5, double, true, 2>"
+PROFILE = os.path.join(ROOT, "profiles", "bk3_patch_q4_64cube_fp64.md")
+DEFAULT_KERNEL = "bk3_patch_kernel<5, 5, double, 0>"
 
 
 def measured_traffic():
@@ -752,7 +752,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (measured_traffic() if args.schedule is None and args.gload is None
                                  else None),
-                     "traffic_source": "profiles/bk3_cells_q4_64cube_fp64.md (rocprofv3 PMC, "
+                     "traffic_source": "profiles/bk3_patch_q4_64cube_fp64.md (rocprofv3 PMC, "
                                        "calibrated against a 1 GiB copy)",
                      "kernel": DEFAULT_KERNEL,
                      "kernel_time_us": t_kernel * 1e6, "algorithmic_bytes": alg_bytes,
