@@ -763,7 +763,7 @@ def main():
         "fp64_fma_tflops": fma64 / 1e12, "fp32_fma_tflops": fma32 / 1e12,
         "fp64_vendor_tflops": FP64_PEAK_TFLOPS, "fp32_vendor_tflops": 157.3,
         "note": "independent FMA chains in every lane (pmg_measure_fma_rate); FP32 is "
-                "issued unpacked here (packed FP32 disabled), hence at the FP64 rate",
+                "issued unpacked here (packed FP32 disabled), about twice the FP64 rate",
         "hbm_copy_note": "configs.rows.hbm_copy_1gib is the measured copy rate"}
     # The reference's own logs hold 14.16 GDoF/s for this vmult
     # (same DoF count, nq = p+1) on a GH200; they are run logs, not a
