@@ -65,7 +65,7 @@ def load_pmg():
 
 
 PROFILE = os.path.join(ROOT, "profiles", "bk3_patch_q4_64cube_fp64.md")
-DEFAULT_KERNEL = "bk3_patch_kernel<5, 5, double, 0>"
+DEFAULT_KERNEL = "bk3_patch_kernel<5, 5, double, 0, false>"
 
 
 def measured_traffic():

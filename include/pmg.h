@@ -70,7 +70,9 @@ enum pmg_index_array {
   PMG_INDEX_PATCH_SLOTS = 7,       /* [uint16] [C][(p+1)^3] patch slot, 0xFFFF constrained */
   PMG_INDEX_PATCH_LINES = 8,       /* [uint16] [C][(p+1)^2] first slot of each x-line */
   PMG_INDEX_PATCH_LINE_FLAGS = 9,  /* [uint8]  [C] */
-  PMG_INDEX_VERTICES = 10          /* [double] [C][8][3] */
+  PMG_INDEX_VERTICES = 10,         /* [double] [C][8][3] */
+  PMG_INDEX_PATCH_COLORS = 11,     /* [uint8]  [C] colour of the cell within its batch */
+  PMG_INDEX_PATCH_NCOLORS = 12     /* [uint8]  [batches] colours (0: the batch adds with atomics) */
 };
 /* Index arrays of a transfer (pmg_transfer_index_data) */
 enum pmg_transfer_index_array {

@@ -147,19 +147,21 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 // One batch of E cells per workgroup.  The batch's patch (the union of its
 // cells' unconstrained DoFs, sorted) is read once into LDS, each cell reads
 // its source column from the patch, and the cells' results are summed into
-// the patch with LDS atomics; the patch is then added to y once per DoF, in
+// the patch, colour by colour of a batch-local cell colouring (or with LDS
+// atomics where no colouring is given); the patch is then added to y once per DoF, in
 // sorted DoF order, so a wave's device atomics cover long runs of consecutive
 // DoFs.  Every DoF is added atomically, also those no other batch touches:
 // plain stores into cache lines that other batches' atomics also update were
 // measured slower on MI355X (Q4 64^3 FP64 549 -> 500 us, FP32 330 -> 321 us,
 // Q6 48^3 FP32 480 -> 406 us with all-atomic write-out), so pdof's bit 31
 // (shared with another batch) is not used here.
-template <int NM, int NQ, typename T, int GMODE>
+template <int NM, int NQ, typename T, int GMODE, bool COLORED>
 __global__ void __launch_bounds__(256)
 bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ pcount,
                  const uint16_t *__restrict__ pidx, const uint16_t *__restrict__ pline,
-                 const uint8_t *__restrict__ pflag, const int64_t n_cells, const Shape1D<T> sh)
+                 const uint8_t *__restrict__ pflag, const uint8_t *__restrict__ pcolor,
+                 const uint8_t *__restrict__ pncol, const int64_t n_cells, const Shape1D<T> sh)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells, PMAX = E * NM3;
@@ -179,6 +181,9 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   // registers (they are reused for the write-out), so the workgroup pays one
   // memory latency per dependent step instead of one per loop iteration.
   const int np = static_cast<int>(pcount[batch]);
+  // colours of the batch's cells (uniform over the workgroup; 0: LDS atomics)
+  const int n_colors = COLORED ? static_cast<int>(pncol[batch]) : 0;
+  const int color = COLORED && n_colors > 0 && active ? static_cast<int>(pcolor[cell]) : 0;
   const uint32_t *pd = pdof + batch * PMAX;
   uint16_t pi[NM];
   if (col) {
@@ -217,12 +222,26 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   }
   bk3_cell_apply<NM, NQ, T, GMODE>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3, active, ta, tb,
                                    patch, np);
-  if (col) {
+  if (COLORED && n_colors > 0) {
+    // colour by colour: no two cells of a colour share a slot, so plain
+    // read-modify-writes replace the LDS float atomics (measured costly at
+    // FP32), in a fixed order
+    for (int c = 0; c < n_colors; ++c) {
+      if (col && color == c) {
 #pragma unroll
-    for (int k = 0; k < NM; ++k)
-      if (pi[k] != 0xFFFFu) atomicAdd(patch + pi[k], v[k]);
+        for (int k = 0; k < NM; ++k)
+          if (pi[k] != 0xFFFFu) patch[pi[k]] += v[k];
+      }
+      __syncthreads();
+    }
+  } else {
+    if (col) {
+#pragma unroll
+      for (int k = 0; k < NM; ++k)
+        if (pi[k] != 0xFFFFu) atomicAdd(patch + pi[k], v[k]);
+    }
+    __syncthreads();
   }
-  __syncthreads();
 #pragma unroll
   for (int j = 0; j < PL; ++j) {
     const int i = tid + j * LANES;
@@ -241,8 +260,19 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
   (void)assign;  // y += patch either way: callers of vmult zero the free rows first
-  hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE>), dim3(grid), dim3(Tile::lanes), 0, s,
-                     x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, n_cells, sh);
+  // the colour-by-colour sum is compiled for the default G-load policy only
+  bool colored = false;
+  if constexpr (GMODE == kGEarlyNT) {
+    colored = sc.pncol != nullptr && sc.pcolor != nullptr;
+    if (colored)
+      hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true>), dim3(grid), dim3(Tile::lanes),
+                         0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, sc.pcolor,
+                         sc.pncol, n_cells, sh);
+  }
+  if (!colored)
+    hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, false>), dim3(grid), dim3(Tile::lanes),
+                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, nullptr,
+                       nullptr, n_cells, sh);
   PMG_HIP_CHECK(hipGetLastError());
 }
 

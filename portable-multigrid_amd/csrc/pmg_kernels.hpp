@@ -129,6 +129,11 @@ struct Bk3Schedule {
   // x-line are consecutive; null when too few cells qualify
   const uint16_t *pline = nullptr;
   const uint8_t *pflag = nullptr;
+  // batch-local cell colouring of the patch schedule (null: the cells add
+  // into the LDS patch with LDS atomics): colour by colour, the cells of one
+  // colour add with plain read-modify-writes (no two share a slot);
+  // pncol[b] = 0 keeps the atomics for batch b
+  const uint8_t *pcolor = nullptr, *pncol = nullptr;
   // gather data (kGather): inverse map of the gn DoFs, goff[gn + 1] into
   // glist (entry positions cell * nm^3 + local, ascending per DoF), and the
   // per-cell buffer gbuf [n_cells][nm^3] of the operator precision
@@ -186,10 +191,12 @@ int64_t compress_patch_lines_device(int64_t n_cells, int nm, const uint16_t *pid
                                     uint8_t *flag, hipStream_t s);
 // list[i] = to[list[i]] (positions -> the numbering a map gives them)
 void remap_list_device(int64_t n, const uint32_t *to, uint32_t *list, hipStream_t s);
-// patches of the batches of E cells (Bk3Schedule::pdof, pcount, pidx)
+// patches of the batches of E cells (Bk3Schedule::pdof, pcount, pidx) and
+// the batch-local cell colouring (pcolor per cell, pncol colours per batch)
 bool patch_schedule_fits(int64_t n_cells, int nm3, int E, int64_t n_dofs);
 void build_patches_device(int64_t n_cells, int nm3, int E, const uint32_t *map, int64_t n_dofs,
-                          uint32_t *pdof, uint32_t *pcount, uint16_t *pidx, hipStream_t s);
+                          uint32_t *pdof, uint32_t *pcount, uint16_t *pidx, uint8_t *pcolor,
+                          uint8_t *pncol, hipStream_t s);
 
 // y += sum_cells A_e x.  assign is accepted for the callers' intent (vmult,
 // which zeroes the free rows first) and does not change the result.

@@ -93,7 +93,7 @@ class Operator {
 struct PatchArrays {
   DeviceArray<uint32_t> pdof, pcount;
   DeviceArray<uint16_t> pidx, pline;
-  DeviceArray<uint8_t> pflag;
+  DeviceArray<uint8_t> pflag, pcolor, pncol;
 };
 
 class LaplaceOperatorBK3 : public Operator {

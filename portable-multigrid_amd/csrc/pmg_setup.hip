@@ -248,29 +248,43 @@ constexpr int kPatchMaxEntries = 2 * kPatchThreads;
 // every entry's slot goes to pidx (0xFFFF for invalid entries), each slot's
 // DoF to pdof with bit 31 set when the run is shorter than the DoF's
 // multiplicity (a cell outside the batch has it); slots past the count are 0.
+// Then a greedy colouring of the batch's cells in cell order (a cell takes
+// the smallest colour no earlier cell sharing one of its slots has) into
+// pcolor, and the number of colours into pncol[b]; pncol[b] = 0 when a cell
+// names one DoF twice (its lanes would update one slot together).
 __global__ void __launch_bounds__(kPatchThreads)
 patch_build_kernel(const int64_t n_cells, const int nm3, const int E, const int sort_n,
                    const uint32_t *__restrict__ map, const uint32_t *__restrict__ mult,
                    uint32_t *__restrict__ pdof, uint32_t *__restrict__ pcount,
-                   uint16_t *__restrict__ pidx)
+                   uint16_t *__restrict__ pidx, uint8_t *__restrict__ pcolor,
+                   uint8_t *__restrict__ pncol)
 {
   using Scan = hipcub::BlockScan<uint32_t, kPatchThreads>;
   __shared__ uint64_t key[kPatchMaxEntries];
-  __shared__ uint16_t first[kPatchMaxEntries], last[kPatchMaxEntries];
+  __shared__ uint16_t first[kPatchMaxEntries], last[kPatchMaxEntries], eslot[kPatchMaxEntries];
+  __shared__ unsigned long long smask[kPatchMaxEntries];
+  __shared__ unsigned long long forbid;
+  __shared__ int dup;
   __shared__ typename Scan::TempStorage scan_tmp;
   constexpr uint64_t kNone = ~uint64_t(0);
   const int64_t b = blockIdx.x;
   const int64_t c0 = b * E;
-  const int ne = static_cast<int>(std::min<int64_t>(E, n_cells - c0)) * nm3;
+  const int nce = static_cast<int>(std::min<int64_t>(E, n_cells - c0));
+  const int ne = nce * nm3;
   const int pmax = E * nm3;
   const int t = threadIdx.x;
   uint16_t *bidx = pidx + c0 * nm3;
+  if (t == 0) dup = 0;
   for (int k = t; k < sort_n; k += kPatchThreads) {
     uint64_t v = kNone;
     if (k < ne) {
       const uint32_t d = map[c0 * nm3 + k];
-      if (d != kInvalid) v = (uint64_t(d) << 16) | uint64_t(k);
-      else bidx[k] = 0xFFFF;
+      if (d != kInvalid) {
+        v = (uint64_t(d) << 16) | uint64_t(k);
+      } else {
+        bidx[k] = 0xFFFF;
+        eslot[k] = 0xFFFF;
+      }
     }
     key[k] = v;
   }
@@ -301,9 +315,13 @@ patch_build_kernel(const int64_t n_cells, const int nm3, const int E, const int 
     const uint64_t v = key[k];
     const uint32_t s = slot[q] + head[q] - 1;
     bidx[v & 0xFFFF] = static_cast<uint16_t>(s);
+    eslot[v & 0xFFFF] = static_cast<uint16_t>(s);
     if (head[q]) first[s] = static_cast<uint16_t>(k);
     if (k + 1 == sort_n || (key[k + 1] >> 16) != (v >> 16)) last[s] = static_cast<uint16_t>(k + 1);
+    // a run's entries are sorted by entry, so two of one cell are neighbours
+    if (!head[q] && int(key[k - 1] & 0xFFFF) / nm3 == int(v & 0xFFFF) / nm3) dup = 1;
   }
+  for (int n = t; n < pmax; n += kPatchThreads) smask[n] = 0ull;
   __syncthreads();
   for (int n = t; n < pmax; n += kPatchThreads) {
     uint32_t out = 0;
@@ -314,7 +332,26 @@ patch_build_kernel(const int64_t n_cells, const int nm3, const int E, const int 
     }
     pdof[b * pmax + n] = out;
   }
-  if (t == 0) pcount[b] = count;
+  int n_colors = 0;
+  for (int e = 0; e < nce; ++e) {
+    if (t == 0) forbid = 0ull;
+    __syncthreads();
+    const uint16_t s = t < nm3 ? eslot[e * nm3 + t] : uint16_t(0xFFFF);
+    if (s != 0xFFFF) {
+      const unsigned long long f = smask[s];
+      if (f) atomicOr(&forbid, f);
+    }
+    __syncthreads();
+    const int c = __builtin_ctzll(~forbid);  // < 64: at most 63 earlier cells
+    if (s != 0xFFFF) atomicOr(&smask[s], 1ull << c);
+    if (t == 0) pcolor[c0 + e] = static_cast<uint8_t>(c);
+    n_colors = c + 1 > n_colors ? c + 1 : n_colors;
+    __syncthreads();
+  }
+  if (t == 0) {
+    pcount[b] = count;
+    pncol[b] = dup ? 0 : static_cast<uint8_t>(n_colors);
+  }
 }
 
 struct BoxGrid {
@@ -441,9 +478,10 @@ bool patch_schedule_fits(int64_t n_cells, int nm3, int E, int64_t n_dofs) {
 }
 
 void build_patches_device(int64_t n_cells, int nm3, int E, const uint32_t *map, int64_t n_dofs,
-                          uint32_t *pdof, uint32_t *pcount, uint16_t *pidx, hipStream_t s) {
-  PMG_REQUIRE(patch_schedule_fits(n_cells, nm3, E, n_dofs), PMG_ERR_SIZE,
-              "patch schedule: batch too large");
+                          uint32_t *pdof, uint32_t *pcount, uint16_t *pidx, uint8_t *pcolor,
+                          uint8_t *pncol, hipStream_t s) {
+  PMG_REQUIRE(patch_schedule_fits(n_cells, nm3, E, n_dofs) && E <= 64 && nm3 <= kPatchThreads,
+              PMG_ERR_SIZE, "patch schedule: batch too large");
   const int64_t nb = (n_cells + E - 1) / E;
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "patch schedule: too many batches");
   int sort_n = 2;
@@ -454,7 +492,7 @@ void build_patches_device(int64_t n_cells, int nm3, int E, const uint32_t *map, 
                      n_cells * nm3, map, mult.ptr);
   PMG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(patch_build_kernel, dim3(static_cast<unsigned>(nb)), dim3(kPatchThreads), 0, s,
-                     n_cells, nm3, E, sort_n, map, mult.ptr, pdof, pcount, pidx);
+                     n_cells, nm3, E, sort_n, map, mult.ptr, pdof, pcount, pidx, pcolor, pncol);
   PMG_HIP_CHECK(hipGetLastError());
   PMG_HIP_CHECK(hipStreamSynchronize(s));  // mult is released on return
 }

@@ -80,6 +80,33 @@ def batch_cells(nq):
     return max(256 // (nq * nq), 1)
 
 
+def colors_np(imap, n_cells, nm3, E):
+    """Greedy colouring of each batch's cells in cell order (the smallest
+    colour no earlier cell of the batch sharing a DoF has) and the colour
+    count per batch, 0 where a cell names one DoF twice."""
+    nb = -(-n_cells // E)
+    pcolor = np.zeros(n_cells, np.uint8)
+    pncol = np.zeros(nb, np.uint8)
+    for b in range(nb):
+        masks, dup, ncol = {}, False, 0
+        for c in range(b * E, min(n_cells, (b + 1) * E)):
+            ent = imap[c * nm3:(c + 1) * nm3]
+            ent = [int(d) for d in ent if d != INV]
+            dup = dup or len(set(ent)) != len(ent)
+            forbidden = 0
+            for d in set(ent):
+                forbidden |= masks.get(d, 0)
+            col = 0
+            while (forbidden >> col) & 1:
+                col += 1
+            for d in set(ent):
+                masks[d] = masks.get(d, 0) | (1 << col)
+            pcolor[c] = col
+            ncol = max(ncol, col + 1)
+        pncol[b] = 0 if dup else ncol
+    return pcolor, pncol
+
+
 def check_operator_index_data(pmg, op, dofmap, cons, p, nq, n_dofs):
     """op's device arrays against the restatements over the caller's map."""
     n_cells = len(dofmap) // (p + 1) ** 3
@@ -98,6 +125,9 @@ def check_operator_index_data(pmg, op, dofmap, cons, p, nq, n_dofs):
     assert np.array_equal(op.index_data("patch_counts"), pcount)
     assert np.array_equal(op.index_data("patch_dofs"), pdof)
     assert np.array_equal(op.index_data("patch_slots"), pidx)
+    pc, pn = colors_np(imap, n_cells, nm3, batch_cells(nq))
+    assert np.array_equal(op.index_data("patch_colors"), pc)
+    assert np.array_equal(op.index_data("patch_ncolors"), pn)
     pl, pf = lines_np(pidx.astype(np.uint32), n_cells, nm, 0xFFFF, 0xFFFF)
     if pl is None:
         assert op.index_data("patch_lines").size == 0
@@ -152,6 +182,56 @@ def test_array_operator_device_elimination_on_shuffled_numbering(pmg, p, nq, n_e
                                             n_q_points_1d=nq)
     assert np.array_equal(op.index_data("constrained"), cons)
     check_operator_index_data(pmg, op, dofmap, cons, p, nq, n_dofs)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("accum", ["atomic", "colored"])
+@pytest.mark.parametrize("p,nq,dims", [(4, 5, (12, 3, 3)), (6, 7, (5, 3, 2)), (3, 4, (20, 2, 3)),
+                                       (1, 2, (4, 4, 5))])
+def test_patch_accumulation_modes_match_oracle(pmg, orc, monkeypatch, prec, accum, p, nq, dims):
+    """The patch schedule with its cells summed into the LDS patch by LDS
+    atomics or colour by colour gives the oracle's vmult."""
+    import torch
+    print(f"[patch accumulation] {accum} {prec} p={p} nq={nq} {dims}", flush=True)
+    monkeypatch.setenv("PMG_PATCH_ACCUM", accum)
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    b = pmg.box(*dims, deform=1, amplitude=0.05)
+    op = pmg.LaplaceOperatorBK3(b, p, nq, precision)
+    op.set_schedule(pmg.SCHEDULE_PATCH)
+    assert op.index_data("patch_ncolors").min() >= 1
+    ref = orc.LaplaceBK3(orc.mesh(*dims, deform=1, amplitude=0.05), p, nq)
+    x = seeded_uniform(op.m(), 231)
+    y = op.initialize_dof_vector()
+    op.vmult(y, _t(x, op.dtype))
+    assert rel_err(y.double().cpu().numpy(), ref.vmult(x)) < tol
+
+
+def test_patch_colouring_falls_back_on_repeated_dofs(pmg, monkeypatch):
+    """A cell that names one DoF twice keeps LDS atomics for its batch
+    (pncol 0); the vmult still equals the atomic schedule's."""
+    import torch
+    monkeypatch.setenv("PMG_PATCH_ACCUM", "colored")
+    b, p, nq = pmg.box(12, 3, 3, deform=1, amplitude=0.05), 2, 3
+    n_dofs = pmg.box_n_dofs(b, p)
+    dofmap = pmg.box_dofmap(b, p, True)
+    nm3 = (p + 1) ** 3
+    c = 37  # an interior cell: its entries 13 and 14 are both valid
+    assert dofmap[c * nm3 + 13] != INV and dofmap[c * nm3 + 14] != INV
+    dofmap[c * nm3 + 14] = dofmap[c * nm3 + 13]
+    op = pmg.LaplaceOperatorBK3.from_arrays(p, dofmap, n_dofs, pmg.box_boundary_dofs(b, p),
+                                            cell_vertices=pmg.box_cell_vertices(b),
+                                            n_q_points_1d=nq)
+    op.set_schedule(pmg.SCHEDULE_PATCH)
+    pn = op.index_data("patch_ncolors")
+    E = batch_cells(nq)
+    assert pn[c // E] == 0 and (np.delete(pn, c // E) > 0).all()
+    x = _t(seeded_uniform(n_dofs, 241))
+    y1, y2 = op.initialize_dof_vector(), op.initialize_dof_vector()
+    op.vmult(y1, x)
+    op.set_schedule(pmg.SCHEDULE_ATOMIC)
+    op.vmult(y2, x)
+    assert rel_err(y1.cpu().numpy(), y2.cpu().numpy()) < 1e-14
 
 
 def test_device_range_checks_name_the_entry(pmg):
