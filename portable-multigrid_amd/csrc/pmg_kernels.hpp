@@ -363,9 +363,10 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
 // with r.r partials; x += alpha_prev p_prev (not if first) and p = z + beta p;
 // the last x += alpha p after the loop.  alpha is kept in sc[4] by
 // launch_cg_scalars(op 0).
+// stage (optional): the new r also written there in stage_precision
 template <typename T>
 void launch_cg_update_r(int64_t n, T *r, const T *q, const double *sc, double *partials,
-                        hipStream_t s);
+                        hipStream_t s, void *stage = nullptr, int stage_precision = PMG_FP64);
 template <typename T, typename TZ = T>
 void launch_cg_update_xp(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first,
                          hipStream_t s);

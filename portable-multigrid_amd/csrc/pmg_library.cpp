@@ -887,13 +887,15 @@ void MGTransfer::setup_dg_inverse(const uint32_t *map_fine) {
 // buffer [cell][nc^3], and each coarse DoF gathers its entries through the
 // inverse map (CSR over buffer positions; constrained entries in none).  It
 // costs a second launch and the buffer's bytes, so it is used where it
-// measured faster: large levels (>= 2^20 entries) in FP32 or of a p-transfer
-// (Q4 h 64^3 -> 32^3: FP32 76 -> 58 us, FP64 83 -> 87 us; Q6 -> Q3 on 48^3:
-// FP32 134 -> 114 us, FP64 148 -> 139 us).  PMG_RESTRICT_GATHER=0|1 forces
-// either path.
+// measured faster: p-transfers of >= 2^18 entries and FP32 h-transfers of
+// >= 2^20 entries (Q4 h 64^3 -> 32^3: FP32 76 -> 58 us, FP64 83 -> 87 us;
+// Q6 -> Q3 on 48^3: FP32 134 -> 114 us, FP64 148 -> 139 us; Q3 -> Q1 on
+// 48^3, 2^19.8 entries: FP32 44 -> 31 us, FP64 44 -> 37 us; the small Q1
+// h-levels of 2^12..2^17 entries lose 2-4 us each).  PMG_RESTRICT_GATHER=0|1
+// forces either path.
 void MGTransfer::setup_restrict_gather() {
   const int64_t nc3 = int64_t(nmc_) * nmc_ * nmc_;
-  const bool big = n_cells_ * nc3 >= (int64_t(1) << 20);
+  const bool big = n_cells_ * nc3 >= (int64_t(1) << (kind_ == PMG_TRANSFER_P ? 18 : 20));
   bool use_gather = big && (precision_ == PMG_FP32 || kind_ == PMG_TRANSFER_P);
   if (const char *genv = std::getenv("PMG_RESTRICT_GATHER")) use_gather = std::string(genv) == "1";
   if (use_gather && n_cells_ * nc3 < (int64_t(1) << 32) && n_dofs_c_ > 0) {
@@ -1274,14 +1276,16 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
     }
     op.vmult(q, p, s);
     launch_dot<T>(n, p, q, partials, sc + 1, s);
-    launch_cg_update_r<T>(n, r, q, sc, partials, s);
+    // the residual update also writes the next V-cycle input (its staging),
+    // so the GPU stays busy while the host waits for r.r
+    if (precond == PMG_PRECOND_MG)
+      launch_cg_update_r<T>(n, r, q, sc, partials, s, mg->input_buffer(), mg->level_precision());
+    else
+      launch_cg_update_r<T>(n, r, q, sc, partials, s);
     launch_reduce_partials(partials, kDotBlocks, sc, 2, 1, s);
     launch_cg_scalars(sc, 0, s, w.host_rr, target2);
     ++it;
     const bool check = it % check_every == 0 || it == max_it;
-    // the next iteration's V-cycle input is queued before the host waits for
-    // r.r, so the GPU stays busy; it skips itself once r.r has converged
-    if (precond == PMG_PRECOND_MG && it < max_it) mg->stage(r, s, check ? sc + 7 : nullptr);
     if (check) {
       if (w.host_rr) {
         PMG_HIP_CHECK(hipStreamSynchronize(s));

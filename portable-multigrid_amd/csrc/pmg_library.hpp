@@ -414,6 +414,9 @@ class VCycleMultigrid {
   // V-cycle on it and returns the finest level's solution
   void stage(const void *src, hipStream_t s, const double *skip) const;
   const void *run(hipStream_t s) const;
+  // the buffer stage() writes (the finest level's defect, level precision):
+  // a producer of the next input may write it directly instead
+  void *input_buffer() const { return def_.back().data(); }
   int outer_precision() const { return outer_precision_; }
   int level_precision() const { return precision_; }
   int64_t m() const { return ops_.back()->m(); }  // size of the finest level

@@ -182,15 +182,18 @@ __global__ void cg_p_k(int64_t n, T *p, const TZ *z, const double *sc, int first
 // p update of iteration k+1, which reads p_k anyway, also applies
 // x += alpha_k p_k (cg_xp_k); a last cg_x_k after the loop.  The same
 // arithmetic as updating x and r together, one pass over p less.
-template <typename T>
+// With stage, the new residual is also written there in the preconditioner's
+// precision (the V-cycle input of the next iteration: one pass over r less).
+template <typename T, typename TS>
 __global__ void __launch_bounds__(kThreads)
-cg_r_k(int64_t n, T *r, const T *q, const double *sc, double *partials) {
+cg_r_k(int64_t n, T *r, const T *q, const double *sc, double *partials, TS *stage) {
   const double pq = sc[kPq];
   const T a = static_cast<T>(pq != 0.0 ? sc[kRho] / pq : 0.0);
   double s = 0;
   for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
     const T ri = r[i] - a * __builtin_nontemporal_load(q + i);
     r[i] = ri;
+    if (stage) stage[i] = static_cast<TS>(ri);
     s += static_cast<double>(ri) * static_cast<double>(ri);
   }
   s = block_sum(s);
@@ -366,8 +369,13 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
 }
 template <typename T>
 void launch_cg_update_r(int64_t n, T *r, const T *q, const double *sc, double *partials,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(cg_r_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, q, sc, partials);
+                        hipStream_t s, void *stage, int stage_precision) {
+  if (stage && stage_precision == PMG_FP32)
+    hipLaunchKernelGGL((cg_r_k<T, float>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, q, sc,
+                       partials, static_cast<float *>(stage));
+  else
+    hipLaunchKernelGGL((cg_r_k<T, double>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, q, sc,
+                       partials, static_cast<double *>(stage));
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T, typename TZ>
@@ -423,9 +431,9 @@ PMG_CGX(float, float)
 PMG_CGX(float, double)
 #undef PMG_CGX
 template void launch_cg_update_r<double>(int64_t, double *, const double *, const double *,
-                                         double *, hipStream_t);
+                                         double *, hipStream_t, void *, int);
 template void launch_cg_update_r<float>(int64_t, float *, const float *, const double *, double *,
-                                        hipStream_t);
+                                        hipStream_t, void *, int);
 template void launch_cg_update_x<double>(int64_t, double *, const double *, const double *,
                                          hipStream_t);
 template void launch_cg_update_x<float>(int64_t, float *, const float *, const double *,
