@@ -75,22 +75,34 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
   // is fetched now; nothing else in the launch writes these DoFs
   const int64_t base = cell * NF3 + tb * NF + ta;
   uint32_t mf[NF];
-  T yv[NF];
+  T yv[NF];  // owner form: current fine values; weighted form: the weights
   if (OWNER && active) {
 #pragma unroll
     for (int c = 0; c < NF; ++c)
       mf[c] = LINES ? owner_entry<NF>(fl, cell, c, tb, ta) : map_f[base + c * NF2];
 #pragma unroll
     for (int c = 0; c < NF; ++c) yv[c] = (mf[c] & 0x80000000u) ? yf[mf[c] & 0x7FFFFFFFu] : T(0);
+  } else if (active) {
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+      mf[c] = map_f[base + c * NF2];
+      yv[c] = w[base + c * NF2];
+    }
   }
 
-  // gather the coarse cell values, lane (i,j): L0[k][j][i]
+  // gather the coarse cell values, lane (i,j): L0[k][j][i]; branch-free (an
+  // invalid entry reads entry 0, zeroed after the load): a load under a lane
+  // condition whose value is used in the branch makes the compiler wait for
+  // it there, one memory latency per entry
   if (active && ta < NC && tb < NC) {
+    uint32_t d[NC];
+    T v[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      const uint32_t d = map_c[cell * NC3 + k * NC2 + tb * NC + ta];
-      L0[k * NC2 + tb * NC + ta] = d == kInvalid ? T(0) : xc[d];
-    }
+    for (int k = 0; k < NC; ++k) d[k] = map_c[cell * NC3 + k * NC2 + tb * NC + ta];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v[k] = xc[d[k] == kInvalid ? 0u : d[k]];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) L0[k * NC2 + tb * NC + ta] = d[k] == kInvalid ? T(0) : v[k];
   }
   __syncthreads();
   // x sweep, lane (j,k): L1[k][j][a]
@@ -135,8 +147,7 @@ prolongate_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *
       if (OWNER) {
         if (mf[c] & 0x80000000u) yf[mf[c] & 0x7FFFFFFFu] = yv[c] + scale * v;
       } else {
-        const T wt = w[base + c * NF2];
-        if (wt != T(0)) atomic_add(yf + map_f[base + c * NF2], wt * v);
+        if (yv[c] != T(0)) atomic_add(yf + mf[c], yv[c] * v);
       }
     }
   }
@@ -165,20 +176,39 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 
   // weighted gather of the fine column over c by lane (a,b) (consecutive fine
   // DoFs across lanes), z sweep in registers: L0[k][b][a]
+  // Branch-free gathers (every fine map entry is a valid DoF): a load under a
+  // lane condition makes the compiler wait for its result inside the branch,
+  // one memory latency per entry.  Entries that are not gathered (no owner
+  // bit, zero weight) are loaded too and zeroed after the load; the branch on
+  // xs is uniform.
   if (active) {
     T f[NF];
     const int64_t base = cell * NF3 + tb * NF + ta;
+    uint32_t m[NF];
+    T wt[NF], v[NF];
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
-      if (OWNER) {
-        const uint32_t m = LINES ? owner_entry<NF>(fl, cell, c, tb, ta) : map_f[base + c * NF2];
-        const uint32_t d = m & 0x7FFFFFFFu;
-        f[c] = (m & 0x80000000u) ? (xs ? xf[d] - xs[d] : xf[d]) : T(0);
-      } else {
-        const T wt = w[base + c * NF2];
-        const uint32_t d = map_f[base + c * NF2];
-        f[c] = wt == T(0) ? T(0) : wt * (xs ? xf[d] - xs[d] : xf[d]);
+      m[c] = OWNER ? (LINES ? owner_entry<NF>(fl, cell, c, tb, ta) : map_f[base + c * NF2])
+                   : map_f[base + c * NF2];
+      wt[c] = OWNER ? T(1) : w[base + c * NF2];
+    }
+    if (xs) {
+      T a[NF], b[NF];
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
+        a[c] = xf[m[c] & 0x7FFFFFFFu];
+        b[c] = xs[m[c] & 0x7FFFFFFFu];
       }
+#pragma unroll
+      for (int c = 0; c < NF; ++c) v[c] = a[c] - b[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NF; ++c) v[c] = xf[m[c] & 0x7FFFFFFFu];
+    }
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+      if (OWNER) f[c] = (m[c] & 0x80000000u) ? v[c] : T(0);
+      else f[c] = wt[c] == T(0) ? T(0) : wt[c] * v[c];
     }
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
