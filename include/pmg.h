@@ -72,7 +72,12 @@ enum pmg_index_array {
   PMG_INDEX_PATCH_LINE_FLAGS = 9,  /* [uint8]  [C] */
   PMG_INDEX_VERTICES = 10,         /* [double] [C][8][3] */
   PMG_INDEX_PATCH_COLORS = 11,     /* [uint8]  [C] colour of the cell within its batch */
-  PMG_INDEX_PATCH_NCOLORS = 12     /* [uint8]  [batches] colours (0: the batch adds with atomics) */
+  PMG_INDEX_PATCH_NCOLORS = 12,    /* [uint8]  [batches] colours (0: the batch adds with atomics) */
+  PMG_INDEX_PATCH_CHUNKS = 13      /* [uint32] [batches][ceil(E (p+1)^3 / 64)][8] the patch DoFs of
+                                      every 64 slots as <= 4 runs: first DoFs of runs 0..3,
+                                      s1 | s2 << 8 | s3 << 16 | n_runs << 24 (run k starts at
+                                      slot s_k, 64 if absent; n_runs 0: more than 4 runs), 0, 0,
+                                      0; empty when fewer than half of the chunks are runs */
 };
 /* Index arrays of a transfer (pmg_transfer_index_data) */
 enum pmg_transfer_index_array {

@@ -41,7 +41,7 @@ OPERATOR_INDEX = {"map": (0, np.uint32), "constrained": (1, np.uint32),
                   "patch_counts": (6, np.uint32), "patch_slots": (7, np.uint16),
                   "patch_lines": (8, np.uint16), "patch_line_flags": (9, np.uint8),
                   "vertices": (10, np.float64), "patch_colors": (11, np.uint8),
-                  "patch_ncolors": (12, np.uint8)}
+                  "patch_ncolors": (12, np.uint8), "patch_chunks": (13, np.uint32)}
 TRANSFER_INDEX = {"coarse_map": (0, np.uint32), "fine_map": (1, np.uint32),
                   "fine_lines": (2, np.uint32), "restrict_offsets": (3, np.uint32),
                   "restrict_list": (4, np.uint32), "dg_offsets": (5, np.uint32),

@@ -159,7 +159,7 @@ template <int NM, int NQ, typename T, int GMODE, bool COLORED>
 __global__ void __launch_bounds__(256)
 bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ pcount,
-                 const uint16_t *__restrict__ pidx, const uint16_t *__restrict__ pline,
+                 const uint32_t *__restrict__ pchunk, const uint16_t *__restrict__ pidx, const uint16_t *__restrict__ pline,
                  const uint8_t *__restrict__ pflag, const uint8_t *__restrict__ pcolor,
                  const uint8_t *__restrict__ pncol, const int64_t n_cells, const Shape1D<T> sh)
 {
@@ -195,24 +195,63 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 #pragma unroll
   for (int k = 0; k < NM; ++k) pr[k] = pl[k * NM];
   const uint8_t pf = (pflag ? pflag : reinterpret_cast<const uint8_t *>(pidx))[lcell];
-  // the lane's patch entries (kept for the write-out); pdof is zero past the
-  // count, and indices past PMAX are clamped into the batch's own entries
+  // the lane's patch entries (kept for the write-out); indices past PMAX are
+  // clamped into the batch's own entries, values past the count are not used
   const uint32_t *pd = pdof + batch * PMAX;
   uint32_t pw[PL];
+  if (pchunk) {
+    // the 64 slots a wave handles in pass j form chunk j LANES/64 + wave,
+    // described by at most 4 runs of consecutive DoFs: one record of 8 words
+    // per wave (scalar loads) instead of 64 entries; chunks of more runs load
+    // their entries.  All records are loaded before any is used, and taken
+    // as wave-uniform values (readfirstlane), which keeps the compiler from
+    // sinking their loads into the selects below.
+    constexpr int QC = (PMAX + 63) / 64;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t lane = static_cast<uint32_t>(tid & 63);
+    uint4 rr[PL];
+    uint32_t rs[PL];
 #pragma unroll
-  for (int j = 0; j < PL; ++j) {
-    const int i = tid + j * LANES;
-    pw[j] = pd[i < PMAX ? i : PMAX - 1];
+    for (int j = 0; j < PL; ++j) {
+      const int q = j * (LANES / 64) + wv;
+      const uint32_t *rc = pchunk + (batch * QC + (q < QC ? q : QC - 1)) * 8;
+      rr[j] = *reinterpret_cast<const uint4 *>(rc);
+      rs[j] = rc[4];
+    }
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+      const int q = j * (LANES / 64) + wv;
+      const int i = tid + j * LANES;
+      const uint32_t r0 = __builtin_amdgcn_readfirstlane(rr[j].x);
+      const uint32_t r1 = __builtin_amdgcn_readfirstlane(rr[j].y);
+      const uint32_t r2 = __builtin_amdgcn_readfirstlane(rr[j].z);
+      const uint32_t r3 = __builtin_amdgcn_readfirstlane(rr[j].w);
+      const uint32_t sw = __builtin_amdgcn_readfirstlane(rs[j]);
+      const uint32_t s1 = sw & 0xFFu, s2 = (sw >> 8) & 0xFFu, s3 = (sw >> 16) & 0xFFu;
+      uint32_t d = lane >= s3   ? r3 + (lane - s3)
+                   : lane >= s2 ? r2 + (lane - s2)
+                   : lane >= s1 ? r1 + (lane - s1)
+                                : r0 + lane;
+      if (q >= QC) d = 0u;
+      else if ((sw >> 24) == 0) d = pd[i < PMAX ? i : PMAX - 1];  // more than 4 runs
+      pw[j] = d;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+      const int i = tid + j * LANES;
+      pw[j] = pd[i < PMAX ? i : PMAX - 1];
+    }
   }
   const int np = static_cast<int>(pcount[batch]);
   // colours of the batch's cells (uniform over the workgroup; 0: LDS atomics)
   const int n_colors = COLORED ? static_cast<int>(pncol[batch]) : 0;
   const int color = COLORED && n_colors > 0 && active ? static_cast<int>(pcolor[cell]) : 0;
-  // source values of every entry, those past the count included (x[0] or a
-  // duplicate: only the first np are stored)
+  // source values of every slot, those past the count reading x[0] (only the
+  // first np are stored)
   T xv[PL];
 #pragma unroll
-  for (int j = 0; j < PL; ++j) xv[j] = x[pw[j] & 0x7FFFFFFFu];
+  for (int j = 0; j < PL; ++j) xv[j] = x[tid + j * LANES < np ? pw[j] & 0x7FFFFFFFu : 0u];
   uint16_t pi[NM];
 #pragma unroll
   for (int k = 0; k < NM; ++k) pi[k] = static_cast<uint16_t>(pr[k] + lta);
@@ -281,12 +320,12 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
     colored = sc.pncol != nullptr && sc.pcolor != nullptr;
     if (colored)
       hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true>), dim3(grid), dim3(Tile::lanes),
-                         0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, sc.pcolor,
+                         0, s, x, y, G, sc.pdof, sc.pcount, sc.pchunk, sc.pidx, sc.pline, sc.pflag, sc.pcolor,
                          sc.pncol, n_cells, sh);
   }
   if (!colored)
     hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, false>), dim3(grid), dim3(Tile::lanes),
-                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pidx, sc.pline, sc.pflag, nullptr,
+                       0, s, x, y, G, sc.pdof, sc.pcount, sc.pchunk, sc.pidx, sc.pline, sc.pflag, nullptr,
                        nullptr, n_cells, sh);
   PMG_HIP_CHECK(hipGetLastError());
 }

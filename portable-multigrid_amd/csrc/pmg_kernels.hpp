@@ -118,6 +118,9 @@ struct Bk3Schedule {
   // cell-local DoF its patch index pidx[cell][i] (0xFFFF: constrained)
   const uint32_t *pdof = nullptr, *pcount = nullptr;
   const uint16_t *pidx = nullptr;
+  // run records of the patch entries, 8 words per 64 slots (see
+  // patch_chunk_kernel); null when fewer than half of the chunks are runs
+  const uint32_t *pchunk = nullptr;
   // compressed DoF indices (atomic and colored schedules): per cell
   // lines[cell][k][j] = first DoF of x-line (j,k) when lflag[cell] says every
   // x-line of the cell is a run of consecutive DoFs (no constrained entry);
@@ -194,6 +197,10 @@ void remap_list_device(int64_t n, const uint32_t *to, uint32_t *list, hipStream_
 // patches of the batches of E cells (Bk3Schedule::pdof, pcount, pidx) and
 // the batch-local cell colouring (pcolor per cell, pncol colours per batch)
 bool patch_schedule_fits(int64_t n_cells, int nm3, int E, int64_t n_dofs);
+// Run records of the patch entries, 8 words per 64 slots of every batch (see
+// patch_chunk_kernel); returns the number of chunks described by <= 4 runs
+int64_t build_patch_chunks_device(int64_t nb, int pmax, const uint32_t *pdof,
+                                  const uint32_t *pcount, uint32_t *rec, hipStream_t s);
 void build_patches_device(int64_t n_cells, int nm3, int E, const uint32_t *map, int64_t n_dofs,
                           uint32_t *pdof, uint32_t *pcount, uint16_t *pidx, uint8_t *pcolor,
                           uint8_t *pncol, hipStream_t s);

@@ -383,6 +383,16 @@ static void build_patch_schedule(const uint32_t *map, int64_t n_cells, int degre
   sc.pdof = pa.pdof.ptr;
   sc.pcount = pa.pcount.ptr;
   sc.pidx = pa.pidx.ptr;
+  // the patch entries as runs, 8 words per 64 slots instead of 256: used when
+  // at least half of the chunks are at most 4 runs (box numberings: all but
+  // a few)
+  const int64_t pmax = int64_t(E) * nm3, n_chunks = nb * ((pmax + 63) / 64);
+  pa.pchunk.resize(static_cast<size_t>(n_chunks * 8));
+  const int64_t n_runs =
+      build_patch_chunks_device(nb, static_cast<int>(pmax), pa.pdof.ptr, pa.pcount.ptr,
+                                pa.pchunk.ptr, nullptr);
+  if (2 * n_runs >= n_chunks) sc.pchunk = pa.pchunk.ptr;
+  else pa.pchunk.release();
   bool colored = precision == PMG_FP32;
   if (const char *env = std::getenv("PMG_PATCH_ACCUM")) colored = std::string(env) == "colored";
   sc.pcolor = colored ? pa.pcolor.ptr : nullptr;
@@ -472,6 +482,7 @@ LaplaceOperatorBK3::IndexArray LaplaceOperatorBK3::index_array(int which) const 
     case PMG_INDEX_VERTICES: return arr(vertices_);
     case PMG_INDEX_PATCH_COLORS: return arr(patches_.pcolor);
     case PMG_INDEX_PATCH_NCOLORS: return arr(patches_.pncol);
+    case PMG_INDEX_PATCH_CHUNKS: return arr(patches_.pchunk);
     default: throw Error(PMG_ERR_INVALID_ARG, "unknown index array");
   }
 }

@@ -91,7 +91,7 @@ class Operator {
 // Continuous Q_p Laplace operator (LaplaceOperatorBK3).
 // Device arrays of a patch schedule (Bk3Schedule::pdof .. pflag)
 struct PatchArrays {
-  DeviceArray<uint32_t> pdof, pcount;
+  DeviceArray<uint32_t> pdof, pcount, pchunk;
   DeviceArray<uint16_t> pidx, pline;
   DeviceArray<uint8_t> pflag, pcolor, pncol;
 };

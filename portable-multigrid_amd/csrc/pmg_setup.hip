@@ -473,6 +473,68 @@ void remap_list_device(int64_t n, const uint32_t *to, uint32_t *list, hipStream_
   PMG_HIP_CHECK(hipGetLastError());
 }
 
+// Run records of the patch entries, one per 64 consecutive slots of a batch
+// (the slots one wave handles in the patch kernel): 8 words {first DoF of runs
+// 0..3, s1 | s2 << 8 | s3 << 16 | nr << 24, 0, 0, 0}, runs of consecutive DoFs
+// starting at slots 0 = s0 < s1 < s2 < s3 (64 for runs that do not exist), nr
+// the number of runs (1..4) or 0 when the chunk's valid slots need more than
+// 4 runs (the kernel then loads the chunk's entries).  Slots past the count
+// are not described.
+__global__ void __launch_bounds__(256)
+patch_chunk_kernel(const int64_t n_chunks, const int pmax, const int qc,
+                   const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ pcount,
+                   uint32_t *__restrict__ rec, unsigned long long *described)
+{
+  const int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  bool ok = false;
+  if (c < n_chunks) {
+    const int64_t b = c / qc;
+    const int q = static_cast<int>(c - b * qc);
+    const int n = static_cast<int>(pcount[b]) - q * 64;
+    const uint32_t *pd = pdof + b * pmax + q * 64;
+    uint32_t d0[4] = {0u, 0u, 0u, 0u}, s[4] = {0u, 64u, 64u, 64u};
+    int nr = 1;
+    uint32_t prev = 0;
+    for (int u = 0; u < n && u < 64 && nr > 0; ++u) {
+      const uint32_t d = pd[u] & 0x7FFFFFFFu;
+      if (u == 0) {
+        d0[0] = d;
+      } else if (d != prev + 1) {
+        if (nr == 4) {
+          nr = 0;  // too many runs
+        } else {
+          d0[nr] = d;
+          s[nr] = static_cast<uint32_t>(u);
+          ++nr;
+        }
+      }
+      prev = d;
+    }
+    uint32_t *r = rec + c * 8;
+    for (int k = 0; k < 4; ++k) r[k] = d0[k];
+    r[4] = s[1] | (s[2] << 8) | (s[3] << 16) | (static_cast<uint32_t>(nr) << 24);
+    r[5] = r[6] = r[7] = 0u;
+    ok = nr > 0;
+  }
+  const unsigned long long votes = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && votes)
+    atomicAdd(described, static_cast<unsigned long long>(__popcll(votes)));
+}
+
+int64_t build_patch_chunks_device(int64_t nb, int pmax, const uint32_t *pdof,
+                                  const uint32_t *pcount, uint32_t *rec, hipStream_t s) {
+  const int qc = (pmax + 63) / 64;
+  const int64_t n_chunks = nb * qc;
+  if (n_chunks <= 0) return 0;
+  DeviceArray<unsigned long long> cnt(1);
+  PMG_HIP_CHECK(hipMemsetAsync(cnt.ptr, 0, sizeof(unsigned long long), s));
+  PMG_REQUIRE(n_chunks < (int64_t(1) << 39), PMG_ERR_SIZE, "too many patch chunks");
+  hipLaunchKernelGGL(patch_chunk_kernel, dim3(grid_for(n_chunks, int64_t(1) << 31)), dim3(256), 0, s, n_chunks, pmax,
+                     qc, pdof, pcount, rec, cnt.ptr);
+  PMG_HIP_CHECK(hipGetLastError());
+  return static_cast<int64_t>(read_flag(cnt.ptr, s));
+}
+
 bool patch_schedule_fits(int64_t n_cells, int nm3, int E, int64_t n_dofs) {
   return n_cells > 0 && n_dofs < (int64_t(1) << 31) && int64_t(E) * nm3 <= kPatchMaxEntries;
 }

@@ -107,6 +107,30 @@ def colors_np(imap, n_cells, nm3, E):
     return pcolor, pncol
 
 
+def chunks_np(pdof, pcount, pmax):
+    """Run records of every 64 patch slots of every batch: first DoFs of runs
+    0..3 of consecutive DoFs among the valid slots (bit 31 cleared), their
+    first slots s1..s3 (64 if absent) and the number of runs (0 if more than
+    4) packed in word 4."""
+    nb, qc = len(pcount), (pmax + 63) // 64
+    pd = (pdof.reshape(nb, pmax) & 0x7FFFFFFF).astype(np.int64)
+    rec = np.zeros((nb * qc, 8), np.uint32)
+    for b in range(nb):
+        for q in range(qc):
+            d = pd[b, q * 64:min(q * 64 + 64, int(pcount[b]))]
+            r = rec[b * qc + q]
+            starts = [0] + [u for u in range(1, len(d)) if d[u] != d[u - 1] + 1] if len(d) else []
+            use = starts[:4]  # a chunk of more runs keeps its first 4 and says 0 runs
+            s = use + [64] * (4 - len(use))
+            if not use:
+                s[0] = 0
+            for k, u in enumerate(use):
+                r[k] = d[u]
+            nr = 1 if not use else (len(starts) if len(starts) <= 4 else 0)
+            r[4] = s[1] | (s[2] << 8) | (s[3] << 16) | (nr << 24)
+    return rec
+
+
 def check_operator_index_data(pmg, op, dofmap, cons, p, nq, n_dofs):
     """op's device arrays against the restatements over the caller's map."""
     n_cells = len(dofmap) // (p + 1) ** 3
@@ -128,6 +152,11 @@ def check_operator_index_data(pmg, op, dofmap, cons, p, nq, n_dofs):
     pc, pn = colors_np(imap, n_cells, nm3, batch_cells(nq))
     assert np.array_equal(op.index_data("patch_colors"), pc)
     assert np.array_equal(op.index_data("patch_ncolors"), pn)
+    ch = chunks_np(pdof, pcount, batch_cells(nq) * nm3)
+    if 2 * int((ch[:, 4] >> 24 != 0).sum()) >= len(ch):
+        assert np.array_equal(op.index_data("patch_chunks").reshape(-1, 8), ch)
+    else:
+        assert op.index_data("patch_chunks").size == 0
     pl, pf = lines_np(pidx.astype(np.uint32), n_cells, nm, 0xFFFF, 0xFFFF)
     if pl is None:
         assert op.index_data("patch_lines").size == 0
@@ -376,3 +405,37 @@ def test_box_transfer_rejects_bad_arguments(pmg):
     bad.nx = 0
     assert lib.pmg_operator_create_box(ctypes.byref(bad), 2, 3, pmg.FP64, ctypes.byref(h)) != pmg.OK
     assert lib.pmg_operator_create_box(ctypes.byref(b), 7, 8, pmg.FP64, ctypes.byref(h)) != pmg.OK
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_patch_chunk_runs_with_fallback_chunks(pmg, prec):
+    """A box numbering with a few DoFs relabelled: most 64-slot chunks of the
+    patch lists are <= 4 runs, some are not and load their entries; the patch
+    vmult equals the atomic schedule's on the same operator."""
+    import torch
+    print(f"[patch chunks] partial relabel {prec}", flush=True)
+    precision = pmg.FP64 if prec == "fp64" else pmg.FP32
+    p, nq, dims = 4, 5, (12, 3, 4)
+    b = pmg.box(*dims, deform=1, amplitude=0.05)
+    n_dofs = pmg.box_n_dofs(b, p)
+    rng = np.random.Generator(np.random.PCG64(1234))
+    relabel = np.arange(n_dofs, dtype=np.uint32)
+    pick = rng.choice(n_dofs, n_dofs // 100, replace=False)
+    relabel[pick] = relabel[rng.permutation(pick)]
+    dofmap = relabel[pmg.box_dofmap(b, p, False)]
+    cons = np.sort(relabel[pmg.box_boundary_dofs(b, p)]).astype(np.uint32)
+    op = pmg.LaplaceOperatorBK3.from_arrays(p, dofmap, n_dofs, cons,
+                                            cell_vertices=pmg.box_cell_vertices(b),
+                                            n_q_points_1d=nq, precision=precision)
+    op.set_schedule(pmg.SCHEDULE_PATCH)
+    ch = op.index_data("patch_chunks").reshape(-1, 8)
+    nr = ch[:, 4] >> 24
+    assert ch.size > 0 and (nr == 0).any() and (nr > 0).sum() * 2 >= len(nr)
+    x = _t(seeded_uniform(n_dofs, 77), op.dtype)
+    y_patch = op.initialize_dof_vector()
+    op.vmult(y_patch, x)
+    op.set_schedule(pmg.SCHEDULE_ATOMIC)
+    y_atomic = op.initialize_dof_vector()
+    op.vmult(y_atomic, x)
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    assert rel_err(y_patch.double().cpu().numpy(), y_atomic.double().cpu().numpy()) < tol
