@@ -76,20 +76,27 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   uint32_t idx[NM];
   T u[NM], v[NM];
   if (active && ta < NM && tb < NM) {
-    bool full = true;
-    if (lines) {
-      const uint32_t *lc = lines + cell * NM2 + tb;
+    // branch-free loads (a load under a lane condition whose value is used in
+    // the branch makes the compiler wait for it there): without compressed
+    // lines the line and flag loads read the map and are ignored, invalid
+    // entries read x[0] and are zeroed after the load
+    const uint32_t *lc = (lines ? lines : map) + cell * NM2 + tb;
+    uint32_t ln[NM];
 #pragma unroll
-      for (int k = 0; k < NM; ++k) idx[k] = lc[k * NM] + ta;
-      full = lflag[cell] == 0;
-    }
-    if (full) {
+    for (int k = 0; k < NM; ++k) ln[k] = lc[k * NM];
+    const uint8_t lf = (lflag ? lflag : reinterpret_cast<const uint8_t *>(map))[cell];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) idx[k] = ln[k] + ta;
+    if (!lines || lf == 0) {
       const uint32_t *mc = map + cell * NM3 + tb * NM + ta;
 #pragma unroll
       for (int k = 0; k < NM; ++k) idx[k] = mc[k * NM2];
     }
+    T xv[NM];
 #pragma unroll
-    for (int k = 0; k < NM; ++k) u[k] = (idx[k] == kInvalid) ? T(0) : x[idx[k]];
+    for (int k = 0; k < NM; ++k) xv[k] = x[idx[k] == kInvalid ? 0u : idx[k]];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) u[k] = idx[k] == kInvalid ? T(0) : xv[k];
   }
   bk3_cell_apply<NM, NQ, T, GMODE, WT || (64 % NQ2 == 0)>(u, v, g, Gc, sh, P0, P0 + NQ3,
                                                          P0 + 2 * NQ3, active, ta, tb);
