@@ -67,7 +67,7 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   const bool active = (!WT || wc < CW) && (e < E) && (pos < n_range);
   const int64_t cell = active ? (cell_ids ? static_cast<int64_t>(cell_ids[pos]) : pos) : 0;
   T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
-  const T *Gc = G + g_index(NQ, cell, 0, active ? t : 0);
+  const T *Gc = G + g_index<T>(NQ, cell, 0, active ? t : 0);
 
   T g[6][NQ];
   if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
@@ -268,7 +268,7 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
     for (int k = 0; k < NM; ++k) pi[k] = pc[k * NM2];
   }
   T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
-  const T *Gc = G + g_index(NQ, active ? cell : 0, 0, t);
+  const T *Gc = G + g_index<T>(NQ, active ? cell : 0, 0, t);
   T g[6][NQ];
   if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
 #pragma unroll
@@ -467,7 +467,7 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
   const bool act1 = one && e1 < E && e1 < n_cells;
   const bool col1 = act1 && ta < NM && tb < NM;
   T *P1 = lds + (act1 ? e1 : 0) * lds_cell_stride<T>(3 * NQ3);
-  const T *G1 = G + g_index(NQ, act1 ? e1 : 0, 0, t1);
+  const T *G1 = G + g_index<T>(NQ, act1 ? e1 : 0, 0, t1);
   T g1[6][NQ];
   uint32_t idx1[NM];
   if (act1) bk3_load_g<NQ, T, kGEarly>(G1, g1);
@@ -1033,7 +1033,7 @@ geometry_kernel(const int nq, const int64_t n_cells, const double *__restrict__ 
   int comp = 0;
   for (int a = 0; a < 3; ++a)
     for (int b = a; b < 3; ++b, ++comp)
-      G[g_index(nq, c, comp, q)] =
+      G[g_index<T>(nq, c, comp, q)] =
           static_cast<T>(s * (K[a][0] * K[b][0] + K[a][1] * K[b][1] + K[a][2] * K[b][2]));
 }
 
@@ -1091,7 +1091,7 @@ bk3_diagonal_kernel(const int nm, const int nq, const int64_t n_cells, const TG 
   constexpr double wc[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};
   for (int c = 0; c < 6; ++c) {
     __syncthreads();
-    for (int o = lane; o < nq3; o += 64) g[o] = static_cast<double>(G[g_index(nq, cell, c, o)]);
+    for (int o = lane; o < nq3; o += 64) g[o] = static_cast<double>(G[g_index<TG>(nq, cell, c, o)]);
     __syncthreads();
     const double *X = f[fx[c]], *Y = f[fy[c]], *Z = f[fz[c]];
     for (int o = lane; o < nm * nq2; o += 64) {  // t1[(q2 nq + q1) nm + a] = sum_q0 X(a,q0) g
@@ -1207,15 +1207,16 @@ grad_coefficient_kernel(const int64_t n_cells, const T *__restrict__ G,
   cell_gradient<NM, NQ, T>(lam, map, cell, sh, lds + 3 * NQ3, lds + 4 * NQ3, lds + 5 * NQ3, ta,
                            tb, lz);
   __syncthreads();
-  constexpr int64_t CS = static_cast<int64_t>(NQ) * Bk3Tile<NQ>::lanes;  // component stride
   for (int r = 0; r < NQ; ++r) {
     const int id = r * NQ2 + tb * NQ + ta;
-    const T *Gq = G + g_index(NQ, cell, 0, id);
+    T gq[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) gq[c] = G[g_index<T>(NQ, cell, c, id)];
     const T ux = lds[2 * NQ3 + id], uy = lds[NQ3 + id], uzz = uz[r];
     const T lx = lds[5 * NQ3 + id], ly = lds[4 * NQ3 + id], lzz = lz[r];
-    const T fx = Gq[0] * ux + Gq[CS] * uy + Gq[2 * CS] * uzz;
-    const T fy = Gq[CS] * ux + Gq[3 * CS] * uy + Gq[4 * CS] * uzz;
-    const T fz = Gq[2 * CS] * ux + Gq[4 * CS] * uy + Gq[5 * CS] * uzz;
+    const T fx = gq[0] * ux + gq[1] * uy + gq[2] * uzz;
+    const T fy = gq[1] * ux + gq[3] * uy + gq[4] * uzz;
+    const T fz = gq[2] * ux + gq[4] * uy + gq[5] * uzz;
     out[(perm ? static_cast<int64_t>(perm[cell]) : cell) * NQ3 + id] = -(fx * lx + fy * ly + fz * lzz);
   }
 }

@@ -41,17 +41,34 @@ enum : int {
   kGLateNT = 2    // non-temporal loads issued where G is used
 };
 
-// Geometric factors of the lane's q-point column (ta,tb) over all planes r:
-// batch-interleaved G, component c, plane r at Gc[(c * NQ + r) * LANES], so
-// for a batch of consecutive cells a wave reads 64 consecutive values per load.
+// Geometric factors of the lane's q-point column (ta,tb) over all planes r,
+// Gc = G + g_index<T>(nq, cell, 0, q of plane 0): batch-interleaved G,
+// component c, plane r at Gc[(c * NQ + r) * LANES] (FP32) or the component
+// pair c/2 at Gc[((c / 2) * NQ + r) * 2 LANES] (FP64), so for a batch of
+// consecutive cells every load instruction of a wave reads 64 consecutive
+// values (pairs).
 template <int NQ, typename T, int GMODE>
 __device__ __forceinline__ void bk3_load_g(const T *Gc, T (&g)[6][NQ]) {
   constexpr int LANES = Bk3Tile<NQ>::lanes;
+  if constexpr (sizeof(T) == 8) {
+    // component pairs (0,1), (2,3), (4,5): one 16-byte load each
+    typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int c = 0; c < 6; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int r = 0; r < NQ; ++r)
-      g[c][r] = GMODE == kGEarly ? Gc[(c * NQ + r) * LANES] : ldg_stream(Gc + (c * NQ + r) * LANES);
+      for (int r = 0; r < NQ; ++r) {
+        const T2 *p = reinterpret_cast<const T2 *>(Gc + (c * NQ + r) * 2 * LANES);
+        const T2 v = GMODE == kGEarly ? *p : __builtin_nontemporal_load(p);
+        g[2 * c][r] = v.x;
+        g[2 * c + 1][r] = v.y;
+      }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+      for (int r = 0; r < NQ; ++r)
+        g[c][r] = GMODE == kGEarly ? Gc[(c * NQ + r) * LANES] : ldg_stream(Gc + (c * NQ + r) * LANES);
+  }
 }
 
 // Cell stiffness applied to one cell held by an NQ x NQ lane tile (lane

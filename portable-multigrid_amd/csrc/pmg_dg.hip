@@ -71,7 +71,7 @@ dg_apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict_
   T *V = P2 + NQ3;      // value coefficient at the Gauss points [r][q][p]
   T *A = V + NQ3;       // face scratch, 2 NQ^2
   T *B = A + 2 * NQ2;   // face scratch, 2 NQ^2
-  const T *Gc = G + g_index(NQ, active ? cell : 0, 0, t);
+  const T *Gc = G + g_index<T>(NQ, active ? cell : 0, 0, t);
 
   // ---- own column (i,j) = (ta,tb) over k; V = 0
   T u[NM], out[NM];

@@ -72,19 +72,27 @@ __host__ __device__ constexpr int g_batch_cells(int nq) {
   return (256 / (nq * nq)) > 0 ? 256 / (nq * nq) : 1;
 }
 // Storage of the geometric factors G (6 symmetric components per q-point),
-// batch-interleaved:
-//   G[((batch * 6 + comp) * nq + r) * lanes + e * nq^2 + (q mod nq^2)],
-// r = q / nq^2, e = cell mod E, so that every G load instruction of a wave
-// reads 64 consecutive values.  lanes = E nq^2 rounded up to 64.
+// batch-interleaved: with the lane l = e nq^2 + (q mod nq^2) of q-point q of
+// cell e of a batch (r = q / nq^2),
+//   FP32: G[((batch * 6 + comp) * nq + r) * lanes + l],
+//   FP64: G[((batch * 3 + comp / 2) * nq + r) * 2 lanes + 2 l + comp % 2],
+// so that every G load instruction of a wave reads 64 consecutive values
+// (FP32) or pairs of components (FP64: one 16-byte load for two components,
+// measured faster than two 8-byte loads; FP32 lost with 8-byte pairs).
+// lanes = E nq^2 rounded up to 64.
 __host__ __device__ inline int g_batch_lanes(int nq) {
   return ((g_batch_cells(nq) * nq * nq + 63) / 64) * 64;
 }
+template <typename T>
 __host__ __device__ inline int64_t g_index(int nq, int64_t cell, int comp, int q) {
   const int nq2 = nq * nq, E = g_batch_cells(nq);
   const int r = q / nq2, t = q - r * nq2;
   const int64_t b = cell / E;
   const int e = static_cast<int>(cell - b * E);
-  return ((b * 6 + comp) * nq + r) * g_batch_lanes(nq) + e * nq2 + t;
+  const int l = e * nq2 + t;
+  if (sizeof(T) == 8)
+    return ((b * 3 + comp / 2) * nq + r) * 2 * g_batch_lanes(nq) + 2 * l + (comp & 1);
+  return ((b * 6 + comp) * nq + r) * g_batch_lanes(nq) + l;
 }
 __host__ __device__ inline int64_t g_size(int nq, int64_t n_cells) {
   const int64_t nb = (n_cells + g_batch_cells(nq) - 1) / g_batch_cells(nq);

@@ -263,7 +263,8 @@ LaplaceOperatorBK3::LaplaceOperatorBK3(int degree, int nq, int precision, int64_
         const int64_t from = caller_order_ ? c : int64_t(perm[c]);
         for (int comp = 0; comp < 6; ++comp)
           for (int q = 0; q < nq3; ++q)
-            dst[g_index(nq, c, comp, q)] = static_cast<std::remove_pointer_t<decltype(dst)>>(
+            dst[g_index<std::remove_pointer_t<decltype(dst)>>(nq, c, comp, q)] =
+                static_cast<std::remove_pointer_t<decltype(dst)>>(
                 G[(from * 6 + comp) * nq3 + q]);
       }
     };
