@@ -84,8 +84,11 @@ enum pmg_transfer_index_array {
   PMG_TRANSFER_INDEX_COARSE_MAP = 0,        /* [uint32] [C][nc^3], constrained invalid */
   PMG_TRANSFER_INDEX_FINE_MAP = 1,          /* [uint32] [C][nf^3], bit 31 owner mark */
   PMG_TRANSFER_INDEX_FINE_LINES = 2,        /* [uint32] [C][nf^2][2] {first DoF, owner bits} */
-  PMG_TRANSFER_INDEX_RESTRICT_OFFSETS = 3,  /* [uint32] coarse DoF -> buffer entries (CSR) */
-  PMG_TRANSFER_INDEX_RESTRICT_LIST = 4,     /* [uint32] */
+  PMG_TRANSFER_INDEX_RESTRICT_OFFSETS = 3,  /* [uint32] coarse DoF i -> its restriction buffer
+                                               range [off[i], off[i+1]) */
+  PMG_TRANSFER_INDEX_RESTRICT_POSITIONS = 4, /* [uint32] [C][nc^3] buffer position of each coarse
+                                               map entry (entries ordered by coarse DoF, then
+                                               by entry index), constrained entries invalid */
   PMG_TRANSFER_INDEX_DG_OFFSETS = 5,        /* [uint32] CG DoF -> DG copies (CSR) */
   PMG_TRANSFER_INDEX_DG_LIST = 6            /* [uint32] */
 };

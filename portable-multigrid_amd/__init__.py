@@ -44,7 +44,7 @@ OPERATOR_INDEX = {"map": (0, np.uint32), "constrained": (1, np.uint32),
                   "patch_ncolors": (12, np.uint8), "patch_chunks": (13, np.uint32)}
 TRANSFER_INDEX = {"coarse_map": (0, np.uint32), "fine_map": (1, np.uint32),
                   "fine_lines": (2, np.uint32), "restrict_offsets": (3, np.uint32),
-                  "restrict_list": (4, np.uint32), "dg_offsets": (5, np.uint32),
+                  "restrict_positions": (4, np.uint32), "dg_offsets": (5, np.uint32),
                   "dg_list": (6, np.uint32)}
 
 OK, ERR_INVALID_ARG, ERR_SIZE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED, ERR_ALLOC = range(7)

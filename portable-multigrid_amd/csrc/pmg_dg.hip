@@ -433,7 +433,11 @@ dg_cg_restrict_kernel(const int64_t n, const uint32_t *__restrict__ cg_map,
 // Restriction as a gather over the inverse map: CG DoF i sums the DG values
 // of its copies list[off[i] .. off[i+1]) (CSR built at set-up), no atomics.
 // DoFs with no copy (constrained) are left untouched, as by the scatter form.
-template <typename T>
+// The copies are read in groups (entry 0; 1-3 where there are more; 4-7):
+// all list entries of the groups first, then all values, so a DoF with 8
+// copies waits two memory latencies instead of two per copy, and only lanes
+// whose DoF has that many copies issue a group.
+template <typename T, bool SUB>
 __global__ void __launch_bounds__(256)
 dg_cg_restrict_gather_kernel(const int64_t n_cg, const uint32_t *__restrict__ off,
                              const uint32_t *__restrict__ list, const T *__restrict__ src,
@@ -443,11 +447,33 @@ dg_cg_restrict_gather_kernel(const int64_t n_cg, const uint32_t *__restrict__ of
   if (i >= n_cg) return;
   const uint32_t k0 = off[i], k1 = off[i + 1];
   if (k0 == k1) return;
-  T acc = T(0);
-  if (sub)
-    for (uint32_t k = k0; k < k1; ++k) acc += src[list[k]] - sub[list[k]];
-  else
-    for (uint32_t k = k0; k < k1; ++k) acc += src[list[k]];
+  const uint32_t ne = k1 - k0, kl = k1 - 1;
+  auto val = [&](uint32_t e) { return SUB ? src[e] - sub[e] : src[e]; };
+  uint32_t e[8] = {};
+  e[0] = list[k0];
+  if (ne > 1) {
+#pragma unroll
+    for (int j = 1; j < 4; ++j) e[j] = list[k0 + j <= kl ? k0 + j : kl];
+  }
+  if (ne > 4) {
+#pragma unroll
+    for (int j = 4; j < 8; ++j) e[j] = list[k0 + j <= kl ? k0 + j : kl];
+  }
+  T a[8] = {};
+  a[0] = val(e[0]);
+  if (ne > 1) {
+#pragma unroll
+    for (int j = 1; j < 4; ++j) a[j] = val(e[j]);
+  }
+  if (ne > 4) {
+#pragma unroll
+    for (int j = 4; j < 8; ++j) a[j] = val(e[j]);
+  }
+  T acc = a[0];
+#pragma unroll
+  for (int j = 1; j < 8; ++j)
+    if (static_cast<uint32_t>(j) < ne) acc += a[j];
+  for (uint32_t k = k0 + 8; k < k1; ++k) acc += val(list[k]);
   dst[i] += acc;
 }
 
@@ -455,8 +481,13 @@ template <typename T>
 void launch_dg_cg_restrict_gather(int64_t n_cg, const uint32_t *off, const uint32_t *list,
                                   const T *src, T *dst, hipStream_t s, const T *sub) {
   if (n_cg == 0) return;
-  hipLaunchKernelGGL(dg_cg_restrict_gather_kernel<T>, dim3(static_cast<unsigned>((n_cg + 255) / 256)),
-                     dim3(256), 0, s, n_cg, off, list, src, sub, dst);
+  const dim3 grid(static_cast<unsigned>((n_cg + 255) / 256));
+  if (sub)
+    hipLaunchKernelGGL((dg_cg_restrict_gather_kernel<T, true>), grid, dim3(256), 0, s, n_cg, off,
+                       list, src, sub, dst);
+  else
+    hipLaunchKernelGGL((dg_cg_restrict_gather_kernel<T, false>), grid, dim3(256), 0, s, n_cg, off,
+                       list, src, sub, dst);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_dg_cg_restrict_gather<double>(int64_t, const uint32_t *, const uint32_t *,

@@ -273,7 +273,7 @@ void launch_dg_face_geometry(int nq, int64_t n_faces, const uint32_t *finfo, con
                              const double *qp, const double *qw, int metric, T *fdata,
                              hipStream_t s);
 // DG -> CG restriction as a gather: dst[i] += sum of (src - sub)[list[off[i] .. off[i+1])]
-// (sub optional); also the second half of an h/p restriction through a cell buffer
+// (sub optional)
 template <typename T>
 void launch_dg_cg_restrict_gather(int64_t n_cg, const uint32_t *off, const uint32_t *list,
                                   const T *src, T *dst, hipStream_t s, const T *sub = nullptr);
@@ -311,15 +311,23 @@ template <typename T>
 void launch_prolongate(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                        const uint32_t *map_c, const uint32_t *map_f, const T *w, T scale,
                        const T *xc, T *yf, hipStream_t s, const uint2 *fine_lines = nullptr);
-// yc += R (xf - xf_sub); xf_sub optional.  cell_buf (optional, n_cells nmc^3
-// entries): the per-cell coarse values are stored there instead of being
-// added to yc; the caller sums them with a gather (launch_dg_cg_restrict_gather
-// over the inverse coarse map)
+// yc += R (xf - xf_sub); xf_sub optional.  cell_buf (optional): the per-cell
+// coarse values are stored to cell_buf[map_c[entry]] instead of being added
+// to yc, map_c then holding buffer positions in coarse-DoF order
+// (launch_csr_positions; 0xFFFFFFFF: no store), and the caller sums each
+// coarse DoF's range with launch_restrict_sum
 template <typename T>
 void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
                      const uint32_t *map_c, const uint32_t *map_f, const T *w, const T *xf,
                      T *yc, hipStream_t s, const T *xf_sub = nullptr, T *cell_buf = nullptr,
                      const uint2 *fine_lines = nullptr);
+// yc[i] += sum of buf[off[i] .. off[i+1]), in order
+template <typename T>
+void launch_restrict_sum(int64_t n_c, const uint32_t *off, const T *buf, T *yc, hipStream_t s);
+// pos[n_pos]: pos[list[k]] = k for the ne entries of an inverse map, every
+// other position 0xFFFFFFFF.  Synchronous on s.
+void launch_csr_positions(int64_t n_pos, int64_t ne, const uint32_t *list, uint32_t *pos,
+                          hipStream_t s);
 
 // ---------------------------------------------------------------- vectors
 template <typename T> void launch_fill(int64_t n, T *x, T v, hipStream_t s);

@@ -895,24 +895,33 @@ void MGTransfer::setup_dg_inverse(const uint32_t *map_fine) {
                                                      inv_list_.ptr, nullptr);
 }
 
-// Restriction without coarse atomics: cells store their coarse values to a
-// buffer [cell][nc^3], and each coarse DoF gathers its entries through the
-// inverse map (CSR over buffer positions; constrained entries in none).  It
-// costs a second launch and the buffer's bytes, so it is used where it
-// measured faster: p-transfers of >= 2^18 entries and FP32 h-transfers of
-// >= 2^20 entries (Q4 h 64^3 -> 32^3: FP32 76 -> 58 us, FP64 83 -> 87 us;
-// Q6 -> Q3 on 48^3: FP32 134 -> 114 us, FP64 148 -> 139 us; Q3 -> Q1 on
-// 48^3, 2^19.8 entries: FP32 44 -> 31 us, FP64 44 -> 37 us; the small Q1
-// h-levels of 2^12..2^17 entries lose 2-4 us each).  PMG_RESTRICT_GATHER=0|1
-// forces either path.
+// Restriction without coarse atomics: cells store their coarse values into a
+// buffer ordered by coarse DoF (each (cell, coarse DoF) entry at its position
+// in the inverse map, constrained entries nowhere), and each coarse DoF sums
+// its contiguous range.  It costs a second launch and the buffer's bytes, so
+// it is used where it measured faster: p-transfers of >= 2^18 entries and
+// FP32 h-transfers of >= 2^20 entries.  Q4 h 64^3 -> 32^3 FP32: atomics 76
+// -> buffer 55 us; Q6 -> Q3 on 48^3: FP32 89 us, FP64 119 us against 134 /
+// 148 us with atomics; the small Q1 h-levels of 2^12..2^17 entries lose a
+// few us each.  FP64 h-transfers keep the atomics: alone the buffer is
+// faster (64^3 -> 32^3: 67 -> 63 us), but the FP64 V-cycle at 64^3 Q4 ran
+// 45 us slower with it (4447 -> 4492 us).  The DoF-ordered buffer replaced a cell-ordered
+// one summed through the inverse list: its stores scatter (restriction kernel
+// Q6 -> Q3 FP32 60 -> 72 us) but the sum streams (37 -> 17 us), a net gain
+// on every level measured (Q6 -> Q3 FP32 97 -> 89 us, FP64 121 -> 118 us;
+// h 64^3 -> 32^3 FP32 61 -> 53 us).  PMG_RESTRICT_GATHER=0|1 forces either
+// path.
 void MGTransfer::setup_restrict_gather() {
   const int64_t nc3 = int64_t(nmc_) * nmc_ * nmc_;
   const bool big = n_cells_ * nc3 >= (int64_t(1) << (kind_ == PMG_TRANSFER_P ? 18 : 20));
   bool use_gather = big && (precision_ == PMG_FP32 || kind_ == PMG_TRANSFER_P);
   if (const char *genv = std::getenv("PMG_RESTRICT_GATHER")) use_gather = std::string(genv) == "1";
   if (use_gather && n_cells_ * nc3 < (int64_t(1) << 32) && n_dofs_c_ > 0) {
-    inverse_csr(map_c_.ptr, n_cells_ * nc3, n_dofs_c_, cinv_off_, cinv_list_);
-    cell_buf_.reinit(precision_, n_cells_ * nc3);
+    DeviceArray<uint32_t> list;
+    inverse_csr(map_c_.ptr, n_cells_ * nc3, n_dofs_c_, cinv_off_, list);
+    cpos_.resize(static_cast<size_t>(n_cells_ * nc3));
+    launch_csr_positions(n_cells_ * nc3, static_cast<int64_t>(list.n), list.ptr, cpos_.ptr, nullptr);
+    cell_buf_.reinit(precision_, std::max<int64_t>(static_cast<int64_t>(list.n), 1));
   }
 }
 
@@ -1114,7 +1123,7 @@ LaplaceOperatorBK3::IndexArray MGTransfer::index_array(int which) const {
     case PMG_TRANSFER_INDEX_FINE_MAP: return arr(map_f_);
     case PMG_TRANSFER_INDEX_FINE_LINES: return arr(fine_lines_);
     case PMG_TRANSFER_INDEX_RESTRICT_OFFSETS: return arr(cinv_off_);
-    case PMG_TRANSFER_INDEX_RESTRICT_LIST: return arr(cinv_list_);
+    case PMG_TRANSFER_INDEX_RESTRICT_POSITIONS: return arr(cpos_);
     case PMG_TRANSFER_INDEX_DG_OFFSETS: return arr(inv_off_);
     case PMG_TRANSFER_INDEX_DG_LIST: return arr(inv_list_);
     default: throw Error(PMG_ERR_INVALID_ARG, "unknown index array");
@@ -1193,22 +1202,23 @@ void MGTransfer::restrict_sub(void *dst, const void *src, const void *sub, hipSt
   }
   const bool gather = cinv_off_.ptr != nullptr;
   const uint2 *fl = reinterpret_cast<const uint2 *>(fine_lines_.ptr);
+  const uint32_t *targets = gather ? cpos_.ptr : map_c_.ptr;
   if (f64) {
-    launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, map_c_.ptr, map_f_.ptr,
+    launch_restrict<double>(nmc_, nmf_, n_cells_, M64_, targets, map_f_.ptr,
                             owner_ ? nullptr : D(w_restrict_.data()), D(src),
                             static_cast<double *>(dst), s, D(sub),
                             gather ? static_cast<double *>(cell_buf_.data()) : nullptr, fl);
     if (gather)
-      launch_dg_cg_restrict_gather<double>(n_dofs_c_, cinv_off_.ptr, cinv_list_.ptr,
-                                           D(cell_buf_.data()), static_cast<double *>(dst), s);
+      launch_restrict_sum<double>(n_dofs_c_, cinv_off_.ptr, D(cell_buf_.data()),
+                                  static_cast<double *>(dst), s);
   } else {
-    launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, map_c_.ptr, map_f_.ptr,
+    launch_restrict<float>(nmc_, nmf_, n_cells_, M32_, targets, map_f_.ptr,
                            owner_ ? nullptr : F(w_restrict_.data()), F(src),
                            static_cast<float *>(dst), s, F(sub),
                            gather ? static_cast<float *>(cell_buf_.data()) : nullptr, fl);
     if (gather)
-      launch_dg_cg_restrict_gather<float>(n_dofs_c_, cinv_off_.ptr, cinv_list_.ptr,
-                                          F(cell_buf_.data()), static_cast<float *>(dst), s);
+      launch_restrict_sum<float>(n_dofs_c_, cinv_off_.ptr, F(cell_buf_.data()),
+                                 static_cast<float *>(dst), s);
   }
 }
 

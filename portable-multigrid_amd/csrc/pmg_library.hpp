@@ -323,7 +323,7 @@ class MGTransfer {
   // h/p restriction through a cell buffer: inverse coarse map (CSR over
   // positions cell * nc^3 + i) and the buffer; a transfer's restrictions
   // must therefore be ordered (they share the buffer)
-  DeviceArray<uint32_t> cinv_off_, cinv_list_;
+  DeviceArray<uint32_t> cinv_off_, cpos_;  // restriction buffer: ranges and positions
   DeviceArray<uint32_t> fine_lines_;  // owner mode: x-line words {first DoF, owner bits}
   mutable DeviceVector cell_buf_;
   DeviceVector w_prolong_, w_restrict_;

@@ -163,8 +163,9 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
 {
   // restricts xf - xs (xs optional: the V-cycle's residual d - A s, computed
   // here instead of in a pass of its own).  buf (optional): the cell's coarse
-  // values are stored to buf[cell][nc^3] instead of added to yc with atomics;
-  // a gather over the inverse coarse map then sums them.
+  // values are stored to buf[map_c[entry]] instead of added to yc with
+  // atomics; map_c then holds buffer positions in coarse-DoF order, and a
+  // contiguous sum per coarse DoF follows (restrict_sum_kernel).
   constexpr int NF2 = NF * NF, NF3 = NF2 * NF, NC2 = NC * NC, NC3 = NC2 * NC;
   constexpr int E = XferTile<NF>::cells, LC = xfer_lds_cell<NC, NF>();
   __shared__ T lds[E * LC];
@@ -172,7 +173,16 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
   const int tid = threadIdx.x, e = tid / NF2, t = tid - e * NF2, ta = t % NF, tb = t / NF;
   const int64_t cell = static_cast<int64_t>(lb) * E + e;
   const bool active = e < E && cell < n_cells;
+  const bool writer = active && ta < NC && tb < NC;
   T *L0 = lds + (active ? e : 0) * LC, *L1 = L0 + NC * NF2;
+  // the coarse targets of lane (i,j) over k, requested at entry so that the
+  // write-out does not wait for them (lanes that write nothing load entry 0)
+  uint32_t dc[NC];
+  {
+    const int64_t cb = writer ? cell * NC3 + tb * NC + ta : 0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) dc[k] = map_c[cb + k * NC2];
+  }
 
   // weighted gather of the fine column over c by lane (a,b) (consecutive fine
   // DoFs across lanes), z sweep in registers: L0[k][b][a]
@@ -248,19 +258,58 @@ restrict_kernel(const int64_t n_cells, const Transfer1D<T> M, const uint32_t *__
   }
   __syncthreads();
   // scatter-add to the coarse DoFs by lane (i,j) over k (skip constrained),
-  // or store the cell's values contiguously
-  if (active && ta < NC && tb < NC) {
+  // or store into the buffer position
+  if (writer) {
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       const int l = k * NC2 + tb * NC + ta;
-      if (buf) {
-        buf[cell * NC3 + l] = L0[l];
-      } else {
-        const uint32_t d = map_c[cell * NC3 + l];
-        if (d != kInvalid) atomic_add(yc + d, L0[l]);
+      if (dc[k] != kInvalid) {
+        if (buf) buf[dc[k]] = L0[l];
+        else atomic_add(yc + dc[k], L0[l]);
       }
     }
   }
+}
+
+// Second half of a restriction through the buffer: coarse DoF i sums its
+// entries buf[off[i] .. off[i+1]) in order (the order of the inverse map, so
+// the result does not depend on scheduling).  Loads are issued in groups
+// without dependent steps between them (indices clamped into the DoF's own
+// range): entry 0, then entries 1-3 where there are more, then 4-7; only
+// lanes whose DoF has that many entries issue a group, so a wave touches no
+// more cache lines than its entries occupy.
+template <typename T>
+__global__ void __launch_bounds__(256)
+restrict_sum_kernel(const int64_t n, const uint32_t *__restrict__ off, const T *__restrict__ buf,
+                    T *__restrict__ dst)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k0 = off[i], k1 = off[i + 1];
+  if (k0 == k1) return;
+  const uint32_t ne = k1 - k0, kl = k1 - 1;
+  T acc = buf[k0];
+  if (ne > 1) {
+    T v[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = buf[k0 + 1 + j <= kl ? k0 + 1 + j : kl];
+    if (ne > 4) {
+      T w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) w[j] = buf[k0 + 4 + j <= kl ? k0 + 4 + j : kl];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc += v[j];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4u + j < ne) acc += w[j];
+      for (uint32_t k = k0 + 8; k < k1; ++k) acc += buf[k];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (1u + j < ne) acc += v[j];
+    }
+  }
+  dst[i] += acc;
 }
 
 template <int NC, int NF, typename T>
@@ -332,6 +381,18 @@ void launch_restrict(int nmc, int nmf, int64_t n_cells, const Transfer1D<T> &M,
   xfer_dispatch<T>(false, nmc, nmf, n_cells, M, map_c, map_f, w ? nullptr : fine_lines, w, T(1),
                    xf, xf_sub, yc, cell_buf, s);
 }
+template <typename T>
+void launch_restrict_sum(int64_t n_c, const uint32_t *off, const T *buf, T *yc, hipStream_t s) {
+  if (n_c <= 0) return;
+  PMG_REQUIRE((n_c + 255) / 256 < (int64_t(1) << 31), PMG_ERR_SIZE, "too many coarse DoFs");
+  hipLaunchKernelGGL(restrict_sum_kernel<T>, dim3(static_cast<unsigned>((n_c + 255) / 256)),
+                     dim3(256), 0, s, n_c, off, buf, yc);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_restrict_sum<double>(int64_t, const uint32_t *, const double *, double *,
+                                          hipStream_t);
+template void launch_restrict_sum<float>(int64_t, const uint32_t *, const float *, float *,
+                                         hipStream_t);
 
 template void launch_prolongate<double>(int, int, int64_t, const Transfer1D<double> &,
                                         const uint32_t *, const uint32_t *, const double *,
@@ -393,7 +454,26 @@ __global__ void fine_lines_kernel(const int64_t nl, const int nmf, const uint32_
     if (!ok) atomicOr(bad, 1u);
   }
 }
+// pos[list[k]] = k: the buffer position of every entry of an inverse map
+__global__ void csr_positions_kernel(const int64_t ne, const uint32_t *__restrict__ list,
+                                     uint32_t *__restrict__ pos)
+{
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < ne;
+       k += int64_t(gridDim.x) * blockDim.x)
+    pos[list[k]] = static_cast<uint32_t>(k);
+}
 }  // namespace
+
+void launch_csr_positions(int64_t n_pos, int64_t ne, const uint32_t *list, uint32_t *pos,
+                          hipStream_t s) {
+  if (n_pos <= 0) return;
+  PMG_HIP_CHECK(hipMemsetAsync(pos, 0xFF, static_cast<size_t>(n_pos) * sizeof(uint32_t), s));
+  if (ne > 0) {
+    hipLaunchKernelGGL(csr_positions_kernel, dim3(setup_grid(ne)), dim3(256), 0, s, ne, list, pos);
+    PMG_HIP_CHECK(hipGetLastError());
+  }
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+}
 
 void launch_owner_marks(int64_t nw, int64_t n_dofs, uint32_t *mf, const uint8_t *cons,
                         uint32_t *first, hipStream_t s) {

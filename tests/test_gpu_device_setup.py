@@ -361,14 +361,14 @@ def test_box_transfer_index_data_matches_host_build(pmg, monkeypatch, kind, p, p
         cons = pmg.box_boundary_dofs(pmg.refine(b), p)
         a = pmg._Transfer(pmg.TRANSFER_H, precision, p, p, mc, mf, None, d.n_c, d.n_f, 1.0,
                           cons=cons)
-        names = ["coarse_map", "fine_map", "fine_lines", "restrict_offsets", "restrict_list"]
+        names = ["coarse_map", "fine_map", "fine_lines", "restrict_offsets", "restrict_positions"]
     elif kind == "p":
         d = pmg.PolynomialTransfer(b, p, pf, precision, device_weights=True)
         mc, mf, _ = pmg.box_p_transfer_maps(b, p, pf, weights=False)
         cons = pmg.box_boundary_dofs(b, pf)
         a = pmg._Transfer(pmg.TRANSFER_P, precision, p, pf, mc, mf, None, d.n_c, d.n_f, 1.0,
                           cons=cons)
-        names = ["coarse_map", "fine_map", "fine_lines", "restrict_offsets", "restrict_list"]
+        names = ["coarse_map", "fine_map", "fine_lines", "restrict_offsets", "restrict_positions"]
     else:
         d = pmg.ContinuousTransfer(b, p, precision)
         a = pmg._Transfer(pmg.TRANSFER_DG_CG, precision, p, p, pmg.box_dofmap(b, p, True), None,
@@ -379,6 +379,10 @@ def test_box_transfer_index_data_matches_host_build(pmg, monkeypatch, kind, p, p
         ad, dd = a.index_data(name), d.index_data(name)
         assert ad.size > 0, name
         assert np.array_equal(dd, ad), name
+    if kind != "dg":
+        off, pos = restrict_buffer_layout(d.index_data("coarse_map"), d.n_c)
+        assert np.array_equal(d.index_data("restrict_offsets"), off)
+        assert np.array_equal(d.index_data("restrict_positions"), pos)
     xc, xf = _t(seeded_uniform(d.n_c, 221), dt), _t(seeded_uniform(d.n_f, 222), dt)
     ya, yd = (torch.zeros(d.n_f, dtype=dt, device="cuda") for _ in range(2))
     a.prolongate_and_add(ya, xc)
@@ -388,6 +392,21 @@ def test_box_transfer_index_data_matches_host_build(pmg, monkeypatch, kind, p, p
     a.restrict_and_add(ra, xf)
     d.restrict_and_add(rd, xf)
     assert torch.equal(ra, rd)  # the gather restriction sums in a fixed order
+
+
+def restrict_buffer_layout(mc, n_c):
+    """The restriction buffer's layout restated: coarse DoF i owns the range
+    [off[i], off[i+1]), its entries ordered by their index in the coarse map;
+    pos[e] is entry e's place (invalid for constrained entries)."""
+    mc = np.asarray(mc, dtype=np.uint32)
+    valid = mc != np.uint32(0xFFFFFFFF)
+    off = np.zeros(n_c + 1, dtype=np.uint32)
+    off[1:] = np.cumsum(np.bincount(mc[valid].astype(np.int64), minlength=n_c))
+    ent = np.flatnonzero(valid)
+    order = ent[np.argsort(mc[ent], kind="stable")]
+    pos = np.full(mc.size, 0xFFFFFFFF, dtype=np.uint32)
+    pos[order] = np.arange(order.size, dtype=np.uint32)
+    return off, pos
 
 
 def test_box_transfer_rejects_bad_arguments(pmg):

@@ -80,6 +80,46 @@ def test_restriction_paths_match_oracle(pmg, orc, monkeypatch, gather, kind, pre
         assert rel_err(yc.double().cpu().numpy(), ref.restrict_add(xf, yc0.copy())) < tol
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_restriction_buffer_sums_long_ranges(pmg, monkeypatch, prec):
+    """A coarse map in which one DoF collects the entries of six interior
+    vertices (48 entries: more than the 8 the buffer sum loads without
+    branches) and five DoFs have none: the buffered restriction equals the
+    atomic one, on two calls in a row."""
+    import torch
+    print(f"[restriction buffer] long ranges {prec}", flush=True)
+    precision, dt = (pmg.FP64, torch.float64) if prec == "fp64" else (pmg.FP32, torch.float32)
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    b = pmg.box(4, deform=1, amplitude=0.05)
+    pc, pf = 1, 3
+    mc, mf, _ = pmg.box_p_transfer_maps(b, pc, pf, weights=False)
+    n_c, n_f = pmg.box_n_dofs(b, pc), pmg.box_n_dofs(b, pf)
+    cons = pmg.box_boundary_dofs(b, pf)
+    valid = mc != np.uint32(0xFFFFFFFF)
+    counts = np.bincount(mc[valid].astype(np.int64), minlength=n_c)
+    glue = np.flatnonzero(counts == 8)[:6]
+    assert glue.size == 6
+    mc = mc.copy()
+    mc[np.isin(mc, glue)] = glue[0]
+    assert np.count_nonzero(mc == glue[0]) == 48
+    monkeypatch.setenv("PMG_RESTRICT_GATHER", "0")
+    ta = pmg._Transfer(pmg.TRANSFER_P, precision, pc, pf, mc, mf, None, n_c, n_f, 1.0, cons=cons)
+    monkeypatch.setenv("PMG_RESTRICT_GATHER", "1")
+    tg = pmg._Transfer(pmg.TRANSFER_P, precision, pc, pf, mc, mf, None, n_c, n_f, 1.0, cons=cons)
+    assert ta.index_data("restrict_offsets").size == 0
+    off = tg.index_data("restrict_offsets")
+    assert int(off[glue[0] + 1]) - int(off[glue[0]]) == 48
+    for seed in (47, 48):
+        xf, yc0 = seeded_uniform(n_f, seed), seeded_uniform(n_c, seed + 10)
+        ya, yg = _t(yc0, dt), _t(yc0, dt)
+        ta.restrict_and_add(ya, _t(xf, dt))
+        tg.restrict_and_add(yg, _t(xf, dt))
+        a, g = ya.double().cpu().numpy(), yg.double().cpu().numpy()
+        assert rel_err(g, a) < tol
+        y0 = _t(yc0, dt).double().cpu().numpy()
+        assert np.array_equal(g[glue[1:]], y0[glue[1:]])  # DoFs without entries
+
+
 def test_transfer_fp32_and_doubled_prolongation(pmg, orc):
     import torch
     bc = pmg.box(2)
