@@ -30,9 +30,13 @@
  *   adjoint gradient  difference of the bilinear form under a kappa change,
  *                     central finite differences of J          test_oracle_bk3.py,
  *                                                             test_oracle_adjoint.py
- * Parity unpinned (deal.II internals no reference test pins): the
- * eigenvalue-estimate details of deal.II PreconditionChebyshev (initial
- * vector, stopping rule), see pmo_cheby_setup.
+ *   eigenvalue estimate  dense Rayleigh-Ritz on the same Krylov space,
+ *   and auto degree      closed-form Chebyshev bound             test_oracle_pinning.py
+ *   SIP face measure     20x20 Gauss area of non-planar faces   test_oracle_pinning.py
+ *   RHS and L2 error     closed-form integrals                  test_oracle_pinning.py
+ * Every function is pinned.  The eigenvalue estimate restates deal.II
+ * PreconditionChebyshev's published algorithm (start vector, stopping rule,
+ * safety factor); its Ritz values are fixed by the Rayleigh-Ritz check.
  */
 #ifndef PMG_ORACLE_H
 #define PMG_ORACLE_H

@@ -14,8 +14,11 @@
  *   largest Ritz value, alpha = lambda_max/range (range > 1) or
  *   min(0.9 lambda_max, lambda_min); theta = (lambda_max + alpha)/2,
  *   delta = (lambda_max - alpha)/2; first-kind three-term recurrence.
- *   Parity unpinned against deal.II itself (no reference test pins the
- *   eigenvalue estimate); pinned by the textbook Chebyshev error bound.
+ *   Pinned by the textbook Chebyshev error bound, by a dense Rayleigh-Ritz
+ *   projection of P^{-1}A on the same Krylov space (same start vector and
+ *   iteration count: a wrong count or start vector fails), and by the
+ *   closed-form automatic degree (tests/test_oracle_pinning.py); no
+ *   reference test pins deal.II's own floating-point details.
  * CG: deal.II SolverCG as used at tests/common/portable_multigrid_solver.h:147-169
  *   and source/geometric_multigrid_bk3/program.cc:343-351: convergence when
  *   ||r_k|| <= tol ||r_0||, iteration count = completed iterations.
