@@ -344,21 +344,43 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
 // evaluated on the device, so a coarse solve costs one launch instead of
 // ~10 launches and a host round trip per iteration.  Every lane derives the
 // loop decisions from the same LDS-broadcast scalars, so barriers stay uniform.
-__device__ inline double wg_sum_bcast(double v, double *red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s += red[w];
-    red[8] = s;
+//
+// Workgroup sums broadcast to every lane with one barrier: the wave sums go
+// to one of two LDS slots, used alternately, and every lane adds them in wave
+// order (the same additions one lane made when it broadcast the total).  A
+// slot is rewritten two calls later, after the barrier of the call between,
+// which every lane reaches only after reading it.  red: 2 x 8 x 2 doubles.
+struct WgSum {
+  double *red;
+  int slot;
+  __device__ explicit WgSum(double *r) : red(r), slot(0) {}
+  // a and b summed over the workgroup (both in one barrier)
+  __device__ void sum2(double &a, double &b) {
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off, 64);
+      b += __shfl_down(b, off, 64);
+    }
+    double *r = red + slot * 16;
+    slot ^= 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+      r[2 * wave] = a;
+      r[2 * wave + 1] = b;
+    }
+    __syncthreads();
+    a = 0;
+    b = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+      a += r[2 * w];
+      b += r[2 * w + 1];
+    }
   }
-  __syncthreads();
-  const double s = red[8];
-  __syncthreads();
-  return s;
-}
+  __device__ double sum(double v) {
+    double u = 0;
+    sum2(v, u);
+    return v;
+  }
+};
 
 template <int NM, int NQ, typename T>
 __global__ void __launch_bounds__(256)
@@ -369,7 +391,8 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
 {
   constexpr int E = Bk3Tile<NQ>::cells;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
-  __shared__ double red[9];
+  __shared__ double red[32];
+  WgSum wg(red);
   const int tid = threadIdx.x, nt = blockDim.x;
 
   auto apply = [&](const T *in, T *out) {
@@ -390,7 +413,7 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
     r[i] = ri;
     s += double(ri) * double(ri);
   }
-  const double r0 = sqrt(wg_sum_bcast(s, red));
+  const double r0 = sqrt(wg.sum(s));
   double res = r0, rho_old = 1.0;
   int it = 0;
   while (it < max_it && res > rel_tol * r0) {
@@ -400,7 +423,7 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
       z[i] = zi;
       s += double(zi) * double(r[i]);
     }
-    const double rho = wg_sum_bcast(s, red);
+    const double rho = wg.sum(s);
     // first iteration: p = z exactly (p holds stale memory, 0 * NaN would leak)
     const T beta = it == 0 ? T(0) : static_cast<T>(rho / rho_old);
     for (int64_t i = tid; i < n; i += nt) p[i] = it == 0 ? z[i] : z[i] + beta * p[i];
@@ -408,7 +431,7 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
     apply(p, q);
     s = 0;
     for (int64_t i = tid; i < n; i += nt) s += double(p[i]) * double(q[i]);
-    const double pq = wg_sum_bcast(s, red);
+    const double pq = wg.sum(s);
     const T alpha = pq != 0.0 ? static_cast<T>(rho / pq) : T(0);
     s = 0;
     for (int64_t i = tid; i < n; i += nt) {
@@ -417,7 +440,7 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
       r[i] = ri;
       s += double(ri) * double(ri);
     }
-    res = sqrt(wg_sum_bcast(s, red));
+    res = sqrt(wg.sum(s));
     rho_old = rho;
     ++it;
     if (pq == 0.0) break;
@@ -434,8 +457,12 @@ coarse_pcg_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map, con
 // atomics) and the others in registers (lane entries i = tid + j 256, j <
 // NPL): x, r, z and the inverse diagonal.  An iteration then goes to memory
 // only for G and the map (L2-resident), instead of a global round trip per
-// vector access.  Same arithmetic in the same order as coarse_pcg_kernel.
-// Requires n <= 256 NPL.
+// vector access.  Same arithmetic in the same order as coarse_pcg_kernel,
+// with fewer barriers per iteration (the loop is latency-bound): the norm of
+// the updated residual and the next rho = z.r are summed in one reduction,
+// q is cleared while p is written, and a constrained row's q_i = p_i is read
+// through a per-lane mask instead of being copied into q.  Requires n <= 256
+// NPL.
 template <int NM, int NQ, typename T, int NPL>
 __global__ void __launch_bounds__(256)
 coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
@@ -447,7 +474,8 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
   constexpr int E = Bk3Tile<NQ>::cells, LANES = 256;
   __shared__ T lds[E * lds_cell_stride<T>(3 * NQ * NQ * NQ)];
   __shared__ T pv[NPL * LANES], qv[NPL * LANES];
-  __shared__ double red[9];
+  __shared__ double red[32];
+  WgSum wg(red);
   const int tid = threadIdx.x;
   T xr[NPL], rr[NPL], zr[NPL], dr[NPL];
 #pragma unroll
@@ -455,8 +483,25 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
     const int64_t i = tid + j * LANES;
     xr[j] = i < n ? x[i] : T(0);
     dr[j] = i < n ? pinv[i] : T(0);
-    if (i < n) pv[i] = xr[j];
+    if (i < n) {
+      pv[i] = xr[j];
+      qv[i] = T(0);
+    }
   }
+  __syncthreads();
+  // constrained rows: marked in q once, read into a per-lane bit mask
+  for (int64_t c = tid; c < n_cons; c += LANES) qv[cons[c]] = T(1);
+  __syncthreads();
+  uint32_t cmask = 0;
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    const int64_t i = tid + j * LANES;
+    if (i < n && qv[i] != T(0)) cmask |= 1u << j;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NPL; ++j)
+    if (tid + j * LANES < n) qv[tid + j * LANES] = T(0);
   __syncthreads();
 
   // A level of at most E cells is one batch: its geometric factors and DoF
@@ -476,11 +521,9 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
     for (int k = 0; k < NM; ++k) idx1[k] = map[e1 * NM3 + k * NM2 + tb * NM + ta];
   }
 
-  auto apply = [&]() {  // qv = A pv, constrained rows copied
-#pragma unroll
-    for (int j = 0; j < NPL; ++j)
-      if (tid + j * LANES < n) qv[tid + j * LANES] = T(0);
-    __syncthreads();
+  // qv += A pv over the unconstrained rows (qv cleared and pv complete on
+  // entry); ends with a barrier
+  auto apply = [&]() {
     if (one) {
       T u[NM], v[NM];
       if (col1) {
@@ -501,59 +544,69 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
         __syncthreads();
       }
     }
-    for (int64_t i = tid; i < n_cons; i += LANES) qv[cons[i]] = pv[cons[i]];
-    __syncthreads();
   };
+  // entry i of q = A p (constrained rows: the identity)
+  auto qval = [&](int j, int64_t i) { return (cmask >> j) & 1u ? pv[i] : qv[i]; };
 
   apply();
-  double s = 0;
+  double srr = 0, szr = 0;
 #pragma unroll
   for (int j = 0; j < NPL; ++j) {
     const int64_t i = tid + j * LANES;
-    const T ri = i < n ? b[i] - qv[i] : T(0);
+    const T ri = i < n ? b[i] - qval(j, i) : T(0);
     rr[j] = ri;
-    s += double(ri) * double(ri);
+    srr += double(ri) * double(ri);
   }
-  const double r0 = sqrt(wg_sum_bcast(s, red));
-  double res = r0, rho_old = 1.0;
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    zr[j] = dr[j] * rr[j];
+    szr += double(zr[j]) * double(rr[j]);
+  }
+  wg.sum2(srr, szr);
+  const double r0 = sqrt(srr);
+  double res = r0, rho = szr, rho_old = 1.0;
   int it = 0;
   while (it < max_it && res > rel_tol * r0) {
-    s = 0;
-#pragma unroll
-    for (int j = 0; j < NPL; ++j) {
-      zr[j] = dr[j] * rr[j];
-      s += double(zr[j]) * double(rr[j]);
-    }
-    const double rho = wg_sum_bcast(s, red);
     // first iteration: p = z exactly
     const T beta = it == 0 ? T(0) : static_cast<T>(rho / rho_old);
 #pragma unroll
     for (int j = 0; j < NPL; ++j) {
       const int64_t i = tid + j * LANES;
-      if (i < n) pv[i] = it == 0 ? zr[j] : zr[j] + beta * pv[i];
+      if (i < n) {
+        pv[i] = it == 0 ? zr[j] : zr[j] + beta * pv[i];
+        qv[i] = T(0);
+      }
     }
     __syncthreads();
     apply();
-    s = 0;
+    double s = 0;
 #pragma unroll
     for (int j = 0; j < NPL; ++j) {
       const int64_t i = tid + j * LANES;
-      if (i < n) s += double(pv[i]) * double(qv[i]);
+      if (i < n) s += double(pv[i]) * double(qval(j, i));
     }
-    const double pq = wg_sum_bcast(s, red);
+    const double pq = wg.sum(s);
     const T alpha = pq != 0.0 ? static_cast<T>(rho / pq) : T(0);
-    s = 0;
+    srr = 0;
+    szr = 0;
 #pragma unroll
     for (int j = 0; j < NPL; ++j) {
       const int64_t i = tid + j * LANES;
       if (i < n) {
         xr[j] += alpha * pv[i];
-        rr[j] -= alpha * qv[i];
+        rr[j] -= alpha * qval(j, i);
       }
-      s += double(rr[j]) * double(rr[j]);
+      srr += double(rr[j]) * double(rr[j]);
     }
-    res = sqrt(wg_sum_bcast(s, red));
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+      zr[j] = dr[j] * rr[j];
+      szr += double(zr[j]) * double(rr[j]);
+    }
+    wg.sum2(srr, szr);
+    res = sqrt(srr);
     rho_old = rho;
+    rho = szr;
     ++it;
     if (pq == 0.0) break;
   }
@@ -573,7 +626,7 @@ static bool coarse_pcg_lds(int64_t n_cells, const T *G, const uint32_t *map, con
                            const T *b, T *x, double rel_tol, int max_it, double *info,
                            hipStream_t s) {
   constexpr size_t lds = sizeof(T) * (Bk3Tile<NQ>::cells * lds_cell_stride<T>(3 * NQ * NQ * NQ) +
-                                      2 * NPL * 256) + 9 * sizeof(double);
+                                      2 * NPL * 256) + 32 * sizeof(double);
   if (n > 256 * NPL || lds > 64 * 1024) return false;
   hipLaunchKernelGGL((coarse_pcg_lds_kernel<NM, NQ, T, NPL>), dim3(1), dim3(256), 0, s, G, map,
                      n_cells, sh, cons, n_cons, pinv, n, b, x, rel_tol, max_it, info);
