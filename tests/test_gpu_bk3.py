@@ -85,6 +85,30 @@ def test_diagonal_matches_dense(pmg, orc, n, p, nq, deform):
     assert rel_err(1.0 / inv, d) < 1e-12
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_vmult_above_fused_prep_threshold_matches_oracle(pmg, orc, prec):
+    """A level of more than 2^20 DoFs (26^3 Q4: 1 157 625), where vmult
+    zero-fills the output and copies the constrained rows in passes of their
+    own: the result against the oracle, with the output vector full of NaN
+    beforehand and the constrained rows equal to the source exactly."""
+    import torch
+    print(f"[bk3 large] 26^3 Q4 {prec}", flush=True)
+    precision, dt = (pmg.FP64, torch.float64) if prec == "fp64" else (pmg.FP32, torch.float32)
+    op, ref = _pair(pmg, orc, 26, 4, 5, 1, precision)
+    assert op.m() > 2 ** 20
+    x = seeded_uniform(op.m(), 2601)
+    xs = x if prec == "fp64" else x.astype(np.float32).astype(np.float64)
+    y = ref.vmult(xs)
+    src = torch.tensor(x, dtype=dt, device="cuda")
+    dst = op.initialize_dof_vector()
+    dst.fill_(float("nan"))
+    op.vmult(dst, src)
+    out = dst.double().cpu().numpy()
+    assert rel_err(out, y) < (1e-12 if prec == "fp64" else 2e-6)
+    cons = pmg.box_boundary_dofs(pmg.box(26, deform=1, amplitude=0.05), 4).astype(np.int64)
+    assert np.array_equal(out[cons], src.double().cpu().numpy()[cons])
+
+
 def test_large_vmult_properties(pmg):
     """Size-independent checks at the 64^3 Q4 workload: symmetry <Ax,y> = <x,Ay>
     and A 1 = 0 on rows whose stencil touches no constrained DoF."""
