@@ -359,7 +359,7 @@ constexpr int kDotBlocks = 4096;
 // stream_a: read a non-temporally (its last use for a while)
 template <typename TA, typename TB = TA>
 void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s,
-                bool stream_a = false);
+                bool stream_a = false, const double *skip = nullptr);
 // t = b - t   (residual)
 template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStream_t s);
 
@@ -380,7 +380,7 @@ void launch_cg_update_xr(int64_t n, T *x, T *r, const T *p, const T *q, const do
                          double *partials, hipStream_t s);
 template <typename T>
 void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *partials,
-                           hipStream_t s);
+                           hipStream_t s, const double *skip = nullptr);
 // p = z (first) or z + beta p
 template <typename T, typename TZ = T>
 void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int first, hipStream_t s);
@@ -391,19 +391,25 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
 // stage (optional): the new r also written there in stage_precision
 template <typename T>
 void launch_cg_update_r(int64_t n, T *r, const T *q, const double *sc, double *partials,
-                        hipStream_t s, void *stage = nullptr, int stage_precision = PMG_FP64);
+                        hipStream_t s, void *stage = nullptr, int stage_precision = PMG_FP64,
+                        const double *skip = nullptr);
 template <typename T, typename TZ = T>
 void launch_cg_update_xp(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first,
-                         hipStream_t s);
+                         hipStream_t s, const double *skip = nullptr);
 template <typename T>
 void launch_cg_update_x(int64_t n, T *x, const T *p, const double *sc, hipStream_t s);
 // final reduction of kDotBlocks partial sums into sc[slot] (optionally sqrt)
 void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
-                            int n_sums, hipStream_t s);
+                            int n_sums, hipStream_t s, const double *skip = nullptr);
+// CG scalar slots: the iteration that converged and the converged flag
+constexpr int kCgConvItSlot = 6, kCgConvergedSlot = 7;
 // scalar bookkeeping for CG on the device: see kernel for slot layout
-// also: sc[7] = 1 once sc[2] = r.r <= target2 (or NaN); host[0] = r.r (pinned, optional)
+// also: sc[7] = 1 once sc[2] = r.r <= target2 (or NaN), sc[6] = host[1] = it
+// then; host[0] = r.r (pinned, optional).  skip (the CG kernels above): the
+// converged flag sc + 7; the kernels then change nothing.  op 2 clears the
+// flag before a solve.
 void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host = nullptr,
-                       double target2 = -1.0);
+                       double target2 = -1.0, int it = 0);
 
 // Measured FMA issue rate of the device (flop/s) in the given precision:
 // independent FMA chains in every lane of 16 workgroups per CU (measurement

@@ -1239,7 +1239,7 @@ void CGWork::reinit(int precision, int64_t n) {
 }
 
 CGSpace CGWork::space() {
-  host.allocate(1);
+  host.allocate(2);  // r.r and the iteration that converged
   CGSpace w;
   w.sc = sc.ptr;
   w.partials = partials.ptr;
@@ -1280,6 +1280,15 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
   int it = 0;
   static const bool verbose = std::getenv("PMG_CG_VERBOSE") != nullptr;
   const double target2 = target * target;
+  // the converged flag: once set, the CG kernels of iterations queued after
+  // the converged one change nothing, so the host may check every few
+  // iterations and the result is the same as checking after each one
+  const double *skip = sc + kCgConvergedSlot;
+  launch_cg_scalars(sc, 2, s, w.host_rr);
+  if (check_every <= 0) {
+    check_every = precond == PMG_PRECOND_MG ? 1 : kCgCheckEvery;
+    if (const char *e = std::getenv("PMG_CG_CHECK_EVERY")) check_every = std::max(1, std::atoi(e));
+  }
   if (precond == PMG_PRECOND_MG) mg->stage(r, s, nullptr);
   while (it < max_it) {
     if (precond == PMG_PRECOND_MG) {
@@ -1287,40 +1296,49 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
       // (the same values as after a conversion, one pass less)
       const void *zl = mg->run(s);
       if (mg->level_precision() == PMG_FP64) {
-        launch_dot<T, double>(n, r, static_cast<const double *>(zl), partials, sc + 0, s, true);
-        launch_cg_update_xp<T, double>(n, x, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s);
+        launch_dot<T, double>(n, r, static_cast<const double *>(zl), partials, sc + 0, s, true, skip);
+        launch_cg_update_xp<T, double>(n, x, p, static_cast<const double *>(zl), sc, it == 0 ? 1 : 0, s,
+                                       skip);
       } else {
-        launch_dot<T, float>(n, r, static_cast<const float *>(zl), partials, sc + 0, s, true);
-        launch_cg_update_xp<T, float>(n, x, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s);
+        launch_dot<T, float>(n, r, static_cast<const float *>(zl), partials, sc + 0, s, true, skip);
+        launch_cg_update_xp<T, float>(n, x, p, static_cast<const float *>(zl), sc, it == 0 ? 1 : 0, s,
+                                      skip);
       }
     } else {
-      launch_cg_precond_dot<T>(n, r, pinv, z, partials, s);
-      launch_reduce_partials(partials, kDotBlocks, sc, 0, 1, s);
-      launch_cg_update_xp<T>(n, x, p, z, sc, it == 0 ? 1 : 0, s);
+      launch_cg_precond_dot<T>(n, r, pinv, z, partials, s, skip);
+      launch_reduce_partials(partials, kDotBlocks, sc, 0, 1, s, skip);
+      launch_cg_update_xp<T>(n, x, p, z, sc, it == 0 ? 1 : 0, s, skip);
     }
     op.vmult(q, p, s);
-    launch_dot<T>(n, p, q, partials, sc + 1, s);
+    launch_dot<T>(n, p, q, partials, sc + 1, s, false, skip);
     // the residual update also writes the next V-cycle input (its staging),
     // so the GPU stays busy while the host waits for r.r
     if (precond == PMG_PRECOND_MG)
-      launch_cg_update_r<T>(n, r, q, sc, partials, s, mg->input_buffer(), mg->level_precision());
+      launch_cg_update_r<T>(n, r, q, sc, partials, s, mg->input_buffer(), mg->level_precision(),
+                            skip);
     else
-      launch_cg_update_r<T>(n, r, q, sc, partials, s);
-    launch_reduce_partials(partials, kDotBlocks, sc, 2, 1, s);
-    launch_cg_scalars(sc, 0, s, w.host_rr, target2);
+      launch_cg_update_r<T>(n, r, q, sc, partials, s, nullptr, PMG_FP64, skip);
+    launch_reduce_partials(partials, kDotBlocks, sc, 2, 1, s, skip);
+    launch_cg_scalars(sc, 0, s, w.host_rr, target2, it + 1);
     ++it;
     const bool check = it % check_every == 0 || it == max_it;
     if (check) {
+      double conv_it = 0;
       if (w.host_rr) {
         PMG_HIP_CHECK(hipStreamSynchronize(s));
         rr = w.host_rr[0];
+        conv_it = w.host_rr[1];
       } else {
-        PMG_HIP_CHECK(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        double scv[8];
+        PMG_HIP_CHECK(hipMemcpyAsync(scv, sc, sizeof(scv), hipMemcpyDeviceToHost, s));
         PMG_HIP_CHECK(hipStreamSynchronize(s));
+        rr = scv[2];
+        conv_it = scv[kCgConvItSlot];
       }
       res.res = std::sqrt(rr);
       if (verbose) std::fprintf(stderr, "[pmg cg] it %d  |r|/|r0| = %.3e\n", it, res.res / res.res0);
-      if (!(res.res > target)) {  // also stops on NaN
+      if (conv_it > 0) {  // r.r <= target^2 at that iteration, or NaN
+        it = static_cast<int>(conv_it);
         res.converged = res.res <= target;
         break;
       }

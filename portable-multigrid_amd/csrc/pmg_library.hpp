@@ -353,8 +353,9 @@ class PreconditionChebyshev {
 
 // Work space of one CG run as raw device pointers: 8 scalars, kDotBlocks
 // partial sums, and r, z, p, q (m() entries of the operator precision).
-// host_rr: a pinned host double the device stores r.r into at every
-// iteration, or null (r.r is then copied back at each convergence check).
+// host_rr: two pinned host doubles the device stores r.r and the iteration
+// that converged into, or null (the scalars are then copied back at each
+// convergence check).
 struct CGSpace {
   double *sc = nullptr, *partials = nullptr;
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
@@ -379,13 +380,18 @@ struct CGResult {
 
 class VCycleMultigrid;
 
+// check_every: iterations queued between host checks of the residual (the
+// result does not depend on it: kernels queued after convergence change
+// nothing); 0 = kCgCheckEvery without, 1 with the multigrid preconditioner
+// (a V-cycle queued past convergence costs more than a check).
+constexpr int kCgCheckEvery = 8;
 CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, CGWork &work,
-                  hipStream_t s, int check_every = 1);
+                  hipStream_t s, int check_every = 0);
 // The same on caller-provided work space (allocates nothing itself).
 CGResult solve_cg(const Operator &op, int precond, const VCycleMultigrid *mg,
                   const void *b, void *x, double rel_tol, int max_it, const CGSpace &space,
-                  hipStream_t s, int check_every = 1);
+                  hipStream_t s, int check_every = 0);
 
 // solve_cg on the library's fixed device work array (pmg_noalloc.hip):
 // allocates nothing; throws PMG_ERR_SIZE above cg_noalloc_max_dofs(precision).

@@ -92,9 +92,12 @@ dot_k(int64_t n, const TA *a, const TB *b, double *partials, bool stream_a) {
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// sc[slot + k] = sum_b partials[k * n_partials + b] (optionally also its sqrt)
+// sc[slot + k] = sum_b partials[k * n_partials + b]; nothing once *skip != 0
+// (a CG iteration queued after convergence)
 __global__ void __launch_bounds__(kThreads)
-reduce_k(const double *partials, int n_partials, double *sc, int slot, int n_sums) {
+reduce_k(const double *partials, int n_partials, double *sc, int slot, int n_sums,
+         const double *skip) {
+  if (skip && *skip != 0.0) return;
   for (int k = 0; k < n_sums; ++k) {
     double s = 0;
     for (int i = threadIdx.x; i < n_partials; i += kThreads) s += partials[k * n_partials + i];
@@ -133,7 +136,9 @@ __global__ void cheby_k(int mode, int64_t n, T *x, T *old, const T *b, const T *
 
 // CG scalar slots (device doubles):
 //  0 rho = r.z      1 p.q       2 r.r       3 rho_old   4 alpha   5 beta
-enum { kRho = 0, kPq = 1, kRr = 2, kRhoOld = 3, kAlpha = 4, kBeta = 5, kConverged = 7 };
+//  6 the iteration that converged   7 converged flag
+enum { kRho = 0, kPq = 1, kRr = 2, kRhoOld = 3, kAlpha = 4, kBeta = 5, kConvIt = kCgConvItSlot,
+       kConverged = kCgConvergedSlot };
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -155,9 +160,13 @@ cg_xr_k(int64_t n, T *x, T *r, const T *p, const T *q, const double *sc, double 
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// The CG kernels below return at once when skip (the converged flag) is set:
+// the host may queue iterations beyond the one that converges, and they
+// then change nothing.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-cg_pz_k(int64_t n, const T *r, const T *pinv, T *z, double *partials) {
+cg_pz_k(int64_t n, const T *r, const T *pinv, T *z, double *partials, const double *skip) {
+  if (skip && *skip != 0.0) return;
   double s = 0;
   for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
     const T zi = pinv ? pinv[i] * r[i] : r[i];
@@ -186,7 +195,9 @@ __global__ void cg_p_k(int64_t n, T *p, const TZ *z, const double *sc, int first
 // precision (the V-cycle input of the next iteration: one pass over r less).
 template <typename T, typename TS>
 __global__ void __launch_bounds__(kThreads)
-cg_r_k(int64_t n, T *r, const T *q, const double *sc, double *partials, TS *stage) {
+cg_r_k(int64_t n, T *r, const T *q, const double *sc, double *partials, TS *stage,
+       const double *skip) {
+  if (skip && *skip != 0.0) return;
   const double pq = sc[kPq];
   const T a = static_cast<T>(pq != 0.0 ? sc[kRho] / pq : 0.0);
   double s = 0;
@@ -200,7 +211,9 @@ cg_r_k(int64_t n, T *r, const T *q, const double *sc, double *partials, TS *stag
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 template <typename T, typename TZ>
-__global__ void cg_xp_k(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first) {
+__global__ void cg_xp_k(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first,
+                        const double *skip) {
+  if (skip && *skip != 0.0) return;
   const double ro = sc[kRhoOld];
   const T beta = (first || ro == 0.0) ? T(0) : static_cast<T>(sc[kRho] / ro);
   const T a = first ? T(0) : static_cast<T>(sc[kAlpha]);
@@ -222,20 +235,34 @@ __global__ void cg_x_k(int64_t n, T *x, const T *p, const double *sc) {
     x[i] += a * p[i];
 }
 
-// end of an iteration: alpha = rho / p.q kept for the deferred x update,
-// rho_old = rho; host[0] = r.r (optional, pinned);
-// sc[kConverged] = 1 once r.r <= target2 (or is NaN), which skips the
-// speculatively queued input pass of the next iteration
-__global__ void cg_scalars_k(double *sc, int op, double *host, double target2) {
+// end of iteration it: alpha = rho / p.q kept for the deferred x update,
+// rho_old = rho; host[0] = r.r (optional, pinned); sc[kConverged] = 1 once
+// r.r <= target2 (or is NaN), with sc[kConvIt] = host[1] = it, which makes
+// the CG kernels of iterations queued after it return at once, and skips the
+// speculatively queued input pass of the next iteration.  op 2: clear the
+// flag and the iteration (start of a solve); nothing once converged.
+__global__ void cg_scalars_k(double *sc, int op, double *host, double target2, int it) {
   if (threadIdx.x == 0) {
+    if (op == 2) {
+      sc[kConverged] = 0.0;
+      sc[kConvIt] = 0.0;
+      if (host) host[1] = 0.0;
+      return;
+    }
+    if (sc[kConverged] != 0.0) return;
     if (op == 0) {
       const double pq = sc[kPq];
       sc[kAlpha] = pq != 0.0 ? sc[kRho] / pq : 0.0;
       sc[kRhoOld] = sc[kRho];
     }
     const double rr = sc[kRr];
-    sc[kConverged] = !(rr > target2) ? 1.0 : 0.0;
-    if (host) host[0] = rr;
+    const bool done = !(rr > target2);
+    sc[kConverged] = done ? 1.0 : 0.0;
+    if (done) sc[kConvIt] = it;
+    if (host) {
+      host[0] = rr;
+      if (done) host[1] = it;
+    }
   }
 }
 template <typename TD, typename TS>
@@ -326,17 +353,18 @@ template <typename T> void launch_residual(int64_t n, T *t, const T *b, hipStrea
 }
 template <typename TA, typename TB>
 void launch_dot(int64_t n, const TA *a, const TB *b, double *partials, double *out, hipStream_t s,
-                bool stream_a) {
+                bool stream_a, const double *skip) {
   hipLaunchKernelGGL((dot_k<TA, TB>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, a, b, partials,
                      stream_a);
   PMG_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, kDotBlocks, out, 0, 1);
+  hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, kDotBlocks, out, 0, 1,
+                     skip);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_reduce_partials(const double *partials, int n_partials, double *sc, int slot,
-                            int n_sums, hipStream_t s) {
+                            int n_sums, hipStream_t s, const double *skip) {
   hipLaunchKernelGGL(reduce_k, dim3(1), dim3(kThreads), 0, s, partials, n_partials, sc, slot,
-                     n_sums);
+                     n_sums, skip);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T>
@@ -357,8 +385,9 @@ void launch_cg_update_xr(int64_t n, T *x, T *r, const T *p, const T *q, const do
 }
 template <typename T>
 void launch_cg_precond_dot(int64_t n, const T *r, const T *pinv, T *z, double *partials,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(cg_pz_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, pinv, z, partials);
+                           hipStream_t s, const double *skip) {
+  hipLaunchKernelGGL(cg_pz_k<T>, dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, pinv, z, partials,
+                     skip);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T, typename TZ>
@@ -369,21 +398,21 @@ void launch_cg_update_p(int64_t n, T *p, const TZ *z, const double *sc, int firs
 }
 template <typename T>
 void launch_cg_update_r(int64_t n, T *r, const T *q, const double *sc, double *partials,
-                        hipStream_t s, void *stage, int stage_precision) {
+                        hipStream_t s, void *stage, int stage_precision, const double *skip) {
   if (stage && stage_precision == PMG_FP32)
     hipLaunchKernelGGL((cg_r_k<T, float>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, q, sc,
-                       partials, static_cast<float *>(stage));
+                       partials, static_cast<float *>(stage), skip);
   else
     hipLaunchKernelGGL((cg_r_k<T, double>), dim3(kDotBlocks), dim3(kThreads), 0, s, n, r, q, sc,
-                       partials, static_cast<double *>(stage));
+                       partials, static_cast<double *>(stage), skip);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T, typename TZ>
 void launch_cg_update_xp(int64_t n, T *x, T *p, const TZ *z, const double *sc, int first,
-                         hipStream_t s) {
+                         hipStream_t s, const double *skip) {
   if (n <= 0) return;
   hipLaunchKernelGGL((cg_xp_k<T, TZ>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, p, z, sc,
-                     first);
+                     first, skip);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template <typename T>
@@ -392,8 +421,8 @@ void launch_cg_update_x(int64_t n, T *x, const T *p, const double *sc, hipStream
   hipLaunchKernelGGL(cg_x_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, p, sc);
   PMG_HIP_CHECK(hipGetLastError());
 }
-void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double target2) {
-  hipLaunchKernelGGL(cg_scalars_k, dim3(1), dim3(64), 0, s, sc, op, host, target2);
+void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double target2, int it) {
+  hipLaunchKernelGGL(cg_scalars_k, dim3(1), dim3(64), 0, s, sc, op, host, target2, it);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -406,34 +435,34 @@ void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double t
   template void launch_set_indexed<T>(int64_t, const uint32_t *, T *, T, hipStream_t);        \
   template void launch_invert<T>(int64_t, T *, hipStream_t);                                  \
   template void launch_residual<T>(int64_t, T *, const T *, hipStream_t);                     \
-  template void launch_dot<T>(int64_t, const T *, const T *, double *, double *, hipStream_t, bool); \
+  template void launch_dot<T>(int64_t, const T *, const T *, double *, double *, hipStream_t, bool, const double *); \
   template void launch_cheby_update<T>(int, int64_t, T *, T *, const T *, const T *,          \
                                        const T *, double, double, bool, hipStream_t, T *,     \
                                        const uint8_t *);                                      \
   template void launch_cg_update_xr<T>(int64_t, T *, T *, const T *, const T *,               \
                                        const double *, double *, hipStream_t);                \
   template void launch_cg_precond_dot<T>(int64_t, const T *, const T *, T *, double *,        \
-                                         hipStream_t);                                        \
+                                         hipStream_t, const double *);                        \
   template void launch_cg_update_p<T>(int64_t, T *, const T *, const double *, int, hipStream_t);
 PMG_INST(double)
 PMG_INST(float)
 // CG vectors of one precision with a preconditioned vector z of the other
 template void launch_dot<double, float>(int64_t, const double *, const float *, double *, double *,
-                                        hipStream_t, bool);
+                                        hipStream_t, bool, const double *);
 template void launch_dot<float, double>(int64_t, const float *, const double *, double *, double *,
-                                        hipStream_t, bool);
+                                        hipStream_t, bool, const double *);
 #define PMG_CGX(T, TZ)                                                                             \
   template void launch_cg_update_xp<T, TZ>(int64_t, T *, T *, const TZ *, const double *, int,    \
-                                           hipStream_t);
+                                           hipStream_t, const double *);
 PMG_CGX(double, double)
 PMG_CGX(double, float)
 PMG_CGX(float, float)
 PMG_CGX(float, double)
 #undef PMG_CGX
 template void launch_cg_update_r<double>(int64_t, double *, const double *, const double *,
-                                         double *, hipStream_t, void *, int);
+                                         double *, hipStream_t, void *, int, const double *);
 template void launch_cg_update_r<float>(int64_t, float *, const float *, const double *, double *,
-                                        hipStream_t, void *, int);
+                                        hipStream_t, void *, int, const double *);
 template void launch_cg_update_x<double>(int64_t, double *, const double *, const double *,
                                          hipStream_t);
 template void launch_cg_update_x<float>(int64_t, float *, const float *, const double *,

@@ -342,6 +342,32 @@ def test_jacobi_pcg_matches_oracle(pmg, orc):
     assert rel_err(x.cpu().numpy(), xr) < 1e-7
 
 
+@pytest.mark.parametrize("every", ["1", "5", "8"])
+def test_cg_result_does_not_depend_on_the_check_interval(pmg, monkeypatch, every):
+    """Iterations queued past the converged one change nothing: the iteration
+    count and the solution are those of a check after every iteration."""
+    print(f"[cg] check every {every}", flush=True)
+    b = pmg.box(4, deform=1, amplitude=0.05)
+    op = pmg.LaplaceOperatorBK3(b, 3, 4)
+    op.compute_diagonal()
+    rhs = _t(seeded_uniform(op.m(), 83))
+    monkeypatch.setenv("PMG_CG_CHECK_EVERY", "1")
+    x1 = op.initialize_dof_vector()
+    its1, r01, res1 = pmg.solve_cg(op, rhs, x1, pmg.PRECOND_JACOBI, None, 1e-9, 2000)
+    monkeypatch.setenv("PMG_CG_CHECK_EVERY", every)
+    x2 = op.initialize_dof_vector()
+    its2, r02, res2 = pmg.solve_cg(op, rhs, x2, pmg.PRECOND_JACOBI, None, 1e-9, 2000)
+    # equal up to the round-off of the FP atomics, which differs run to run;
+    # one more iteration would move x by about the 1e-9 tolerance
+    assert its2 == its1
+    assert abs(res2 - res1) <= 1e-12 * r01
+    assert rel_err(x2.cpu().numpy(), x1.cpu().numpy()) < 1e-11
+    # not converged within max_it: the count is max_it either way
+    with pytest.raises(pmg.PMGError) as e:
+        pmg.solve_cg(op, rhs, op.initialize_dof_vector(), pmg.PRECOND_JACOBI, None, 1e-9, its1 - 3)
+    assert e.value.status == pmg.ERR_NOT_CONVERGED
+
+
 def test_coarse_chebyshev_vcycle_runs(pmg):
     """The reference's coarse solver (Chebyshev, automatic degree to 1e-3)."""
     boxes = [pmg.box(2), pmg.box(4)]
