@@ -173,13 +173,13 @@ def level_table(pmg, mg, ops, src, n_apps=5):
             "applications": n_apps}
 
 
-def mg_solve(pmg, fine_box, degree, levels, level_precision, fine_op, b, tol):
+def mg_solve(pmg, fine_box, degree, levels, level_precision, fine_op, b, tol, affine=False):
     import torch
     t0 = time.perf_counter()
     mg, ops, trs, sms, boxes = pmg.geometric_hierarchy(
         fine_box, degree, levels, precision=level_precision, smoother_degree=3,
         smoothing_range=15.0, eig_cg_n_iterations=10, coarse=pmg.COARSE_PCG, coarse_tol=1e-12,
-        outer_precision=pmg.FP64)
+        outer_precision=pmg.FP64, affine=affine)
     setup = time.perf_counter() - t0
     results = []
     for rep in range(2):  # first solve warms caches and code objects
@@ -192,8 +192,10 @@ def mg_solve(pmg, fine_box, degree, levels, level_precision, fine_op, b, tol):
     t, its, r0, r, x = results[-1]
     err = fine_op.l2_error(x, 1)
     out = {"time_s": t, "iterations": its, "rel_residual": r / r0, "l2_error": err,
-           "setup_s": setup, "roofline": solve_roofline(pmg, ops, fine_op, its, t),
-           "levels_table": level_table(pmg, mg, ops, b)}
+           "setup_s": setup}
+    if not affine:  # the byte model is that of G at every q-point
+        out["roofline"] = solve_roofline(pmg, ops, fine_op, its, t)
+        out["levels_table"] = level_table(pmg, mg, ops, b)
     del mg, ops, trs, sms
     return out, x
 
@@ -529,6 +531,15 @@ def bench_rows(pmg):
         x = torch.rand(N, dtype=dt[prec], device="cuda")
         y = op.initialize_dof_vector()
         rows[f"a1_a2_bk3_vmult_q4_64cube_{tag}"] = _roof(time_events(lambda: op.vmult(y, x), 20), by)
+        # the same vmult with the affine geometry (6 values per cell instead of
+        # G at every q-point), on its own byte model s(2N + 6C)
+        op.use_affine_geometry(True)
+        t_aff = time_events(lambda: op.vmult(y, x), 20)
+        op.use_affine_geometry(False)
+        rows[f"a1_a2_bk3_vmult_q4_64cube_{tag}_affine"] = dict(
+            _roof(t_aff, sz[prec] * (2 * N + 6 * op.n_cells)), gdofs=N / t_aff / 1e9,
+            note="pmg_operator_use_affine_geometry: G_q = A_cell w_p w_q w_r; not the "
+                 "reference's byte model (G at every q-point)")
         rows[f"a3_a4_operator_setup_q4_64cube_{tag}"] = {
             "time_ms": t_setup * 1e3,
             "note": "box map, constraints, vertices, lines, patches and G built on the device "
@@ -647,6 +658,24 @@ def bench_mgcg_128(pmg):
             "speedup_mixed": r64["time_s"] / r32["time_s"], "levels": 7}
 
 
+def bench_mgcg_affine(pmg):
+    """The primary 64^3 Q4 MG-CG (6 h-levels, Chebyshev 3, coarse PCG 1e-12,
+    sin RHS, 1e-10) with the affine geometry on every level and the outer
+    operator: the unit-cube cells are affine, so each vmult reads 6 values per
+    cell instead of G at every q-point.  Not the reference's algorithm for G;
+    the primary mgcg entry is the comparison."""
+    b = pmg.box(64)
+    op = pmg.LaplaceOperatorBK3(b, 4, 5, pmg.FP64)
+    op.use_affine_geometry()
+    rhs = op.assemble_rhs(1)
+    r64, x64 = mg_solve(pmg, b, 4, 6, pmg.FP64, op, rhs, 1e-10, affine=True)
+    r32, x32 = mg_solve(pmg, b, 4, 6, pmg.FP32, op, rhs, 1e-10, affine=True)
+    r32["rel_diff_vs_fp64_solution"] = ((x32 - x64).abs().max() / x64.abs().max()).item()
+    return {"dofs": op.m(), "fp64": r64, "fp32_vcycle_fp64_cg": r32,
+            "speedup_mixed": r64["time_s"] / r32["time_s"], "levels": 6,
+            "note": "affine geometry on every level (pmg_operator_use_affine_geometry)"}
+
+
 def bench_reference_logged(pmg):
     """The configuration of the reference's own GPU run logs (context, not a
     target): [-1,0.9]^3 refined to 64^3 cells, continuous Q4 with q = p+1,
@@ -709,7 +738,7 @@ def main():
     ap.add_argument("--config", action="append", default=None,
                     help="run only these configuration legs (repeatable): rows, bk3_q4_16cube_jacobi_pcg, "
                          "sipdg_q3_32cube_deformed, ph_mg_q6_48cube_deformed, adjoint_q4_32cube, "
-                         "mgcg_q4_128cube, reference_logged_ph_q4_64cube")
+                         "mgcg_q4_128cube, mgcg_q4_64cube_affine, reference_logged_ph_q4_64cube")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -816,6 +845,7 @@ def main():
                          ("ph_mg_q6_48cube_deformed", bench_config4),
                          ("adjoint_q4_32cube", bench_config5),
                          ("mgcg_q4_128cube", bench_mgcg_128),
+                         ("mgcg_q4_64cube_affine", bench_mgcg_affine),
                          ("reference_logged_ph_q4_64cube", bench_reference_logged)]:
             if args.config and name not in args.config:
                 continue

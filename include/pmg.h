@@ -242,6 +242,16 @@ int pmg_operator_vmult_dummy(pmg_operator op, void *dst, const void *src, int gh
  * DEVICE array [C][nq^3] of doubles; G <- kappa * G_geometry.  Requires an
  * operator created from cell vertices.  kappa = NULL restores kappa = 1. */
 int pmg_operator_set_coefficient(pmg_operator op, const double *kappa, pmg_stream stream);
+/* Affine geometry for vmult (enable 1) or G at every q-point (enable 0, the
+ * default).  On affine cells G_q = A_c w_p w_q w_r with A_c = |det J| J^-1 J^-T
+ * constant per cell, so the atomic and patch schedules read 6 values per
+ * cell instead of 6 nq^3; the result equals the G form up to rounding.
+ * Requires an operator created from cell vertices whose cells are all affine
+ * (PMG_ERR_INVALID_ARG otherwise) and kappa = 1; setting a coefficient turns
+ * it off.  No reference counterpart: the reference stores G at every
+ * quadrature point (portable_laplace_operator_bk3.h:456-515); this serves
+ * callers with affine meshes (boxes, sheared boxes). */
+int pmg_operator_use_affine_geometry(pmg_operator op, int enable);
 /* Adjoint gradient: out[c][q] = -(grad_xi u)^T G_q (grad_xi lambda) with the
  * operator's current G (= kappa G_geometry), i.e. dJ/dg_q for kappa = exp(g)
  * when lambda solves A lambda = dJ/du.  u, lambda: device vectors of the

@@ -143,6 +143,10 @@ class LaplaceOperatorBK3 : public LaplaceOperatorBase {
   void set_coefficient(const double *kappa_device, pmg_stream s = nullptr) {
     check(pmg_operator_set_coefficient(h_.get(), kappa_device, s));
   }
+  // affine cells: vmult reads 6 values per cell instead of G at every q-point
+  void use_affine_geometry(bool enable = true) {
+    check(pmg_operator_use_affine_geometry(h_.get(), enable ? 1 : 0));
+  }
   void gradient_coefficient(const void *u, const void *lambda, void *out,
                             pmg_stream s = nullptr) const {
     check(pmg_operator_gradient_coefficient(h_.get(), u, lambda, out, s));

@@ -150,6 +150,7 @@ def lib():
         "pmg_cg_solve_noalloc": (_i, [_vp, _i, _vp, _vp, _vp, _d, _i, P(_i), P(_d), P(_d),
                                       _vp]),
         "pmg_operator_set_coefficient": (_i, [_vp, _vp, _vp]),
+        "pmg_operator_use_affine_geometry": (_i, [_vp, _i]),
         "pmg_operator_gradient_coefficient": (_i, [_vp, _vp, _vp, _vp, _vp]),
         "pmg_operator_algorithmic_bytes": (_i, [_vp, P(_d), P(_d)]),
         "pmg_operator_assemble_rhs": (_i, [_vp, _i, _i, _vp, _vp]),
@@ -417,6 +418,11 @@ class LaplaceOperatorBK3:
         else:
             _check(lib().pmg_operator_set_coefficient(
                 self.h, _dev_ptr(kappa, FP64, self.n_cells * self.nq ** 3), _stream()))
+
+    def use_affine_geometry(self, enable=True):
+        """vmult from 6 values per affine cell instead of G at every q-point
+        (raises PMGError if a cell is not affine or a coefficient is set)."""
+        _check(lib().pmg_operator_use_affine_geometry(self.h, 1 if enable else 0))
 
     def gradient_coefficient(self, u, lam, out):
         _check(lib().pmg_operator_gradient_coefficient(
@@ -735,7 +741,8 @@ def vector_dot(a, b, precision=FP64):
 def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=3,
                     smoothing_range=15.0, eig_cg_n_iterations=10, coarse=COARSE_PCG,
                     coarse_tol=1e-12, prolongation_scale=1.0, outer_precision=None,
-                    dg_penalty_kind=0, dg_metric=0, coefficient=None, coarse_precision=None):
+                    dg_penalty_kind=0, dg_metric=0, coefficient=None, coarse_precision=None,
+                    affine=False):
     """V-cycle over `levels` = [(box, degree) or (box, degree, "dg"), ...]
     from coarsest to finest.
 
@@ -750,7 +757,9 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
     diagonal and smoother are set up.
     coarse_precision: the coarsest operator's precision.  None: FP64 when FP32
     levels precondition an FP64 outer solve (the coarse-grid solve stays in
-    FP64, like the diagonal inversion and the outer CG), else `precision`."""
+    FP64, like the diagonal inversion and the outer CG), else `precision`.
+    affine: the continuous levels' vmults use the affine geometry (6 values
+    per cell; every cell must be affine, and no coefficient)."""
     if coarse_precision is None:
         mixed = precision == FP32 and outer_precision == FP64 and len(levels) > 1
         coarse_precision = FP64 if mixed else precision
@@ -766,6 +775,8 @@ def build_hierarchy(levels, n_q_points_1d=None, precision=FP64, smoother_degree=
             op = LaplaceOperatorBK3(b, p, nq, coarse_precision if l == 0 else precision)
             if coefficient is not None:
                 op.set_coefficient(coefficient(b, op.nq))
+            if affine:
+                op.use_affine_geometry()
         op.compute_diagonal()
         ops.append(op)
         if l == 0:
@@ -914,7 +925,8 @@ def coarsen_boxes(fine_box, n_levels):
 def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precision=FP64,
                         smoother_degree=3, smoothing_range=15.0, eig_cg_n_iterations=10,
                         coarse=COARSE_PCG, coarse_tol=1e-12, prolongation_scale=1.0,
-                        outer_precision=None, coefficient=None, coarse_precision=None):
+                        outer_precision=None, coefficient=None, coarse_precision=None,
+                        affine=False):
     """h-only hierarchy at fixed degree (source/geometric_multigrid_bk3/program.cc):
     level 0 has fine_box coarsened n_levels-1 times."""
     boxes = coarsen_boxes(fine_box, n_levels)
@@ -922,5 +934,6 @@ def geometric_hierarchy(fine_box, degree, n_levels, n_q_points_1d=None, precisio
     mg, ops, trs, sms = build_hierarchy([(b, degree) for b in boxes], nqf, precision,
                                         smoother_degree, smoothing_range, eig_cg_n_iterations,
                                         coarse, coarse_tol, prolongation_scale, outer_precision,
-                                        coefficient=coefficient, coarse_precision=coarse_precision)
+                                        coefficient=coefficient, coarse_precision=coarse_precision,
+                                        affine=affine)
     return mg, ops, trs, sms, boxes

@@ -70,7 +70,11 @@ bk3_batch(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
   const T *Gc = G + g_index<T>(NQ, cell, 0, active ? t : 0);
 
   T g[6][NQ];
-  if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
+  if (GMODE == kGAffine) {
+    if (active) bk3_affine_g<NQ, T>(G + cell * 6, sh, ta, tb, g);
+  } else if (GMODE != kGLateNT && active) {
+    bk3_load_g<NQ, T, GMODE>(Gc, g);
+  }
 
   // gather column (i,j) = (ta,tb) over k; invalid (constrained) entries read 0
   uint32_t idx[NM];
@@ -270,7 +274,11 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
   T *P0 = lds + (active ? e : 0) * lds_cell_stride<T>(3 * NQ3);
   const T *Gc = G + g_index<T>(NQ, active ? cell : 0, 0, t);
   T g[6][NQ];
-  if (GMODE != kGLateNT && active) bk3_load_g<NQ, T, GMODE>(Gc, g);
+  if (GMODE == kGAffine) {
+    if (active) bk3_affine_g<NQ, T>(G + cell * 6, sh, ta, tb, g);
+  } else if (GMODE != kGLateNT && active) {
+    bk3_load_g<NQ, T, GMODE>(Gc, g);
+  }
 #pragma unroll
   for (int j = 0; j < PL; ++j)
     if (tid + j * LANES < np) patch[tid + j * LANES] = xv[j];
@@ -321,9 +329,10 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
   PMG_REQUIRE(nb < (int64_t(1) << 31), PMG_ERR_SIZE, "too many cells for one launch");
   const unsigned grid = pad_to_xcd(static_cast<unsigned>(nb));
   (void)assign;  // y += patch either way: callers of vmult zero the free rows first
-  // the colour-by-colour sum is compiled for the default G-load policy only
+  // the colour-by-colour sum is compiled for the default G-load policy and
+  // affine geometry only
   bool colored = false;
-  if constexpr (GMODE == kGEarlyNT) {
+  if constexpr (GMODE == kGEarlyNT || GMODE == kGAffine) {
     colored = sc.pncol != nullptr && sc.pcolor != nullptr;
     if (colored)
       hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true>), dim3(grid), dim3(Tile::lanes),
@@ -925,7 +934,10 @@ template <int NM, int NQ, typename T, bool FULL>
 static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_t n_cells,
                        const Shape1D<T> &sh, hipStream_t s, const Bk3Schedule *sc, bool assign) {
   if (sc && sc->kind == kPatch && sc->pdof) {
-    if (sc->gmode == kGEarlyNT)
+    if (sc->gaff)
+      bk3_launch_patch<NM, NQ, T, kGAffine>(assign, x, y, static_cast<const T *>(sc->gaff), *sc,
+                                            n_cells, sh, s);
+    else if (sc->gmode == kGEarlyNT)
       bk3_launch_patch<NM, NQ, T, kGEarlyNT>(assign, x, y, G, *sc, n_cells, sh, s);
     else if (FULL && sc->gmode == kGEarly)
       bk3_launch_patch<NM, NQ, T, (FULL ? kGEarly : kGLateNT)>(assign, x, y, G, *sc, n_cells, sh, s);
@@ -974,7 +986,10 @@ static void bk3_launch(const T *x, T *y, const T *G, const uint32_t *map, int64_
   const int gmode = sc ? sc->gmode : kGLateNT;
   const uint32_t *lines = sc ? sc->lines : nullptr;
   const uint8_t *lflag = sc ? sc->lflag : nullptr;
-  if (FULL && gmode == kGEarly)
+  if (sc && sc->gaff)
+    bk3_launch_range<NM, NQ, T, kScatterAtomic, kGAffine>(x, y, static_cast<const T *>(sc->gaff),
+                                                          map, lines, lflag, nullptr, n_cells, sh, s);
+  else if (FULL && gmode == kGEarly)
     bk3_launch_range<NM, NQ, T, kScatterAtomic, (FULL ? kGEarly : kGLateNT)>(x, y, G, map, lines, lflag,
                                                                    nullptr, n_cells, sh, s);
   else if (FULL && gmode == kGEarlyNT)
@@ -1107,6 +1122,69 @@ template void launch_geometry<double>(int, int64_t, const double *, const double
 template void launch_geometry<float>(int, int64_t, const double *, const double *,
                                      const double *, const double *, const uint32_t *, float *,
                                      hipStream_t);
+
+// Affine cells: the trilinear map of vertices v_0..v_7 (v_i at the corner
+// (i & 1, i >> 1 & 1, i >> 2 & 1)) is affine when its twist terms vanish,
+// and J is then the edge vectors v_1 - v_0, v_2 - v_0, v_4 - v_0 everywhere;
+// A = |det J| J^-1 J^-T (the G of geometry_kernel without the weights).
+template <typename T>
+__global__ void affine_geometry_kernel(const int64_t n_cells, const double *__restrict__ cv,
+                                       T *__restrict__ A, unsigned long long *__restrict__ bad)
+{
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const double *v = cv + c * 24;
+  auto at = [&](int i, int d) { return v[3 * i + d]; };
+  double J[3][3], tw = 0, h = 0;
+  for (int d = 0; d < 3; ++d) {
+    J[d][0] = at(1, d) - at(0, d);
+    J[d][1] = at(2, d) - at(0, d);
+    J[d][2] = at(4, d) - at(0, d);
+    tw = fmax(tw, fabs(at(0, d) - at(1, d) - at(2, d) + at(3, d)));
+    tw = fmax(tw, fabs(at(0, d) - at(1, d) - at(4, d) + at(5, d)));
+    tw = fmax(tw, fabs(at(0, d) - at(2, d) - at(4, d) + at(6, d)));
+    tw = fmax(tw, fabs(-at(0, d) + at(1, d) + at(2, d) - at(3, d) + at(4, d) - at(5, d) - at(6, d) +
+                       at(7, d)));
+  }
+  for (int k = 0; k < 3; ++k)
+    h = fmax(h, sqrt(J[0][k] * J[0][k] + J[1][k] * J[1][k] + J[2][k] * J[2][k]));
+  if (!(tw <= 1e-10 * h)) atomicAdd(bad, 1ull);
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / det;
+  const double K[3][3] = {{c00 * id, c10 * id, c20 * id},
+                          {c01 * id, c11 * id, c21 * id},
+                          {c02 * id, c12 * id, c22 * id}};
+  int comp = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b, ++comp)
+      A[c * 6 + comp] =
+          static_cast<T>(fabs(det) * (K[a][0] * K[b][0] + K[a][1] * K[b][1] + K[a][2] * K[b][2]));
+}
+
+template <typename T>
+int64_t launch_affine_geometry(int64_t n_cells, const double *cell_vertices, T *A, hipStream_t s) {
+  if (n_cells <= 0) return 0;
+  DeviceArray<unsigned long long> bad(1);
+  PMG_HIP_CHECK(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(affine_geometry_kernel<T>, dim3(static_cast<unsigned>((n_cells + 255) / 256)),
+                     dim3(256), 0, s, n_cells, cell_vertices, A, bad.ptr);
+  PMG_HIP_CHECK(hipGetLastError());
+  unsigned long long h = 0;
+  PMG_HIP_CHECK(hipMemcpyAsync(&h, bad.ptr, sizeof(h), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  return static_cast<int64_t>(h);
+}
+template int64_t launch_affine_geometry<double>(int64_t, const double *, double *, hipStream_t);
+template int64_t launch_affine_geometry<float>(int64_t, const double *, float *, hipStream_t);
 
 // ---------------------------------------------------------------- diagonal
 // (A_e)_ii = sum_q g_i(q)^T G_q g_i(q), g_i(q) = (phi'phi phi, phi phi' phi,

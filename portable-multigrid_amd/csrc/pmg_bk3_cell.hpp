@@ -38,8 +38,27 @@ __device__ inline T ldg_stream(const T *p) {
 enum : int {
   kGEarlyNT = 0,  // non-temporal loads issued at kernel entry
   kGEarly = 1,    // ordinary loads issued at kernel entry
-  kGLateNT = 2    // non-temporal loads issued where G is used
+  kGLateNT = 2,   // non-temporal loads issued where G is used
+  kGAffine = 3    // affine cells: G_q = A w_p w_q w_r from the cell's 6 values
 };
+
+// The column's G of an affine cell: lane (ta,tb) = (p,q) takes
+// g[c][r] = A_c w_p w_q w_r from the cell's A (Ac: 6 values, the same for
+// the cell's lanes) and the 1D weights.
+template <int NQ, typename T>
+__device__ __forceinline__ void bk3_affine_g(const T *Ac, const Shape1D<T> &sh, int ta, int tb,
+                                             T (&g)[6][NQ]) {
+  T a[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a[c] = Ac[c];
+  const T wab = sh.W[ta] * sh.W[tb];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const T ac = a[c] * wab;
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) g[c][r] = ac * sh.W[r];
+  }
+}
 
 // Geometric factors of the lane's q-point column (ta,tb) over all planes r,
 // Gc = G + g_index<T>(nq, cell, 0, q of plane 0): batch-interleaved G,

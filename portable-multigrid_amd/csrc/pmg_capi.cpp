@@ -267,6 +267,13 @@ int pmg_operator_set_coefficient(pmg_operator op, const double *kappa, pmg_strea
   });
 }
 
+int pmg_operator_use_affine_geometry(pmg_operator op, int enable) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    bk3_of(op).use_affine_geometry(enable != 0);
+  });
+}
+
 int pmg_operator_gradient_coefficient(pmg_operator op, const void *u, const void *lam, void *out,
                                       pmg_stream s) {
   return guard([&] {

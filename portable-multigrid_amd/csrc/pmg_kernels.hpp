@@ -31,6 +31,7 @@ struct Shape1D {
   T SE[kEO * kEO], SO[kEO * kEO], SM[kEO];  // (S[k][r] +- S[m-k][r]) / 2, S[hm][r]
   T FE[kEO * kEO], FO[kEO * kEO], FM[kEO];  // (D[n][p] +- D[N-n][p]) / 2, D[hq][p]
   T TE[kEO * kEO], TO[kEO * kEO], TC[kEO], TM[kEO];  // (D[r][n] -+ D[N-r][n]) / 2, D[r][hq], D[hq][n]
+  T W[kMaxNq];  // 1D quadrature weights (affine geometry: G_q = A w_p w_q w_r)
 };
 
 // Fills the even-odd tables of sh from S [nm][nq] and D [nq][nq] (row-major).
@@ -157,6 +158,10 @@ struct Bk3Schedule {
   // the second pass writes the constrained rows too (zero, or y unchanged
   // when adding): callers restore them
   int grid[3] = {0, 0, 0};
+  // affine geometry (atomic and patch schedules): per cell the 6 components
+  // of A = |det J| J^-1 J^-T, [cell][6] in the operator precision, used
+  // instead of G (G_q = A w_p w_q w_r); null: G
+  const void *gaff = nullptr;
 };
 // true if map is the lexicographic box map of dims[0] x dims[1] x dims[2]
 // cells at this degree (entries of constrained DoFs may be invalid)
@@ -223,6 +228,12 @@ void launch_bk3_cells(int degree, int nq, const T *x, T *y, const T *G, const ui
                       const Bk3Schedule *sched = nullptr, bool assign = false);
 // G[c][6][nq^3] = kappa * JxW * J^{-1}J^{-T} of the trilinear map of the cell
 // vertices; kappa (optional) is indexed by the caller's cell perm[c]
+// Affine geometry: A[c][6] = |det J| (J^-1 J^-T) of every cell whose
+// trilinear map is affine (twist terms of its vertices within 1e-10 of its
+// longest edge); returns the number of cells that are not affine (their A is
+// left undefined).  Synchronous on s.
+template <typename T>
+int64_t launch_affine_geometry(int64_t n_cells, const double *cell_vertices, T *A, hipStream_t s);
 template <typename T>
 void launch_geometry(int nq, int64_t n_cells, const double *cell_vertices, const double *qpts,
                      const double *qwts, const double *kappa, const uint32_t *perm, T *G,

@@ -103,6 +103,10 @@ void LaplaceOperatorBK3::setup_shape() {
   sh32_ = to_shape<float>(shape_.S, shape_.D);
   fill_even_odd(sh64_, shape_.S.data(), shape_.D.data(), degree_ + 1, shape_.n_q_points_1d);
   fill_even_odd(sh32_, shape_.S.data(), shape_.D.data(), degree_ + 1, shape_.n_q_points_1d);
+  for (int q = 0; q < nq_; ++q) {
+    sh64_.W[q] = shape_.quad.weights[q];
+    sh32_.W[q] = static_cast<float>(shape_.quad.weights[q]);
+  }
   shd64_ = to_shape<double>(shape_.S, shape_.Sd);  // exact derivatives in the D slot
 }
 
@@ -722,6 +726,27 @@ void LaplaceOperatorBK3::set_coefficient(const double *kappa, hipStream_t s) {
   if (precision_ == PMG_FP64) fill_geometry_t<double>(kappa, s);
   else fill_geometry_t<float>(kappa, s);
   has_diag_ = false;
+  has_kappa_ = kappa != nullptr;
+  if (has_kappa_) sched_.gaff = nullptr;  // G_q now varies beyond the weights
+}
+
+void LaplaceOperatorBK3::use_affine_geometry(bool on) {
+  if (!on) {
+    sched_.gaff = nullptr;
+    return;
+  }
+  PMG_REQUIRE(vertices_.ptr != nullptr, PMG_ERR_INVALID_ARG,
+              "affine geometry needs an operator created from cell vertices");
+  PMG_REQUIRE(!has_kappa_, PMG_ERR_INVALID_ARG,
+              "affine geometry needs a constant coefficient (a kappa was set)");
+  gaff_.reinit(precision_, std::max<int64_t>(n_cells_ * 6, 1));
+  const int64_t bad =
+      precision_ == PMG_FP64
+          ? launch_affine_geometry<double>(n_cells_, vertices_.ptr, static_cast<double *>(gaff_.data()), nullptr)
+          : launch_affine_geometry<float>(n_cells_, vertices_.ptr, static_cast<float *>(gaff_.data()), nullptr);
+  PMG_REQUIRE(bad == 0, PMG_ERR_INVALID_ARG,
+              std::to_string(bad) + " of " + std::to_string(n_cells_) + " cells are not affine");
+  sched_.gaff = gaff_.data();
 }
 
 void LaplaceOperatorBK3::gradient_coefficient(const void *u, const void *lam, void *out,

@@ -113,6 +113,13 @@ class LaplaceOperatorBK3 : public Operator {
   void compute_diagonal() override;
   const void *inverse_diagonal() const override;
   void set_coefficient(const double *kappa, hipStream_t s);
+  // Affine geometry for the vmult's atomic and patch schedules: per cell
+  // A = |det J| J^-1 J^-T (6 values) instead of G at every q-point (G_q =
+  // A w_p w_q w_r).  Needs cell vertices, affine cells (throws otherwise) and
+  // no coefficient; set_coefficient with a kappa turns it off.  G stays for
+  // the diagonal, the coarse solve, the gradient and the other schedules.
+  void use_affine_geometry(bool on);
+  bool uses_affine_geometry() const { return sched_.gaff != nullptr; }
   void gradient_coefficient(const void *u, const void *lam, void *out, hipStream_t s) const;
   void assemble_rhs(int f_kind, int nq, void *b, hipStream_t s) const override;
   double l2_error(const void *u, int u_kind, int nq, hipStream_t s) const override;
@@ -165,7 +172,9 @@ class LaplaceOperatorBK3 : public Operator {
   DeviceArray<uint32_t> map_, constrained_, perm_;
   DeviceArray<double> vertices_, qpts_, qwts_;
   DeviceVector G_, inv_diag_;
+  DeviceVector gaff_;  // affine geometry: [cell][6]
   bool has_diag_ = false;
+  bool has_kappa_ = false;
   // cell colouring for the atomics-free schedule, DoF patches for the patch schedule
   DeviceArray<uint32_t> cell_ids_;
   PatchArrays patches_;

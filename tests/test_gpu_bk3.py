@@ -158,6 +158,55 @@ def test_coefficient_and_gradient_match_oracle(pmg, orc):
     assert rel_err(out.cpu().numpy(), g_ref) < 1e-12
 
 
+@pytest.mark.parametrize("deform", [0, 2])
+@pytest.mark.parametrize("p,nq", [(1, 2), (2, 3), (4, 5), (4, 6), (6, 7)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_affine_geometry_vmult_matches_oracle(pmg, orc, deform, p, nq, prec):
+    """Affine cells (the undeformed box and the affine shear): vmult from 6
+    values per cell against the oracle, which uses G at every q-point, in the
+    patch and the atomic schedule."""
+    import torch
+    print(f"[bk3 affine] deform={deform} p={p} nq={nq} {prec}", flush=True)
+    precision, dt = (pmg.FP64, torch.float64) if prec == "fp64" else (pmg.FP32, torch.float32)
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    op, ref = _pair(pmg, orc, 3, p, nq, deform, precision)
+    op.use_affine_geometry()
+    x = seeded_uniform(op.m(), 3000 + 10 * p + nq)
+    y = ref.vmult(x if prec == "fp64" else x.astype(np.float32).astype(np.float64))
+    src = torch.tensor(x, dtype=dt, device="cuda")
+    for sched in (pmg.SCHEDULE_PATCH, pmg.SCHEDULE_ATOMIC):
+        op.set_schedule(sched, -1)
+        dst = op.initialize_dof_vector()
+        dst.fill_(float("nan"))
+        op.vmult(dst, src)
+        assert rel_err(dst.double().cpu().numpy(), y) < tol, sched
+
+
+def test_affine_geometry_refused_and_turned_off(pmg, orc):
+    """A deformed (non-affine) mesh is refused; a coefficient turns the
+    affine form off, and vmult then uses kappa G at every q-point."""
+    import torch
+    print("[bk3 affine] refusal and coefficient", flush=True)
+    op, _ = _pair(pmg, orc, 3, 3, 4, 1, pmg.FP64)
+    with pytest.raises(pmg.PMGError) as e:
+        op.use_affine_geometry()
+    assert e.value.status == pmg.ERR_INVALID_ARG
+    n, p, nq = 3, 3, 4
+    op, ref = _pair(pmg, orc, n, p, nq, 0, pmg.FP64)
+    op.use_affine_geometry()
+    kappa = np.exp(seeded_uniform(op.n_cells * nq ** 3, 25, -0.5, 0.5))
+    op.set_coefficient(torch.tensor(kappa, device="cuda"))
+    ref.G = orc.scale_G(ref.G, kappa, nq)
+    x = seeded_uniform(op.m(), 26)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, torch.tensor(x, device="cuda"))
+    assert rel_err(dst.cpu().numpy(), ref.vmult(x)) < 1e-12
+    with pytest.raises(pmg.PMGError):
+        op.use_affine_geometry()  # not with a coefficient
+    op.set_coefficient(None)
+    op.use_affine_geometry()  # kappa = 1 again
+
+
 def test_bad_arguments_raise(pmg):
     import torch
     b = pmg.box(2)
