@@ -524,7 +524,8 @@ void LaplaceOperatorBK3::vmult_t(T *dst, const T *src, hipStream_t s) const {
   // and an indexed copy of the constrained rows (fewer bytes than the fused
   // pass's byte mask and conditional loads, measured faster at 17M DoFs; a
   // bit mask instead of the byte mask lost too: Q4 64^3 FP64 vmult 417 ->
-  // 437 us, its dependent mask load makes the pure write stream wait).
+  // 437 us, its dependent mask load makes the pure write stream wait; so did
+  // finding a box operator's boundary rows from the index, 420 -> 428 us).
   // The gather and grid schedules overwrite every free row themselves: no fill.
   const bool gather = sched_.kind == kGather || sched_.kind == kGrid;
   const bool fused = !gather && n_constrained_ > 0 && n_dofs_ <= kFusedPrepMaxDofs;
