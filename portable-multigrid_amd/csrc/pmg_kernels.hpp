@@ -421,6 +421,10 @@ constexpr int kCgConvItSlot = 6, kCgConvergedSlot = 7;
 // flag before a solve.
 void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host = nullptr,
                        double target2 = -1.0, int it = 0);
+// launch_reduce_partials into sc[2] (gated by the converged flag) followed by
+// launch_cg_scalars(op 0), in one launch
+void launch_cg_rr_scalars(const double *partials, int n_partials, double *sc, hipStream_t s,
+                          double *host, double target2, int it);
 
 // Measured FMA issue rate of the device (flop/s) in the given precision:
 // independent FMA chains in every lane of 16 workgroups per CU (measurement

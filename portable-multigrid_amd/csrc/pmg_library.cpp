@@ -1344,8 +1344,7 @@ CGResult solve_cg_t(const Operator &op, int precond, const VCycleMultigrid *mg,
                             skip);
     else
       launch_cg_update_r<T>(n, r, q, sc, partials, s, nullptr, PMG_FP64, skip);
-    launch_reduce_partials(partials, kDotBlocks, sc, 2, 1, s, skip);
-    launch_cg_scalars(sc, 0, s, w.host_rr, target2, it + 1);
+    launch_cg_rr_scalars(partials, kDotBlocks, sc, s, w.host_rr, target2, it + 1);
     ++it;
     const bool check = it % check_every == 0 || it == max_it;
     if (check) {

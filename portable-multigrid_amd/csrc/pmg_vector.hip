@@ -265,6 +265,29 @@ __global__ void cg_scalars_k(double *sc, int op, double *host, double target2, i
     }
   }
 }
+// The end of a CG iteration in one launch: sc[kRr] = the sum of the r.r
+// partials (as reduce_k), then the bookkeeping of cg_scalars_k (op 0)
+__global__ void __launch_bounds__(kThreads)
+cg_rr_scalars_k(const double *partials, int n_partials, double *sc, double *host, double target2,
+                int it) {
+  if (sc[kConverged] != 0.0) return;
+  double s = 0;
+  for (int i = threadIdx.x; i < n_partials; i += kThreads) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    sc[kRr] = s;
+    const double pq = sc[kPq];
+    sc[kAlpha] = pq != 0.0 ? sc[kRho] / pq : 0.0;
+    sc[kRhoOld] = sc[kRho];
+    const bool done = !(s > target2);
+    sc[kConverged] = done ? 1.0 : 0.0;
+    if (done) sc[kConvIt] = it;
+    if (host) {
+      host[0] = s;
+      if (done) host[1] = it;
+    }
+  }
+}
 template <typename TD, typename TS>
 __global__ void convert_unless_k(int64_t n, TD *d, const TS *s, const double *skip) {
   if (skip && *skip != 0.0) return;
@@ -419,6 +442,12 @@ template <typename T>
 void launch_cg_update_x(int64_t n, T *x, const T *p, const double *sc, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(cg_x_k<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, p, sc);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_cg_rr_scalars(const double *partials, int n_partials, double *sc, hipStream_t s,
+                          double *host, double target2, int it) {
+  hipLaunchKernelGGL(cg_rr_scalars_k, dim3(1), dim3(kThreads), 0, s, partials, n_partials, sc, host,
+                     target2, it);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_cg_scalars(double *sc, int op, hipStream_t s, double *host, double target2, int it) {
