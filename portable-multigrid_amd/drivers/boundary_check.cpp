@@ -11,6 +11,9 @@
 //                 constrained-row copy composed by hand
 //   reinit_p, reinit_r  a transfer reinit'ed to another box against a fresh
 //                 transfer of that box (prolongation, restriction)
+//   affine        vmult with use_affine_geometry on an undeformed box against
+//                 the same operator with G at every q-point
+//   affine_refused 1 if the deformed operator refused the affine geometry
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -158,9 +161,28 @@ int main() {
     const double reinit_p = max_abs_diff(a, bb) / max_abs(bb);
     const double reinit_r = max_abs_diff(ra, rb) / max_abs(rb);
 
+    // ---- affine geometry (a library option with no reference counterpart)
+    int refused = 0;
+    try {
+      op.use_affine_geometry();
+    } catch (const Error &e) {
+      refused = e.status == PMG_ERR_INVALID_ARG;
+    }
+    const pmg_box ba = make_box(4, 0);
+    LaplaceOperatorBK3 opa(ba, 4, 5, PMG_FP64);
+    const int64_t Na = opa.m();
+    DeviceBuffer xa(Na), ya(Na), yb(Na);
+    xa.set(pseudo_random(Na, 31));
+    opa.vmult(ya.p, xa.p);
+    opa.use_affine_geometry();
+    opa.vmult(yb.p, xa.p);
+    const std::vector<double> ga = ya.get(), gb = yb.get();
+    const double affine = max_abs_diff(gb, ga) / max_abs(ga);
+
     std::printf("el=%.3e dummy_11=%.3e dummy_01=%.3e dummy_10=%.3e dummy_00=%.3e "
-                "reinit_p=%.3e reinit_r=%.3e\n",
-                el_err, dummy[1][1], dummy[0][1], dummy[1][0], dummy[0][0], reinit_p, reinit_r);
+                "reinit_p=%.3e reinit_r=%.3e affine=%.3e affine_refused=%d\n",
+                el_err, dummy[1][1], dummy[0][1], dummy[1][0], dummy[0][0], reinit_p, reinit_r,
+                affine, refused);
     return 0;
   } catch (const Error &e) {
     std::fprintf(stderr, "pmg error %d: %s\n", e.status, e.what());

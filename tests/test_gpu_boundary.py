@@ -157,8 +157,9 @@ def test_vcycle_after_transfer_reinit(pmg, monkeypatch, prec):
 
 
 def test_cpp_mirror_boundary_members():
-    """drivers/boundary_check (C++ on include/pmg.hpp) calls el, vmult_dummy
-    and MGTransferBase::reinit and reports their deviations."""
+    """drivers/boundary_check (C++ on include/pmg.hpp) calls el, vmult_dummy,
+    MGTransferBase::reinit and the affine-geometry option and reports their
+    deviations."""
     exe = os.path.join(PKG_DIR, "drivers", "boundary_check")
     if not os.path.exists(exe):
         pytest.fail(f"{exe} is not built (run __graft_entry__.build())")
@@ -170,3 +171,4 @@ def test_cpp_mirror_boundary_members():
     for k in ("dummy_11", "dummy_01", "dummy_10", "dummy_00"):
         assert r[k] < 1e-14, k
     assert r["reinit_p"] < 1e-14 and r["reinit_r"] < 1e-14
+    assert r["affine"] < 1e-13 and r["affine_refused"] == 1
