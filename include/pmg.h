@@ -341,6 +341,12 @@ int pmg_dg_operator_create(int dim, int degree, int n_q_points_1d, int precision
                            const uint32_t *inner_faces, int64_t n_boundary_faces,
                            const uint32_t *boundary_faces, const double *penalty,
                            int penalty_kind, int metric, pmg_operator *op);
+/* The same operator for a box (pmg_box_dg_faces' faces; boundary id of box
+ * side f: bit f of neumann_sides, 0 Dirichlet, 1 Neumann) with vertices,
+ * connectivity, penalty (penalty_kind 0 or 1), G and face data built on the
+ * device: no host arrays and no host loops over cells or faces. */
+int pmg_dg_operator_create_box(const pmg_box *box, int degree, int n_q_points_1d, int precision,
+                               int penalty_kind, int metric, int neumann_sides, pmg_operator *op);
 /* penalty values in use, [F + Fb] (host array) */
 int pmg_dg_operator_penalty(pmg_operator op, double *penalty);
 

@@ -124,6 +124,7 @@ def lib():
         "pmg_dg_operator_create": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp, _vp,
                                         _i, _i, P(_vp)]),
         "pmg_dg_operator_penalty": (_i, [_vp, _vp]),
+        "pmg_dg_operator_create_box": (_i, [_BP, _i, _i, _i, _i, _i, _i, P(_vp)]),
         "pmg_operator_create": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp, P(_vp)]),
         "pmg_operator_create_from_G": (_i, [_i, _i, _i, _i, _i64, _vp, _i64, _vp, _i64, _vp,
                                             P(_vp)]),
@@ -479,11 +480,25 @@ class LaplaceOperatorDG(LaplaceOperatorBK3):
     otherwise."""
 
     def __init__(self, b=None, degree=1, n_q_points_1d=None, precision=FP64, penalty=None,
-                 penalty_kind=0, metric=0, neumann_sides=0, _raw=None):
+                 penalty_kind=0, metric=0, neumann_sides=0, _raw=None, device_setup=True):
+        """device_setup (a box with the computed penalty): vertices, faces,
+        penalty and geometry built on the device (pmg_dg_operator_create_box);
+        False builds the face lists on the host and passes them in."""
         self.degree = degree
         self.nq = degree + 1 if n_q_points_1d is None else n_q_points_1d
         self.precision = precision
         self.h = ctypes.c_void_p()
+        if _raw is None and penalty is None and device_setup:
+            self.box = b
+            _check(lib().pmg_dg_operator_create_box(ctypes.byref(b), degree, self.nq, precision,
+                                                    penalty_kind, metric, neumann_sides,
+                                                    ctypes.byref(self.h)))
+            self.n_cells = box_n_cells(b)
+            n = (b.nx, b.ny, b.nz)
+            self.n_faces = (3 * n[0] * n[1] * n[2] - n[1] * n[2] - n[0] * n[2] - n[0] * n[1]
+                            + 2 * (n[1] * n[2] + n[0] * n[2] + n[0] * n[1]))
+            self._m = self.n_cells * (degree + 1) ** 3
+            return
         if _raw is None:
             self.box = b
             cv = box_cell_vertices(b)

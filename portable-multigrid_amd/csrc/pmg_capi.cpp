@@ -129,6 +129,19 @@ int pmg_dg_operator_create(int dim, int degree, int nq, int precision, int64_t n
   });
 }
 
+int pmg_dg_operator_create_box(const pmg_box *box, int degree, int nq, int precision,
+                               int penalty_kind, int metric, int neumann_sides, pmg_operator *op) {
+  return guard([&] {
+    PMG_NOT_NULL(op);
+    *op = nullptr;
+    PMG_NOT_NULL(box);
+    auto h = std::make_unique<pmg_operator_s>();
+    h->impl = std::make_unique<pmg::LaplaceOperatorDG>(*box, degree, nq, precision, penalty_kind,
+                                                       metric, neumann_sides);
+    *op = h.release();
+  });
+}
+
 int pmg_dg_operator_penalty(pmg_operator op, double *penalty) {
   return guard([&] {
     PMG_NOT_NULL(op);

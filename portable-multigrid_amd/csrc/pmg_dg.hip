@@ -399,6 +399,143 @@ template void launch_dg_face_geometry<float>(int, int64_t, const uint32_t *, con
                                              const double *, const double *, int, float *,
                                              hipStream_t);
 
+// ---------------------------------------------------------------- box set-up
+// SIP-DG topology of an nx x ny x nz box on the device, in the numbering of
+// pmg_box_dg_faces: inner faces direction by direction (x, y, z), each the
+// face (L, L + e_d) of the cells L with L_d < n_d - 1 in lexicographic order;
+// then the boundary faces direction by direction in lexicographic cell
+// order, a cell's lower face before its upper one.  One lane per cell:
+// cface/cnbr of its six faces, and finfo of the faces it is the "-" side of
+// (inner faces: the lower cell; boundary faces: the cell).  Boundary id of
+// box side f: bit f of neumann_sides (0 Dirichlet, 1 Neumann).
+__global__ void __launch_bounds__(256)
+dg_box_topology_kernel(const int nx, const int ny, const int nz, const int neumann_sides,
+                       uint32_t *__restrict__ cface, uint32_t *__restrict__ cnbr,
+                       uint32_t *__restrict__ finfo)
+{
+  const int64_t nc = int64_t(nx) * ny * nz;
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int n[3] = {nx, ny, nz};
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (int64_t(nx) * ny);
+  const int64_t ijk[3] = {i, j, k};
+  const int64_t stride[3] = {1, nx, int64_t(nx) * ny};
+  const int64_t off_in[3] = {0, int64_t(nx - 1) * ny * nz,
+                             int64_t(nx - 1) * ny * nz + int64_t(nx) * (ny - 1) * nz};
+  const int64_t n_inner = off_in[2] + int64_t(nx) * ny * (nz - 1);
+  const int64_t off_bd[3] = {0, 2 * int64_t(ny) * nz, 2 * int64_t(ny) * nz + 2 * int64_t(nx) * nz};
+  // rank of cell (a, b, e) among the cells with x_d < n_d - 1, lexicographic
+  auto inner_index = [&](int d, int64_t a, int64_t b, int64_t e) -> int64_t {
+    if (d == 0) return off_in[0] + a + int64_t(nx - 1) * (b + int64_t(ny) * e);
+    if (d == 1) return off_in[1] + a + int64_t(nx) * (b + int64_t(ny - 1) * e);
+    return off_in[2] + a + int64_t(nx) * (b + int64_t(ny) * e);
+  };
+  auto bdry_index = [&](int d, int s) -> int64_t {
+    if (d == 0) return off_bd[0] + 2 * (j + int64_t(ny) * k) + s;
+    if (d == 1)
+      return off_bd[1] + 2 * int64_t(nx) * k + (ny > 1 ? (s ? nx + i : i) : 2 * i + s);
+    return off_bd[2] + (nz > 1 ? (s ? int64_t(nx) * ny + i + int64_t(nx) * j : i + int64_t(nx) * j)
+                               : 2 * (i + int64_t(nx) * j) + s);
+  };
+  for (int f = 0; f < 6; ++f) {
+    const int d = f >> 1, s = f & 1;
+    const bool nb = s ? ijk[d] + 1 < n[d] : ijk[d] > 0;
+    if (nb) {
+      const int64_t other = c + (s ? stride[d] : -stride[d]);
+      // the lower cell of the pair names the face
+      int64_t low[3] = {i, j, k};
+      if (!s) --low[d];
+      const int64_t F = inner_index(d, low[0], low[1], low[2]);
+      cface[6 * c + f] = static_cast<uint32_t>(F) | (s ? 0u : (1u << 30));
+      cnbr[6 * c + f] = static_cast<uint32_t>(other);
+      if (s) {
+        finfo[3 * F] = static_cast<uint32_t>(c);
+        finfo[3 * F + 1] = static_cast<uint32_t>(other);
+        finfo[3 * F + 2] = static_cast<uint32_t>(f);
+      }
+    } else {
+      const int64_t F = n_inner + bdry_index(d, s);
+      const uint32_t role = ((neumann_sides >> f) & 1) ? 3u : 2u;
+      cface[6 * c + f] = static_cast<uint32_t>(F) | (role << 30);
+      cnbr[6 * c + f] = 0u;
+      finfo[3 * F] = static_cast<uint32_t>(c);
+      finfo[3 * F + 1] = kInvalid;
+      finfo[3 * F + 2] = static_cast<uint32_t>(f);
+    }
+  }
+}
+
+// h[c][f] = |K| / |F_f| of every trilinear cell: |K| and the face areas by
+// the 4-point Gauss rule per direction (g4p, g4w) of the host set-up
+__global__ void __launch_bounds__(256)
+dg_cell_h_kernel(const int64_t n_cells, const double *__restrict__ cv,
+                 const double *__restrict__ g4p, const double *__restrict__ g4w,
+                 double *__restrict__ h)
+{
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const double *v = cv + 24 * c;
+  double vol = 0;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b)
+      for (int e = 0; e < 4; ++e) {
+        const double xi[3] = {g4p[a], g4p[b], g4p[e]};
+        double J[3][3];
+        trilinear_jacobian(v, xi, J);
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        vol += fabs(det) * g4w[a] * g4w[b] * g4w[e];
+      }
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2, t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+    double area = 0;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        double xi[3], J[3][3];
+        xi[d] = f % 2;
+        xi[t1] = g4p[a];
+        xi[t2] = g4p[b];
+        trilinear_jacobian(v, xi, J);
+        const double n0 = J[1][t1] * J[2][t2] - J[2][t1] * J[1][t2];
+        const double n1 = J[2][t1] * J[0][t2] - J[0][t1] * J[2][t2];
+        const double n2 = J[0][t1] * J[1][t2] - J[1][t1] * J[0][t2];
+        area += sqrt(n0 * n0 + n1 * n1 + n2 * n2) * g4w[a] * g4w[b];
+      }
+    h[6 * c + f] = vol / area;
+  }
+}
+
+// sigma[F] = fac (1/h- + 1/h+), h+ = h- on boundary faces
+__global__ void __launch_bounds__(256)
+dg_penalty_kernel(const int64_t n_faces, const uint32_t *__restrict__ finfo,
+                  const double *__restrict__ h, const double fac, double *__restrict__ sigma)
+{
+  const int64_t F = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (F >= n_faces) return;
+  const uint32_t cm = finfo[3 * F], cp = finfo[3 * F + 1], fm = finfo[3 * F + 2];
+  const double hm = h[6 * int64_t(cm) + fm];
+  const double hp = cp == kInvalid ? hm : h[6 * int64_t(cp) + (fm ^ 1u)];
+  sigma[F] = fac * (1.0 / hm + 1.0 / hp);
+}
+
+void launch_dg_box_setup(const pmg_box &box, int neumann_sides, const double *cv,
+                         const double *g4p, const double *g4w, double penalty_fac,
+                         uint32_t *cface, uint32_t *cnbr, uint32_t *finfo, double *h,
+                         double *sigma, int64_t n_faces, hipStream_t s) {
+  const int64_t nc = int64_t(box.nx) * box.ny * box.nz;
+  if (nc <= 0) return;
+  const dim3 gc(static_cast<unsigned>((nc + 255) / 256)), b(256);
+  hipLaunchKernelGGL(dg_box_topology_kernel, gc, b, 0, s, box.nx, box.ny, box.nz, neumann_sides,
+                     cface, cnbr, finfo);
+  PMG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(dg_cell_h_kernel, gc, b, 0, s, nc, cv, g4p, g4w, h);
+  PMG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(dg_penalty_kernel, dim3(static_cast<unsigned>((n_faces + 255) / 256)), b, 0,
+                     s, n_faces, finfo, h, penalty_fac, sigma);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
 // ---------------------------------------------------------------- DG <-> CG
 // ContinuousTransfer (portable_continuous_transfer.h:112-160, 219-266): DG DoF
 // e = c*NM^3 + i (or dg_map[e]) and CG DoF cg_map[e] share a support point.

@@ -281,6 +281,14 @@ void launch_dg_apply(int degree, int nq, bool add, const T *x, T *y, const T *G,
                      const uint32_t *cface, const uint32_t *cnbr, const T *fdata, const T *fsig,
                      int64_t n_cells, const Shape1D<T> &sh, const DgShape<T> &fs, hipStream_t s);
 // fdata[F][7][nq^2] from cell vertices; finfo[F] = (cell-, cell+ or invalid, face of cell-)
+// SIP-DG box set-up on the device: connectivity cface/cnbr [C][6] and
+// finfo [F][3] in the numbering of pmg_box_dg_faces (boundary id of side f:
+// bit f of neumann_sides), cell h [C][6] = |K| / |F_f| by the 4-point Gauss
+// rule (g4p, g4w) and the penalty sigma [F] = penalty_fac (1/h- + 1/h+)
+void launch_dg_box_setup(const pmg_box &box, int neumann_sides, const double *cv,
+                         const double *g4p, const double *g4w, double penalty_fac,
+                         uint32_t *cface, uint32_t *cnbr, uint32_t *finfo, double *h,
+                         double *sigma, int64_t n_faces, hipStream_t s);
 template <typename T>
 void launch_dg_face_geometry(int nq, int64_t n_faces, const uint32_t *finfo, const double *cv,
                              const double *qp, const double *qw, int metric, T *fdata,

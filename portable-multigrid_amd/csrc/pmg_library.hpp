@@ -235,6 +235,10 @@ class LaplaceOperatorDG : public Operator {
                     const double *cell_vertices, int64_t n_inner, const uint32_t *inner_faces,
                     int64_t n_boundary, const uint32_t *boundary_faces, const double *penalty,
                     int penalty_kind, int metric);
+  // a box built on the device (faces in the numbering of pmg_box_dg_faces;
+  // boundary id of side f: bit f of neumann_sides)
+  LaplaceOperatorDG(const pmg_box &box, int degree, int nq, int precision, int penalty_kind,
+                    int metric, int neumann_sides);
   void vmult(void *dst, const void *src, hipStream_t s) const override;
   void apply_add(void *dst, const void *src, hipStream_t s) const override;
   // cell-integral diagonal only, as LaplaceOperatorDG::compute_diagonal
@@ -258,6 +262,8 @@ class LaplaceOperatorDG : public Operator {
  private:
   template <typename T> void apply_t(bool add, T *dst, const T *src, hipStream_t s) const;
   const uint32_t *identity_map() const;
+  void setup_1d();
+  void setup_geometry(const uint32_t *finfo_device, int metric);
   int degree_, nq_, precision_;
   int64_t n_cells_, nm3_, n_faces_ = 0;
   ShapeData shape_;

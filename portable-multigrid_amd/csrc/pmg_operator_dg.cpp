@@ -159,7 +159,61 @@ LaplaceOperatorDG::LaplaceOperatorDG(int degree, int nq, int precision, int64_t 
     }
   }
 
-  // 1D data
+  setup_1d();
+  cface_.upload(cface);
+  cnbr_.upload(cnbr);
+  vertices_.upload(cell_vertices, static_cast<size_t>(24 * n_cells));
+  DeviceArray<uint32_t> finfo_d;
+  finfo_d.upload(finfo);
+  setup_geometry(finfo_d.ptr, metric);
+}
+
+// A box on the device: vertices, connectivity (the numbering of
+// pmg_box_dg_faces), the penalty from the cell and face measures, G and the
+// face data, without host loops or host arrays.
+LaplaceOperatorDG::LaplaceOperatorDG(const pmg_box &box, int degree, int nq, int precision,
+                                     int penalty_kind, int metric, int neumann_sides)
+    : degree_(degree), nq_(nq), precision_(precision),
+      n_cells_(int64_t(box.nx) * box.ny * box.nz) {
+  check_precision(precision);
+  PMG_REQUIRE(degree >= 1 && degree <= 6, PMG_ERR_UNSUPPORTED, "degree must be in [1,6]");
+  PMG_REQUIRE(nq == degree + 1 || nq == degree + 2, PMG_ERR_UNSUPPORTED,
+              "n_q_points_1d must be degree+1 or degree+2");
+  PMG_REQUIRE(box.nx > 0 && box.ny > 0 && box.nz > 0, PMG_ERR_INVALID_ARG,
+              "box needs at least one cell per direction");
+  PMG_REQUIRE(box.deform >= 0 && box.deform <= 2, PMG_ERR_INVALID_ARG, "unknown deformation");
+  PMG_REQUIRE(metric == 0 || metric == 1, PMG_ERR_INVALID_ARG, "metric must be 0 or 1");
+  PMG_REQUIRE(penalty_kind == 0 || penalty_kind == 1, PMG_ERR_INVALID_ARG,
+              "penalty_kind must be 0 or 1");
+  nm3_ = int64_t(degree + 1) * (degree + 1) * (degree + 1);
+  PMG_REQUIRE(n_cells_ * nm3_ < int64_t(0xFFFFFFFFu), PMG_ERR_SIZE,
+              "DG DoF count exceeds 32-bit indexing");
+  const int64_t nx = box.nx, ny = box.ny, nz = box.nz;
+  const int64_t n_inner = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1);
+  n_faces_ = n_inner + 2 * (ny * nz + nx * nz + nx * ny);
+  PMG_REQUIRE(n_faces_ < (int64_t(1) << 30), PMG_ERR_SIZE, "too many faces");
+  vertices_.resize(static_cast<size_t>(24 * n_cells_));
+  box_vertices_device(box, vertices_.ptr, nullptr);
+  cface_.resize(static_cast<size_t>(6 * n_cells_));
+  cnbr_.resize(static_cast<size_t>(6 * n_cells_));
+  DeviceArray<uint32_t> finfo_d(static_cast<size_t>(3 * n_faces_));
+  DeviceArray<double> h(static_cast<size_t>(6 * n_cells_)), sig(static_cast<size_t>(n_faces_));
+  const Quadrature1D g4 = gauss_legendre(4);
+  DeviceArray<double> g4p, g4w;
+  g4p.upload(g4.points);
+  g4w.upload(g4.weights);
+  const double fac = penalty_kind == 0 ? 0.5 * degree * (degree + 1.0)
+                                       : (degree + 1.0) * (degree + 1.0);
+  launch_dg_box_setup(box, neumann_sides, vertices_.ptr, g4p.ptr, g4w.ptr, fac, cface_.ptr,
+                      cnbr_.ptr, finfo_d.ptr, h.ptr, sig.ptr, n_faces_, nullptr);
+  sigma_ = sig.download();
+  setup_1d();
+  setup_geometry(finfo_d.ptr, metric);
+}
+
+// 1D data: shape matrices, their even-odd tables, and the face tables
+void LaplaceOperatorDG::setup_1d() {
+  const int degree = degree_, nq = nq_;
   shape_ = make_shape_data(degree, nq);
   sh64_ = make_shape<double>(shape_.S, shape_.D);
   sh32_ = make_shape<float>(shape_.S, shape_.D);
@@ -195,12 +249,13 @@ LaplaceOperatorDG::LaplaceOperatorDG(int degree, int nq, int precision, int64_t 
     }
   }
 
-  // device data
-  cface_.upload(cface);
-  cnbr_.upload(cnbr);
-  vertices_.upload(cell_vertices, static_cast<size_t>(24 * n_cells));
-  DeviceArray<uint32_t> finfo_d;
-  finfo_d.upload(finfo);
+}
+
+// G and the face data (finfo: device [F][3]), the penalty in the operator
+// precision
+void LaplaceOperatorDG::setup_geometry(const uint32_t *finfo_d, int metric) {
+  const int nq = nq_, precision = precision_;
+  const int64_t n_cells = n_cells_;
   DeviceArray<double> qp, qw;
   qp.upload(shape_.quad.points);
   qw.upload(shape_.quad.weights);
@@ -209,17 +264,17 @@ LaplaceOperatorDG::LaplaceOperatorDG(int degree, int nq, int precision, int64_t 
   if (precision == PMG_FP64) {
     launch_geometry<double>(nq, n_cells, vertices_.ptr, qp.ptr, qw.ptr, nullptr, nullptr,
                             static_cast<double *>(G_.data()), nullptr);
-    launch_dg_face_geometry<double>(nq, n_faces_, finfo_d.ptr, vertices_.ptr, qp.ptr, qw.ptr,
+    launch_dg_face_geometry<double>(nq, n_faces_, finfo_d, vertices_.ptr, qp.ptr, qw.ptr,
                                     metric, static_cast<double *>(fdata_.data()), nullptr);
     upload_as<double>(fsig_, precision, sigma_);
   } else {
     launch_geometry<float>(nq, n_cells, vertices_.ptr, qp.ptr, qw.ptr, nullptr, nullptr,
                            static_cast<float *>(G_.data()), nullptr);
-    launch_dg_face_geometry<float>(nq, n_faces_, finfo_d.ptr, vertices_.ptr, qp.ptr, qw.ptr,
+    launch_dg_face_geometry<float>(nq, n_faces_, finfo_d, vertices_.ptr, qp.ptr, qw.ptr,
                                    metric, static_cast<float *>(fdata_.data()), nullptr);
     upload_as<float>(fsig_, precision, sigma_);
   }
-  PMG_HIP_CHECK(hipDeviceSynchronize());  // finfo_d, qp, qw are freed on return
+  PMG_HIP_CHECK(hipDeviceSynchronize());  // qp, qw and the caller's finfo are freed after
 }
 
 template <typename T>

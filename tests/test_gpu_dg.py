@@ -199,6 +199,38 @@ def test_dg_discretisation_error_converges(pmg):
     assert rate > 2.6, errs
 
 
+@pytest.mark.parametrize("cells,p,deform,neumann,kind,prec", [
+    ((3, 3, 3), 2, 1, 0, 0, "fp64"), ((2, 3, 5), 3, 2, 0b101001, 1, "fp64"),
+    ((1, 2, 3), 1, 0, 0b000110, 0, "fp64"), ((4, 1, 2), 2, 1, 0, 1, "fp32"),
+    ((2, 2, 1), 4, 1, 0b111111, 0, "fp32")])
+def test_dg_box_device_setup_matches_host_setup(pmg, cells, p, deform, neumann, kind, prec):
+    """pmg_dg_operator_create_box (vertices, faces, penalty and geometry on
+    the device) against the operator from host face lists: the same penalty
+    per face (the same face numbering) and the same vmult and diagonal, to
+    rounding."""
+    import torch
+    print(f"[dg box set-up] {cells} p={p} deform={deform} neumann={neumann} {prec}", flush=True)
+    precision, dt = (pmg.FP64, torch.float64) if prec == "fp64" else (pmg.FP32, torch.float32)
+    b = pmg.box(*cells, deform=deform, amplitude=AMP[deform])
+    kw = dict(penalty_kind=kind, neumann_sides=neumann)
+    d = pmg.LaplaceOperatorDG(b, p, None, precision, **kw)
+    h = pmg.LaplaceOperatorDG(b, p, None, precision, device_setup=False, **kw)
+    assert d.n_faces == h.n_faces and d.m() == h.m()
+    assert rel_err(d.penalty(), h.penalty()) < 1e-14
+    x = _t(seeded_uniform(d.m(), 231), dt)
+    yd, yh = d.initialize_dof_vector(), h.initialize_dof_vector()
+    d.vmult(yd, x)
+    h.vmult(yh, x)
+    assert rel_err(yd.double().cpu().numpy(), yh.double().cpu().numpy()) < (
+        1e-13 if prec == "fp64" else 1e-6)
+    d.compute_diagonal()
+    h.compute_diagonal()
+    # the device and host vertex generators round the deformation's sines
+    # differently: the two operators agree to rounding, not bit for bit
+    assert rel_err(d.get_matrix_diagonal_inverse().double().cpu().numpy(),
+                   h.get_matrix_diagonal_inverse().double().cpu().numpy()) < 1e-13
+
+
 def test_dg_rejects_bad_face_lists(pmg):
     b = pmg.box(2)
     cv = pmg.box_cell_vertices(b)
