@@ -517,8 +517,15 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
   // indices are loaded once into registers instead of once per iteration.
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   const bool one = n_cells <= E;
-  const int e1 = tid / NQ2, t1 = tid - e1 * NQ2, ta = t1 % NQ, tb = t1 / NQ;
-  const bool act1 = one && e1 < E && e1 < n_cells;
+  // wave-aligned tiles (whole cells per wave) when the level's cells fit
+  // them (2^3 cells at nq = 5: 2 per wave): the sweeps then sync per wave
+  // instead of per workgroup (a tile that straddles waves needs barriers)
+  constexpr int CW = Bk3Tile<NQ>::per_wave;
+  const bool wt = (64 % NQ2 != 0) && n_cells <= (LANES / 64) * CW;
+  const int wl = tid & 63, wc = wl / NQ2;
+  const int e1 = wt ? (tid >> 6) * CW + wc : tid / NQ2;
+  const int t1 = wt ? wl - wc * NQ2 : tid - e1 * NQ2, ta = t1 % NQ, tb = t1 / NQ;
+  const bool act1 = one && (!wt || wc < CW) && e1 < E && e1 < n_cells;
   const bool col1 = act1 && ta < NM && tb < NM;
   T *P1 = lds + (act1 ? e1 : 0) * lds_cell_stride<T>(3 * NQ3);
   const T *G1 = G + g_index<T>(NQ, act1 ? e1 : 0, 0, t1);
@@ -539,8 +546,12 @@ coarse_pcg_lds_kernel(const T *__restrict__ G, const uint32_t *__restrict__ map,
 #pragma unroll
         for (int k = 0; k < NM; ++k) u[k] = idx1[k] == kInvalid ? T(0) : pv[idx1[k]];
       }
-      bk3_cell_apply<NM, NQ, T, kGEarly>(u, v, g1, G1, sh, P1, P1 + NQ3, P1 + 2 * NQ3, act1, ta,
-                                         tb);
+      if (wt)
+        bk3_cell_apply<NM, NQ, T, kGEarly, true>(u, v, g1, G1, sh, P1, P1 + NQ3, P1 + 2 * NQ3, act1,
+                                                 ta, tb);
+      else
+        bk3_cell_apply<NM, NQ, T, kGEarly>(u, v, g1, G1, sh, P1, P1 + NQ3, P1 + 2 * NQ3, act1, ta,
+                                           tb);
       if (col1) {
 #pragma unroll
         for (int k = 0; k < NM; ++k)
