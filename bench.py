@@ -430,6 +430,19 @@ def bench_config4(pmg):
     r32["roofline"] = solve_roofline(pmg, ops32, outer, r32["iterations"], r32["time_s"])
     r32["levels_table"] = level_table(pmg, mg32, ops32, rhs)
     r32["rel_diff_vs_fp64_solution"] = ((x32 - x64).abs().max() / x64.abs().max()).item()
+    # the same mixed solve with the Q6 FP32 level's x and y sweeps on the
+    # matrix cores (PMG_BK3_MFMA=1)
+    del mg32, ops32[:-1]
+    os.environ["PMG_BK3_MFMA"] = "1"
+    try:
+        mgx, opsx, _, _ = pmg.build_hierarchy(levels, precision=pmg.FP32, outer_precision=pmg.FP64)
+    finally:
+        os.environ.pop("PMG_BK3_MFMA", None)
+    rx, xx = solve_timed(pmg, outer, rhs, mgx, pmg.PRECOND_MG, 1e-10)
+    rx["rel_diff_vs_fp64_solution"] = ((xx - x64).abs().max() / x64.abs().max()).item()
+    r32["matrix_core_sweeps"] = {k: rx[k] for k in ("time_s", "iterations",
+                                                    "rel_diff_vs_fp64_solution") if k in rx}
+    del mgx, opsx
     return {"workload": "ph-MG Q6, 48^3 deformed cells, p 6->3->1, h 48->3 at Q1",
             "dofs": outer.m(), "levels": len(levels), "fp64": r64, "fp32_vcycle_fp64_cg": r32,
             "speedup_mixed": r64["time_s"] / r32["time_s"],
@@ -637,6 +650,32 @@ def bench_rows(pmg):
             "fine_level_matvecs": 6, "levels": "h 2^3..64^3 at Q4"}
         del mg, ops, trs, sms, r, z
         torch.cuda.empty_cache()
+
+    # the Q6 fine level of the ph-MG configuration (48^3 deformed, nq 7) in
+    # FP32: the patch kernel's x and y sweeps on the vector units (default) or
+    # as batched contractions on the matrix cores (PMG_BK3_MFMA=1)
+    b48 = pmg.box(48, deform=1, amplitude=0.05)
+    ys = {}
+    for mode in ("vector", "matrix_cores"):
+        if mode == "matrix_cores":
+            os.environ["PMG_BK3_MFMA"] = "1"
+        try:
+            op = pmg.LaplaceOperatorBK3(b48, 6, 7, pmg.FP32)
+        finally:
+            os.environ.pop("PMG_BK3_MFMA", None)
+        x = torch.rand(op.m(), dtype=torch.float32, device="cuda",
+                       generator=torch.Generator(device="cuda").manual_seed(48))
+        y = op.initialize_dof_vector()
+        sfx = "" if mode == "vector" else "_matrix_cores"
+        rows[f"a1_a2_bk3_vmult_q6_48cube_deformed_fp32{sfx}"] = dict(
+            _roof(time_events(lambda: op.vmult(y, x), 20), op.algorithmic_bytes()[0]),
+            sweeps=mode)
+        ys[mode] = y.double()
+        del op, x, y
+        torch.cuda.empty_cache()
+    rows["a1_a2_bk3_vmult_q6_48cube_deformed_fp32_matrix_cores"]["rel_diff_vs_vector"] = (
+        (ys["matrix_cores"] - ys["vector"]).abs().max() / ys["vector"].abs().max()).item()
+    del ys
     return rows
 
 

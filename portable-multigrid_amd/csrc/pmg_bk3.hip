@@ -165,14 +165,16 @@ bk3_cells_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 // plain stores into cache lines that other batches' atomics also update were
 // measured slower on MI355X (Q4 64^3 FP64 549 -> 500 us, FP32 330 -> 321 us,
 // Q6 48^3 FP32 480 -> 406 us with all-atomic write-out), so pdof's bit 31
-// (shared with another batch) is not used here.
-template <int NM, int NQ, typename T, int GMODE, bool COLORED>
+// (shared with another batch) is not used here.  MX: the x and y sweeps of
+// the cell apply on the matrix cores (bk3_forward / bk3_backward).
+template <int NM, int NQ, typename T, int GMODE, bool COLORED, bool MX = false>
 __global__ void __launch_bounds__(256)
 bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ G,
                  const uint32_t *__restrict__ pdof, const uint32_t *__restrict__ pcount,
                  const uint32_t *__restrict__ pchunk, const uint16_t *__restrict__ pidx, const uint16_t *__restrict__ pline,
                  const uint8_t *__restrict__ pflag, const uint8_t *__restrict__ pcolor,
-                 const uint8_t *__restrict__ pncol, const int64_t n_cells, const Shape1D<T> sh)
+                 const uint8_t *__restrict__ pncol, const int64_t n_cells, const Shape1D<T> sh,
+                 const float *__restrict__ mxa)
 {
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ, NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int E = Bk3Tile<NQ>::cells, PMAX = E * NM3;
@@ -289,8 +291,9 @@ bk3_patch_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict
 #pragma unroll
     for (int k = 0; k < NM; ++k) u[k] = pi[k] == 0xFFFFu ? T(0) : patch[pi[k]];
   }
-  bk3_cell_apply<NM, NQ, T, GMODE>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3, active, ta, tb,
-                                   patch, np);
+  const int n_act = static_cast<int>(n_cells - cell_base < E ? n_cells - cell_base : E);
+  bk3_cell_apply<NM, NQ, T, GMODE, (64 % NQ2 == 0), MX>(u, v, g, Gc, sh, P0, P0 + NQ3, P0 + 2 * NQ3,
+                                                         active, ta, tb, patch, np, lds, n_act, mxa);
   if (COLORED && n_colors > 0) {
     // colour by colour: no two cells of a colour share a slot, so plain
     // read-modify-writes replace the LDS float atomics (measured costly at
@@ -334,15 +337,32 @@ static void bk3_launch_patch(bool assign, const T *x, T *y, const T *G, const Bk
   bool colored = false;
   if constexpr (GMODE == kGEarlyNT || GMODE == kGAffine) {
     colored = sc.pncol != nullptr && sc.pcolor != nullptr;
+    // the x and y sweeps on the matrix cores: FP32 at nq >= 6, tiles that
+    // straddle waves (the sweeps mix the batch's cells over all waves)
+    if constexpr (std::is_same<T, float>::value && NQ >= 6 && 64 % (NQ * NQ) != 0 &&
+                  GMODE == kGEarlyNT) {
+      if (sc.mxa) {
+        if (colored)
+          hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true, true>), dim3(grid),
+                             dim3(Tile::lanes), 0, s, x, y, G, sc.pdof, sc.pcount, sc.pchunk,
+                             sc.pidx, sc.pline, sc.pflag, sc.pcolor, sc.pncol, n_cells, sh, sc.mxa);
+        else
+          hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, false, true>), dim3(grid),
+                             dim3(Tile::lanes), 0, s, x, y, G, sc.pdof, sc.pcount, sc.pchunk,
+                             sc.pidx, sc.pline, sc.pflag, nullptr, nullptr, n_cells, sh, sc.mxa);
+        PMG_HIP_CHECK(hipGetLastError());
+        return;
+      }
+    }
     if (colored)
       hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, true>), dim3(grid), dim3(Tile::lanes),
                          0, s, x, y, G, sc.pdof, sc.pcount, sc.pchunk, sc.pidx, sc.pline, sc.pflag, sc.pcolor,
-                         sc.pncol, n_cells, sh);
+                         sc.pncol, n_cells, sh, nullptr);
   }
   if (!colored)
     hipLaunchKernelGGL((bk3_patch_kernel<NM, NQ, T, GMODE, false>), dim3(grid), dim3(Tile::lanes),
                        0, s, x, y, G, sc.pdof, sc.pcount, sc.pchunk, sc.pidx, sc.pline, sc.pflag, nullptr,
-                       nullptr, n_cells, sh);
+                       nullptr, n_cells, sh, nullptr);
   PMG_HIP_CHECK(hipGetLastError());
 }
 

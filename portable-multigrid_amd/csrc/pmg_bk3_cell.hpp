@@ -3,6 +3,8 @@
 // NQ x NQ lane tile (reference: BK3::Parallel::KokkosKernel, bk3_kokkos_kernel.h:195-669).
 #pragma once
 
+#include <type_traits>
+
 #include "pmg_common.hpp"
 #include "pmg_kernels.hpp"
 
@@ -51,7 +53,15 @@ __device__ __forceinline__ void bk3_affine_g(const T *Ac, const Shape1D<T> &sh, 
   T a[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) a[c] = Ac[c];
-  const T wab = sh.W[ta] * sh.W[tb];
+  // w_ta and w_tb by selects over constant indices: a lane-dependent index
+  // into the kernel argument made the compiler copy it to private memory
+  T wa = T(0), wb = T(0);
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    wa = ta == i ? sh.W[i] : wa;
+    wb = tb == i ? sh.W[i] : wb;
+  }
+  const T wab = wa * wb;
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const T ac = a[c] * wab;
@@ -224,16 +234,80 @@ __device__ __forceinline__ void eo_deriv_t(const Shape1D<T> &sh, const T (&x)[NQ
   }
 }
 
+// ---- batched 1D contractions on the matrix cores (FP32)
+// v_mfma_f32_16x16x4f32, one 16 x 16 block with K = 4: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and receives D[4(l/16)+v][l%16], v < 4.
+typedef float mx_f4 __attribute__((ext_vector_type(4)));
+
+// For every column c < ncol (one line of one of the batch's cells):
+//   d(c, m, sum_{k<K} a(m, k) b(c, k))    for m < M <= 16,
+// and with TWO also d2(c, m, sum_k a2(m, k) b(c, k)) from the same operands.
+// Columns go in tiles of 16, tile t to wave t mod (waves); every lane of the
+// workgroup calls it (the tile loop is uniform per wave, and the matrix
+// instruction runs with all 64 lanes).  The matrices are zero-padded to
+// 16 x 4 ceil(K / 4); b is read only for c < ncol, k < K.  The bk3 sweeps
+// take their matrices from mx_a, six zero-padded 16 x 16 tables A[m * 16 + k]
+// (fill_mx_tables), read by lane-dependent index: indexing the Shape1D
+// kernel argument that way would copy it to private memory.
+template <int M, int K, bool TWO, typename FA, typename FA2, typename FB, typename FD, typename FD2>
+__device__ __forceinline__ void mx_sweep(const int ncol, FA a, FA2 a2, FB b, FD d, FD2 d2) {
+  static_assert(M <= 16, "one block of 16 rows");
+  constexpr int KS = (K + 3) / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  float av[KS], av2[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int k = 4 * s + q4;
+    av[s] = r16 < M && k < K ? a(r16, k) : 0.f;
+    if constexpr (TWO) av2[s] = r16 < M && k < K ? a2(r16, k) : 0.f;
+  }
+  const int ntile = (ncol + 15) >> 4;
+  for (int t = wave; t < ntile; t += nw) {
+    const int c = 16 * t + r16;
+    const bool cv = c < ncol;
+    float bv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + q4;
+      bv[s] = cv && k < K ? b(c, k) : 0.f;
+    }
+    mx_f4 acc = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s], acc, 0, 0, 0);
+      if constexpr (TWO) acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av2[s], bv[s], acc2, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = 4 * q4 + v;
+      if (cv && m < M) {
+        d(c, m, acc[v]);
+        if constexpr (TWO) d2(c, m, acc2[v]);
+      }
+    }
+  }
+}
+
 // bk3_forward: u[k] is the source column (i,j) = (ta,tb) over k (read only
 // where ta,tb < NM).  Leaves the values at the Gauss points in
 // P0[r][q][p] (p along x) and d/dxi_0 of them in P2.  zbuf[0..zn) (LDS,
 // optional) is cleared by the whole workgroup after every lane has consumed u.
-template <int NM, int NQ, typename T, bool WAVE = (64 % (NQ * NQ) == 0)>
+// MX: the x and y sweeps (phases 2 and 3) run as batched contractions on the
+// matrix cores over the mx_cells cells of the batch, whose LDS regions start
+// at mx_lds with the stride lds_cell_stride(3 NQ^3); FP32 only, and every
+// phase ends with a workgroup barrier.  The x-gradient is then formed from
+// the nodal values directly, with the product matrix S D, instead of from the
+// rounded values at the Gauss points.
+template <int NM, int NQ, typename T, bool WAVE = (64 % (NQ * NQ) == 0), bool MX = false>
 __device__ __forceinline__ void
 bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const bool active,
-            const int ta, const int tb, T *zbuf = nullptr, const int zn = 0)
+            const int ta, const int tb, T *zbuf = nullptr, const int zn = 0, T *mx_lds = nullptr,
+            const int mx_cells = 0, const float *mx_a = nullptr)
 {
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
+  static_assert(!MX || (std::is_same<T, float>::value && !WAVE), "MX: FP32, workgroup barriers");
+  [[maybe_unused]] constexpr int NQ3 = NQ2 * NQ, CS = lds_cell_stride<T>(3 * NQ3);
   // ---- phase 1: interpolate the column in z
   if (active && ta < NM && tb < NM) {
     T w[NQ];
@@ -249,6 +323,42 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
     __syncthreads();
   } else {
     cell_sync<NQ, WAVE>();
+  }
+
+  if constexpr (MX) {
+    // phase 2: column (r,i) of a cell, B[r][q][i] = sum_j S[j][q] A[r][j][i]
+    constexpr int L = NQ * NM;
+    mx_sweep<NQ, NM, false>(
+        mx_cells * L, [&](int m, int k) { return mx_a[m * 16 + k]; }, [](int, int) { return 0.f; },
+        [&](int c, int k) {
+          const int e = c / L, l = c - e * L, r = l / NM, i = l - r * NM;
+          return mx_lds[e * CS + r * NM2 + k * NM + i];
+        },
+        [&](int c, int m, float v) {
+          const int e = c / L, l = c - e * L, r = l / NM, i = l - r * NM;
+          mx_lds[e * CS + NQ3 + (r * NQ + m) * NM + i] = v;
+        },
+        [](int, int, float) {});
+    __syncthreads();
+    // phase 3: column (r,q), U[r][q][p] = sum_i S[i][p] B[r][q][i] and
+    // dU/dxi_0 = sum_i (S D)[i][p] B[r][q][i]
+    mx_sweep<NQ, NM, true>(
+        mx_cells * NQ2, [&](int m, int k) { return mx_a[m * 16 + k]; },
+        [&](int m, int k) { return mx_a[256 + m * 16 + k]; },
+        [&](int c, int k) {
+          const int e = c / NQ2, l = c - e * NQ2;
+          return mx_lds[e * CS + NQ3 + l * NM + k];
+        },
+        [&](int c, int m, float v) {
+          const int e = c / NQ2, l = c - e * NQ2;
+          mx_lds[e * CS + l * NQ + m] = v;
+        },
+        [&](int c, int m, float v) {
+          const int e = c / NQ2, l = c - e * NQ2;
+          mx_lds[e * CS + 2 * NQ3 + l * NQ + m] = v;
+        });
+    __syncthreads();
+    return;
   }
 
   // ---- phase 2: y-interpolation, lane (i,r) = (ta,tb): B[r][q][i]
@@ -284,12 +394,36 @@ bk3_forward(const T (&u)[NM], const Shape1D<T> &sh, T *P0, T *P1, T *P2, const b
 // and the test with grad phi; with HASV also sum_q phi(q) V[q] for a value
 // coefficient V[r][q][p] at the Gauss points (LDS).  out[k] receives the
 // result for column (i,j) = (ta,tb) over k.
-template <int NM, int NQ, typename T, int GMODE, bool HASV, bool WAVE = (64 % (NQ * NQ) == 0)>
+// MX: as for bk3_forward, phases 4, 6, 7 and 8 on the matrix cores (no V);
+// phase 7 applies the product matrix S D^T to the y-flux and S to the rest in
+// one contraction of length 2 NQ.
+template <int NM, int NQ, typename T, int GMODE, bool HASV, bool WAVE = (64 % (NQ * NQ) == 0),
+          bool MX = false>
 __device__ __forceinline__ void
 bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *P0, T *P1,
-             T *P2, const T *V, const bool active, const int ta, const int tb)
+             T *P2, const T *V, const bool active, const int ta, const int tb,
+             T *mx_lds = nullptr, const int mx_cells = 0, const float *mx_a = nullptr)
 {
   constexpr int NQ2 = NQ * NQ, NM2 = NM * NM;
+  static_assert(!MX || (std::is_same<T, float>::value && !WAVE && !HASV), "MX: FP32, no V");
+  [[maybe_unused]] constexpr int NQ3 = NQ2 * NQ, CS = lds_cell_stride<T>(3 * NQ3);
+  auto none = [](int, int) { return 0.f; };
+  auto drop = [](int, int, float) {};
+  if constexpr (MX) {
+    // phase 4: column (r,p), du/dxi_1 [r][q][p] = sum_n D[n][q] U[r][n][p]
+    mx_sweep<NQ, NQ, false>(
+        mx_cells * NQ2, [&](int m, int k) { return mx_a[2 * 256 + m * 16 + k]; }, none,
+        [&](int c, int k) {
+          const int e = c / NQ2, l = c - e * NQ2, r = l / NQ, p = l - r * NQ;
+          return mx_lds[e * CS + r * NQ2 + k * NQ + p];
+        },
+        [&](int c, int m, float v) {
+          const int e = c / NQ2, l = c - e * NQ2, r = l / NQ, p = l - r * NQ;
+          mx_lds[e * CS + NQ3 + r * NQ2 + m * NQ + p] = v;
+        },
+        drop);
+    __syncthreads();
+  } else {
   // ---- phase 4: y-gradient, lane (p,r) = (ta,tb)
   if (active) {
     T c[NQ], w[NQ];
@@ -300,6 +434,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
     for (int q = 0; q < NQ; ++q) P1[tb * NQ2 + q * NQ + ta] = w[q];  // du/dxi_1
   }
   cell_sync<NQ, WAVE>();
+  }
 
   // ---- phase 5: z-gradient, flux G grad u, z-integration; lane (p,q) = (ta,tb)
   if (active) {
@@ -323,6 +458,51 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
   }
   cell_sync<NQ, WAVE>();
 
+  if constexpr (MX) {
+    // phase 6: column (r,q), Z[r][q][p] += sum_n D[p][n] F_x[r][q][n]
+    mx_sweep<NQ, NQ, false>(
+        mx_cells * NQ2, [&](int m, int k) { return mx_a[3 * 256 + m * 16 + k]; }, none,
+        [&](int c, int k) {
+          const int e = c / NQ2, l = c - e * NQ2;
+          return mx_lds[e * CS + 2 * NQ3 + l * NQ + k];
+        },
+        [&](int c, int m, float v) {
+          const int e = c / NQ2, l = c - e * NQ2;
+          mx_lds[e * CS + l * NQ + m] += v;
+        },
+        drop);
+    __syncthreads();
+    // phase 7: column (r,p), W[r][j][p] = sum_q S[j][q] (sum_n D[q][n]
+    // F_y[r][n][p] + Z[r][q][p]); the column is rewritten in place (a column
+    // is read and written by the lanes of one tile only)
+    mx_sweep<NM, 2 * NQ, false>(
+        mx_cells * NQ2, [&](int m, int k) { return mx_a[4 * 256 + m * 16 + k]; }, none,
+        [&](int c, int k) {
+          const int e = c / NQ2, l = c - e * NQ2, r = l / NQ, p = l - r * NQ;
+          return k < NQ ? mx_lds[e * CS + NQ3 + r * NQ2 + k * NQ + p]
+                        : mx_lds[e * CS + r * NQ2 + (k - NQ) * NQ + p];
+        },
+        [&](int c, int m, float v) {
+          const int e = c / NQ2, l = c - e * NQ2, r = l / NQ, p = l - r * NQ;
+          mx_lds[e * CS + NQ3 + r * NQ2 + m * NQ + p] = v;
+        },
+        drop);
+    __syncthreads();
+    // phase 8: column (r,j), Z[r][j][i] = sum_p S[i][p] W[r][j][p]
+    constexpr int L = NQ * NM;
+    mx_sweep<NM, NQ, false>(
+        mx_cells * L, [&](int m, int k) { return mx_a[5 * 256 + m * 16 + k]; }, none,
+        [&](int c, int k) {
+          const int e = c / L, l = c - e * L, r = l / NM, j = l - r * NM;
+          return mx_lds[e * CS + NQ3 + r * NQ2 + j * NQ + k];
+        },
+        [&](int c, int m, float v) {
+          const int e = c / L, l = c - e * L, r = l / NM, j = l - r * NM;
+          mx_lds[e * CS + 2 * NQ3 + r * NM2 + j * NM + m] = v;
+        },
+        drop);
+    __syncthreads();
+  } else {
   // ---- phase 6: x-integration accumulated onto the z part, lane (q,r)
   if (active) {
     T f[NQ], w[NQ];
@@ -362,6 +542,7 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
     for (int i = 0; i < NM; ++i) P2[tb * NM2 + ta * NM + i] = w[i];  // Z[r][j][i]
   }
   cell_sync<NQ, WAVE>();
+  }
 
   // ---- phase 9: z back-interpolation, lane (i,j) = (ta,tb)
   if (active && ta < NM && tb < NM) {
@@ -373,14 +554,17 @@ bk3_backward(T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh, T *
 }
 
 // Both halves: out = A_e u (see bk3_forward / bk3_backward).
-template <int NM, int NQ, typename T, int GMODE, bool WAVE = (64 % (NQ * NQ) == 0)>
+template <int NM, int NQ, typename T, int GMODE, bool WAVE = (64 % (NQ * NQ) == 0), bool MX = false>
 __device__ __forceinline__ void
 bk3_cell_apply(const T (&u)[NM], T (&out)[NM], T (&g)[6][NQ], const T *Gc, const Shape1D<T> &sh,
                T *P0, T *P1, T *P2, const bool active, const int ta, const int tb,
-               T *zbuf = nullptr, const int zn = 0)
+               T *zbuf = nullptr, const int zn = 0, T *mx_lds = nullptr, const int mx_cells = 0,
+               const float *mx_a = nullptr)
 {
-  bk3_forward<NM, NQ, T, WAVE>(u, sh, P0, P1, P2, active, ta, tb, zbuf, zn);
-  bk3_backward<NM, NQ, T, GMODE, false, WAVE>(out, g, Gc, sh, P0, P1, P2, nullptr, active, ta, tb);
+  bk3_forward<NM, NQ, T, WAVE, MX>(u, sh, P0, P1, P2, active, ta, tb, zbuf, zn, mx_lds, mx_cells,
+                                   mx_a);
+  bk3_backward<NM, NQ, T, GMODE, false, WAVE, MX>(out, g, Gc, sh, P0, P1, P2, nullptr, active, ta,
+                                                  tb, mx_lds, mx_cells, mx_a);
 }
 
 }  // namespace pmg

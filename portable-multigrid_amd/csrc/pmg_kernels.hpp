@@ -148,6 +148,10 @@ struct Bk3Schedule {
   // colour add with plain read-modify-writes (no two share a slot);
   // pncol[b] = 0 keeps the atomics for batch b
   const uint8_t *pcolor = nullptr, *pncol = nullptr;
+  // patch schedule, FP32 at nq >= 6: the x and y sweeps of the cell apply as
+  // batched contractions on the matrix cores, with these operand tables
+  // (6 x 16 x 16 floats; null: vector sweeps)
+  const float *mxa = nullptr;
   // gather data (kGather): inverse map of the gn DoFs, goff[gn + 1] into
   // glist (entry positions cell * nm^3 + local, ascending per DoF), and the
   // per-cell buffer gbuf [n_cells][nm^3] of the operator precision

@@ -418,9 +418,47 @@ static void build_patch_schedule(const uint32_t *map, int64_t n_cells, int degre
   }
 }
 
-// The main map's patches, built on first use.
+// The matrices of the cell apply's x and y sweeps in the operand form of the
+// matrix-core contractions: six zero-padded 16 x 16 tables A[m * 16 + k],
+// out[m] = sum_k A[m][k] in[k], with S[i][q] = shape.S[i nq + q] and
+// D[n][q] = shape.D[n nq + q]; products formed in double and rounded once.
+static std::vector<float> fill_mx_tables(const ShapeData &shape, int nm, int nq) {
+  std::vector<float> a(6 * 256, 0.f);
+  auto S = [&](int i, int q) { return shape.S[i * nq + q]; };
+  auto D = [&](int n, int q) { return shape.D[n * nq + q]; };
+  auto set = [&](int t, int m, int k, double v) { a[t * 256 + m * 16 + k] = static_cast<float>(v); };
+  for (int m = 0; m < nq; ++m)
+    for (int k = 0; k < nm; ++k) {
+      set(0, m, k, S(k, m));  // values at the Gauss points
+      double sd = 0;
+      for (int n = 0; n < nq; ++n) sd += S(k, n) * D(n, m);
+      set(1, m, k, sd);  // their derivative, from the nodal values
+    }
+  for (int m = 0; m < nq; ++m)
+    for (int k = 0; k < nq; ++k) {
+      set(2, m, k, D(k, m));  // derivative at the Gauss points
+      set(3, m, k, D(m, k));  // test with the derivative
+    }
+  for (int m = 0; m < nm; ++m)
+    for (int k = 0; k < nq; ++k) {
+      double sd = 0;
+      for (int q = 0; q < nq; ++q) sd += S(m, q) * D(q, k);
+      set(4, m, k, sd);       // test with the basis after the derivative
+      set(4, m, nq + k, S(m, k));  // test with the basis
+      set(5, m, k, S(m, k));
+    }
+  return a;
+}
+
+// The main map's patches, built on first use.  PMG_BK3_MFMA=1 runs the x and
+// y sweeps of the FP32 patch kernel at nq >= 6 on the matrix cores.
 void LaplaceOperatorBK3::build_patches_once() {
   build_patch_schedule(map_.ptr, n_cells_, degree_, nq_, precision_, n_dofs_, patches_, sched_);
+  const char *env = std::getenv("PMG_BK3_MFMA");
+  if (env && std::string(env) == "1" && precision_ == PMG_FP32 && nq_ >= 6 && 2 * nq_ <= 16) {
+    patches_.mxa.upload(fill_mx_tables(shape_, degree_ + 1, nq_));
+    sched_.mxa = patches_.mxa.ptr;
+  }
 }
 
 void LaplaceOperatorBK3::set_box_structure(int nx, int ny, int nz) {

@@ -94,6 +94,7 @@ struct PatchArrays {
   DeviceArray<uint32_t> pdof, pcount, pchunk;
   DeviceArray<uint16_t> pidx, pline;
   DeviceArray<uint8_t> pflag, pcolor, pncol;
+  DeviceArray<float> mxa;  // matrix-core operand tables (fill_mx_tables)
 };
 
 class LaplaceOperatorBK3 : public Operator {

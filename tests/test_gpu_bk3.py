@@ -239,6 +239,34 @@ def test_schedules_agree(pmg, orc, p, nq, sched, gload):
     assert rel_err(dst.cpu().numpy(), ref.vmult(x)) < 1e-12
 
 
+@pytest.mark.parametrize("p,nq", [(4, 6), (5, 6), (5, 7), (6, 7)])
+@pytest.mark.parametrize("accum", ["colored", "atomic"])
+def test_patch_matrix_core_sweeps_match_oracle(pmg, orc, p, nq, accum, monkeypatch):
+    """FP32 patch kernel with the x and y sweeps as batched contractions on
+    the matrix cores (PMG_BK3_MFMA=1), both patch accumulations, on 27
+    deformed cells (the last batch partial): the oracle's result to the FP32
+    bar, and not bitwise the vector sweeps' result (the matrix path ran)."""
+    import torch
+    print(f"[bk3 mfma] p={p} nq={nq} accum={accum}", flush=True)
+    monkeypatch.setenv("PMG_PATCH_ACCUM", accum)
+    op, ref = _pair(pmg, orc, 3, p, nq, 1, pmg.FP32)
+    x = seeded_uniform(op.m(), 700 + 10 * p + nq)
+    y = ref.vmult(x.astype(np.float32).astype(np.float64))
+    xs = torch.tensor(x, dtype=torch.float32, device="cuda")
+    op.set_schedule(pmg.SCHEDULE_PATCH, 0)
+    plain = op.initialize_dof_vector()
+    op.vmult(plain, xs)
+    monkeypatch.setenv("PMG_BK3_MFMA", "1")
+    opm, _ = _pair(pmg, orc, 3, p, nq, 1, pmg.FP32)
+    opm.set_schedule(pmg.SCHEDULE_PATCH, 0)
+    mx = opm.initialize_dof_vector()
+    for _ in range(2):  # a second application gives the same result
+        opm.vmult(mx, xs)
+        assert rel_err(mx.cpu().numpy(), y) < 2e-6
+    assert rel_err(plain.cpu().numpy(), y) < 2e-6
+    assert not torch.equal(mx, plain)
+
+
 @pytest.mark.parametrize("order", ["grouped", "morton"])
 def test_internal_cell_orders(pmg, orc, order, monkeypatch):
     """Internal cell reorderings (and the patch schedule on compact groups)
